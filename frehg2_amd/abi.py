@@ -1,0 +1,154 @@
+"""ctypes mirror of include/frehg_gw.h (the C-ABI of the MI355X groundwater step).
+
+The structures here are byte-for-byte the C structs of the public header; the
+loader finds the in-tree HIP library and fails loudly when it is missing, so no
+code path can silently fall back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libfrehg_gw.so"
+
+ABI_VERSION = 1
+
+FREHG_OK = 0
+FREHG_EINVAL = 1
+FREHG_EDEVICE = 2
+FREHG_EDIVERGED = 3
+FREHG_EUNSUPPORTED = 4
+
+BC_NOFLUX, BC_HEAD, BC_FLUX, BC_GRAVITY, BC_HEAD_HYDROSTATIC = 0, 1, 2, 3, 4
+IC_MOISTURE, IC_HEAD, IC_WATER_TABLE = 0, 1, 2
+PC_JACOBI, PC_ZLINE, PC_ICC0 = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+
+F_HEAD, F_WC, F_HEAD_N, F_WC_N = 0, 1, 2, 3
+F_KX, F_KY, F_KZF = 4, 5, 6
+F_QX, F_QY, F_QZF = 7, 8, 9
+F_VLOSS, F_DZ3D, F_BOT3D = 10, 11, 12
+F_AREA_X, F_AREA_Y, F_SIN_X, F_COS_X, F_SIN_Y, F_COS_Y = 13, 14, 15, 16, 17, 18
+F_CAPACITY = 19
+
+FACE_FIELDS = (F_KZF, F_QZF)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class GwDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("dz_stretch", C.c_double),
+        ("follow_terrain", C.c_int), ("terrain_layers", C.c_int),
+        ("bottom_elevation", _dp), ("bottom_elevation_const", C.c_double),
+        ("ksx", _dp), ("ksy", _dp), ("ksz", _dp), ("vg_alpha", _dp), ("vg_n", _dp),
+        ("theta_s", _dp), ("theta_r", _dp), ("aev", _dp),
+        ("specific_storage", C.c_double),
+        ("use_full3d", C.c_int), ("predictor_corrector", C.c_int),
+        ("face_conductivity", C.c_int), ("saturated_contact", C.c_int),
+        ("reallocation", C.c_int), ("reallocation_surplus", C.c_int),
+        ("dt_init", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+        ("dq_grow", C.c_double), ("dq_shrink", C.c_double), ("courant_max", C.c_double),
+        ("ic_form", C.c_int), ("ic_values", _dp), ("ic_const", C.c_double),
+        ("top_code", _ip), ("bot_code", _ip), ("xm_code", _ip), ("xp_code", _ip),
+        ("ym_code", _ip), ("yp_code", _ip),
+        ("top_value", _dp), ("bot_value", _dp), ("xm_value", _dp), ("xp_value", _dp),
+        ("ym_value", _dp), ("yp_value", _dp),
+        ("rtol", C.c_double), ("atol", C.c_double), ("max_iterations", C.c_int),
+        ("preconditioner", C.c_int), ("precision", C.c_int),
+        ("initial_guess_previous", C.c_int), ("inner_rtol", C.c_double),
+    ]
+
+
+class GwAudit(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "boundary_in", "ss_storage", "realloc_adjust", "realloc_dropped", "vloss",
+        "cpl_exchanged", "cpl_bounce", "cpl_vent", "cpl_evap", "cpl_vent_thin", "pc_clamp")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("converged", C.c_int),
+                ("residual_norm", C.c_double), ("rhs_norm", C.c_double)]
+
+
+class GwTiming(C.Structure):
+    _fields_ = [("step_ms", C.c_double), ("solve_ms", C.c_double), ("spmv_ms", C.c_double),
+                ("spmv_launches", C.c_int64), ("spmv_bytes", C.c_double),
+                ("steps", C.c_int64), ("pcg_iterations", C.c_int64)]
+
+
+# Exported symbols of the product library, with argtypes (the C-ABI surface).
+_vp = C.c_void_p
+SYMBOLS = {
+    "frehg_abi_version": ([], C.c_int),
+    "frehg_last_error": ([], C.c_char_p),
+    "frehg_gw_create": ([C.POINTER(GwDesc), C.POINTER(_vp)], C.c_int),
+    "frehg_gw_destroy": ([_vp], C.c_int),
+    "frehg_gw_step": ([_vp, C.c_double, C.c_double], C.c_int),
+    "frehg_gw_next_dt": ([_vp, _dp], C.c_int),
+    "frehg_gw_get_audit": ([_vp, C.POINTER(GwAudit)], C.c_int),
+    "frehg_gw_owned_volume": ([_vp, _dp], C.c_int),
+    "frehg_gw_last_solve": ([_vp, C.POINTER(SolveStats)], C.c_int),
+    "frehg_gw_active_cells": ([_vp, C.POINTER(C.c_int64)], C.c_int),
+    "frehg_gw_field_size": ([_vp, C.c_int, C.POINTER(C.c_int64)], C.c_int),
+    "frehg_gw_get_field": ([_vp, C.c_int, _dp], C.c_int),
+    "frehg_gw_set_field": ([_vp, C.c_int, _dp], C.c_int),
+    "frehg_gw_set_bc_values": ([_vp, C.c_int, _dp], C.c_int),
+    "frehg_gw_attach_coupling": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int],
+                                 C.c_int),
+    "frehg_gw_set_timing": ([_vp, C.c_int], C.c_int),
+    "frehg_gw_get_timing": ([_vp, C.POINTER(GwTiming)], C.c_int),
+    "frehg_gw_synchronize": ([_vp], C.c_int),
+    "frehg_ls_create": ([C.c_int64, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(_vp)],
+                        C.c_int),
+    "frehg_ls_destroy": ([_vp], C.c_int),
+    "frehg_ls_set_pattern": ([_vp, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+                             C.c_int),
+    "frehg_ls_set_values": ([_vp, _vp], C.c_int),
+    "frehg_ls_solve": ([_vp, _vp, _vp, C.POINTER(SolveStats)], C.c_int),
+    "frehg_ls_is_symmetric": ([_vp, C.c_double, _ip], C.c_int),
+}
+
+
+class FrehgError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[frehg error {code}] {message}")
+        self.code = code
+
+
+_LIB = None
+
+
+def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
+    """Load the in-tree HIP library; raise if it is not built."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FrehgError(FREHG_EDEVICE,
+                         f"HIP library {p} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(str(p))
+    for name, (args, res) in SYMBOLS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if lib.frehg_abi_version() != ABI_VERSION:
+        raise FrehgError(FREHG_EINVAL, "ABI version mismatch")
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def check(lib, rc: int):
+    if rc != FREHG_OK:
+        msg = lib.frehg_last_error()
+        raise FrehgError(rc, msg.decode() if msg else "")

@@ -1,0 +1,260 @@
+/*
+ * frehg_gw.h — C-ABI of the MI355X-native Richards groundwater step.
+ *
+ * Two seams of the reference are replaced:
+ *
+ *  1. The step seam, frehg::gw::RichardsSolver (src/gw/RichardsSolver.hpp:166-423):
+ *     construction from a grid/soil/IC/BC description, step(t, dtg), nextDt(),
+ *     head(), waterContent(), audit(), ownedVolume(), lastSolve(), the flux and
+ *     face-conductivity accessors, attachCoupling().
+ *  2. The solve seam, frehg::LinearSystem (src/core/LinearSystem.hpp:109-151):
+ *     setPattern / setValues / solve on a COO system, replacing the PETSc
+ *     KSPCG call (src/core/LinearSystem.cpp:149,308,320).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a nonzero FREHG_E* code on
+ *    failure; frehg_last_error() then returns a message (thread-local).
+ *    This replaces the reference's throwing log::fatal (src/core/Logger.hpp:53-57).
+ *  - Host arrays are plain C pointers.  Arrays documented "device" are HIP
+ *    device pointers; work on them is ordered on the handle's stream.
+ *  - 3D interior arrays are flattened (j*nx + i)*nz + k with k = 0 the top
+ *    layer (the reference's flat-file order, src/gw/RichardsSolver.cpp:224-228).
+ *    2D column arrays are j*nx + i.
+ *  - Halo-layout fields (the getters) are (ny+2) x (nx+2) x nz with one ghost
+ *    cell in j and i and k unit stride, exactly the reference's Field3
+ *    (src/core/Types.hpp:35-43); z-face fields carry nz+1 planes, plane k the
+ *    face above cell k (src/gw/RichardsSolver.hpp:584-587).
+ */
+#ifndef FREHG_GW_H
+#define FREHG_GW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FREHG_GW_ABI_VERSION 1
+
+/* Error codes. */
+#define FREHG_OK 0
+#define FREHG_EINVAL 1      /* invalid argument or descriptor */
+#define FREHG_EDEVICE 2     /* HIP runtime error */
+#define FREHG_EDIVERGED 3   /* linear solve did not converge (fatal in the reference) */
+#define FREHG_EUNSUPPORTED 4
+
+/* Boundary-condition codes: GwBcCode (src/gw/RichardsSolver.hpp:66-72). */
+#define FREHG_BC_NOFLUX 0
+#define FREHG_BC_HEAD 1
+#define FREHG_BC_FLUX 2
+#define FREHG_BC_GRAVITY 3
+#define FREHG_BC_HEAD_HYDROSTATIC 4
+
+/* initial_conditions.groundwater forms (src/gw/RichardsSolver.cpp:393-413). */
+#define FREHG_IC_MOISTURE 0
+#define FREHG_IC_HEAD 1
+#define FREHG_IC_WATER_TABLE 2
+
+/* Preconditioners of the PCG that replaces KSPCG. */
+#define FREHG_PC_JACOBI 0   /* point Jacobi */
+#define FREHG_PC_ZLINE 1    /* z-line block Jacobi: one tridiagonal solve per column */
+#define FREHG_PC_ICC0 2     /* incomplete Cholesky (CPU oracle only; the reference default) */
+
+/* Arithmetic of the linear solve. */
+#define FREHG_PREC_FP64 0   /* plain FP64 PCG */
+#define FREHG_PREC_MIXED 1  /* FP32 inner PCG inside FP64 iterative refinement */
+
+/*
+ * Everything RichardsSolver's constructor reads from FrehgConfig, Grid,
+ * TerrainMetric and BoundarySet, as plain arrays
+ * (src/gw/RichardsSolver.cpp:61-205; YAML keys src/core/Config.cpp:305-420).
+ * Zero-initialise and fill; NULL array pointers select the *_const value.
+ */
+typedef struct frehg_gw_desc {
+  /* domain.* (src/core/Config.hpp DomainConfig; Grid, TerrainMetric) */
+  int nx, ny, nz;
+  double dx, dy, dz;
+  double dz_stretch;               /* 0 is read as 1 */
+  int follow_terrain;              /* domain.follow_terrain */
+  int terrain_layers;              /* 0 scaled (legacy), 1 uniform */
+  const double* bottom_elevation;  /* ny*nx raster or NULL */
+  double bottom_elevation_const;
+
+  /* soil: per-cell fields, interior cells (j*nx+i)*nz+k.  The reference's
+   * soil.types x soil.map staging (src/gw/RichardsSolver.cpp:207-329)
+   * expands to exactly these arrays; continuous per-cell fields are the
+   * documented extension. */
+  const double *ksx, *ksy, *ksz, *vg_alpha, *vg_n, *theta_s, *theta_r, *aev;
+
+  /* groundwater.* */
+  double specific_storage;
+  int use_full3d;
+  int predictor_corrector;         /* 1 consistent (default); 0 legacy is rejected */
+  int face_conductivity;           /* 0 arithmetic (default), 1 harmonic */
+  int saturated_contact;           /* 1 head (default); 0 legacy is rejected */
+  int reallocation;                /* 0 legacy (default), 1 disabled */
+  int reallocation_surplus;        /* 0 drop (default), 1 redistribute */
+  double dt_init, dt_min, dt_max;  /* groundwater.timestep */
+  double dq_grow, dq_shrink, courant_max;
+
+  /* initial_conditions.groundwater */
+  int ic_form;                     /* FREHG_IC_* */
+  const double* ic_values;         /* per cell (moisture/head) or per column (water_table) */
+  double ic_const;
+
+  /* Rasterized boundary markers (GwBcCode per column / edge cell, ny*nx each).
+   * Side arrays are read at their edge cells only (x-: i = 0, x+: i = nx-1,
+   * y-: j = 0, y+: j = ny-1).  NULL code arrays mean no-flux. */
+  const int *top_code, *bot_code, *xm_code, *xp_code, *ym_code, *yp_code;
+  const double *top_value, *bot_value, *xm_value, *xp_value, *ym_value, *yp_value;
+
+  /* solver.groundwater.* */
+  double rtol, atol;
+  int max_iterations;
+  int preconditioner;              /* FREHG_PC_* */
+  int precision;                   /* FREHG_PREC_*: solver.groundwater.precision (extension) */
+  int initial_guess_previous;      /* 1: start the PCG from h^n; 0: from zero (PETSc default) */
+  double inner_rtol;               /* mixed only: inner FP32 solve tolerance (0 -> 1e-4) */
+} frehg_gw_desc;
+
+/* GwStepAudit (src/gw/RichardsSolver.hpp:86-130), the terms this path books. */
+typedef struct frehg_gw_audit {
+  double boundary_in;
+  double ss_storage;
+  double realloc_adjust;
+  double realloc_dropped;
+  double vloss;
+  double cpl_exchanged;
+  double cpl_bounce;
+  double cpl_vent;
+  double cpl_evap;
+  double cpl_vent_thin;
+  double pc_clamp;                 /* mode-3 clamp volume of this step */
+} frehg_gw_audit;
+
+/* SolveStats (src/core/LinearSystem.hpp:49-53) plus refinement detail. */
+typedef struct frehg_gw_solve_stats {
+  int iterations;                  /* total PCG iterations (inner ones in mixed mode) */
+  int refinements;                 /* FP64 residual sweeps (mixed mode), else 0 */
+  int converged;                   /* 1 converged, 0 not */
+  double residual_norm;            /* final true residual 2-norm */
+  double rhs_norm;                 /* ||b||_2 */
+} frehg_gw_solve_stats;
+
+/* Field identifiers for frehg_gw_get_field / frehg_gw_set_field. */
+#define FREHG_F_HEAD 0             /* h     (H)  RichardsSolver::head() */
+#define FREHG_F_WC 1               /* theta (H)  waterContent() */
+#define FREHG_F_HEAD_N 2           /* h^n   (H) */
+#define FREHG_F_WC_N 3             /* theta^n (H) waterContentStart() */
+#define FREHG_F_KX 4               /* face K x+ (H)  faceConductivityX() */
+#define FREHG_F_KY 5               /* face K y+ (H) */
+#define FREHG_F_KZF 6              /* face K z (F)   faceConductivityZFace() */
+#define FREHG_F_QX 7               /* Darcy volume flux x (H) fluxXVolumetric() */
+#define FREHG_F_QY 8
+#define FREHG_F_QZF 9              /* (F) fluxZFaceVolumetric() */
+#define FREHG_F_VLOSS 10           /* cumulative clamp volume (H) vlossField() */
+#define FREHG_F_DZ3D 11            /* geometry (H) */
+#define FREHG_F_BOT3D 12
+#define FREHG_F_AREA_X 13
+#define FREHG_F_AREA_Y 14
+#define FREHG_F_SIN_X 15
+#define FREHG_F_COS_X 16
+#define FREHG_F_SIN_Y 17
+#define FREHG_F_COS_Y 18
+#define FREHG_F_CAPACITY 19        /* C(h^n) of the last assembly (H) */
+
+/* Opaque handles. */
+typedef struct frehg_gw_solver frehg_gw_solver;
+typedef struct frehg_linear_system frehg_linear_system;
+
+/* Library identity and errors. */
+int frehg_abi_version(void);
+const char* frehg_last_error(void);
+
+/* ---- step seam (replaces frehg::gw::RichardsSolver) ------------------- */
+
+/* RichardsSolver::RichardsSolver (src/gw/RichardsSolver.cpp:61-205) together
+ * with TerrainMetric (src/gw/TerrainMetric.cpp:69-99) and
+ * Grid::buildGlobalIds (src/core/Grid.cpp:160-245): stages geometry, soil,
+ * boundary markers and the initial state on device. */
+int frehg_gw_create(const frehg_gw_desc* desc, frehg_gw_solver** out);
+int frehg_gw_destroy(frehg_gw_solver* s);
+
+/* RichardsSolver::step(t, dtg) (src/gw/RichardsSolver.cpp:729-794). */
+int frehg_gw_step(frehg_gw_solver* s, double t, double dtg);
+/* RichardsSolver::nextDt() (src/gw/RichardsSolver.hpp:261). */
+int frehg_gw_next_dt(const frehg_gw_solver* s, double* dt);
+/* RichardsSolver::audit() (src/gw/RichardsSolver.hpp:419). */
+int frehg_gw_get_audit(const frehg_gw_solver* s, frehg_gw_audit* out);
+/* RichardsSolver::ownedVolume() (src/gw/RichardsSolver.cpp:892-912). */
+int frehg_gw_owned_volume(frehg_gw_solver* s, double* volume);
+/* RichardsSolver::lastSolve() (src/gw/RichardsSolver.hpp:402). */
+int frehg_gw_last_solve(const frehg_gw_solver* s, frehg_gw_solve_stats* out);
+/* Active cell count (Grid::activeCount3Local, src/core/Grid.hpp:137). */
+int frehg_gw_active_cells(const frehg_gw_solver* s, int64_t* n);
+/* Element count of a field in halo layout (H or F). */
+int frehg_gw_field_size(const frehg_gw_solver* s, int field, int64_t* n);
+/* Copy a halo-layout field to host memory (the head()/waterContent()/flux
+ * accessors, src/gw/RichardsSolver.hpp:265-358). */
+int frehg_gw_get_field(frehg_gw_solver* s, int field, double* host_out);
+/* Overwrite h or theta from host memory and rebuild the derived state
+ * (checkpoint restore + RichardsSolver::refreshDerivedState,
+ * src/gw/RichardsSolver.cpp:880-890).  Only FREHG_F_HEAD / FREHG_F_WC. */
+int frehg_gw_set_field(frehg_gw_solver* s, int field, const double* host_in);
+/* Refresh per-column boundary values (updateBoundaryValues,
+ * src/gw/RichardsSolver.cpp:526-566); which = 0 top, 1 bottom, 2 x-, 3 x+,
+ * 4 y-, 5 y+; values is ny*nx. */
+int frehg_gw_set_bc_values(frehg_gw_solver* s, int which, const double* host_values);
+
+/* GwCoupling (src/gw/RichardsSolver.hpp:138-163): device arrays in 2D halo
+ * layout (ny+2)*(nx+2), aliased and mutated by the step exactly where the
+ * reference mutates them.  bounce may be NULL.  RichardsSolver::attachCoupling. */
+int frehg_gw_attach_coupling(frehg_gw_solver* s, double* eta, double* dept, double* seep_accum,
+                             double* avail, double* gain, double* bounce, double min_depth,
+                             int vent_all);
+
+/* Per-phase and per-kernel timing (HIP events on the solver's stream). */
+typedef struct frehg_gw_timing {
+  double step_ms;          /* accumulated wall of whole steps (events) */
+  double solve_ms;         /* accumulated PCG time */
+  double spmv_ms;          /* accumulated duration of the SpMV kernel launches */
+  int64_t spmv_launches;
+  double spmv_bytes;       /* algorithmic bytes per SpMV launch */
+  int64_t steps;
+  int64_t pcg_iterations;
+} frehg_gw_timing;
+/* Enable (1) or disable (0) per-launch event timing of the SpMV kernel and
+ * reset the accumulators. */
+int frehg_gw_set_timing(frehg_gw_solver* s, int enable);
+int frehg_gw_get_timing(frehg_gw_solver* s, frehg_gw_timing* out);
+/* Block until all work on the solver's stream is done. */
+int frehg_gw_synchronize(frehg_gw_solver* s);
+
+/* ---- solve seam (replaces frehg::LinearSystem / PETSc KSPCG) ---------- */
+
+/* LinearSystem(comm, prefix, nLocalRows, nGlobalRows, settings)
+ * (src/core/LinearSystem.hpp:109-110; defaults :72-97). */
+int frehg_ls_create(int64_t n_rows, double rtol, double atol, int max_iterations,
+                    int preconditioner, frehg_linear_system** out);
+int frehg_ls_destroy(frehg_linear_system* ls);
+/* LinearSystem::setPattern (src/core/LinearSystem.cpp:273-296): host COO
+ * index arrays; entries with row or column -1 are ignored, duplicates sum. */
+int frehg_ls_set_pattern(frehg_linear_system* ls, int64_t n_coo, const int64_t* rows,
+                         const int64_t* cols);
+/* LinearSystem::setValues (src/core/LinearSystem.cpp:298-309): device array
+ * of n_coo values in pattern order. */
+int frehg_ls_set_values(frehg_linear_system* ls, const double* values_device);
+/* LinearSystem::solve (src/core/LinearSystem.cpp:332-484): device rhs and
+ * solution arrays of n_rows; zero initial guess, CG, true-residual test
+ * ||b - A x||_2 <= max(rtol ||b||_2, atol).  Non-convergence returns
+ * FREHG_EDIVERGED (fatal in the reference, :426-431). */
+int frehg_ls_solve(frehg_linear_system* ls, const double* rhs_device, double* sol_device,
+                   frehg_gw_solve_stats* stats);
+/* LinearSystem::isSymmetric (src/core/LinearSystem.hpp:137). */
+int frehg_ls_is_symmetric(frehg_linear_system* ls, double tol, int* symmetric);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FREHG_GW_H */

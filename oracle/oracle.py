@@ -1,0 +1,246 @@
+"""ctypes wrapper of the CPU FP64 oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this module, and only as the checker / CPU baseline.  It mirrors the Python
+interface of frehg2_amd.gw.RichardsSolver so the parity tests can drive both
+with the same case.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from frehg2_amd import abi
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "liboracle_gw.so"
+
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_SIG = {
+    "oracle_last_error": ([], C.c_char_p),
+    "oracle_set_threads": ([C.c_int], None),
+    "oracle_get_threads": ([], C.c_int),
+    "oracle_vg_theta": ([C.c_double] * 6, C.c_double),
+    "oracle_vg_head": ([C.c_double] * 6, C.c_double),
+    "oracle_vg_capacity": ([C.c_double] * 6, C.c_double),
+    "oracle_vg_conductivity": ([C.c_double] * 7, C.c_double),
+    "oracle_vg_dkdtheta": ([C.c_double] * 7, C.c_double),
+    "oracle_face_k_harmonic": ([C.c_double] * 16, C.c_double),
+    "oracle_gw_create": ([C.POINTER(abi.GwDesc), C.POINTER(_vp)], C.c_int),
+    "oracle_gw_destroy": ([_vp], C.c_int),
+    "oracle_gw_step": ([_vp, C.c_double, C.c_double], C.c_int),
+    "oracle_gw_next_dt": ([_vp, _dp], C.c_int),
+    "oracle_gw_audit": ([_vp, C.POINTER(abi.GwAudit)], C.c_int),
+    "oracle_gw_owned_volume": ([_vp, _dp], C.c_int),
+    "oracle_gw_last_solve": ([_vp, C.POINTER(abi.SolveStats)], C.c_int),
+    "oracle_gw_active_cells": ([_vp, C.POINTER(C.c_int64)], C.c_int),
+    "oracle_gw_field_size": ([_vp, C.c_int, C.POINTER(C.c_int64)], C.c_int),
+    "oracle_gw_get_field": ([_vp, C.c_int, _dp], C.c_int),
+    "oracle_gw_set_field": ([_vp, C.c_int, _dp], C.c_int),
+    "oracle_gw_set_bc_values": ([_vp, C.c_int, _dp], C.c_int),
+    "oracle_gw_attach_coupling": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int],
+                                  C.c_int),
+    "oracle_gw_ktop": ([_vp, C.POINTER(C.c_int)], C.c_int),
+    "oracle_gw_geometry_scalars": ([_vp, _dp], C.c_int),
+    "oracle_gw_system": ([_vp, _dp, _dp, C.POINTER(C.c_int64)], C.c_int),
+    "oracle_gw_assemble_only": ([_vp, C.c_double], C.c_int),
+    "oracle_ls_create": ([C.c_int64, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(_vp)],
+                         C.c_int),
+    "oracle_ls_destroy": ([_vp], C.c_int),
+    "oracle_ls_set_pattern": ([_vp, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+                              C.c_int),
+    "oracle_ls_set_values": ([_vp, _dp], C.c_int),
+    "oracle_ls_solve": ([_vp, _dp, _dp, C.POINTER(abi.SolveStats)], C.c_int),
+    "oracle_ls_is_symmetric": ([_vp, C.c_double, C.POINTER(C.c_int)], C.c_int),
+}
+
+_LIB = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        for name, (args, res) in _SIG.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise abi.FrehgError(rc, lib().oracle_last_error().decode())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def set_threads(n: int) -> None:
+    lib().oracle_set_threads(int(n))
+
+
+class OracleRichards:
+    """The oracle's RichardsSolver (same methods as frehg2_amd.gw.RichardsSolver)."""
+
+    def __init__(self, case):
+        self.case = case
+        desc, keep = case.to_desc()
+        h = _vp()
+        _check(lib().oracle_gw_create(C.byref(desc), C.byref(h)))
+        self._h = h
+        self.nx, self.ny, self.nz = case.nx, case.ny, case.nz
+        self.t = 0.0
+        self._coupling = None
+        if case.coupling is not None:
+            self.attach_coupling(**case.coupling)
+
+    def close(self):
+        if self._h:
+            lib().oracle_gw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field_shape(self, fid):
+        nk = self.nz + 1 if fid in abi.FACE_FIELDS else self.nz
+        return (self.ny + 2, self.nx + 2, nk)
+
+    def get(self, fid) -> np.ndarray:
+        out = np.empty(self.field_shape(fid), dtype=np.float64)
+        _check(lib().oracle_gw_get_field(self._h, fid, _dptr(out)))
+        return out
+
+    def set(self, fid, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.field_shape(fid))
+        _check(lib().oracle_gw_set_field(self._h, fid, _dptr(a)))
+
+    def set_bc_values(self, which: int, values) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(self.ny, self.nx)
+        _check(lib().oracle_gw_set_bc_values(self._h, which, _dptr(a)))
+
+    def attach_coupling(self, eta, dept, seep_accum, avail, gain, bounce=None, min_depth=0.0,
+                        vent_all=0):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).copy() for a in
+                (eta, dept, seep_accum, avail, gain)]
+        b = np.ascontiguousarray(bounce, dtype=np.float64).copy() if bounce is not None else None
+        self._coupling = dict(zip(("eta", "dept", "seep_accum", "avail", "gain"), arrs))
+        self._coupling["bounce"] = b
+        ptrs = [_dptr(a) for a in arrs] + [_dptr(b) if b is not None else None]
+        _check(lib().oracle_gw_attach_coupling(self._h, *ptrs, float(min_depth), int(vent_all)))
+
+    def coupling_state(self, name) -> np.ndarray:
+        return self._coupling[name].copy()
+
+    def step(self, dt=None, t=None):
+        if dt is None:
+            dt = self.next_dt()
+        self.t = self.t + dt if t is None else t
+        _check(lib().oracle_gw_step(self._h, self.t, dt))
+        return dt
+
+    def next_dt(self) -> float:
+        v = C.c_double()
+        _check(lib().oracle_gw_next_dt(self._h, C.byref(v)))
+        return v.value
+
+    def audit(self) -> dict:
+        a = abi.GwAudit()
+        _check(lib().oracle_gw_audit(self._h, C.byref(a)))
+        return a.as_dict()
+
+    def volume(self) -> float:
+        v = C.c_double()
+        _check(lib().oracle_gw_owned_volume(self._h, C.byref(v)))
+        return v.value
+
+    def last_solve(self) -> dict:
+        s = abi.SolveStats()
+        _check(lib().oracle_gw_last_solve(self._h, C.byref(s)))
+        return {n: getattr(s, n) for n, _ in s._fields_}
+
+    def active_cells(self) -> int:
+        v = C.c_int64()
+        _check(lib().oracle_gw_active_cells(self._h, C.byref(v)))
+        return v.value
+
+    def ktop(self) -> np.ndarray:
+        out = np.empty((self.ny, self.nx), dtype=np.int32)
+        _check(lib().oracle_gw_ktop(self._h, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
+    def geometry_scalars(self) -> dict:
+        out = np.empty(3)
+        _check(lib().oracle_gw_geometry_scalars(self._h, _dptr(out)))
+        return {"offset": out[0], "box_top": out[1], "pc_clamp_total": out[2]}
+
+    def assemble_only(self, dtg: float):
+        _check(lib().oracle_gw_assemble_only(self._h, dtg))
+        return self.system()
+
+    def system(self):
+        n = self.active_cells()
+        vals = np.empty(7 * n)
+        rhs = np.empty(n)
+        cols = np.empty(7 * n, dtype=np.int64)
+        _check(lib().oracle_gw_system(self._h, _dptr(vals), _dptr(rhs),
+                                      cols.ctypes.data_as(C.POINTER(C.c_int64))))
+        return vals.reshape(n, 7), rhs, cols.reshape(n, 7)
+
+
+def vg(fn: str, soil, *args) -> float:
+    """Call one oracle closure: soil = (alpha, n, theta_s, theta_r, aev)."""
+    return getattr(lib(), f"oracle_vg_{fn}")(*soil, *args)
+
+
+class OracleLinearSystem:
+    def __init__(self, n, rtol=1e-8, atol=1e-14, max_iterations=500, pc=abi.PC_JACOBI):
+        h = _vp()
+        _check(lib().oracle_ls_create(n, rtol, atol, max_iterations, pc, C.byref(h)))
+        self._h = h
+        self.n = n
+
+    def __del__(self):
+        try:
+            lib().oracle_ls_destroy(self._h)
+        except Exception:
+            pass
+
+    def set_pattern(self, rows, cols):
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        c = np.ascontiguousarray(cols, dtype=np.int64)
+        self.n_coo = r.size
+        _check(lib().oracle_ls_set_pattern(self._h, r.size, r.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           c.ctypes.data_as(C.POINTER(C.c_int64))))
+
+    def set_values(self, vals):
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        _check(lib().oracle_ls_set_values(self._h, _dptr(v)))
+
+    def solve(self, b):
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n)
+        st = abi.SolveStats()
+        _check(lib().oracle_ls_solve(self._h, _dptr(bb), _dptr(x), C.byref(st)))
+        return x, {n: getattr(st, n) for n, _ in st._fields_}
+
+    def is_symmetric(self, tol):
+        s = C.c_int()
+        _check(lib().oracle_ls_is_symmetric(self._h, tol, C.byref(s)))
+        return bool(s.value)
