@@ -1,0 +1,188 @@
+// gw_device.h — device-side building blocks of the MI355X Richards step:
+// grid indexing in the halo layout, the van Genuchten-Mualem closures,
+// wave64 / block reductions with compensated (Neumaier) accumulation and the
+// XCD-aware work mapping.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace frehg {
+
+constexpr int kWave = 64;
+constexpr int kXcds = 8;  // MI355X: workgroups are dispatched round-robin over 8 XCDs
+
+// Halo layout (src/core/Types.hpp:35-43): (j, i, k), one ghost in j and i,
+// k unit stride; z-face fields carry nz + 1 planes (plane k = face above k).
+struct Grid {
+  int nx, ny, nz;
+  int nx2, ny2;
+  double dx, dy, az;
+  __host__ __device__ int64_t c3(int j, int i, int k) const {
+    return (static_cast<int64_t>(j) * nx2 + i) * nz + k;
+  }
+  __host__ __device__ int64_t cF(int j, int i, int k) const {
+    return (static_cast<int64_t>(j) * nx2 + i) * (nz + 1) + k;
+  }
+  __host__ __device__ int c2(int j, int i) const { return j * nx2 + i; }
+  __host__ __device__ int64_t cellsH() const { return static_cast<int64_t>(ny2) * nx2 * nz; }
+  __host__ __device__ int64_t cellsF() const { return static_cast<int64_t>(ny2) * nx2 * (nz + 1); }
+  __host__ __device__ int columns() const { return nx * ny; }
+};
+
+// ---------------------------------------------------------------------------
+// van Genuchten-Mualem closures (src/gw/VanGenuchten.hpp:45-128).  The
+// expression order of each closure is the reference's; the device pow/log/exp
+// are ROCm's own implementations.
+// ---------------------------------------------------------------------------
+struct VgSoil {
+  double a, n, ts, tr, aev;
+};
+
+__device__ __forceinline__ double vg_theta(const VgSoil& s, double h) {
+  const double m = 1.0 - 1.0 / s.n;
+  double wc;
+  if (h > s.aev) {
+    wc = s.ts;
+  } else {
+    const double se = pow(1.0 + pow(fabs(s.a * h), s.n), -m);
+    wc = s.tr + (s.ts - s.tr) * se;
+  }
+  if (wc > s.ts) wc = s.ts;
+  else if (wc < s.tr) wc = s.tr;
+  return wc;
+}
+
+__device__ __forceinline__ double vg_head(const VgSoil& s, double wc) {
+  const double eps = 1.0e-7;
+  const double m = 1.0 - 1.0 / s.n;
+  if (wc - s.tr < eps) wc = s.tr + eps;
+  if (wc < s.ts) {
+    const double ratio = (s.ts - s.tr) / (wc - s.tr);
+    return -(1.0 / s.a) * pow(pow(ratio, 1.0 / m) - 1.0, 1.0 / s.n);
+  }
+  return 0.0;
+}
+
+__device__ __forceinline__ double vg_capacity(const VgSoil& s, double h) {
+  if (h > 0.0) return 0.0;
+  const double m = 1.0 - 1.0 / s.n;
+  const double ah = fabs(s.a * h);
+  const double top = s.a * s.n * m * (s.ts - s.tr) * pow(ah, s.n - 1.0);
+  const double bot = pow(1.0 + pow(ah, s.n), m + 1.0);
+  return top / bot;
+}
+
+// The Mualem factors shared by every Ks the cell carries: K(h, ks) =
+// ((ks * sqrt_se) * br) * br, capped at ks and = ks for h > 0.
+struct MualemFactors {
+  double sq, br;
+  bool sat;
+};
+__device__ __forceinline__ MualemFactors vg_mualem(const VgSoil& s, double h) {
+  const double m = 1.0 - 1.0 / s.n;
+  const double se = pow(1.0 + pow(fabs(s.a * h), s.n), -m);
+  const double br = 1.0 - pow(1.0 - pow(se, 1.0 / m), m);
+  return MualemFactors{pow(se, 0.5), br, h > 0.0};
+}
+__device__ __forceinline__ double vg_k_from(const MualemFactors& f, double ks) {
+  double k = ks * f.sq * f.br * f.br;
+  if (k > ks) k = ks;
+  if (f.sat) k = ks;
+  return k;
+}
+__device__ __forceinline__ double vg_conductivity(const VgSoil& s, double h, double ks) {
+  return vg_k_from(vg_mualem(s, h), ks);
+}
+
+__device__ __forceinline__ double vg_dkdtheta(const VgSoil& s, double wc, double ks) {
+  if (wc > 0.9999 * s.ts && wc < s.ts) wc = 0.9999 * s.ts;
+  const double m = 1.0 - 1.0 / s.n;
+  const double se = (wc - s.tr) / (s.ts - s.tr);
+  const double t0 = pow(1.0 - pow(se, 1.0 / m), m);
+  const double t1 = 0.5 * ks * pow(se, -0.5) * (1.0 - t0) * (1.0 - t0);
+  const double t2 = 2.0 * ks * pow(se, (2.0 - m) / (2.0 * m)) * (1.0 - t0) *
+                    pow(1.0 - pow(se, 1.0 / m), m - 1.0);
+  return (t1 + t2) / (s.ts - s.tr);
+}
+
+// FaceConductivity.hpp:37-58 from the flanks' closures ka = K(h_a, ks_a).
+__device__ __forceinline__ double face_k_harmonic(double ka, double ksa, double la, double kb,
+                                                  double ksb, double lb) {
+  if (ksa <= 0.0 || ksb <= 0.0) return 0.0;
+  const double ksh = (la + lb) / (la / ksa + lb / ksb);
+  return ksh * 0.5 * (ka / ksa + kb / ksb);
+}
+
+// ---------------------------------------------------------------------------
+// Reductions.  Neumaier-compensated accumulation per thread, a fixed-shape
+// wave64 butterfly, then a fixed-order combine of the block's waves: the
+// result depends only on the launch shape, never on timing.
+// ---------------------------------------------------------------------------
+struct KSum {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double v) {
+    const double t = s + v;
+    if (fabs(s) >= fabs(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return s + c; }
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+// Block-wide reduction of NV values; every thread receives the results.
+// op: 0 sum, 1 max, 2 min.  Requires blockDim.x a multiple of 64, <= 1024.
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], const int (&op)[NV]) {
+  __shared__ double sh[16][NV];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    v[q] = op[q] == 0 ? wave_sum(v[q]) : (op[q] == 1 ? wave_max(v[q]) : wave_min(v[q]));
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sh[wid][q] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double r = sh[0][q];
+    for (int w = 1; w < nw; ++w)
+      r = op[q] == 0 ? r + sh[w][q] : (op[q] == 1 ? fmax(r, sh[w][q]) : fmin(r, sh[w][q]));
+    v[q] = r;
+  }
+  __syncthreads();
+}
+
+// XCD-aware remap of a 1D grid: the blocks an XCD receives (b % 8 == x)
+// cover one contiguous range of logical blocks, so stencil neighbours a few
+// rows apart are served from the same L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t b = blockIdx.x;
+  const int64_t nb = gridDim.x;
+  const int64_t x = b % kXcds;
+  const int64_t per = nb / kXcds;
+  const int64_t rem = nb % kXcds;
+  return x * per + (x < rem ? x : rem) + b / kXcds;
+}
+
+}  // namespace frehg

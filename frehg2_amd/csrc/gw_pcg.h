@@ -1,0 +1,87 @@
+// gw_pcg.h — the structured-grid PCG that replaces PETSc KSPCG for the
+// Richards head system (src/core/LinearSystem.cpp:147-227, :332-484).
+//
+// The operator is the symmetric 7-point stencil stored as four halo-layout
+// arrays: Ad (diagonal), Axp / Ayp / Azp (coupling of a cell with its i+1 /
+// j+1 / k+1 neighbour, zero where that neighbour is missing or inactive).
+// Inactive interior rows are identity rows with a zero right-hand side.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gw_device.h"
+
+namespace frehg {
+
+template <class T>
+struct Stencil {
+  T *Ad = nullptr, *Axp = nullptr, *Ayp = nullptr, *Azp = nullptr;
+};
+
+struct PcgStats {
+  int iterations = 0;
+  int refinements = 0;
+  int converged = 0;
+  double residual_norm = 0.0;
+  double rhs_norm = 0.0;
+  double spmv_ms = 0.0;       // accumulated SpMV-kernel time (timing mode)
+  int64_t spmv_launches = 0;
+};
+
+struct PcgOptions {
+  double rtol = 1e-8, atol = 1e-14;
+  int max_iterations = 1000;
+  bool zline = true;          // z-line block Jacobi, else point Jacobi
+  bool mixed = false;         // FP32 inner PCG inside FP64 refinement
+  double inner_rtol = 1e-4;   // relative reduction of each inner FP32 solve
+  int max_refinements = 40;
+  bool timing = false;        // bracket each SpMV launch with HIP events
+};
+
+// Device-resident PCG workspace for one grid (allocated once).
+class Pcg {
+ public:
+  Pcg(const Grid& g, hipStream_t stream);
+  ~Pcg();
+  Pcg(const Pcg&) = delete;
+  Pcg& operator=(const Pcg&) = delete;
+
+  // Solve A x = b; x holds the initial guess on entry (zero on inactive
+  // rows) and the solution on exit.  Returns 0, or FREHG_EDIVERGED.
+  int solve(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+            PcgStats* stats);
+
+  // Algorithmic bytes of one launch of the stencil kernel (the dominant one).
+  double spmvBytesPerLaunch(bool mixed) const;
+
+ private:
+  int solveFp64(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                PcgStats* st);
+  int solveMixed(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                 PcgStats* st);
+  template <class T>
+  int innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T* z, T* q,
+                      const T* invd, const T* w, bool zline, double tol2, int maxIt,
+                      const PcgOptions& opt, PcgStats* st, int* itDone);
+  void reduceHost(int nvals, double* out);
+
+  Grid g_;
+  hipStream_t stream_;
+  int blocks_ = 0;            // grid size of the column kernels (multiple of 8)
+  // FP64 vectors (halo layout)
+  double *r_ = nullptr, *z_ = nullptr, *p0_ = nullptr, *p1_ = nullptr, *q_ = nullptr;
+  double *invd_ = nullptr, *w_ = nullptr;
+  // FP32 copies for the mixed form
+  float *Ad32_ = nullptr, *Axp32_ = nullptr, *Ayp32_ = nullptr, *Azp32_ = nullptr;
+  float *r32_ = nullptr, *z32_ = nullptr, *p032_ = nullptr, *p132_ = nullptr, *q32_ = nullptr;
+  float *d32_ = nullptr, *invd32_ = nullptr, *w32_ = nullptr;
+  double* parts_ = nullptr;   // 4 x blocks_ partial sums
+  double* red_ = nullptr;     // reduced values (device)
+  double* redHost_ = nullptr; // pinned mirror
+  void* state_ = nullptr;     // PcgState (device)
+  void* stateHost_ = nullptr; // pinned mirror
+  hipEvent_t evA_[2] = {nullptr, nullptr};
+};
+
+}  // namespace frehg

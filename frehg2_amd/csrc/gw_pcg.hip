@@ -1,0 +1,834 @@
+// gw_pcg.hip — structured-grid preconditioned CG on gfx950.
+//
+// Replaces PETSc KSPCG (src/core/LinearSystem.cpp:149, :320) for the Richards
+// head system.  Convergence test: ||b - A x||_2 <= max(rtol ||b||_2, atol) on
+// the recursive residual, confirmed on the true residual (a restart if the
+// two disagree).  PETSc's default tests the preconditioned norm; the true
+// residual is the preconditioner-independent form of the same tolerance.
+//
+// Work mapping ("column waves"): one 64-lane wavefront owns one z-column, lane
+// l holding rows k = l + 64 m (m < R = ceil(nz / 64)), so every load and store
+// along k is coalesced, k +- 1 neighbours come from lane shuffles and the
+// z-line (tridiagonal) preconditioner is two wave-level affine scans in
+// registers.  Columns are split into 8 contiguous ranges, one per XCD, so the
+// i +- 1 / j +- 1 neighbour columns a wave reads were brought into the same L2
+// by the waves next to it.
+//
+// One iteration is two kernels:
+//   dir:    p = z + beta p (beta from the previous update's partial sums,
+//           recomputed at the 4 lateral neighbours), q = A p, partial p.q
+//   update: alpha = rz / p.q, x += alpha p, r -= alpha q, z = M^-1 r,
+//           partial r.z and r.r
+// Each kernel reduces the other's per-block partials redundantly in every
+// block, in a fixed order, so the scalars are deterministic and no host
+// round trip sits inside the iteration.  Block 0 mirrors the scalars to a
+// state struct the host polls every few iterations.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/frehg_gw.h"
+#include "gw_pcg.h"
+
+namespace frehg {
+namespace {
+
+struct PcgState {
+  double rz[2];
+  double rr;
+  double alpha;
+  int it;
+  int done;
+};
+
+constexpr int kBlock = 256;
+constexpr int kWpb = kBlock / kWave;
+constexpr int kChunk = 16;   // rows per LDS tile in the factorization
+constexpr int kPoll = 8;     // iterations between host polls
+
+#define HIP_OK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      throw std::runtime_error(std::string("HIP: ") + hipGetErrorString(e_) + " at " + \
+                               __FILE__ + ":" + std::to_string(__LINE__));            \
+  } while (0)
+
+__device__ __forceinline__ void column_loop(int ncol, int64_t& first, int64_t& end,
+                                            int64_t& stride) {
+  const int xcd = blockIdx.x % kXcds;
+  const int bpx = gridDim.x / kXcds;
+  const int local = blockIdx.x / kXcds;
+  const int64_t c0 = static_cast<int64_t>(xcd) * ncol / kXcds;
+  end = static_cast<int64_t>(xcd + 1) * ncol / kXcds;
+  first = c0 + static_cast<int64_t>(local) * kWpb + threadIdx.x / kWave;
+  stride = static_cast<int64_t>(bpx) * kWpb;
+}
+
+__device__ __forceinline__ int64_t column_base(const Grid& g, int64_t col) {
+  const int j = static_cast<int>(col / g.nx) + 1;
+  const int i = static_cast<int>(col % g.nx) + 1;
+  return g.c3(j, i, 0);
+}
+
+// Redundant in-block reduction of the previous kernel's partial sums.
+template <int NV>
+__device__ __forceinline__ void reduce_parts(const double* parts, int nparts, const int (&slot)[NV],
+                                             double (&out)[NV]) {
+  int op[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    out[q] = 0.0;
+    op[q] = 0;
+  }
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] += parts[slot[q] * nparts + b];
+  }
+  block_reduce<NV>(out, op);
+}
+
+template <int NV>
+__device__ __forceinline__ void write_parts(double* parts, const int (&slot)[NV],
+                                            double (&v)[NV]) {
+  int op[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) op[q] = 0;
+  block_reduce<NV>(v, op);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) parts[slot[q] * gridDim.x + blockIdx.x] = v[q];
+  }
+}
+
+// Value of row k-1 / k+1 of a column held as v[m] at row lane + 64 m.
+template <class T, int R>
+__device__ __forceinline__ void shift_rows(const T (&v)[R], int lane, T (&below)[R],
+                                           T (&above)[R]) {
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const T up = __shfl_up(v[m], 1, kWave);
+    const T upSeg = (m > 0) ? __shfl(v[m > 0 ? m - 1 : 0], kWave - 1, kWave) : T(0);
+    above[m] = lane > 0 ? up : upSeg;  // row k - 1
+    const T dn = __shfl_down(v[m], 1, kWave);
+    const T dnSeg = (m + 1 < R) ? __shfl(v[m + 1 < R ? m + 1 : m], 0, kWave) : T(0);
+    below[m] = lane < kWave - 1 ? dn : dnSeg;  // row k + 1
+  }
+}
+
+// z = M^-1 r for one column: point Jacobi (LINE = false) or the exact
+// tridiagonal solve of the column block (LINE = true) with the LDL^T factors
+// invd_k = 1 / delta_k and w_k = Azp_k / delta_k:
+//   forward  y_k = r_k - w_{k-1} y_{k-1}
+//   backward z_k = y_k invd_k - w_k z_{k+1}
+// Both recurrences are affine, y_k = a_k + b_k y_{k-1}, and run as
+// Hillis-Steele scans over the 64 lanes of each row segment.
+template <class T, int R, bool LINE>
+__device__ __forceinline__ void apply_precond(int lane, int nz, int64_t base, const T* __restrict__ invd,
+                                              const T* __restrict__ w, const T (&r)[R], T (&z)[R]) {
+  T iv[R];
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int k = lane + kWave * m;
+    iv[m] = k < nz ? invd[base + k] : T(0);
+  }
+  if constexpr (!LINE) {
+#pragma unroll
+    for (int m = 0; m < R; ++m) z[m] = r[m] * iv[m];
+    return;
+  } else {
+    T wv[R], wPrev[R], dummy[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      wv[m] = k < nz ? w[base + k] : T(0);
+    }
+    shift_rows<T, R>(wv, lane, dummy, wPrev);
+    T y[R];
+    T carry = T(0);
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      T a = r[m];
+      T b = -wPrev[m];
+#pragma unroll
+      for (int s = 1; s < kWave; s <<= 1) {
+        const T ap = __shfl_up(a, s, kWave);
+        const T bp = __shfl_up(b, s, kWave);
+        if (lane >= s) {
+          a = a + b * ap;
+          b = b * bp;
+        }
+      }
+      y[m] = a + b * carry;
+      carry = __shfl(y[m], kWave - 1, kWave);
+    }
+    carry = T(0);
+#pragma unroll
+    for (int m = R - 1; m >= 0; --m) {
+      const int k = lane + kWave * m;
+      T a = k < nz ? y[m] * iv[m] : T(0);
+      T b = k < nz ? -wv[m] : T(0);
+#pragma unroll
+      for (int s = 1; s < kWave; s <<= 1) {
+        const T an = __shfl_down(a, s, kWave);
+        const T bn = __shfl_down(b, s, kWave);
+        if (lane + s < kWave) {
+          a = a + b * an;
+          b = b * bn;
+        }
+      }
+      z[m] = a + b * carry;
+      carry = __shfl(z[m], 0, kWave);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Factorization of the column blocks (once per solve).  One wave owns 64
+// consecutive columns; rows are staged through LDS in 16-row tiles so the
+// global loads and stores stay coalesced while each lane runs its column's
+// sequential LDL^T recurrence  delta_k = Ad_k - Azp_{k-1} w_{k-1}.
+// ---------------------------------------------------------------------------
+template <class T, bool LINE>
+__global__ __launch_bounds__(64) void k_factor(Grid g, const T* __restrict__ Ad,
+                                               const T* __restrict__ Azp, T* __restrict__ invd,
+                                               T* __restrict__ w) {
+  __shared__ int64_t sbase[kWave];
+  __shared__ T sA[kWave][kChunk + 1];
+  __shared__ T sE[kWave][kChunk + 1];
+  const int lane = threadIdx.x;
+  const int ncol = g.columns();
+  const int64_t col0 = static_cast<int64_t>(blockIdx.x) * kWave;
+  const int64_t mycol = col0 + lane;
+  sbase[lane] = mycol < ncol ? column_base(g, mycol) : -1;
+  __syncthreads();
+  T ePrev = T(0), wPrev = T(0);
+  for (int k0 = 0; k0 < g.nz; k0 += kChunk) {
+#pragma unroll 4
+    for (int m = 0; m < kChunk; ++m) {
+      const int e = m * kWave + lane;
+      const int cc = e / kChunk, kk = e % kChunk, k = k0 + kk;
+      if (sbase[cc] >= 0 && k < g.nz) {
+        sA[cc][kk] = Ad[sbase[cc] + k];
+        if (LINE) sE[cc][kk] = Azp[sbase[cc] + k];
+      }
+    }
+    __syncthreads();
+    if (mycol < ncol) {
+      for (int kk = 0; kk < kChunk && k0 + kk < g.nz; ++kk) {
+        if (LINE) {
+          const T e = sE[lane][kk];
+          const T d = sA[lane][kk] - ePrev * wPrev;
+          const T inv = T(1) / d;
+          const T wv = e * inv;
+          sA[lane][kk] = inv;
+          sE[lane][kk] = wv;
+          ePrev = e;
+          wPrev = wv;
+        } else {
+          sA[lane][kk] = T(1) / sA[lane][kk];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int m = 0; m < kChunk; ++m) {
+      const int e = m * kWave + lane;
+      const int cc = e / kChunk, kk = e % kChunk, k = k0 + kk;
+      if (sbase[cc] >= 0 && k < g.nz) {
+        invd[sbase[cc] + k] = sA[cc][kk];
+        if (LINE) w[sbase[cc] + k] = sE[cc][kk];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// r = b - A x, z = M^-1 r; partials rz (slot 0), rr (1), bb (3).
+template <int R, bool LINE>
+__global__ __launch_bounds__(kBlock) void k_residual64(Grid g, Stencil<double> A,
+                                                       const double* __restrict__ b,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ r,
+                                                       double* __restrict__ z,
+                                                       const double* __restrict__ invd,
+                                                       const double* __restrict__ w,
+                                                       double* __restrict__ parts) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
+  KSum rz, rr, bb;
+  int64_t col, end, stride;
+  column_loop(g.columns(), col, end, stride);
+  for (; col < end; col += stride) {
+    const int64_t base = column_base(g, col);
+    double xv[R], xdn[R], xup[R], az[R], azUp[R], dummy[R], rv[R], zv[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      xv[m] = k < g.nz ? x[base + k] : 0.0;
+      az[m] = k < g.nz ? A.Azp[base + k] : 0.0;
+    }
+    shift_rows<double, R>(xv, lane, xdn, xup);
+    shift_rows<double, R>(az, lane, dummy, azUp);
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      rv[m] = 0.0;
+      if (k < g.nz) {
+        const int64_t c = base + k;
+        const double q = A.Ad[c] * xv[m] + A.Axp[c] * x[c + sx] + A.Axp[c - sx] * x[c - sx] +
+                         A.Ayp[c] * x[c + sy] + A.Ayp[c - sy] * x[c - sy] + az[m] * xdn[m] +
+                         azUp[m] * xup[m];
+        const double bv = b[c];
+        rv[m] = bv - q;
+        bb.add(bv * bv);
+      }
+    }
+    apply_precond<double, R, LINE>(lane, g.nz, base, invd, w, rv, zv);
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      if (k < g.nz) {
+        r[base + k] = rv[m];
+        z[base + k] = zv[m];
+        rz.add(rv[m] * zv[m]);
+        rr.add(rv[m] * rv[m]);
+      }
+    }
+  }
+  double v[3] = {rz.value(), rr.value(), bb.value()};
+  const int slots[3] = {0, 1, 3};
+  write_parts<3>(parts, slots, v);
+}
+
+// Search direction and operator product: p = z + beta p_old, q = A p, p.q.
+template <class T, int R>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, Stencil<T> A, const T* __restrict__ z,
+                                                    const T* __restrict__ pOld, T* __restrict__ pNew,
+                                                    T* __restrict__ q, double* __restrict__ parts,
+                                                    PcgState* __restrict__ st, int n, int restart,
+                                                    double tol2) {
+  double red[2];
+  const int rs[2] = {0, 1};
+  reduce_parts<2>(parts, gridDim.x, rs, red);
+  const double rzv = red[0], rrv = red[1];
+  if (rrv <= tol2) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->it = n;
+      st->rr = rrv;
+    }
+    return;
+  }
+  const double betaD = restart ? 0.0 : rzv / st->rz[(n + 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rz[n & 1] = rzv;
+    st->it = n;
+    st->rr = rrv;
+  }
+  const T beta = static_cast<T>(betaD);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
+  KSum pq;
+  int64_t col, end, stride;
+  column_loop(g.columns(), col, end, stride);
+  for (; col < end; col += stride) {
+    const int64_t base = column_base(g, col);
+    T pn[R], pdn[R], pup[R], az[R], azUp[R], dummy[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      pn[m] = T(0);
+      az[m] = T(0);
+      if (k < g.nz) {
+        const int64_t c = base + k;
+        pn[m] = restart ? z[c] : z[c] + beta * pOld[c];
+        az[m] = A.Azp[c];
+      }
+    }
+    shift_rows<T, R>(pn, lane, pdn, pup);
+    shift_rows<T, R>(az, lane, dummy, azUp);
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      if (k < g.nz) {
+        const int64_t c = base + k;
+        const T pxp = restart ? z[c + sx] : z[c + sx] + beta * pOld[c + sx];
+        const T pxm = restart ? z[c - sx] : z[c - sx] + beta * pOld[c - sx];
+        const T pyp = restart ? z[c + sy] : z[c + sy] + beta * pOld[c + sy];
+        const T pym = restart ? z[c - sy] : z[c - sy] + beta * pOld[c - sy];
+        const T qv = A.Ad[c] * pn[m] + A.Axp[c] * pxp + A.Axp[c - sx] * pxm + A.Ayp[c] * pyp +
+                     A.Ayp[c - sy] * pym + az[m] * pdn[m] + azUp[m] * pup[m];
+        pNew[c] = pn[m];
+        q[c] = qv;
+        pq.add(static_cast<double>(pn[m]) * static_cast<double>(qv));
+      }
+    }
+  }
+  double v[1] = {pq.value()};
+  const int slots[1] = {2};
+  write_parts<1>(parts, slots, v);
+}
+
+// alpha = rz / p.q; x += alpha p; r -= alpha q; z = M^-1 r; partials rz, rr.
+template <class T, int R, bool LINE>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(Grid g, T* __restrict__ x,
+                                                       const T* __restrict__ p, T* __restrict__ r,
+                                                       const T* __restrict__ q, T* __restrict__ z,
+                                                       const T* __restrict__ invd,
+                                                       const T* __restrict__ w,
+                                                       double* __restrict__ parts,
+                                                       PcgState* __restrict__ st, int n) {
+  if (st->done) return;
+  double red[1];
+  const int rs[1] = {2};
+  reduce_parts<1>(parts, gridDim.x, rs, red);
+  const double alphaD = st->rz[n & 1] / red[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alphaD;
+  const T alpha = static_cast<T>(alphaD);
+  const int lane = threadIdx.x & (kWave - 1);
+  KSum rz, rr;
+  int64_t col, end, stride;
+  column_loop(g.columns(), col, end, stride);
+  for (; col < end; col += stride) {
+    const int64_t base = column_base(g, col);
+    T rv[R], zv[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      rv[m] = T(0);
+      if (k < g.nz) {
+        const int64_t c = base + k;
+        x[c] = x[c] + alpha * p[c];
+        rv[m] = r[c] - alpha * q[c];
+      }
+    }
+    apply_precond<T, R, LINE>(lane, g.nz, base, invd, w, rv, zv);
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      if (k < g.nz) {
+        r[base + k] = rv[m];
+        z[base + k] = zv[m];
+        rz.add(static_cast<double>(rv[m]) * static_cast<double>(zv[m]));
+        rr.add(static_cast<double>(rv[m]) * static_cast<double>(rv[m]));
+      }
+    }
+  }
+  double v[2] = {rz.value(), rr.value()};
+  const int slots[2] = {0, 1};
+  write_parts<2>(parts, slots, v);
+}
+
+// Mixed form, outer sweep: xo = xi + sc d (all 7 stencil points), r = b - A xo
+// in FP64; partials rr (slot 1), bb (slot 3).  With hasD = 0, xo is xi.
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_refine64(Grid g, Stencil<double> A,
+                                                     const double* __restrict__ b,
+                                                     const double* __restrict__ xi,
+                                                     double* __restrict__ xo,
+                                                     const float* __restrict__ d, double sc,
+                                                     int hasD, double* __restrict__ r,
+                                                     double* __restrict__ parts) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
+  KSum rr, bb;
+  int64_t col, end, stride;
+  column_loop(g.columns(), col, end, stride);
+  auto xn = [&](int64_t c) { return hasD ? xi[c] + sc * static_cast<double>(d[c]) : xi[c]; };
+  for (; col < end; col += stride) {
+    const int64_t base = column_base(g, col);
+    double xv[R], xdn[R], xup[R], az[R], azUp[R], dummy[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      xv[m] = k < g.nz ? xn(base + k) : 0.0;
+      az[m] = k < g.nz ? A.Azp[base + k] : 0.0;
+    }
+    shift_rows<double, R>(xv, lane, xdn, xup);
+    shift_rows<double, R>(az, lane, dummy, azUp);
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      if (k < g.nz) {
+        const int64_t c = base + k;
+        const double qv = A.Ad[c] * xv[m] + A.Axp[c] * xn(c + sx) + A.Axp[c - sx] * xn(c - sx) +
+                          A.Ayp[c] * xn(c + sy) + A.Ayp[c - sy] * xn(c - sy) + az[m] * xdn[m] +
+                          azUp[m] * xup[m];
+        const double bv = b[c];
+        const double rv = bv - qv;
+        if (hasD) xo[c] = xv[m];
+        r[c] = rv;
+        rr.add(rv * rv);
+        bb.add(bv * bv);
+      }
+    }
+  }
+  double v[2] = {rr.value(), bb.value()};
+  const int slots[2] = {1, 3};
+  write_parts<2>(parts, slots, v);
+}
+
+// Mixed form, inner start: r32 = s r64, d = 0, z32 = M32^-1 r32; partials rz, rr.
+template <int R, bool LINE>
+__global__ __launch_bounds__(kBlock) void k_inner_init(Grid g, const double* __restrict__ r64,
+                                                       double s, float* __restrict__ r32,
+                                                       float* __restrict__ z32,
+                                                       float* __restrict__ d32,
+                                                       const float* __restrict__ invd,
+                                                       const float* __restrict__ w,
+                                                       double* __restrict__ parts) {
+  const int lane = threadIdx.x & (kWave - 1);
+  KSum rz, rr;
+  int64_t col, end, stride;
+  column_loop(g.columns(), col, end, stride);
+  for (; col < end; col += stride) {
+    const int64_t base = column_base(g, col);
+    float rv[R], zv[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      rv[m] = k < g.nz ? static_cast<float>(r64[base + k] * s) : 0.0f;
+    }
+    apply_precond<float, R, LINE>(lane, g.nz, base, invd, w, rv, zv);
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      if (k < g.nz) {
+        r32[base + k] = rv[m];
+        z32[base + k] = zv[m];
+        d32[base + k] = 0.0f;
+        rz.add(static_cast<double>(rv[m]) * static_cast<double>(zv[m]));
+        rr.add(static_cast<double>(rv[m]) * static_cast<double>(rv[m]));
+      }
+    }
+  }
+  double v[2] = {rz.value(), rr.value()};
+  const int slots[2] = {0, 1};
+  write_parts<2>(parts, slots, v);
+}
+
+// FP64 stencil -> FP32 stencil over interior cells.
+__global__ __launch_bounds__(kBlock) void k_cast_stencil(Grid g, Stencil<double> A,
+                                                         Stencil<float> B) {
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int k = static_cast<int>(t % g.nz);
+    const int64_t c = column_base(g, t / g.nz) + k;
+    B.Ad[c] = static_cast<float>(A.Ad[c]);
+    B.Axp[c] = static_cast<float>(A.Axp[c]);
+    B.Ayp[c] = static_cast<float>(A.Ayp[c]);
+    B.Azp[c] = static_cast<float>(A.Azp[c]);
+  }
+}
+
+// Fixed-order reduction of nvals partial slots into out[0..nvals).
+__global__ __launch_bounds__(kBlock) void k_reduce_parts(const double* __restrict__ parts,
+                                                         int nparts, double* __restrict__ out) {
+  double v[4];
+  const int slots[4] = {0, 1, 2, 3};
+  reduce_parts<4>(parts, nparts, slots, v);
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; ++q) out[q] = v[q];
+  }
+}
+
+__global__ void k_state_reset(PcgState* st) {
+  st->done = 0;
+  st->it = 0;
+}
+
+template <class F>
+void dispatch_R(int nz, F&& f) {
+  const int R = (nz + kWave - 1) / kWave;
+  switch (R) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    default: throw std::runtime_error("nz > 512 is not supported by the column kernels");
+  }
+}
+
+template <class T>
+T* dalloc(int64_t n) {
+  void* p = nullptr;
+  HIP_OK(hipMalloc(&p, static_cast<size_t>(n) * sizeof(T)));
+  HIP_OK(hipMemset(p, 0, static_cast<size_t>(n) * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+}  // namespace
+
+Pcg::Pcg(const Grid& g, hipStream_t stream) : g_(g), stream_(stream) {
+  if (g.nz > 512) throw std::runtime_error("nz > 512 is not supported by the column kernels");
+  int cus = 256;
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    cus = prop.multiProcessorCount;
+  const int64_t ncol = g.columns();
+  const int64_t wavesWanted = std::max<int64_t>(8, std::min<int64_t>(ncol, int64_t(cus) * 16));
+  blocks_ = static_cast<int>(std::max<int64_t>(8, ((wavesWanted + kWpb - 1) / kWpb + 7) / 8 * 8));
+  const int64_t nH = g.cellsH();
+  r_ = dalloc<double>(nH);
+  z_ = dalloc<double>(nH);
+  p0_ = dalloc<double>(nH);
+  p1_ = dalloc<double>(nH);
+  q_ = dalloc<double>(nH);
+  invd_ = dalloc<double>(nH);
+  w_ = dalloc<double>(nH);
+  parts_ = dalloc<double>(4 * static_cast<int64_t>(blocks_));
+  red_ = dalloc<double>(4);
+  state_ = dalloc<PcgState>(1);
+  HIP_OK(hipHostMalloc(&redHost_, 4 * sizeof(double), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc(&stateHost_, sizeof(PcgState), hipHostMallocDefault));
+  HIP_OK(hipEventCreate(&evA_[0]));
+  HIP_OK(hipEventCreate(&evA_[1]));
+}
+
+Pcg::~Pcg() {
+  for (void* p : {(void*)r_, (void*)z_, (void*)p0_, (void*)p1_, (void*)q_, (void*)invd_, (void*)w_,
+                  (void*)Ad32_, (void*)Axp32_, (void*)Ayp32_, (void*)Azp32_, (void*)r32_,
+                  (void*)z32_, (void*)p032_, (void*)p132_, (void*)q32_, (void*)d32_,
+                  (void*)invd32_, (void*)w32_, (void*)parts_, (void*)red_, state_})
+    if (p) (void)hipFree(p);
+  if (redHost_) (void)hipHostFree(redHost_);
+  if (stateHost_) (void)hipHostFree(stateHost_);
+  for (hipEvent_t e : evA_)
+    if (e) (void)hipEventDestroy(e);
+}
+
+double Pcg::spmvBytesPerLaunch(bool mixed) const {
+  // dir kernel per interior cell: z, p_old, Ad, Axp, Ayp, Azp in; p, q out.
+  const double per = mixed ? 8.0 * 4.0 : 8.0 * 8.0;
+  return per * static_cast<double>(g_.columns()) * g_.nz;
+}
+
+void Pcg::reduceHost(int nvals, double* out) {
+  k_reduce_parts<<<1, kBlock, 0, stream_>>>(parts_, blocks_, red_);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(redHost_, red_, 4 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIP_OK(hipStreamSynchronize(stream_));
+  std::memcpy(out, redHost_, static_cast<size_t>(nvals) * sizeof(double));
+}
+
+int Pcg::solve(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+               PcgStats* st) {
+  *st = PcgStats{};
+  return opt.mixed ? solveMixed(A, b, x, opt, st) : solveFp64(A, b, x, opt, st);
+}
+
+template <class T>
+int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T* z, T* q,
+                         const T* invd, const T* w, bool zline, double tol2, int maxIt,
+                         const PcgOptions& opt, PcgStats* st, int* itDone) {
+  PcgState* dst = static_cast<PcgState*>(state_);
+  PcgState* hst = static_cast<PcgState*>(stateHost_);
+  k_state_reset<<<1, 1, 0, stream_>>>(dst);
+  HIP_OK(hipGetLastError());
+  int n = 0;
+  hipEvent_t evs[2 * kPoll];
+  if (opt.timing)
+    for (auto& e : evs) HIP_OK(hipEventCreate(&e));
+  int done = 0;
+  while (true) {
+    const int chunk = std::min(kPoll, maxIt - n);
+    if (chunk <= 0) break;
+    for (int c = 0; c < chunk; ++c, ++n) {
+      if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c], stream_));
+      dispatch_R(g_.nz, [&](auto RC) {
+        constexpr int R = decltype(RC)::value;
+        k_pcg_dir<T, R><<<blocks_, kBlock, 0, stream_>>>(g_, A, z, pOld, pNew, q, parts_, dst, n,
+                                                         n == 0 ? 1 : 0, tol2);
+      });
+      HIP_OK(hipGetLastError());
+      if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c + 1], stream_));
+      dispatch_R(g_.nz, [&](auto RC) {
+        constexpr int R = decltype(RC)::value;
+        if (zline)
+          k_pcg_update<T, R, true><<<blocks_, kBlock, 0, stream_>>>(g_, x, pNew, r, q, z, invd, w,
+                                                                    parts_, dst, n);
+        else
+          k_pcg_update<T, R, false><<<blocks_, kBlock, 0, stream_>>>(g_, x, pNew, r, q, z, invd,
+                                                                     w, parts_, dst, n);
+      });
+      HIP_OK(hipGetLastError());
+      std::swap(pOld, pNew);
+    }
+    HIP_OK(hipMemcpyAsync(hst, dst, sizeof(PcgState), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    if (opt.timing) {
+      int active = hst->done ? hst->it - (n - chunk) : chunk;
+      active = std::max(0, std::min(active, chunk));
+      for (int c = 0; c < active; ++c) {
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, evs[2 * c], evs[2 * c + 1]));
+        st->spmv_ms += ms;
+        st->spmv_launches += 1;
+      }
+    }
+    if (hst->done) {
+      done = 1;
+      break;
+    }
+  }
+  if (opt.timing)
+    for (auto& e : evs) (void)hipEventDestroy(e);
+  *itDone = done ? hst->it : std::min(n, maxIt);
+  return done;
+}
+
+int Pcg::solveFp64(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                   PcgStats* st) {
+  const int nFactorBlocks = (g_.columns() + kWave - 1) / kWave;
+  if (opt.zline)
+    k_factor<double, true><<<nFactorBlocks, kWave, 0, stream_>>>(g_, A.Ad, A.Azp, invd_, w_);
+  else
+    k_factor<double, false><<<nFactorBlocks, kWave, 0, stream_>>>(g_, A.Ad, A.Azp, invd_, w_);
+  HIP_OK(hipGetLastError());
+  auto residual = [&]() {
+    dispatch_R(g_.nz, [&](auto RC) {
+      constexpr int R = decltype(RC)::value;
+      if (opt.zline)
+        k_residual64<R, true><<<blocks_, kBlock, 0, stream_>>>(g_, A, b, x, r_, z_, invd_, w_,
+                                                               parts_);
+      else
+        k_residual64<R, false><<<blocks_, kBlock, 0, stream_>>>(g_, A, b, x, r_, z_, invd_, w_,
+                                                                parts_);
+    });
+    HIP_OK(hipGetLastError());
+    double red[4];
+    reduceHost(4, red);
+    return std::make_pair(red[1], red[3]);
+  };
+  auto [rr, bb] = residual();
+  const double tol = std::max(opt.rtol * std::sqrt(bb), opt.atol);
+  const double tol2 = tol * tol;
+  st->rhs_norm = std::sqrt(bb);
+  int total = 0, stalls = 0;
+  double* pOld = p0_;
+  double* pNew = p1_;
+  while (true) {
+    if (rr <= tol2) {
+      st->converged = 1;
+      st->residual_norm = std::sqrt(rr);
+      break;
+    }
+    if (total >= opt.max_iterations) {
+      st->residual_norm = std::sqrt(rr);
+      break;
+    }
+    int its = 0;
+    Stencil<double> Ac = A;
+    innerIterations<double>(Ac, x, pOld, pNew, r_, z_, q_, invd_, w_, opt.zline, tol2,
+                            opt.max_iterations - total, opt, st, &its);
+    total += its;
+    std::tie(rr, bb) = residual();  // true residual (restart point if not converged)
+    if (its == 0 && ++stalls > 2 && !(rr <= tol2)) {
+      st->residual_norm = std::sqrt(rr);
+      break;
+    }
+  }
+  st->iterations = total;
+  return st->converged ? FREHG_OK : FREHG_EDIVERGED;
+}
+
+int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                    PcgStats* st) {
+  const int64_t nH = g_.cellsH();
+  if (!Ad32_) {
+    Ad32_ = dalloc<float>(nH);
+    Axp32_ = dalloc<float>(nH);
+    Ayp32_ = dalloc<float>(nH);
+    Azp32_ = dalloc<float>(nH);
+    r32_ = dalloc<float>(nH);
+    z32_ = dalloc<float>(nH);
+    p032_ = dalloc<float>(nH);
+    p132_ = dalloc<float>(nH);
+    q32_ = dalloc<float>(nH);
+    d32_ = dalloc<float>(nH);
+    invd32_ = dalloc<float>(nH);
+    w32_ = dalloc<float>(nH);
+  }
+  Stencil<float> A32{Ad32_, Axp32_, Ayp32_, Azp32_};
+  k_cast_stencil<<<blocks_ * 4, kBlock, 0, stream_>>>(g_, A, A32);
+  HIP_OK(hipGetLastError());
+  const int nFactorBlocks = (g_.columns() + kWave - 1) / kWave;
+  if (opt.zline)
+    k_factor<float, true><<<nFactorBlocks, kWave, 0, stream_>>>(g_, Ad32_, Azp32_, invd32_, w32_);
+  else
+    k_factor<float, false><<<nFactorBlocks, kWave, 0, stream_>>>(g_, Ad32_, Azp32_, invd32_, w32_);
+  HIP_OK(hipGetLastError());
+  // x ping-pongs between the caller's array and the FP64 q_ buffer (idle in this form).
+  double* xa = x;
+  double* xb = q_;
+  auto refine = [&](const double* xi, double* xo, double sc, int hasD) {
+    dispatch_R(g_.nz, [&](auto RC) {
+      constexpr int R = decltype(RC)::value;
+      k_refine64<R><<<blocks_, kBlock, 0, stream_>>>(g_, A, b, xi, xo, d32_, sc, hasD, r_,
+                                                     parts_);
+    });
+    HIP_OK(hipGetLastError());
+    double red[4];
+    reduceHost(4, red);
+    return std::make_pair(red[1], red[3]);
+  };
+  auto [rr, bb] = refine(xa, xb, 0.0, 0);
+  const double tol = std::max(opt.rtol * std::sqrt(bb), opt.atol);
+  const double tol2 = tol * tol;
+  st->rhs_norm = std::sqrt(bb);
+  const double ncell = static_cast<double>(g_.columns()) * g_.nz;
+  int total = 0;
+  while (true) {
+    if (rr <= tol2) {
+      st->converged = 1;
+      break;
+    }
+    if (st->refinements >= opt.max_refinements || total >= opt.max_iterations) break;
+    const double s = 1.0 / std::sqrt(rr / ncell);
+    dispatch_R(g_.nz, [&](auto RC) {
+      constexpr int R = decltype(RC)::value;
+      if (opt.zline)
+        k_inner_init<R, true><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
+                                                               invd32_, w32_, parts_);
+      else
+        k_inner_init<R, false><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
+                                                                invd32_, w32_, parts_);
+    });
+    HIP_OK(hipGetLastError());
+    const double tolIn2 = opt.inner_rtol * opt.inner_rtol * rr * s * s;
+    int its = 0;
+    float* pOld = p032_;
+    float* pNew = p132_;
+    innerIterations<float>(A32, d32_, pOld, pNew, r32_, z32_, q32_, invd32_, w32_, opt.zline,
+                           tolIn2, std::min(opt.max_iterations - total, 400), opt, st, &its);
+    total += its;
+    std::tie(rr, bb) = refine(xa, xb, 1.0 / s, 1);
+    std::swap(xa, xb);
+    st->refinements += 1;
+  }
+  if (xa != x)
+    HIP_OK(hipMemcpyAsync(x, xa, static_cast<size_t>(nH) * sizeof(double),
+                          hipMemcpyDeviceToDevice, stream_));
+  st->iterations = total;
+  st->residual_norm = std::sqrt(rr);
+  return st->converged ? FREHG_OK : FREHG_EDIVERGED;
+}
+
+template int Pcg::innerIterations<double>(const Stencil<double>&, double*, double*&, double*&,
+                                          double*, double*, double*, const double*, const double*,
+                                          bool, double, int, const PcgOptions&, PcgStats*, int*);
+template int Pcg::innerIterations<float>(const Stencil<float>&, float*, float*&, float*&, float*,
+                                         float*, float*, const float*, const float*, bool, double,
+                                         int, const PcgOptions&, PcgStats*, int*);
+
+}  // namespace frehg

@@ -1,0 +1,1746 @@
+// gw_step.hip — the MI355X Richards groundwater step: device kernels for every
+// phase of frehg::gw::RichardsSolver::step (src/gw/RichardsSolver.cpp:729-794),
+// the host orchestration that stages a domain on device, and the C-ABI of
+// include/frehg_gw.h.
+//
+// Kernel map (reference file:line each kernel restates):
+//   k_ghosts            Predictor.cpp:757-817, Corrector.cpp:709-756
+//   k_classify_coupled  Corrector.cpp:55-118 <PC=true>
+//   k_cell_k            VanGenuchten.hpp:99-111, once per cell and phase
+//   k_face_x / _y / _z  Predictor.cpp:105-386 <PC=true,HARM>; with corr = 1
+//                       also the Darcy fluxes of Corrector.cpp:135-533
+//   k_assemble          Predictor.cpp:400-698 <PC=true> into stencil storage
+//   (PCG)               gw_pcg.hip, replacing LinearSystem::solve
+//   k_scatter           Predictor.cpp:745-754
+//   k_coupled_top       Corrector.cpp:539-627
+//   k_update_wc         Corrector.cpp:629-672
+//   k_realloc_classify  Reallocate.cpp:181-391 (saturated_contact: head)
+//   k_realloc_walk      Reallocate.cpp:410-540
+//   k_finalize          Corrector.cpp:758-792
+//   k_step_end          RichardsSolver.cpp:961-1006, AdaptiveStep.cpp:44-98
+// Density / viscosity ratios are 1 (no transport module): their factors are
+// omitted, which is exact.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/frehg_gw.h"
+#include "gw_device.h"
+#include "gw_pcg.h"
+
+namespace frehg {
+
+// The last error message of this thread, shared by both seams.
+thread_local std::string g_error;
+void set_error_message(const char* m) { g_error = m; }
+
+namespace {
+
+struct ApiError : std::runtime_error {
+  int code;
+  ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void fail(int code, const std::string& m) { throw ApiError(code, m); }
+
+#define HIP_CHECK(x)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess)                                                               \
+      fail(FREHG_EDEVICE, std::string("HIP: ") + hipGetErrorString(e_) + " at " +      \
+                              __FILE__ + ":" + std::to_string(__LINE__));               \
+  } while (0)
+
+constexpr int kThreads = 256;
+
+// Reduction slots of one step (each filled by kRb per-block partials).
+enum Slot {
+  S_SS = 0, S_ADJ, S_DISC, S_LOST, S_VENT, S_THIN, S_VLOSS, S_BIN, S_DQMAX, S_COMIN,
+  S_VOL, S_CEX, S_CBO, S_CEV, S_CLAMP, S_COUNT
+};
+__constant__ int kSlotOp[S_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 2, 0, 0, 0, 0, 0};
+
+struct Dev {
+  Grid g;
+  double *h, *hn, *wc, *wcn, *ch;
+  double *ksx, *ksy, *ksz, *vga, *vgn, *wcs, *wcr, *aev;
+  double *Kx, *Ky, *Kz;
+  double *kx, *ky, *kzF, *qx, *qy, *qzF;
+  double *vloss, *room, *sendUp, *sendDown, *wcSnap, *hSnap;
+  double *dz3d, *bot3d, *ax, *ay, *sinx, *cosx, *siny, *cosy;
+  double* bath;
+  int* kt;
+  int *topCode, *botCode, *sxm, *sxp, *sym, *syp, *cplMode;
+  double *topValue, *botValue, *vxm, *vxp, *vym, *vyp;
+  double *eta, *dept, *seep, *avail, *gain, *bounce;
+  double minDepth;
+  int coupled, ventAll;
+  double ss, courantMax;
+  int full3d, harmonic, reallocOff, redistribute, terrain, guessPrev;
+  double *Ad, *Axp, *Ayp, *Azp, *rhs, *x;
+  int* colFlag;
+  double* parts;
+  int rb;
+};
+
+__device__ __forceinline__ bool act(const Dev& d, int j, int i, int k) {
+  return k >= d.kt[d.g.c2(j, i)];
+}
+__device__ __forceinline__ VgSoil soil_at(const Dev& d, int64_t c) {
+  return VgSoil{d.vga[c], d.vgn[c], d.wcs[c], d.wcr[c], d.aev[c]};
+}
+__device__ __forceinline__ bool is_head(int code) {
+  return code == FREHG_BC_HEAD || code == FREHG_BC_HEAD_HYDROSTATIC;
+}
+
+// XCD-aware grid-stride range over [0, n): each XCD walks one contiguous
+// eighth of the index space.
+struct Range {
+  int64_t t, end, stride;
+};
+__device__ __forceinline__ Range xcd_range(int64_t n) {
+  const int xcd = blockIdx.x % kXcds;
+  const int bpx = gridDim.x / kXcds;
+  const int local = blockIdx.x / kXcds;
+  Range r;
+  r.t = static_cast<int64_t>(xcd) * n / kXcds + static_cast<int64_t>(local) * blockDim.x +
+        threadIdx.x;
+  r.end = static_cast<int64_t>(xcd + 1) * n / kXcds;
+  r.stride = static_cast<int64_t>(bpx) * blockDim.x;
+  return r;
+}
+
+template <int NV>
+__device__ __forceinline__ void put_parts(const Dev& d, const int (&slot)[NV], double (&v)[NV]) {
+  int op[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) op[q] = kSlotOp[slot[q]];
+  block_reduce<NV>(v, op);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) d.parts[static_cast<int64_t>(slot[q]) * d.rb + blockIdx.x] = v[q];
+  }
+}
+
+// ---- ghosts (Predictor.cpp:66-87, :757-817; Corrector.cpp:709-756) --------
+__device__ __forceinline__ double ghost_head(int code, double value, double interior, double zc,
+                                             bool coupled, double bed, double depth) {
+  if (code == FREHG_BC_HEAD) return value;
+  if (code == FREHG_BC_HEAD_HYDROSTATIC) {
+    if (coupled && value > bed) return (bed - zc) + depth;
+    return value - zc;
+  }
+  return interior;
+}
+
+__global__ void k_ghosts(Dev d, int doH, int doWc) {
+  const Grid& g = d.g;
+  const int64_t nxe = static_cast<int64_t>(g.ny) * g.nz;
+  const int64_t n = nxe + static_cast<int64_t>(g.nx) * g.nz;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const bool xEdge = t < nxe;
+    const int64_t u = xEdge ? t : t - nxe;
+    const int k = static_cast<int>(u % g.nz);
+    const int a = static_cast<int>(u / g.nz) + 1;
+    for (int side = 0; side < 2; ++side) {
+      int ji, ii, jg, ig, code;
+      double value;
+      if (xEdge) {
+        ji = jg = a;
+        ii = side == 0 ? 1 : g.nx;
+        ig = side == 0 ? 0 : g.nx + 1;
+        code = side == 0 ? d.sxm[g.c2(ji, ii)] : d.sxp[g.c2(ji, ii)];
+        value = side == 0 ? d.vxm[g.c2(ji, ii)] : d.vxp[g.c2(ji, ii)];
+      } else {
+        ii = ig = a;
+        ji = side == 0 ? 1 : g.ny;
+        jg = side == 0 ? 0 : g.ny + 1;
+        code = side == 0 ? d.sym[g.c2(ji, ii)] : d.syp[g.c2(ji, ii)];
+        value = side == 0 ? d.vym[g.c2(ji, ii)] : d.vyp[g.c2(ji, ii)];
+      }
+      const int64_t cin = g.c3(ji, ii, k), cg = g.c3(jg, ig, k);
+      if (doH) {
+        const double zc = d.bot3d[cin] + 0.5 * d.dz3d[cin];
+        const double dep = d.coupled ? d.dept[g.c2(ji, ii)] : 0.0;
+        d.h[cg] = ghost_head(code, value, d.h[cin], zc, d.coupled != 0, d.bath[g.c2(ji, ii)], dep);
+      }
+      if (doWc) d.wc[cg] = is_head(code) ? d.wcs[cin] : d.wc[cin];
+    }
+  }
+}
+
+// ---- coupled-top classification (Corrector.cpp:55-118, PC = true) --------
+__global__ void k_classify_coupled(Dev d, double dtg) {
+  const Grid& g = d.g;
+  const int n = g.columns();
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int j = t / g.nx + 1, i = t % g.nx + 1;
+    const int c2 = g.c2(j, i);
+    const int kt = d.kt[c2] < g.nz ? d.kt[c2] : -1;
+    const double dep = d.dept[c2];
+    if (kt < 0 || dep <= 0.0) {
+      d.cplMode[c2] =
+          (kt >= 0 && d.h[g.c3(j, i, kt)] > 0.5 * d.dz3d[g.c3(j, i, kt)]) ? 3 : 0;
+      continue;
+    }
+    const int64_t c = g.c3(j, i, kt);
+    const double ks = d.ksz[c];
+    const double qEst = 2.0 * ks * (d.h[c] - dep) / d.dz3d[c] - ks;
+    d.cplMode[c2] = (qEst < 0.0 && -qEst * dtg > d.avail[c2]) ? 2 : 1;
+  }
+}
+
+// ---- per-cell conductivities, halo cells included -------------------------
+__global__ void k_cell_k(Dev d) {
+  const int64_t n = d.g.cellsH();
+  for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const MualemFactors f = vg_mualem(soil_at(d, c), d.h[c]);
+    d.Kx[c] = vg_k_from(f, d.ksx[c]);
+    d.Ky[c] = vg_k_from(f, d.ksy[c]);
+    d.Kz[c] = vg_k_from(f, d.ksz[c]);
+  }
+}
+
+__device__ __forceinline__ bool unsat(const Dev& d, int j, int i, int k) {
+  const int64_t c = d.g.c3(j, i, k);
+  return act(d, j, i, k) && d.wc[c] < d.wcs[c];
+}
+
+// ---- lateral faces: K (Predictor.cpp:127-180, 236-268, 318-374) and, with
+// corr, the Darcy flux (Corrector.cpp:176-366).  dir 0 = x, 1 = y.
+template <int DIR>
+__global__ void k_face_lat(Dev d, int corr) {
+  const Grid& g = d.g;
+  // faces: x: j in [1,ny], i in [0,nx]; y: j in [0,ny], i in [1,nx]
+  const int na = DIR == 0 ? g.nx + 1 : g.nx;
+  const int64_t n = static_cast<int64_t>(DIR == 0 ? g.ny : g.ny + 1) * na * g.nz;
+  const double* K = DIR == 0 ? d.Kx : d.Ky;
+  const double* ks = DIR == 0 ? d.ksx : d.ksy;
+  const double* area = DIR == 0 ? d.ax : d.ay;
+  const double* cosv = DIR == 0 ? d.cosx : d.cosy;
+  const double* sinv = DIR == 0 ? d.sinx : d.siny;
+  double* kf = DIR == 0 ? d.kx : d.ky;
+  double* qf = DIR == 0 ? d.qx : d.qy;
+  const int nlast = DIR == 0 ? g.nx : g.ny;
+  const double spacing = DIR == 0 ? g.dx : g.dy;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int64_t t = r.t;
+    const int k = static_cast<int>(t % g.nz);
+    const int64_t col = t / g.nz;
+    int j, i, jn, in, s;  // s: the face's index along DIR
+    if (DIR == 0) {
+      i = static_cast<int>(col % na);
+      j = static_cast<int>(col / na) + 1;
+      jn = j;
+      in = i + 1;
+      s = i;
+    } else {
+      i = static_cast<int>(col % na) + 1;
+      j = static_cast<int>(col / na);
+      jn = j + 1;
+      in = i;
+      s = j;
+    }
+    const int64_t c = g.c3(j, i, k), cn = g.c3(jn, in, k);
+    const double base = d.harmonic ? face_k_harmonic(K[c], ks[c], 1.0, K[cn], ks[cn], 1.0)
+                                   : 0.5 * (K[c] + K[cn]);
+    double v = base;
+    // domain-edge override (Predictor.cpp:236-268)
+    if (s == 0) {
+      const int jj = DIR == 0 ? j : 1, ii = DIR == 0 ? 1 : i;
+      const int code = DIR == 0 ? d.sxm[g.c2(jj, ii)] : d.sym[g.c2(jj, ii)];
+      if (code == FREHG_BC_NOFLUX) v = 0.0;
+      else if (is_head(code)) v = K[g.c3(jj, ii, k)];
+    }
+    if (s == nlast) {
+      const int code = DIR == 0 ? d.sxp[g.c2(j, i)] : d.syp[g.c2(j, i)];
+      if (code == FREHG_BC_NOFLUX) v = 0.0;
+      else if (is_head(code)) v = K[c];
+    }
+    const bool uns = !d.full3d && (unsat(d, j, i, k) || unsat(d, jn, in, k));
+    if (uns) v = 0.0;
+    const bool inA = s >= 1 && s <= nlast, inB = s + 1 >= 1 && s + 1 <= nlast;
+    const bool actA = act(d, j, i, k), actB = act(d, jn, in, k);
+    if ((inA && !actA) || (inB && !actB)) v = 0.0;
+    kf[c] = v;
+    if (!corr) continue;
+    // Darcy flux (Corrector.cpp:176-366), positive toward -axis
+    const double A = area[c];
+    double q;
+    if (s == 0) {
+      const int64_t c1 = DIR == 0 ? g.c3(j, 1, k) : g.c3(1, i, k);
+      const int64_t c0 = DIR == 0 ? g.c3(j, 0, k) : g.c3(0, i, k);
+      const double kface = K[c1];
+      q = A * kface * cosv[c0] * (d.h[c1] - d.h[c0]) / (0.5 * spacing);
+      const int code = DIR == 0 ? d.sxm[g.c2(j, 1)] : d.sym[g.c2(1, i)];
+      const double val = DIR == 0 ? d.vxm[g.c2(j, 1)] : d.vym[g.c2(1, i)];
+      if (code == FREHG_BC_NOFLUX) q = 0.0;
+      else if (code == FREHG_BC_FLUX) q = -val * A;
+    } else {
+      double kface, delta = spacing;
+      const int sgn = (d.bot3d[cn] > d.bot3d[c]) ? 1 : -1;
+      if (s == nlast) {
+        kface = K[c];
+        delta = 0.5 * spacing;
+      } else {
+        kface = base;
+        if (!actA || !actB) kface = 0.0;
+      }
+      q = A * (kface * cosv[c] * (d.h[cn] - d.h[c]) / delta +
+               static_cast<double>(sgn) * kface * sinv[c]);
+      if (uns) q = 0.0;
+      if (s == nlast) {
+        const int code = DIR == 0 ? d.sxp[g.c2(j, i)] : d.syp[g.c2(j, i)];
+        const double val = DIR == 0 ? d.vxp[g.c2(j, i)] : d.vyp[g.c2(j, i)];
+        if (code == FREHG_BC_NOFLUX) q = 0.0;
+        else if (code == FREHG_BC_FLUX) q = val * A;
+      }
+    }
+    qf[c] = q;
+  }
+}
+
+// ---- vertical faces: K (Predictor.cpp:181-232, 280-313, 375-385) and, with
+// corr, the Darcy flux (Corrector.cpp:373-527) with the coupled top modes.
+__global__ void k_face_z(Dev d, int corr, double dtg) {
+  const Grid& g = d.g;
+  const int nzf = g.nz + 1;
+  const int64_t n = static_cast<int64_t>(g.columns()) * nzf;
+  double clamp = 0.0;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % nzf);
+    const int col = static_cast<int>(r.t / nzf);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    const int c2 = g.c2(j, i);
+    const int kt = d.kt[c2];
+    const bool actK = k < g.nz && k >= kt;
+    const bool actM = k >= 1 && k - 1 >= kt;
+    double v;
+    if (k == 0) {
+      v = 0.0;
+    } else if (k == g.nz) {
+      v = d.botCode[c2] == FREHG_BC_NOFLUX ? 0.0 : d.Kz[g.c3(j, i, g.nz - 1)];
+    } else if (actK && !actM) {
+      v = d.Kz[g.c3(j, i, k)];
+    } else if (!actM) {
+      v = 0.0;
+    } else {
+      const int64_t ca = g.c3(j, i, k - 1), cb = g.c3(j, i, k);
+      v = d.harmonic ? face_k_harmonic(d.Kz[ca], d.ksz[ca], d.dz3d[ca], d.Kz[cb], d.ksz[cb],
+                                       d.dz3d[cb])
+                     : 0.5 * (d.Kz[ca] + d.Kz[cb]);
+    }
+    if (actK && (k == 0 || !actM)) {  // top-face rule
+      const int64_t c = g.c3(j, i, k);
+      const double own = d.Kz[c];
+      if (d.coupled) {
+        v = d.dept[c2] > 0.0 ? d.ksz[c] : own;
+      } else {
+        const int code = d.topCode[c2];
+        if (code == FREHG_BC_HEAD) v = d.ksz[c];
+        else if (code == FREHG_BC_NOFLUX) v = 0.0;
+        else v = own;
+      }
+    }
+    if ((k >= 1 && !actM) || (k < g.nz && !actK)) v = 0.0;  // seal, last
+    d.kzF[g.cF(j, i, k)] = v;
+    if (!corr) continue;
+    double q = 0.0;
+    if (actK && !actM) {
+      const int64_t c = g.c3(j, i, k);
+      if (d.coupled) {
+        const double ks = d.ksz[c];
+        const double depth = d.dept[c2];
+        const int mode = d.cplMode[c2];
+        const double delta = 0.5 * d.dz3d[c];
+        if (mode == 2) {
+          const double roomTop = fmax(0.0, (0.9999 * d.wcs[c] - d.wc[c]) * d.dz3d[c]);
+          const double take = fmin(d.avail[c2], roomTop);
+          q = -take * g.az / dtg;
+          d.avail[c2] -= take;
+        } else if (mode == 1) {
+          q = g.az * ks * ((d.h[c] - depth) / delta - 1.0);
+          if (q < 0.0) {
+            const double need = -q * dtg / g.az;
+            if (need > d.avail[c2]) {
+              q = -d.avail[c2] * g.az / dtg;
+              d.avail[c2] = 0.0;
+            } else {
+              d.avail[c2] -= need;
+            }
+          }
+        } else if (mode == 3) {
+          q = g.az * ks * ((d.h[c] - depth) / delta - 1.0);
+          if (q <= 0.0) {
+            clamp += -q * dtg;
+            q = 0.0;
+          }
+        } else {
+          const double kG = vg_conductivity(soil_at(d, c), depth, ks);
+          const double kface = 0.5 * (kG + d.Kz[c]);
+          q = g.az * kface * ((d.h[c] - depth) / delta - 1.0);
+          if (q <= 0.0) q = 0.0;
+          if (d.topCode[c2] == FREHG_BC_FLUX) {
+            const double tv = d.topValue[c2];
+            if ((tv > 0.0 && d.wc[c] > d.wcr[c]) || tv < 0.0) q += tv * g.az;
+          }
+        }
+      } else {
+        const int code = d.topCode[c2];
+        if (code == FREHG_BC_FLUX) {
+          q = d.topValue[c2] * g.az;
+        } else if (code == FREHG_BC_HEAD) {
+          q = g.az * d.ksz[c] * ((d.h[c] - d.topValue[c2]) / (0.5 * d.dz3d[c]) - 1.0);
+        }
+      }
+    } else if (k == g.nz && actM) {
+      const int64_t c = g.c3(j, i, k - 1);
+      const double kface = d.Kz[c];
+      const int code = d.botCode[c2];
+      if (code == FREHG_BC_FLUX) q = d.botValue[c2] * g.az;
+      else if (code == FREHG_BC_GRAVITY) q = -kface * g.az;
+      else if (code == FREHG_BC_HEAD)
+        q = g.az * kface * ((d.botValue[c2] - d.h[c]) / (0.5 * d.dz3d[c]) - 1.0);
+    } else if (actM && actK) {
+      const int64_t ca = g.c3(j, i, k - 1), cb = g.c3(j, i, k);
+      const double kface =
+          d.harmonic ? face_k_harmonic(d.Kz[ca], d.ksz[ca], d.dz3d[ca], d.Kz[cb], d.ksz[cb],
+                                       d.dz3d[cb])
+                     : 0.5 * (d.Kz[ca] + d.Kz[cb]);
+      q = g.az * kface * ((d.h[cb] - d.h[ca]) / (0.5 * (d.dz3d[ca] + d.dz3d[cb])) - 1.0);
+    }
+    d.qzF[g.cF(j, i, k)] = q;
+  }
+  if (corr) {
+    double v[1] = {clamp};
+    const int s[1] = {S_CLAMP};
+    put_parts<1>(d, s, v);
+  }
+}
+
+// ---- assembly (Predictor.cpp:400-698, PC = true) into stencil storage -----
+__global__ void k_assemble(Dev d, double dtg) {
+  const Grid& g = d.g;
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  const double dx = g.dx, dy = g.dy, az = g.az;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % g.nz);
+    const int col = static_cast<int>(r.t / g.nz);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    const int c2 = g.c2(j, i);
+    const int64_t c = g.c3(j, i, k);
+    const VgSoil s = soil_at(d, c);
+    const double chv = vg_capacity(s, d.hn[c]);
+    d.ch[c] = chv;
+    if (!act(d, j, i, k)) {
+      d.Ad[c] = 1.0;
+      d.Axp[c] = 0.0;
+      d.Ayp[c] = 0.0;
+      d.Azp[c] = 0.0;
+      d.rhs[c] = 0.0;
+      d.x[c] = 0.0;
+      continue;
+    }
+    const int64_t cxm = g.c3(j, i - 1, k), cxp = g.c3(j, i + 1, k);
+    const int64_t cym = g.c3(j - 1, i, k), cyp = g.c3(j + 1, i, k);
+    const bool isTop = (k == 0) || !act(d, j, i, k - 1);
+    const double dzc = d.dz3d[c];
+    const double volume = az * dzc;
+    double gxp = -d.kx[c] * dtg * d.cosx[c] / (dx * dx) * (d.ax[c] * dx);
+    double gxm = -d.kx[cxm] * dtg * d.cosx[cxm] / (dx * dx) * (d.ax[cxm] * dx);
+    double gyp = -d.ky[c] * dtg * d.cosy[c] / (dy * dy) * (d.ay[c] * dy);
+    double gym = -d.ky[cym] * dtg * d.cosy[cym] / (dy * dy) * (d.ay[cym] * dy);
+    const bool east = i == g.nx, west = i == 1, north = j == g.ny, south = j == 1;
+    const int cE = d.sxp[c2], cW = d.sxm[c2], cN = d.syp[c2], cS = d.sym[c2];
+    if (east && is_head(cE)) gxp *= 2.0;
+    if (west && is_head(cW)) gxm *= 2.0;
+    if (north && is_head(cN)) gyp *= 2.0;
+    if (south && is_head(cS)) gym *= 2.0;
+    double dzf = (k == g.nz - 1) ? 0.5 * dzc : 0.5 * (dzc + d.dz3d[c + 1]);
+    const double kzTop = d.kzF[g.cF(j, i, k)], kzBot = d.kzF[g.cF(j, i, k + 1)];
+    const double gzp = -kzBot * dtg / (dzc * dzf) * volume;
+    dzf = isTop ? 0.5 * dzc : 0.5 * (dzc + d.dz3d[c - 1]);
+    const double gzm = -kzTop * dtg / (dzc * dzf) * volume;
+
+    const double storage = (chv + d.ss * d.wcn[c] / d.wcs[c]);
+    double gct = storage * volume - (gxp + gxm + gyp + gym);
+    if (east && cE == FREHG_BC_FLUX) gct += gxp;
+    if (west && cW == FREHG_BC_FLUX) gct += gxm;
+    if (north && cN == FREHG_BC_FLUX) gct += gyp;
+    if (south && cS == FREHG_BC_FLUX) gct += gym;
+    const int bot = d.botCode[c2];
+    if (!(k == g.nz - 1 && bot != FREHG_BC_HEAD)) gct -= gzp;
+    const int mode = d.coupled ? d.cplMode[c2] : 0;
+    if (isTop) {
+      bool dir = d.coupled ? (mode == 1) : (d.topCode[c2] == FREHG_BC_HEAD);
+      dir = dir || (d.coupled && mode == 3);
+      if (dir) gct -= gzm;
+    } else {
+      gct -= gzm;
+    }
+
+    double b = storage * d.hn[c] * volume;
+    b -= dtg * volume * kzBot / dzc;
+    b += dtg * volume * kzTop / dzc;
+    if (d.terrain) {
+      double sg = (d.bot3d[cxp] > d.bot3d[c]) ? 1.0 : -1.0;
+      b += sg * dtg * d.ax[c] * d.kx[c] * d.sinx[c];
+      sg = (d.bot3d[c] > d.bot3d[cxm]) ? -1.0 : 1.0;
+      b += sg * dtg * d.ax[cxm] * d.kx[cxm] * d.sinx[cxm];
+      sg = (d.bot3d[cyp] > d.bot3d[c]) ? 1.0 : -1.0;
+      b += sg * dtg * d.ay[c] * d.ky[c] * d.siny[c];
+      sg = (d.bot3d[c] > d.bot3d[cym]) ? -1.0 : 1.0;
+      b += sg * dtg * d.ay[cym] * d.ky[cym] * d.siny[cym];
+    }
+    if (east) {
+      if (cE == FREHG_BC_FLUX) b += d.ax[c] * d.vxp[c2] * dtg;
+      else b -= gxp * d.hn[cxp];
+    }
+    if (west) {
+      if (cW == FREHG_BC_FLUX) b += d.ax[cxm] * d.vxm[c2] * dtg;
+      else b -= gxm * d.hn[cxm];
+    }
+    if (north) {
+      if (cN == FREHG_BC_FLUX) b += d.ay[c] * d.vyp[c2] * dtg;
+      else b -= gyp * d.hn[cyp];
+    }
+    if (south) {
+      if (cS == FREHG_BC_FLUX) b += d.ay[cym] * d.vym[c2] * dtg;
+      else b -= gym * d.hn[cym];
+    }
+    if (k == g.nz - 1) {
+      if (bot == FREHG_BC_FLUX) b += dtg * volume * (d.botValue[c2] + kzBot) / dzc;
+      else if (bot == FREHG_BC_HEAD) b -= gzp * d.botValue[c2];
+    }
+    if (isTop) {
+      if (d.coupled) {
+        if (mode == 1) {
+          b -= gzm * d.dept[c2];
+        } else if (mode == 3) {
+          // Dirichlet ghost head 0: the leg sits in the diagonal only.
+        } else if (mode == 2) {
+          const double roomTop = fmax(0.0, (0.9999 * d.wcs[c] - d.wcn[c]) * dzc);
+          const double take = fmin(d.avail[c2], roomTop);
+          const double qSupply = -take / dtg;
+          b += -dtg * volume * (qSupply + kzTop) / dzc;
+        } else {
+          double qtop = 0.0;
+          if (d.topCode[c2] == FREHG_BC_FLUX) {
+            const double tv = d.topValue[c2];
+            if ((tv > 0.0 && d.wcn[c] < d.wcs[c]) || (tv < 0.0 && d.wcn[c] > d.wcr[c])) qtop = tv;
+          }
+          b += -dtg * volume * (qtop + kzTop) / dzc;
+        }
+      } else if (d.topCode[c2] == FREHG_BC_HEAD) {
+        b -= gzm * d.topValue[c2];
+      } else if (d.topCode[c2] == FREHG_BC_FLUX) {
+        b += -dtg * volume * (d.topValue[c2] + kzTop) / dzc;
+      }
+    }
+    d.Ad[c] = gct;
+    d.Axp[c] = (!east && act(d, j, i + 1, k)) ? gxp : 0.0;
+    d.Ayp[c] = (!north && act(d, j + 1, i, k)) ? gyp : 0.0;
+    d.Azp[c] = (k + 1 < g.nz) ? gzp : 0.0;
+    d.rhs[c] = b;
+    d.x[c] = d.guessPrev ? d.hn[c] : 0.0;
+  }
+}
+
+__global__ void k_scatter(Dev d) {
+  const Grid& g = d.g;
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % g.nz);
+    const int col = static_cast<int>(r.t / g.nz);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    if (act(d, j, i, k)) {
+      const int64_t c = g.c3(j, i, k);
+      d.h[c] = d.x[c];
+    }
+  }
+}
+
+// ---- coupled top bookkeeping (Corrector.cpp:539-627) ----------------------
+__global__ void k_coupled_top(Dev d, double dtg) {
+  const Grid& g = d.g;
+  KSum ex, bo, ev;
+  for (Range r = xcd_range(g.columns()); r.t < r.end; r.t += r.stride) {
+    const int j = static_cast<int>(r.t / g.nx) + 1, i = static_cast<int>(r.t % g.nx) + 1;
+    const int c2 = g.c2(j, i);
+    const int kt = d.kt[c2];
+    if (kt >= g.nz) continue;
+    const int64_t c = g.c3(j, i, kt);
+    const double q = d.qzF[g.cF(j, i, kt)];
+    ex.add(q * dtg);
+    if (q < 0.0 && d.dept[c2] <= d.minDepth) {
+      const double volume = g.az * d.dz3d[c];
+      if ((d.wcs[c] - d.wc[c]) < -q * dtg / volume) {
+        const double dd = -q * dtg / g.az;
+        d.eta[c2] += dd;
+        d.dept[c2] += dd;
+        d.avail[c2] += dd;
+        d.gain[c2] += dd;
+        if (d.bounce) d.bounce[c2] += dd;
+        bo.add(dd * g.az);
+      }
+    }
+    d.seep[c2] += q * dtg / g.az;
+    if (d.topCode[c2] == FREHG_BC_FLUX && d.topValue[c2] > 0.0 && d.wc[c] > d.wcr[c] &&
+        d.dept[c2] <= d.minDepth) {
+      d.seep[c2] -= d.topValue[c2] * dtg;
+      ev.add(d.topValue[c2] * dtg * g.az);
+    }
+  }
+  double v[3] = {ex.value(), bo.value(), ev.value()};
+  const int s[3] = {S_CEX, S_CBO, S_CEV};
+  put_parts<3>(d, s, v);
+}
+
+// ---- theta update (Corrector.cpp:629-672) ----------------------------------
+__global__ void k_update_wc(Dev d, double dtg) {
+  const Grid& g = d.g;
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  const int64_t sy = static_cast<int64_t>(g.nx2) * g.nz;
+  KSum ss;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % g.nz);
+    const int col = static_cast<int>(r.t / g.nz);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    if (!act(d, j, i, k)) continue;
+    const int64_t c = g.c3(j, i, k);
+    const double volume = g.az * d.dz3d[c];
+    const double dh = d.h[c] - d.hn[c];
+    const double coeff = volume * (1.0 + d.ss * dh / d.wcs[c]);
+    const double dqx = dtg * (d.qx[c] - d.qx[c - g.nz]);
+    const double dqy = dtg * (d.qy[c] - d.qy[c - sy]);
+    const double dqz = dtg * (d.qzF[g.cF(j, i, k + 1)] - d.qzF[g.cF(j, i, k)]);
+    const double upd = (d.wcn[c] * volume + dqx + dqy + dqz) / coeff;
+    d.wc[c] = upd;
+    ss.add(upd * volume * d.ss * dh / d.wcs[c]);
+  }
+  double v[1] = {ss.value()};
+  const int s[1] = {S_SS};
+  put_parts<1>(d, s, v);
+}
+
+// ---- post-allocation, phase 1 (Reallocate.cpp:181-391), head contact -------
+struct Split {
+  double up, down;
+};
+__device__ Split gradient_split(const Dev& d, int j, int i, int k) {
+  const Grid& g = d.g;
+  const double* hR = d.hSnap;
+  double d6[6];
+  const int64_t c = g.c3(j, i, k);
+  auto a = [&](int jj, int ii, int kk) { return kk >= 0 && kk < g.nz && act(d, jj, ii, kk); };
+  d6[0] = (a(j, i + 1, k) && d.kx[c] > 0.0) ? (hR[g.c3(j, i + 1, k)] - hR[c]) / g.dx : 0.0;
+  d6[1] = (a(j, i - 1, k) && d.kx[g.c3(j, i - 1, k)] > 0.0) ? (hR[c] - hR[g.c3(j, i - 1, k)]) / g.dx
+                                                            : 0.0;
+  d6[2] = (a(j + 1, i, k) && d.ky[c] > 0.0) ? (hR[g.c3(j + 1, i, k)] - hR[c]) / g.dy : 0.0;
+  d6[3] = (a(j - 1, i, k) && d.ky[g.c3(j - 1, i, k)] > 0.0) ? (hR[c] - hR[g.c3(j - 1, i, k)]) / g.dy
+                                                            : 0.0;
+  if (a(j, i, k + 1) && d.kzF[g.cF(j, i, k + 1)] > 0.0) {
+    const double dzp = 0.5 * (d.dz3d[c] + d.dz3d[c + 1]);
+    d6[4] = (hR[c + 1] - hR[c]) / dzp - 1.0;
+  } else {
+    d6[4] = 0.0;
+  }
+  if (a(j, i, k - 1) && d.kzF[g.cF(j, i, k)] > 0.0) {
+    const double dzm = 0.5 * (d.dz3d[c] + d.dz3d[c - 1]);
+    d6[5] = (hR[c] - hR[c - 1]) / dzm - 1.0;
+  } else {
+    d6[5] = 0.0;
+  }
+  if (!d.full3d) d6[0] = d6[1] = d6[2] = d6[3] = 0.0;
+  double tot = 0.0, gz;
+  if (d6[0] * d6[1] >= 0.0) tot += fabs(0.5 * (d6[0] + d6[1]));
+  else tot += fabs(d6[0]) + fabs(d6[1]);
+  if (d6[2] * d6[3] >= 0.0) tot += fabs(0.5 * (d6[2] + d6[3]));
+  else tot += fabs(d6[2]) + fabs(d6[3]);
+  if (d6[4] * d6[5] >= 0.0) {
+    gz = 0.5 * (d6[4] + d6[5]);
+    tot += fabs(gz);
+  } else {
+    gz = 0.0;
+    tot += fabs(d6[4]) + fabs(d6[5]);
+  }
+  Split sp{0.0, 0.0};
+  if (tot > 0.0) {
+    if (gz > 0.0) sp.up = gz / tot;
+    else if (gz < 0.0) sp.down = -gz / tot;
+    else {
+      sp.down = fabs(d6[4]) / tot;
+      sp.up = fabs(d6[5]) / tot;
+    }
+  }
+  return sp;
+}
+
+__global__ void k_realloc_classify(Dev d) {
+  const Grid& g = d.g;
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  KSum adj, disc;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % g.nz);
+    const int col = static_cast<int>(r.t / g.nz);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    const int64_t c = g.c3(j, i, k);
+    if (!act(d, j, i, k)) {
+      d.room[c] = 0.0;
+      d.sendUp[c] = 0.0;
+      d.sendDown[c] = 0.0;
+      continue;
+    }
+    const VgSoil s = soil_at(d, c);
+    const double volume = g.az * d.dz3d[c];
+    const double before = d.wc[c];
+    d.sendUp[c] = 0.0;
+    d.sendDown[c] = 0.0;
+    if (before >= s.ts) {
+      if (d.reallocOff) {
+        d.room[c] = 0.0;
+        continue;
+      }
+      const double excess = (before - s.ts) * volume;
+      const Split sp = gradient_split(d, j, i, k);
+      const double su = excess * sp.up, sd = excess * sp.down;
+      d.sendUp[c] = su;
+      d.sendDown[c] = sd;
+      adj.add(-(excess - su - sd));
+      disc.add(excess - su - sd);
+      if (su > 0.0 || sd > 0.0) d.colFlag[col] = 1;
+      d.wc[c] = s.ts;
+      d.room[c] = 0.0;
+      continue;
+    }
+    d.room[c] = (s.ts - before) * volume;
+    auto sat = [&](int jj, int ii, int kk) {
+      const int64_t cc = g.c3(jj, ii, kk);
+      return d.wcSnap[cc] >= d.wcs[cc] || d.hSnap[cc] > d.aev[cc];
+    };
+    bool adjSat = false;
+    if (k + 1 < g.nz && act(d, j, i, k + 1) && sat(j, i, k + 1)) adjSat = true;
+    else if (d.kx[c] != 0.0 && sat(j, i + 1, k)) adjSat = true;
+    else if (d.kx[g.c3(j, i - 1, k)] != 0.0 && sat(j, i - 1, k)) adjSat = true;
+    else if (d.ky[c] != 0.0 && sat(j + 1, i, k)) adjSat = true;
+    else if (d.ky[g.c3(j - 1, i, k)] != 0.0 && sat(j - 1, i, k)) adjSat = true;
+    else {
+      const bool isTop = (k == 0) || !act(d, j, i, k - 1);
+      if (isTop) {
+        if (d.coupled) {
+          if (d.dept[g.c2(j, i)] > 0.0) adjSat = true;
+        } else if (d.topCode[g.c2(j, i)] == FREHG_BC_HEAD && d.topValue[g.c2(j, i)] >= 0.0) {
+          adjSat = true;
+        }
+      } else if (act(d, j, i, k - 1) && sat(j, i, k - 1)) {
+        adjSat = true;
+      }
+    }
+    if (!adjSat) {
+      if (before < 0.9999 * s.ts) d.h[c] = vg_head(s, before);
+    } else if (d.reallocOff) {
+      continue;
+    } else {
+      const double after = vg_theta(s, d.h[c]);
+      d.wc[c] = after;
+      d.room[c] = (s.ts - after) * volume;
+      if (d.redistribute && after < before) {
+        const double surplus = (before - after) * volume;
+        const Split sp = gradient_split(d, j, i, k);
+        const double su = surplus * sp.up, sd = surplus * sp.down;
+        d.sendUp[c] = su;
+        d.sendDown[c] = sd;
+        adj.add(-(surplus - su - sd));
+        disc.add(surplus - su - sd);
+        if (su > 0.0 || sd > 0.0) d.colFlag[col] = 1;
+      } else {
+        adj.add((after - before) * volume);
+        if (after < before) disc.add((before - after) * volume);
+      }
+    }
+  }
+  double v[2] = {adj.value(), disc.value()};
+  const int s[2] = {S_ADJ, S_DISC};
+  put_parts<2>(d, s, v);
+}
+
+// ---- post-allocation, phase 2: column send walks (Reallocate.cpp:410-540) --
+__global__ void k_realloc_walk(Dev d, double dtg) {
+  const Grid& g = d.g;
+  KSum lost, vent, thin;
+  for (Range r = xcd_range(g.columns()); r.t < r.end; r.t += r.stride) {
+    const int col = static_cast<int>(r.t);
+    if (!d.colFlag[col]) continue;
+    d.colFlag[col] = 0;
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    const int c2 = g.c2(j, i);
+    const int kt = d.kt[c2];
+    const int64_t b0 = g.c3(j, i, 0);
+    const int64_t f0 = g.cF(j, i, 0);
+    const bool topIsHead = d.topCode[c2] == FREHG_BC_HEAD;
+    for (int k = kt; k < g.nz; ++k) {
+      double dV = d.sendUp[b0 + k];
+      if (dV > 0.0) {
+        int ll = k;
+        while (ll > 0 && ll - 1 >= kt && dV > 0.0) {
+          --ll;
+          const double room = d.room[b0 + ll];
+          if (room > 0.0) {
+            const double take = (room > dV) ? dV : room;
+            d.wc[b0 + ll] += take / (g.az * d.dz3d[b0 + ll]);
+            d.room[b0 + ll] = room - take;
+            d.qzF[f0 + ll + 1] += take / dtg;
+            dV -= take;
+          }
+        }
+        if (dV > 0.0) {
+          if (d.coupled) {
+            const double dd = dV / g.az;
+            if (d.ventAll || d.dept[c2] > 0.0 || dd > d.minDepth) {
+              if (!(d.dept[c2] > 0.0 || dd > d.minDepth)) thin.add(dV);
+              d.eta[c2] += dd;
+              d.dept[c2] += dd;
+              d.avail[c2] += dd;
+              d.gain[c2] += dd;
+              d.qzF[f0 + ll] += dV / dtg;
+              vent.add(dV);
+            } else {
+              lost.add(-dV);
+            }
+            dV = 0.0;
+          } else if (topIsHead) {
+            d.qzF[f0 + ll] += dV / dtg;
+            dV = 0.0;
+          } else {
+            while (ll + 1 < g.nz && dV > 0.0) {
+              ++ll;
+              const double room = d.room[b0 + ll];
+              if (room > 0.0) {
+                const double take = (room > dV) ? dV : room;
+                d.wc[b0 + ll] += take / (g.az * d.dz3d[b0 + ll]);
+                d.room[b0 + ll] = room - take;
+                d.qzF[f0 + ll] -= take / dtg;
+                dV -= take;
+              }
+            }
+            lost.add(-dV);
+            dV = 0.0;
+          }
+        }
+      }
+      dV = d.sendDown[b0 + k];
+      if (dV > 0.0) {
+        int ll = k;
+        while (ll + 1 < g.nz && dV > 0.0) {
+          ++ll;
+          const double room = d.room[b0 + ll];
+          if (room > 0.0) {
+            const double take = (room > dV) ? dV : room;
+            d.wc[b0 + ll] += take / (g.az * d.dz3d[b0 + ll]);
+            d.room[b0 + ll] = room - take;
+            d.qzF[f0 + ll] -= take / dtg;
+            dV -= take;
+          }
+        }
+        lost.add(-dV);
+      }
+    }
+  }
+  double v[3] = {lost.value(), vent.value(), thin.value()};
+  const int s[3] = {S_LOST, S_VENT, S_THIN};
+  put_parts<3>(d, s, v);
+}
+
+// ---- final clamp (Corrector.cpp:758-792) -----------------------------------
+__global__ void k_finalize(Dev d) {
+  const Grid& g = d.g;
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  KSum lost;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % g.nz);
+    const int col = static_cast<int>(r.t / g.nz);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    if (!act(d, j, i, k)) continue;
+    const int64_t c = g.c3(j, i, k);
+    const double volume = g.az * d.dz3d[c];
+    const double w = d.wc[c];
+    if (w > d.wcs[c]) {
+      const double ex = (w - d.wcs[c]) * volume;
+      d.vloss[c] += ex;
+      lost.add(ex);
+      d.wc[c] = d.wcs[c];
+    } else if (w < d.wcr[c]) {
+      const double de = (d.wcr[c] - w) * volume;
+      d.vloss[c] -= de;
+      lost.add(-de);
+      d.wc[c] = d.wcr[c];
+    }
+  }
+  double v[1] = {lost.value()};
+  const int s[1] = {S_VLOSS};
+  put_parts<1>(d, s, v);
+}
+
+// ---- boundary inflow audit + adaptive-step inputs --------------------------
+// (RichardsSolver.cpp:961-1006, AdaptiveStep.cpp:59-98)
+__global__ void k_step_end(Dev d, double dtg) {
+  const Grid& g = d.g;
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  const int64_t sy = static_cast<int64_t>(g.nx2) * g.nz;
+  KSum bin;
+  double dqMax = 0.0, coMin = 1.0e8;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % g.nz);
+    const int col = static_cast<int>(r.t / g.nz);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    if (!act(d, j, i, k)) continue;
+    const int64_t c = g.c3(j, i, k);
+    const int64_t fk = g.cF(j, i, k);
+    const bool top = (k == 0) || !act(d, j, i, k - 1);
+    if (top) bin.add(-d.qzF[fk] * dtg);
+    if (k == g.nz - 1) bin.add(d.qzF[fk + 1] * dtg);
+    if (i == 1) bin.add(-d.qx[c - g.nz] * dtg);
+    if (i == g.nx) bin.add(d.qx[c] * dtg);
+    if (j == 1) bin.add(-d.qy[c - sy] * dtg);
+    if (j == g.ny) bin.add(d.qy[c] * dtg);
+    if (top) continue;
+    const double qin = d.qx[c] / d.ax[c] + d.qy[c] / d.ay[c] + d.qzF[fk + 1] / g.az;
+    const double qou = d.qx[c - g.nz] / d.ax[c] + d.qy[c - sy] / d.ay[c] + d.qzF[fk] / g.az;
+    const double dq = fabs(qin - qou) * dtg / d.dz3d[c];
+    if (dq > dqMax) dqMax = dq;
+    if (d.wc[c] < d.wcs[c]) {
+      const double dk = vg_dkdtheta(soil_at(d, c), d.wc[c], d.ksz[c]);
+      const double co = d.courantMax * d.dz3d[c] / dk;
+      if (co < coMin) coMin = co;
+    }
+  }
+  double v[3] = {bin.value(), dqMax, coMin};
+  const int s[3] = {S_BIN, S_DQMAX, S_COMIN};
+  put_parts<3>(d, s, v);
+}
+
+__global__ void k_volume(Dev d) {
+  const Grid& g = d.g;
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  KSum vol;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % g.nz);
+    const int col = static_cast<int>(r.t / g.nz);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    if (!act(d, j, i, k)) continue;
+    const int64_t c = g.c3(j, i, k);
+    vol.add(d.wc[c] * g.az * d.dz3d[c]);
+  }
+  double v[1] = {vol.value()};
+  const int s[1] = {S_VOL};
+  put_parts<1>(d, s, v);
+}
+
+// Fixed-order reduction of every slot: one block per slot.
+__global__ void k_reduce_slots(const double* __restrict__ parts, int rb, double* __restrict__ out) {
+  const int slot = blockIdx.x;
+  const int op = kSlotOp[slot];
+  double v[1] = {op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY)};
+  for (int b = threadIdx.x; b < rb; b += blockDim.x) {
+    const double p = parts[static_cast<int64_t>(slot) * rb + b];
+    v[0] = op == 0 ? v[0] + p : (op == 1 ? fmax(v[0], p) : fmin(v[0], p));
+  }
+  const int ops[1] = {op};
+  block_reduce<1>(v, ops);
+  if (threadIdx.x == 0) out[slot] = v[0];
+}
+
+__global__ void k_reset_parts(double* parts, int rb) {
+  const int64_t n = static_cast<int64_t>(S_COUNT) * rb;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int slot = static_cast<int>(t / rb);
+    parts[t] = kSlotOp[slot] == 0 ? 0.0 : (kSlotOp[slot] == 1 ? 0.0 : 1.0e8);
+  }
+}
+
+// ---- initial conditions (RichardsSolver.cpp:381-418) ------------------------
+__global__ void k_initial(Dev d, int form, const double* vals, double vconst, double offset) {
+  const Grid& g = d.g;
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % g.nz);
+    const int col = static_cast<int>(r.t / g.nz);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    const int64_t c = g.c3(j, i, k);
+    const VgSoil s = soil_at(d, c);
+    const double zc = d.bot3d[c] + 0.5 * d.dz3d[c];
+    double value = vconst;
+    if (vals) value = (form == FREHG_IC_WATER_TABLE) ? vals[col] : vals[r.t];
+    double hv, w;
+    if (form == FREHG_IC_MOISTURE) {
+      w = value;
+      hv = (w >= s.ts) ? (d.bath[g.c2(j, i)] - zc) : vg_head(s, w);
+    } else if (form == FREHG_IC_HEAD) {
+      hv = value;
+      w = vg_theta(s, hv);
+    } else {
+      hv = (value + offset) - zc;
+      w = vg_theta(s, hv);
+    }
+    d.h[c] = hv;
+    d.wc[c] = w;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+template <class T>
+T* dev_alloc(int64_t n) {
+  void* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, static_cast<size_t>(std::max<int64_t>(n, 1)) * sizeof(T)));
+  HIP_CHECK(hipMemset(p, 0, static_cast<size_t>(std::max<int64_t>(n, 1)) * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+}  // namespace
+
+struct GwSolver {
+  Grid g{};
+  Dev d{};
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Pcg> pcg;
+  PcgOptions popt;
+  std::vector<void*> allocations;
+  // geometry / options
+  double offset = 0, boxTop = 0;
+  double dtMin = 0, dtMax = 0, dqGrow = 0.01, dqShrink = 0.02;
+  double dtgNext = 0, dtgCurrent = 0;
+  int rb = 0, nb2d = 0;
+  int64_t nActive = 0;
+  std::vector<int> codesHost[6];
+  // results
+  frehg_gw_audit audit{};
+  frehg_gw_solve_stats last{};
+  double* slots = nullptr;      // device
+  double* slotsHost = nullptr;  // pinned
+  // timing
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evs0 = nullptr, evs1 = nullptr;
+  frehg_gw_timing tm{};
+
+  template <class T>
+  T* alloc(int64_t n) {
+    T* p = dev_alloc<T>(n);
+    allocations.push_back(p);
+    return p;
+  }
+
+  ~GwSolver() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    pcg.reset();
+    for (void* p : allocations) (void)hipFree(p);
+    if (slotsHost) (void)hipHostFree(slotsHost);
+    for (hipEvent_t e : {ev0, ev1, evs0, evs1})
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int gridCells() const { return rb; }
+
+  void create(const frehg_gw_desc& ds);
+  void buildGeometry(const frehg_gw_desc& ds, std::vector<double>& bath, std::vector<double>& dz,
+                     std::vector<double>& bot, std::vector<int>& ktop);
+  void ghosts(bool doH, bool doWc) {
+    const int64_t n = (static_cast<int64_t>(g.ny) + g.nx) * g.nz;
+    const int blocks = static_cast<int>(std::min<int64_t>((n + kThreads - 1) / kThreads, 4096));
+    k_ghosts<<<std::max(blocks, 1), kThreads, 0, stream>>>(d, doH ? 1 : 0, doWc ? 1 : 0);
+    HIP_CHECK(hipGetLastError());
+  }
+  void faces(int corr, double dtg) {
+    k_cell_k<<<rb, kThreads, 0, stream>>>(d);
+    HIP_CHECK(hipGetLastError());
+    k_face_lat<0><<<rb, kThreads, 0, stream>>>(d, corr);
+    HIP_CHECK(hipGetLastError());
+    k_face_lat<1><<<rb, kThreads, 0, stream>>>(d, corr);
+    HIP_CHECK(hipGetLastError());
+    k_face_z<<<rb, kThreads, 0, stream>>>(d, corr, dtg);
+    HIP_CHECK(hipGetLastError());
+  }
+  void copy(double* dst, const double* src, int64_t n) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, static_cast<size_t>(n) * sizeof(double),
+                             hipMemcpyDeviceToDevice, stream));
+  }
+  void reduceSlots() {
+    k_reduce_slots<<<S_COUNT, kThreads, 0, stream>>>(d.parts, rb, slots);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(slotsHost, slots, S_COUNT * sizeof(double), hipMemcpyDeviceToHost,
+                             stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+  void step(double t, double dtg);
+  double volume() {
+    k_reset_parts<<<64, kThreads, 0, stream>>>(d.parts, rb);
+    k_volume<<<rb, kThreads, 0, stream>>>(d);
+    HIP_CHECK(hipGetLastError());
+    reduceSlots();
+    return slotsHost[S_VOL];
+  }
+  double* fieldPtr(int f, int64_t* n);
+};
+
+void GwSolver::buildGeometry(const frehg_gw_desc& ds, std::vector<double>& bath,
+                             std::vector<double>& dz, std::vector<double>& bot,
+                             std::vector<int>& ktop) {
+  // TerrainMetric (src/gw/TerrainMetric.cpp:101-430), single rank.
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nx2 = g.nx2, ny2 = g.ny2;
+  auto I2 = [&](int j, int i) { return static_cast<size_t>(j) * nx2 + i; };
+  auto I3 = [&](int j, int i, int k) { return (static_cast<size_t>(j) * nx2 + i) * nz + k; };
+  std::vector<double> dzk(nz);
+  const double stretch = ds.dz_stretch > 0 ? ds.dz_stretch : 1.0;
+  double totalDepth = 0;
+  for (int k = 0; k < nz; ++k) {
+    dzk[k] = ds.dz * std::pow(stretch, k);
+    totalDepth += dzk[k];
+  }
+  bath.assign(static_cast<size_t>(ny2) * nx2, 0.0);
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i)
+      bath[I2(j, i)] = ds.bottom_elevation ? ds.bottom_elevation[(j - 1) * nx + (i - 1)]
+                                           : ds.bottom_elevation_const;
+  double lo = 0.0, hi = 0.0;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      lo = std::min(lo, bath[I2(j, i)]);
+      hi = std::max(hi, bath[I2(j, i)]);
+    }
+  offset = lo < 0.0 ? -lo : 0.0;
+  boxTop = hi + offset;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) bath[I2(j, i)] += offset;
+  for (int j = 1; j <= ny; ++j) {
+    bath[I2(j, 0)] = bath[I2(j, 1)];
+    bath[I2(j, nx + 1)] = bath[I2(j, nx)];
+  }
+  for (int i = 0; i <= nx + 1; ++i) {
+    bath[I2(0, i)] = bath[I2(1, i)];
+    bath[I2(ny + 1, i)] = bath[I2(ny, i)];
+  }
+  const size_t nH = static_cast<size_t>(ny2) * nx2 * nz;
+  dz.assign(nH, 0.0);
+  bot.assign(nH, 0.0);
+  ktop.assign(static_cast<size_t>(ny2) * nx2, nz);
+  if (!ds.follow_terrain) {
+    std::vector<double> bot1d(nz);
+    double level = boxTop;
+    for (int k = 0; k < nz; ++k) {
+      level -= dzk[k];
+      bot1d[k] = level;
+    }
+    std::vector<int> activeK(nz);
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const double b = bath[I2(j, i)];
+        for (int k = 0; k < nz; ++k) {
+          bot[I3(j, i, k)] = bot1d[k];
+          dz[I3(j, i, k)] = dzk[k];
+        }
+        for (int k = 0; k < nz; ++k) {
+          double& bk = bot[I3(j, i, k)];
+          if (bk >= b - 1.0e-5) {
+            activeK[k] = 0;
+            if (bk - b < dzk[k]) bk = b;
+          } else {
+            activeK[k] = 1;
+            if (b - bk <= dzk[k]) {
+              if (b - bk >= 0.25 * dzk[k]) {
+                dz[I3(j, i, k)] = b - bk;
+              } else if (k + 1 < nz) {
+                activeK[k] = 0;
+                dz[I3(j, i, k + 1)] += b - bk;
+              }
+            }
+          }
+        }
+        int top = nz;
+        for (int k = 0; k < nz; ++k)
+          if (activeK[k]) {
+            top = k;
+            break;
+          }
+        ktop[I2(j, i)] = top;
+      }
+  } else {
+    const bool uniform = ds.terrain_layers == 1;
+    const double boxBottom = boxTop - totalDepth;
+    double increSum = 0.0, wgt = 1.0;
+    for (int k = 0; k < nz; ++k) {
+      increSum += wgt;
+      wgt *= stretch;
+    }
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const double b = bath[I2(j, i)];
+        if (b <= boxBottom && !uniform)
+          fail(FREHG_EINVAL, "follow_terrain: a bed sits at or below the subsurface box bottom");
+        double dzv = uniform ? dzk[0] : (b - boxBottom) / increSum;
+        double level = b;
+        for (int k = 0; k < nz; ++k) {
+          if (uniform) dzv = dzk[k];
+          level -= dzv;
+          dz[I3(j, i, k)] = dzv;
+          bot[I3(j, i, k)] = level;
+          if (!uniform) dzv *= stretch;
+        }
+        ktop[I2(j, i)] = 0;
+      }
+  }
+  for (std::vector<double>* f : {&dz, &bot})
+    for (int k = 0; k < nz; ++k) {
+      for (int j = 1; j <= ny; ++j) {
+        (*f)[I3(j, 0, k)] = (*f)[I3(j, 1, k)];
+        (*f)[I3(j, nx + 1, k)] = (*f)[I3(j, nx, k)];
+      }
+      for (int i = 0; i <= nx + 1; ++i) {
+        (*f)[I3(0, i, k)] = (*f)[I3(1, i, k)];
+        (*f)[I3(ny + 1, i, k)] = (*f)[I3(ny, i, k)];
+      }
+    }
+}
+
+void GwSolver::create(const frehg_gw_desc& ds) {
+  if (ds.nx < 1 || ds.ny < 1 || ds.nz < 1) fail(FREHG_EINVAL, "extents must be positive");
+  if (!(ds.dx > 0) || !(ds.dy > 0) || !(ds.dz > 0)) fail(FREHG_EINVAL, "spacings must be positive");
+  if (ds.nz > 512) fail(FREHG_EUNSUPPORTED, "nz > 512 is not supported");
+  if (ds.predictor_corrector != 1)
+    fail(FREHG_EUNSUPPORTED, "predictor_corrector: legacy is not supported (consistent only)");
+  if (ds.saturated_contact != 1)
+    fail(FREHG_EUNSUPPORTED, "saturated_contact: legacy is order-dependent; head only");
+  if (ds.preconditioner == FREHG_PC_ICC0)
+    fail(FREHG_EUNSUPPORTED, "icc0 is a CPU-only preconditioner; use zline or jacobi");
+  if (!ds.ksx || !ds.ksy || !ds.ksz || !ds.vg_alpha || !ds.vg_n || !ds.theta_s || !ds.theta_r ||
+      !ds.aev)
+    fail(FREHG_EINVAL, "all eight per-cell soil arrays are required");
+  if (ds.ic_form < 0 || ds.ic_form > 2) fail(FREHG_EINVAL, "unknown initial-condition form");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    fail(FREHG_EDEVICE, "no HIP device is visible");
+  HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  g.nx = ds.nx;
+  g.ny = ds.ny;
+  g.nz = ds.nz;
+  g.nx2 = ds.nx + 2;
+  g.ny2 = ds.ny + 2;
+  g.dx = ds.dx;
+  g.dy = ds.dy;
+  g.az = ds.dx * ds.dy;
+  d.g = g;
+  int cus = 256;
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    cus = prop.multiProcessorCount;
+  const int64_t ncell = static_cast<int64_t>(g.columns()) * g.nz;
+  rb = static_cast<int>(std::min<int64_t>(int64_t(cus) * 8, std::max<int64_t>(
+                                                                 8, (ncell + kThreads - 1) / kThreads)));
+  rb = (rb + 7) / 8 * 8;
+  d.rb = rb;
+
+  std::vector<double> bath, dz, bot;
+  std::vector<int> ktop;
+  buildGeometry(ds, bath, dz, bot, ktop);
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nx2 = g.nx2, ny2 = g.ny2;
+  auto I3 = [&](int j, int i, int k) { return (static_cast<size_t>(j) * nx2 + i) * nz + k; };
+  const int64_t nH = g.cellsH(), nF = g.cellsF(), n2 = static_cast<int64_t>(ny2) * nx2;
+  nActive = 0;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) nActive += nz - ktop[static_cast<size_t>(j) * nx2 + i];
+
+  // metrics (TerrainMetric.cpp:318-408)
+  std::vector<double> axv(nH), ayv(nH), sxv(nH, 0.0), cxv(nH, 1.0), syv(nH, 0.0), cyv(nH, 1.0);
+  if (ds.follow_terrain) {
+    for (int j = 0; j <= ny + 1; ++j)
+      for (int i = 1; i < nx; ++i)
+        for (int k = 0; k < nz; ++k) {
+          const double zc = bot[I3(j, i, k)] + 0.5 * dz[I3(j, i, k)];
+          const double zp = bot[I3(j, i + 1, k)] + 0.5 * dz[I3(j, i + 1, k)];
+          const double hd = std::fabs(zp - zc);
+          const double dist = std::sqrt(hd * hd + g.dx * g.dx);
+          sxv[I3(j, i, k)] = hd / dist;
+          cxv[I3(j, i, k)] = g.dx / dist;
+        }
+    for (int j = 1; j < ny; ++j)
+      for (int i = 0; i <= nx + 1; ++i)
+        for (int k = 0; k < nz; ++k) {
+          const double zc = bot[I3(j, i, k)] + 0.5 * dz[I3(j, i, k)];
+          const double zp = bot[I3(j + 1, i, k)] + 0.5 * dz[I3(j + 1, i, k)];
+          const double hd = std::fabs(zp - zc);
+          const double dist = std::sqrt(hd * hd + g.dy * g.dy);
+          syv[I3(j, i, k)] = hd / dist;
+          cyv[I3(j, i, k)] = g.dy / dist;
+        }
+  }
+  for (int j = 0; j <= ny + 1; ++j)
+    for (int i = 0; i <= nx + 1; ++i)
+      for (int k = 0; k < nz; ++k) {
+        const size_t c = I3(j, i, k);
+        if (ds.follow_terrain) {
+          const double dzE = (i <= nx) ? dz[I3(j, i + 1, k)] : dz[c];
+          const double dzN = (j <= ny) ? dz[I3(j + 1, i, k)] : dz[c];
+          axv[c] = 0.5 * (dz[c] + dzE) * cxv[c] * g.dy;
+          ayv[c] = 0.5 * (dz[c] + dzN) * cyv[c] * g.dx;
+        } else {
+          axv[c] = g.dy * dz[c];
+          ayv[c] = g.dx * dz[c];
+        }
+      }
+
+  auto up = [&](double* dst, const void* src, int64_t n, size_t sz) {
+    HIP_CHECK(hipMemcpy(dst, src, static_cast<size_t>(n) * sz, hipMemcpyHostToDevice));
+  };
+  auto field = [&]() { return alloc<double>(nH); };
+  d.dz3d = field();
+  d.bot3d = field();
+  d.ax = field();
+  d.ay = field();
+  d.sinx = field();
+  d.cosx = field();
+  d.siny = field();
+  d.cosy = field();
+  up(d.dz3d, dz.data(), nH, 8);
+  up(d.bot3d, bot.data(), nH, 8);
+  up(d.ax, axv.data(), nH, 8);
+  up(d.ay, ayv.data(), nH, 8);
+  up(d.sinx, sxv.data(), nH, 8);
+  up(d.cosx, cxv.data(), nH, 8);
+  up(d.siny, syv.data(), nH, 8);
+  up(d.cosy, cyv.data(), nH, 8);
+  d.bath = alloc<double>(n2);
+  up(d.bath, bath.data(), n2, 8);
+  d.kt = alloc<int>(n2);
+  HIP_CHECK(hipMemcpy(d.kt, ktop.data(), n2 * sizeof(int), hipMemcpyHostToDevice));
+  axv.clear();
+  ayv.clear();
+
+  // soil staging with mirrored edge ghosts (RichardsSolver.cpp:207-329, :24-57)
+  {
+    std::vector<double> tmp(nH);
+    const double* src[8] = {ds.ksx, ds.ksy, ds.ksz, ds.vg_alpha, ds.vg_n, ds.theta_s, ds.theta_r,
+                            ds.aev};
+    double** dst[8] = {&d.ksx, &d.ksy, &d.ksz, &d.vga, &d.vgn, &d.wcs, &d.wcr, &d.aev};
+    for (int a = 0; a < 8; ++a) {
+      std::fill(tmp.begin(), tmp.end(), 0.0);
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+          std::memcpy(&tmp[I3(j, i, 0)], src[a] + (static_cast<size_t>(j - 1) * nx + (i - 1)) * nz,
+                      nz * sizeof(double));
+      for (int j = 0; j <= ny + 1; ++j) {
+        std::memcpy(&tmp[I3(j, 0, 0)], &tmp[I3(j, 1, 0)], nz * sizeof(double));
+        std::memcpy(&tmp[I3(j, nx + 1, 0)], &tmp[I3(j, nx, 0)], nz * sizeof(double));
+      }
+      for (int i = 0; i <= nx + 1; ++i) {
+        std::memcpy(&tmp[I3(0, i, 0)], &tmp[I3(1, i, 0)], nz * sizeof(double));
+        std::memcpy(&tmp[I3(ny + 1, i, 0)], &tmp[I3(ny, i, 0)], nz * sizeof(double));
+      }
+      *dst[a] = field();
+      up(*dst[a], tmp.data(), nH, 8);
+    }
+  }
+  // state and work fields
+  d.h = field();
+  d.hn = field();
+  d.wc = field();
+  d.wcn = field();
+  d.ch = field();
+  d.Kx = field();
+  d.Ky = field();
+  d.Kz = field();
+  d.kx = field();
+  d.ky = field();
+  d.qx = field();
+  d.qy = field();
+  d.kzF = alloc<double>(nF);
+  d.qzF = alloc<double>(nF);
+  d.vloss = field();
+  d.room = field();
+  d.sendUp = field();
+  d.sendDown = field();
+  d.wcSnap = field();
+  d.hSnap = field();
+  d.Ad = field();
+  d.Axp = field();
+  d.Ayp = field();
+  d.Azp = field();
+  d.rhs = field();
+  d.x = field();
+  d.colFlag = alloc<int>(g.columns());
+  d.parts = alloc<double>(static_cast<int64_t>(S_COUNT) * rb);
+  slots = alloc<double>(S_COUNT);
+  HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&slotsHost), S_COUNT * sizeof(double),
+                          hipHostMallocDefault));
+
+  // boundary markers (RichardsSolver.cpp:429-566)
+  {
+    const int* cs[6] = {ds.top_code, ds.bot_code, ds.xm_code, ds.xp_code, ds.ym_code, ds.yp_code};
+    const double* vs[6] = {ds.top_value, ds.bot_value, ds.xm_value, ds.xp_value, ds.ym_value,
+                           ds.yp_value};
+    int** cd[6] = {&d.topCode, &d.botCode, &d.sxm, &d.sxp, &d.sym, &d.syp};
+    double** vd[6] = {&d.topValue, &d.botValue, &d.vxm, &d.vxp, &d.vym, &d.vyp};
+    std::vector<double> vv(n2);
+    for (int a = 0; a < 6; ++a) {
+      std::vector<int>& cc = codesHost[a];
+      cc.assign(n2, FREHG_BC_NOFLUX);
+      std::fill(vv.begin(), vv.end(), 0.0);
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+          const bool member = a < 2 || (a == 2 && i == 1) || (a == 3 && i == nx) ||
+                              (a == 4 && j == 1) || (a == 5 && j == ny);
+          if (!member) continue;
+          const size_t c = static_cast<size_t>(j - 1) * nx + (i - 1);
+          const int code = cs[a] ? cs[a][c] : FREHG_BC_NOFLUX;
+          if (code < 0 || code > 4) fail(FREHG_EINVAL, "boundary code out of range");
+          if (a < 2 && code == FREHG_BC_HEAD_HYDROSTATIC)
+            fail(FREHG_EINVAL, "hydrostatic head is a side condition only");
+          if (a != 1 && code == FREHG_BC_GRAVITY)
+            fail(FREHG_EINVAL, "gravity drainage is a bottom condition only");
+          double v = vs[a] ? vs[a][c] : 0.0;
+          if (code == FREHG_BC_HEAD_HYDROSTATIC) v += offset;
+          cc[static_cast<size_t>(j) * nx2 + i] = code;
+          vv[static_cast<size_t>(j) * nx2 + i] = v;
+        }
+      *cd[a] = alloc<int>(n2);
+      *vd[a] = alloc<double>(n2);
+      HIP_CHECK(hipMemcpy(*cd[a], cc.data(), n2 * sizeof(int), hipMemcpyHostToDevice));
+      up(*vd[a], vv.data(), n2, 8);
+    }
+    d.cplMode = alloc<int>(n2);
+  }
+
+  // options
+  d.ss = ds.specific_storage;
+  d.full3d = ds.use_full3d ? 1 : 0;
+  d.harmonic = ds.face_conductivity == 1 ? 1 : 0;
+  d.reallocOff = ds.reallocation == 1 ? 1 : 0;
+  d.redistribute = ds.reallocation_surplus == 1 ? 1 : 0;
+  d.terrain = ds.follow_terrain ? 1 : 0;
+  d.guessPrev = ds.initial_guess_previous ? 1 : 0;
+  d.courantMax = ds.courant_max;
+  dtMin = ds.dt_min;
+  dtMax = ds.dt_max;
+  dqGrow = ds.dq_grow;
+  dqShrink = ds.dq_shrink;
+  dtgNext = ds.dt_init;
+  popt.rtol = ds.rtol;
+  popt.atol = ds.atol;
+  popt.max_iterations = ds.max_iterations > 0 ? ds.max_iterations : 1000;
+  popt.zline = ds.preconditioner == FREHG_PC_ZLINE;
+  popt.mixed = ds.precision == FREHG_PREC_MIXED;
+  popt.inner_rtol = ds.inner_rtol > 0 ? ds.inner_rtol : 1e-4;
+
+  // initial conditions (RichardsSolver.cpp:331-427)
+  double* icDev = nullptr;
+  if (ds.ic_values) {
+    const int64_t nv = ds.ic_form == FREHG_IC_WATER_TABLE ? g.columns()
+                                                         : static_cast<int64_t>(g.columns()) * nz;
+    icDev = dev_alloc<double>(nv);
+    up(icDev, ds.ic_values, nv, 8);
+  }
+  k_initial<<<rb, kThreads, 0, stream>>>(d, ds.ic_form, icDev, ds.ic_const, offset);
+  HIP_CHECK(hipGetLastError());
+  ghosts(true, true);
+  copy(d.hn, d.h, nH);
+  copy(d.wcn, d.wc, nH);
+  HIP_CHECK(hipStreamSynchronize(stream));
+  if (icDev) (void)hipFree(icDev);
+  pcg = std::make_unique<Pcg>(g, stream);
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  HIP_CHECK(hipEventCreate(&evs0));
+  HIP_CHECK(hipEventCreate(&evs1));
+  tm.spmv_bytes = pcg->spmvBytesPerLaunch(popt.mixed);
+}
+
+void GwSolver::step(double /*t*/, double dtg) {
+  if (!(dtg > 0.0)) fail(FREHG_EINVAL, "dtg must be positive");
+  const int64_t nH = g.cellsH();
+  if (timing) HIP_CHECK(hipEventRecord(ev0, stream));
+  dtgCurrent = dtg;
+  audit = frehg_gw_audit{};
+  k_reset_parts<<<64, kThreads, 0, stream>>>(d.parts, rb);
+  HIP_CHECK(hipGetLastError());
+  // step-start refresh (RichardsSolver.cpp:743-748)
+  ghosts(true, true);
+  copy(d.hn, d.h, nH);
+  copy(d.wcn, d.wc, nH);
+  // predictor
+  if (d.coupled) {
+    k_classify_coupled<<<std::max(1, std::min(4096, (g.columns() + kThreads - 1) / kThreads)),
+                         kThreads, 0, stream>>>(d, dtg);
+    HIP_CHECK(hipGetLastError());
+  }
+  faces(0, dtg);
+  k_assemble<<<rb, kThreads, 0, stream>>>(d, dtg);
+  HIP_CHECK(hipGetLastError());
+  if (timing) HIP_CHECK(hipEventRecord(evs0, stream));
+  Stencil<double> A{d.Ad, d.Axp, d.Ayp, d.Azp};
+  PcgStats ps;
+  PcgOptions o = popt;
+  o.timing = timing;
+  const int rc = pcg->solve(A, d.rhs, d.x, o, &ps);
+  if (timing) HIP_CHECK(hipEventRecord(evs1, stream));
+  last.iterations = ps.iterations;
+  last.refinements = ps.refinements;
+  last.converged = ps.converged;
+  last.residual_norm = ps.residual_norm;
+  last.rhs_norm = ps.rhs_norm;
+  if (rc != FREHG_OK)
+    fail(FREHG_EDIVERGED, "gw linear solve did not converge in " + std::to_string(ps.iterations) +
+                              " iterations (residual " + std::to_string(ps.residual_norm) +
+                              ", ||b|| " + std::to_string(ps.rhs_norm) + ")");
+  k_scatter<<<rb, kThreads, 0, stream>>>(d);
+  HIP_CHECK(hipGetLastError());
+  ghosts(true, false);
+  // corrector
+  faces(1, dtg);
+  if (d.coupled) {
+    k_coupled_top<<<rb, kThreads, 0, stream>>>(d, dtg);
+    HIP_CHECK(hipGetLastError());
+  }
+  k_update_wc<<<rb, kThreads, 0, stream>>>(d, dtg);
+  HIP_CHECK(hipGetLastError());
+  // post-allocation and final clamp
+  copy(d.wcSnap, d.wc, nH);
+  copy(d.hSnap, d.h, nH);
+  k_realloc_classify<<<rb, kThreads, 0, stream>>>(d);
+  HIP_CHECK(hipGetLastError());
+  if (!d.reallocOff) {
+    k_realloc_walk<<<rb, kThreads, 0, stream>>>(d, dtg);
+    HIP_CHECK(hipGetLastError());
+  }
+  k_finalize<<<rb, kThreads, 0, stream>>>(d);
+  HIP_CHECK(hipGetLastError());
+  ghosts(false, true);
+  k_step_end<<<rb, kThreads, 0, stream>>>(d, dtg);
+  HIP_CHECK(hipGetLastError());
+  if (timing) HIP_CHECK(hipEventRecord(ev1, stream));
+  reduceSlots();
+  const double* s = slotsHost;
+  audit.ss_storage = s[S_SS];
+  if (d.reallocOff) {
+    audit.realloc_adjust = 0.0;
+    audit.realloc_dropped = 0.0;
+    audit.cpl_vent = 0.0;
+    audit.cpl_vent_thin = 0.0;
+  } else {
+    audit.realloc_adjust = s[S_ADJ] + s[S_LOST];
+    audit.realloc_dropped = s[S_DISC] - s[S_LOST];
+    audit.cpl_vent = s[S_VENT];
+    audit.cpl_vent_thin = s[S_THIN];
+  }
+  audit.vloss = s[S_VLOSS];
+  audit.boundary_in = s[S_BIN];
+  audit.cpl_exchanged = s[S_CEX];
+  audit.cpl_bounce = s[S_CBO];
+  audit.cpl_evap = s[S_CEV];
+  audit.pc_clamp = s[S_CLAMP];
+  // adaptTimeStep (AdaptiveStep.cpp:100-117)
+  double next = dtg;
+  if (s[S_DQMAX] > dqShrink) next = dtg * 0.75;
+  else if (s[S_DQMAX] < dqGrow) next = dtg * 1.25;
+  if (next > s[S_COMIN]) next = s[S_COMIN];
+  if (next > dtMax) next = dtMax;
+  if (next < dtMin) next = dtMin;
+  dtgNext = next;
+  if (timing) {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    tm.step_ms += ms;
+    HIP_CHECK(hipEventElapsedTime(&ms, evs0, evs1));
+    tm.solve_ms += ms;
+    tm.spmv_ms += ps.spmv_ms;
+    tm.spmv_launches += ps.spmv_launches;
+    tm.steps += 1;
+    tm.pcg_iterations += ps.iterations;
+  }
+}
+
+double* GwSolver::fieldPtr(int f, int64_t* n) {
+  *n = g.cellsH();
+  switch (f) {
+    case FREHG_F_HEAD: return d.h;
+    case FREHG_F_WC: return d.wc;
+    case FREHG_F_HEAD_N: return d.hn;
+    case FREHG_F_WC_N: return d.wcn;
+    case FREHG_F_KX: return d.kx;
+    case FREHG_F_KY: return d.ky;
+    case FREHG_F_KZF: *n = g.cellsF(); return d.kzF;
+    case FREHG_F_QX: return d.qx;
+    case FREHG_F_QY: return d.qy;
+    case FREHG_F_QZF: *n = g.cellsF(); return d.qzF;
+    case FREHG_F_VLOSS: return d.vloss;
+    case FREHG_F_DZ3D: return d.dz3d;
+    case FREHG_F_BOT3D: return d.bot3d;
+    case FREHG_F_AREA_X: return d.ax;
+    case FREHG_F_AREA_Y: return d.ay;
+    case FREHG_F_SIN_X: return d.sinx;
+    case FREHG_F_COS_X: return d.cosx;
+    case FREHG_F_SIN_Y: return d.siny;
+    case FREHG_F_COS_Y: return d.cosy;
+    case FREHG_F_CAPACITY: return d.ch;
+    default: fail(FREHG_EINVAL, "unknown field id");
+  }
+}
+
+namespace {
+template <class Fn>
+int guard(Fn fn) {
+  try {
+    fn();
+    return FREHG_OK;
+  } catch (const ApiError& e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return FREHG_EDEVICE;
+  }
+}
+GwSolver* S(frehg_gw_solver* s) {
+  if (!s) fail(FREHG_EINVAL, "null solver handle");
+  return reinterpret_cast<GwSolver*>(s);
+}
+const GwSolver* S(const frehg_gw_solver* s) {
+  if (!s) fail(FREHG_EINVAL, "null solver handle");
+  return reinterpret_cast<const GwSolver*>(s);
+}
+}  // namespace
+
+}  // namespace frehg
+
+using frehg::fail;
+using frehg::guard;
+using frehg::GwSolver;
+
+extern "C" {
+
+int frehg_abi_version(void) { return FREHG_GW_ABI_VERSION; }
+const char* frehg_last_error(void) { return frehg::g_error.c_str(); }
+
+int frehg_gw_create(const frehg_gw_desc* desc, frehg_gw_solver** out) {
+  return guard([&] {
+    if (!desc || !out) frehg::fail(FREHG_EINVAL, "null argument");
+    auto s = std::make_unique<GwSolver>();
+    s->create(*desc);
+    *out = reinterpret_cast<frehg_gw_solver*>(s.release());
+  });
+}
+int frehg_gw_destroy(frehg_gw_solver* s) {
+  delete reinterpret_cast<GwSolver*>(s);
+  return FREHG_OK;
+}
+int frehg_gw_step(frehg_gw_solver* s, double t, double dtg) {
+  return guard([&] { frehg::S(s)->step(t, dtg); });
+}
+int frehg_gw_next_dt(const frehg_gw_solver* s, double* dt) {
+  return guard([&] { *dt = frehg::S(s)->dtgNext; });
+}
+int frehg_gw_get_audit(const frehg_gw_solver* s, frehg_gw_audit* out) {
+  return guard([&] { *out = frehg::S(s)->audit; });
+}
+int frehg_gw_owned_volume(frehg_gw_solver* s, double* v) {
+  return guard([&] { *v = frehg::S(s)->volume(); });
+}
+int frehg_gw_last_solve(const frehg_gw_solver* s, frehg_gw_solve_stats* out) {
+  return guard([&] { *out = frehg::S(s)->last; });
+}
+int frehg_gw_active_cells(const frehg_gw_solver* s, int64_t* n) {
+  return guard([&] { *n = frehg::S(s)->nActive; });
+}
+int frehg_gw_field_size(const frehg_gw_solver* s, int field, int64_t* n) {
+  return guard([&] { (void)const_cast<GwSolver*>(frehg::S(s))->fieldPtr(field, n); });
+}
+int frehg_gw_get_field(frehg_gw_solver* s, int field, double* host_out) {
+  return guard([&] {
+    GwSolver* g = frehg::S(s);
+    int64_t n = 0;
+    const double* p = g->fieldPtr(field, &n);
+    HIP_CHECK(hipMemcpyAsync(host_out, p, static_cast<size_t>(n) * sizeof(double),
+                             hipMemcpyDeviceToHost, g->stream));
+    HIP_CHECK(hipStreamSynchronize(g->stream));
+  });
+}
+int frehg_gw_set_field(frehg_gw_solver* s, int field, const double* host_in) {
+  return guard([&] {
+    GwSolver* g = frehg::S(s);
+    if (field != FREHG_F_HEAD && field != FREHG_F_WC)
+      frehg::fail(FREHG_EINVAL, "only head / water content are settable");
+    int64_t n = 0;
+    double* p = g->fieldPtr(field, &n);
+    HIP_CHECK(hipMemcpyAsync(p, host_in, static_cast<size_t>(n) * sizeof(double),
+                             hipMemcpyHostToDevice, g->stream));
+    g->ghosts(true, true);
+    g->copy(g->d.hn, g->d.h, g->g.cellsH());
+    g->copy(g->d.wcn, g->d.wc, g->g.cellsH());
+    HIP_CHECK(hipStreamSynchronize(g->stream));
+  });
+}
+int frehg_gw_set_bc_values(frehg_gw_solver* s, int which, const double* values) {
+  return guard([&] {
+    GwSolver* g = frehg::S(s);
+    if (which < 0 || which > 5) frehg::fail(FREHG_EINVAL, "bc selector out of range");
+    double* dst[6] = {g->d.topValue, g->d.botValue, g->d.vxm, g->d.vxp, g->d.vym, g->d.vyp};
+    const int nx = g->g.nx, ny = g->g.ny, nx2 = g->g.nx2;
+    std::vector<double> vv(static_cast<size_t>(g->g.ny2) * nx2, 0.0);
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const size_t c = static_cast<size_t>(j) * nx2 + i;
+        const int code = g->codesHost[which][c];
+        const bool member = which < 2 || (which == 2 && i == 1) || (which == 3 && i == nx) ||
+                            (which == 4 && j == 1) || (which == 5 && j == ny);
+        if (!member) continue;
+        double v = values[(j - 1) * nx + (i - 1)];
+        if (code == FREHG_BC_HEAD_HYDROSTATIC) v += g->offset;
+        vv[c] = v;
+      }
+    HIP_CHECK(hipMemcpyAsync(dst[which], vv.data(), vv.size() * sizeof(double),
+                             hipMemcpyHostToDevice, g->stream));
+    HIP_CHECK(hipStreamSynchronize(g->stream));
+  });
+}
+int frehg_gw_attach_coupling(frehg_gw_solver* s, double* eta, double* dept, double* seep,
+                             double* avail, double* gain, double* bounce, double min_depth,
+                             int vent_all) {
+  return guard([&] {
+    GwSolver* g = frehg::S(s);
+    if (!eta || !dept || !seep || !avail || !gain)
+      frehg::fail(FREHG_EINVAL, "coupling arrays eta, dept, seep_accum, avail, gain are required");
+    g->d.eta = eta;
+    g->d.dept = dept;
+    g->d.seep = seep;
+    g->d.avail = avail;
+    g->d.gain = gain;
+    g->d.bounce = bounce;
+    g->d.minDepth = min_depth;
+    g->d.ventAll = vent_all ? 1 : 0;
+    g->d.coupled = 1;
+  });
+}
+int frehg_gw_set_timing(frehg_gw_solver* s, int enable) {
+  return guard([&] {
+    GwSolver* g = frehg::S(s);
+    g->timing = enable != 0;
+    const double bytes = g->tm.spmv_bytes;
+    g->tm = frehg_gw_timing{};
+    g->tm.spmv_bytes = bytes;
+  });
+}
+int frehg_gw_get_timing(frehg_gw_solver* s, frehg_gw_timing* out) {
+  return guard([&] { *out = frehg::S(s)->tm; });
+}
+int frehg_gw_synchronize(frehg_gw_solver* s) {
+  return guard([&] { HIP_CHECK(hipStreamSynchronize(frehg::S(s)->stream)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,178 @@
+"""Python mirror of frehg::gw::RichardsSolver and frehg::LinearSystem over the
+HIP C-ABI (include/frehg_gw.h).
+
+Method names and argument meaning follow the reference
+(src/gw/RichardsSolver.hpp:166-423, src/core/LinearSystem.hpp:100-213):
+``step(t, dtg)``, ``next_dt()``, ``audit()``, ``volume()`` (ownedVolume),
+``last_solve()``, field getters, ``attach_coupling``.  Errors raise
+``FrehgError`` where the reference throws ``FatalError``.  Every call goes
+to the in-tree libfrehg_gw.so; there is no other code path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import FrehgError, check, load_library
+
+_dp = C.POINTER(C.c_double)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class RichardsSolver:
+    """One groundwater domain on the GPU (the reference's RichardsSolver)."""
+
+    def __init__(self, case):
+        self.lib = load_library()
+        self.case = case
+        desc, keep = case.to_desc()
+        h = C.c_void_p()
+        check(self.lib, self.lib.frehg_gw_create(C.byref(desc), C.byref(h)))
+        self._h = h
+        self.nx, self.ny, self.nz = case.nx, case.ny, case.nz
+        self.t = 0.0
+        self._cpl = None
+        if case.coupling is not None:
+            self.attach_coupling(**case.coupling)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.frehg_gw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field_shape(self, fid):
+        nk = self.nz + 1 if fid in abi.FACE_FIELDS else self.nz
+        return (self.ny + 2, self.nx + 2, nk)
+
+    def get(self, fid) -> np.ndarray:
+        out = np.empty(self.field_shape(fid), dtype=np.float64)
+        check(self.lib, self.lib.frehg_gw_get_field(self._h, fid, _dptr(out)))
+        return out
+
+    def set(self, fid, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.field_shape(fid))
+        check(self.lib, self.lib.frehg_gw_set_field(self._h, fid, _dptr(a)))
+
+    def set_bc_values(self, which: int, values) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(self.ny, self.nx)
+        check(self.lib, self.lib.frehg_gw_set_bc_values(self._h, which, _dptr(a)))
+
+    def attach_coupling(self, eta, dept, seep_accum, avail, gain, bounce=None, min_depth=0.0,
+                        vent_all=0):
+        """GwCoupling: 2D (ny+2, nx+2) surface arrays, copied to device buffers
+        that the step then mutates in place (RichardsSolver.hpp:138-163)."""
+        import torch  # device memory for the aliased coupling arrays
+
+        dev = torch.device("cuda")
+        names = ("eta", "dept", "seep_accum", "avail", "gain", "bounce")
+        vals = (eta, dept, seep_accum, avail, gain, bounce)
+        self._cpl = {}
+        for n, v in zip(names, vals):
+            if v is None:
+                self._cpl[n] = None
+                continue
+            t = torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64), device=dev).contiguous()
+            self._cpl[n] = t
+        torch.cuda.synchronize()
+        ptr = lambda n: C.c_void_p(self._cpl[n].data_ptr()) if self._cpl[n] is not None else None
+        check(self.lib, self.lib.frehg_gw_attach_coupling(
+            self._h, ptr("eta"), ptr("dept"), ptr("seep_accum"), ptr("avail"), ptr("gain"),
+            ptr("bounce"), float(min_depth), int(vent_all)))
+
+    def coupling_state(self, name) -> np.ndarray:
+        check(self.lib, self.lib.frehg_gw_synchronize(self._h))
+        return self._cpl[name].cpu().numpy().copy()
+
+    def step(self, dt=None, t=None) -> float:
+        if dt is None:
+            dt = self.next_dt()
+        self.t = self.t + dt if t is None else t
+        check(self.lib, self.lib.frehg_gw_step(self._h, self.t, dt))
+        return dt
+
+    def next_dt(self) -> float:
+        v = C.c_double()
+        check(self.lib, self.lib.frehg_gw_next_dt(self._h, C.byref(v)))
+        return v.value
+
+    def audit(self) -> dict:
+        a = abi.GwAudit()
+        check(self.lib, self.lib.frehg_gw_get_audit(self._h, C.byref(a)))
+        return a.as_dict()
+
+    def volume(self) -> float:
+        v = C.c_double()
+        check(self.lib, self.lib.frehg_gw_owned_volume(self._h, C.byref(v)))
+        return v.value
+
+    def last_solve(self) -> dict:
+        s = abi.SolveStats()
+        check(self.lib, self.lib.frehg_gw_last_solve(self._h, C.byref(s)))
+        return {n: getattr(s, n) for n, _ in s._fields_}
+
+    def active_cells(self) -> int:
+        v = C.c_int64()
+        check(self.lib, self.lib.frehg_gw_active_cells(self._h, C.byref(v)))
+        return v.value
+
+    def set_timing(self, enable: bool) -> None:
+        check(self.lib, self.lib.frehg_gw_set_timing(self._h, int(enable)))
+
+    def timing(self) -> dict:
+        t = abi.GwTiming()
+        check(self.lib, self.lib.frehg_gw_get_timing(self._h, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in t._fields_}
+
+    def synchronize(self) -> None:
+        check(self.lib, self.lib.frehg_gw_synchronize(self._h))
+
+
+class LinearSystem:
+    """The COO solve seam (LinearSystem::setPattern / setValues / solve).
+    Values, rhs and solution are device arrays (torch tensors on cuda)."""
+
+    def __init__(self, n_rows, rtol=1e-8, atol=1e-14, max_iterations=500, pc=abi.PC_JACOBI):
+        self.lib = load_library()
+        h = C.c_void_p()
+        check(self.lib, self.lib.frehg_ls_create(n_rows, rtol, atol, max_iterations, pc,
+                                                 C.byref(h)))
+        self._h = h
+        self.n = n_rows
+
+    def __del__(self):
+        try:
+            self.lib.frehg_ls_destroy(self._h)
+        except Exception:
+            pass
+
+    def set_pattern(self, rows, cols):
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        c = np.ascontiguousarray(cols, dtype=np.int64)
+        check(self.lib, self.lib.frehg_ls_set_pattern(
+            self._h, r.size, r.ctypes.data_as(C.POINTER(C.c_int64)),
+            c.ctypes.data_as(C.POINTER(C.c_int64))))
+
+    def set_values(self, values_device):
+        check(self.lib, self.lib.frehg_ls_set_values(self._h, C.c_void_p(values_device.data_ptr())))
+
+    def solve(self, rhs_device, sol_device) -> dict:
+        st = abi.SolveStats()
+        check(self.lib, self.lib.frehg_ls_solve(self._h, C.c_void_p(rhs_device.data_ptr()),
+                                                C.c_void_p(sol_device.data_ptr()), C.byref(st)))
+        return {n: getattr(st, n) for n, _ in st._fields_}
+
+    def is_symmetric(self, tol) -> bool:
+        s = C.c_int()
+        check(self.lib, self.lib.frehg_ls_is_symmetric(self._h, tol, C.byref(s)))
+        return bool(s.value)

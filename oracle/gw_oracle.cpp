@@ -39,6 +39,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -221,6 +222,8 @@ struct Oracle {
   frehg_gw_solve_stats last{};
   frehg_gw_audit audit{};
   double pcClampTotal = 0;
+  double tSolve = 0, tStep = 0;
+  int64_t itTotal = 0;
 
   int64_t g(int j, int i, int k) const {
     return gid[(static_cast<size_t>(j) * (nx + 2) + i) * nz + k];
@@ -1116,7 +1119,10 @@ void Oracle::solve() {
   M.setup(*this);
   if (guessPrevious)
     for (int64_t r = 0; r < nActive; ++r) sol[r] = hn(rowJ[r], rowI[r], rowK[r]);
+  const auto t0 = std::chrono::steady_clock::now();
   last = pcg(A, M, rhs.data(), sol.data(), rtol, atol, maxIt, guessPrevious);
+  tSolve += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  itTotal += last.iterations;
   if (!last.converged)
     fail(FREHG_EDIVERGED, "gw linear solve did not converge in " + std::to_string(last.iterations) +
                               " iterations (residual " + std::to_string(last.residual_norm) + ")");
@@ -1687,6 +1693,7 @@ double Oracle::ownedVolume() const {
 // --- RichardsSolver::step (RichardsSolver.cpp:729-794) ------------------------
 void Oracle::step(double /*t*/, double dtg) {
   if (!(dtg > 0.0)) fail(FREHG_EINVAL, "dtg must be positive");
+  const auto t0 = std::chrono::steady_clock::now();
   audit = frehg_gw_audit{};
   dtgCurrent = dtg;
   enforceHeadBc();
@@ -1707,6 +1714,7 @@ void Oracle::step(double /*t*/, double dtg) {
   enforceMoistureBc();
   boundaryFlux(dtg);
   adaptTimeStep(dtg);
+  tStep += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // ---------------------------------------------------------------------------
@@ -1883,6 +1891,15 @@ int oracle_gw_attach_coupling(void* p, double* eta, double* dept, double* seep, 
     o.minDepth = minDepth;
     o.ventAll = ventAll != 0;
   });
+}
+// Accumulated wall seconds of the PCG solves and of whole steps, and the
+// total PCG iteration count (the CPU baseline's clock).
+int oracle_gw_timing(void* p, double* out3) {
+  Oracle& o = *static_cast<Oracle*>(p);
+  out3[0] = o.tSolve;
+  out3[1] = o.tStep;
+  out3[2] = static_cast<double>(o.itTotal);
+  return FREHG_OK;
 }
 int oracle_gw_ktop(void* p, int* out) {
   Oracle& o = *static_cast<Oracle*>(p);

@@ -45,6 +45,7 @@ _SIG = {
     "oracle_gw_attach_coupling": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int],
                                   C.c_int),
     "oracle_gw_ktop": ([_vp, C.POINTER(C.c_int)], C.c_int),
+    "oracle_gw_timing": ([_vp, _dp], C.c_int),
     "oracle_gw_geometry_scalars": ([_vp, _dp], C.c_int),
     "oracle_gw_system": ([_vp, _dp, _dp, C.POINTER(C.c_int64)], C.c_int),
     "oracle_gw_assemble_only": ([_vp, C.c_double], C.c_int),
@@ -184,6 +185,11 @@ class OracleRichards:
         out = np.empty((self.ny, self.nx), dtype=np.int32)
         _check(lib().oracle_gw_ktop(self._h, out.ctypes.data_as(C.POINTER(C.c_int))))
         return out
+
+    def timing(self) -> dict:
+        out = np.empty(3)
+        _check(lib().oracle_gw_timing(self._h, _dptr(out)))
+        return {"solve_s": out[0], "step_s": out[1], "iterations": int(out[2])}
 
     def geometry_scalars(self) -> dict:
         out = np.empty(3)

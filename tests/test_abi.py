@@ -1,0 +1,71 @@
+"""The C-ABI library loads without a GPU, exports every function that
+include/frehg_gw.h declares, and its ctypes mirror has the C struct layout."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from frehg2_amd import abi
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "frehg_gw.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(frehg_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_seams():
+    names = declared_functions()
+    for required in ("frehg_gw_create", "frehg_gw_step", "frehg_gw_get_audit",
+                     "frehg_ls_set_pattern", "frehg_ls_solve"):
+        assert required in names
+    assert set(names) == set(abi.SYMBOLS), set(names) ^ set(abi.SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load_library()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    assert lib.frehg_abi_version() == abi.ABI_VERSION
+
+
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    prog = tmp_path / "layout.c"
+    fields = {
+        "frehg_gw_desc": [n for n, _ in abi.GwDesc._fields_],
+        "frehg_gw_audit": [n for n, _ in abi.GwAudit._fields_],
+        "frehg_gw_solve_stats": [n for n, _ in abi.SolveStats._fields_],
+        "frehg_gw_timing": [n for n, _ in abi.GwTiming._fields_],
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    for st, fl in fields.items():
+        lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for f in fl:
+            lines.append(f'printf("{st}.{f} %zu\\n", offsetof({st}, {f}));')
+    lines.append("return 0;}")
+    prog.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(prog)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True,
+                                                        text=True, check=True).stdout.split("\n") if l)
+    for st, cls in (("frehg_gw_desc", abi.GwDesc), ("frehg_gw_audit", abi.GwAudit),
+                    ("frehg_gw_solve_stats", abi.SolveStats), ("frehg_gw_timing", abi.GwTiming)):
+        assert int(out[st]) == C.sizeof(cls), st
+        for f, _ in cls._fields_:
+            assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)
+
+
+def test_missing_device_is_an_error_not_a_fallback():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from frehg2_amd.gw import RichardsSolver
+    from helpers import gw_base_case
+    with pytest.raises(abi.FrehgError) as e:
+        RichardsSolver(gw_base_case(1, 1, 4, 0.1, 0.5))
+    assert e.value.code == abi.FREHG_EDEVICE
