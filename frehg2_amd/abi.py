@@ -107,6 +107,10 @@ SYMBOLS = {
     "frehg_gw_set_timing": ([_vp, C.c_int], C.c_int),
     "frehg_gw_get_timing": ([_vp, C.POINTER(GwTiming)], C.c_int),
     "frehg_gw_synchronize": ([_vp], C.c_int),
+    "frehg_device_malloc": ([C.c_int64, C.POINTER(_vp)], C.c_int),
+    "frehg_device_free": ([_vp], C.c_int),
+    "frehg_memcpy_htod": ([_vp, _vp, C.c_int64], C.c_int),
+    "frehg_memcpy_dtoh": ([_vp, _vp, C.c_int64], C.c_int),
     "frehg_ls_create": ([C.c_int64, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(_vp)],
                         C.c_int),
     "frehg_ls_destroy": ([_vp], C.c_int),
@@ -152,3 +156,39 @@ def check(lib, rc: int):
     if rc != FREHG_OK:
         msg = lib.frehg_last_error()
         raise FrehgError(rc, msg.decode() if msg else "")
+
+
+class DeviceArray:
+    """A float64 device buffer in the library's HIP runtime (copy in / out)."""
+
+    def __init__(self, host, lib=None):
+        import numpy as np
+        self.lib = lib or load_library()
+        a = np.ascontiguousarray(host, dtype=np.float64)
+        self.shape = a.shape
+        self.nbytes = a.nbytes
+        p = C.c_void_p()
+        check(self.lib, self.lib.frehg_device_malloc(self.nbytes, C.byref(p)))
+        self.ptr = p
+        check(self.lib, self.lib.frehg_memcpy_htod(self.ptr, a.ctypes.data_as(C.c_void_p),
+                                                   self.nbytes))
+
+    @classmethod
+    def zeros(cls, shape, lib=None):
+        import numpy as np
+        return cls(np.zeros(shape), lib)
+
+    def numpy(self):
+        import numpy as np
+        out = np.empty(self.shape, dtype=np.float64)
+        check(self.lib, self.lib.frehg_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self.ptr,
+                                                   self.nbytes))
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.lib.frehg_device_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass

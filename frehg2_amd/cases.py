@@ -104,6 +104,7 @@ class GwCase:
     dt: float = 1.0
     coupling: Optional[dict] = None            # 2D (ny+2, nx+2) arrays + min_depth
     bc_schedule: Optional[object] = None       # callable(step) -> {which: values}
+    adaptive: bool = False                     # steps follow next_dt() (else dt)
 
     def soil_array(self, name: str) -> np.ndarray:
         v = getattr(self, name)
@@ -227,7 +228,9 @@ def stiff_32x32x50(steps: int = 100, precision: str = "mixed") -> GwCase:
     """The loam grid with stiffer contrasts: sigma_lnK 2.0 (8-cell correlation),
     n U[1.1,3.0], alpha U[1,8]; uniform h0 = -5 (the dry profile above a table
     that lies below this 1 m column); ponded head 0.05 on top; free drainage;
-    IR to rtol 1e-10."""
+    IR to rtol 1e-10.  At a fixed 60 s step the PCA scheme diverges on these
+    contrasts (the FP64 oracle does too), so the step follows the adaptive
+    controller (AdaptiveStep.cpp) from 1 s within [0.01, 60] s."""
     rng = _rng(3)
     nx = ny = 32
     nz = 50
@@ -236,9 +239,10 @@ def stiff_32x32x50(steps: int = 100, precision: str = "mixed") -> GwCase:
     n = rng.uniform(1.1, 3.0, (ny, nx, nz))
     c = GwCase(name="richards_stiff_32x32x50", nx=nx, ny=ny, nz=nz, dx=1.0, dy=1.0, dz=0.02,
                ksx=ks, ksy=ks, ksz=ks, vg_alpha=alpha, vg_n=n, theta_s=0.43, theta_r=0.078,
-               aev=0.0, specific_storage=1e-5, dt_init=60.0, dt_min=60.0, dt_max=60.0,
-               ic_form="head", ic_const=-5.0, rtol=1e-10, atol=1e-30, steps=steps, dt=60.0,
+               aev=0.0, specific_storage=1e-5, dt_init=1.0, dt_min=0.01, dt_max=60.0,
+               ic_form="head", ic_const=-5.0, rtol=1e-10, atol=1e-30, steps=steps, dt=1.0,
                precision=precision)
+    c.adaptive = True
     c.set_bc("top", abi.BC_HEAD, 0.05)
     c.set_bc("bot", abi.BC_GRAVITY, 0.0)
     return c
@@ -248,7 +252,8 @@ def columns_4096(steps: int = 50, ncol_side: int = 64, lmax: int = 400) -> GwCas
     """4096 independent soil columns of depth U{50..400} cells (dz=0.01) as a
     64x64x400 regular grid with lateral Ks = 0 and each bed lowered by
     (400-L) dz (ktop masking); per-column van Genuchten parameters, initial
-    head U[-10,-0.5], rain pulses of 0-2e-5 m/s, free drainage."""
+    head U[-10,-0.5], rain pulses of 0-2e-5 m/s, free drainage; adaptive step
+    from 1 s within [0.01, 60] s (a fixed 60 s step diverges at dz = 1 cm)."""
     rng = _rng(4)
     ny = nx = ncol_side
     nz = lmax
@@ -270,9 +275,10 @@ def columns_4096(steps: int = 50, ncol_side: int = 64, lmax: int = 400) -> GwCas
                vg_n=np.broadcast_to(n, (ny, nx, nz)),
                theta_s=np.broadcast_to(ths, (ny, nx, nz)),
                theta_r=np.broadcast_to(thr, (ny, nx, nz)), aev=0.0, specific_storage=1e-5,
-               dt_init=60.0, dt_min=60.0, dt_max=60.0, ic_form="head",
+               dt_init=1.0, dt_min=0.01, dt_max=60.0, ic_form="head",
                ic_values=np.broadcast_to(h0, (ny, nx, nz)).copy(), rtol=1e-10, atol=1e-30,
-               steps=steps, dt=60.0)
+               steps=steps, dt=1.0)
+    c.adaptive = True
     c.set_bc("top", abi.BC_FLUX, 0.0)
     c.set_bc("bot", abi.BC_GRAVITY, 0.0)
 
@@ -291,7 +297,8 @@ def coupled_128x128x64(member: int = 0, steps: int = 20, nx: int = 128, ny: int 
     log-normal Ks (ln-mean ln 5e-6, sigma 1.5, 10-cell correlation), alpha
     U[1,8], n U[1.2,2.5]; water table 1 m below a flat surface; a ponded-depth
     field of 0-0.1 m on about half of the columns feeds the coupled top
-    (dept, avail = dept, eta = dept), seepage faces elsewhere; 20 x 60 s."""
+    (dept, avail = dept, eta = dept), seepage faces elsewhere; adaptive step
+    from 1 s within [0.01, 60] s."""
     rng = _rng(5 + 1000 * member)
     ks = np.exp(np.log(5e-6) + 1.5 * smooth_field(rng, (ny, nx, nz), 10.0))
     alpha = rng.uniform(1.0, 8.0, (ny, nx, nz))
@@ -299,9 +306,10 @@ def coupled_128x128x64(member: int = 0, steps: int = 20, nx: int = 128, ny: int 
     pond = np.clip(0.1 * smooth_field(rng, (ny, nx), 16.0) * 0.5, 0.0, 0.1)
     c = GwCase(name="richards_coupled_128x128x64", nx=nx, ny=ny, nz=nz, dx=1.0, dy=1.0, dz=0.05,
                ksx=ks, ksy=ks, ksz=ks, vg_alpha=alpha, vg_n=n, theta_s=0.43, theta_r=0.078,
-               aev=0.0, specific_storage=1e-5, dt_init=60.0, dt_min=60.0, dt_max=60.0,
+               aev=0.0, specific_storage=1e-5, dt_init=1.0, dt_min=0.01, dt_max=60.0,
                ic_form="water_table", ic_const=-1.0, rtol=1e-10, atol=1e-30, steps=steps,
-               dt=60.0, precision=precision)
+               dt=1.0, precision=precision)
+    c.adaptive = True
     c.set_bc("bot", abi.BC_GRAVITY, 0.0)
     dept = np.zeros((ny + 2, nx + 2))
     dept[1:-1, 1:-1] = pond

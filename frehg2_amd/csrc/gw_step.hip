@@ -1739,6 +1739,29 @@ int frehg_gw_set_timing(frehg_gw_solver* s, int enable) {
 int frehg_gw_get_timing(frehg_gw_solver* s, frehg_gw_timing* out) {
   return guard([&] { *out = frehg::S(s)->tm; });
 }
+int frehg_device_malloc(int64_t bytes, void** ptr) {
+  return guard([&] {
+    if (!ptr || bytes < 0) fail(FREHG_EINVAL, "invalid arguments");
+    HIP_CHECK(hipMalloc(ptr, static_cast<size_t>(bytes > 0 ? bytes : 1)));
+    HIP_CHECK(hipMemset(*ptr, 0, static_cast<size_t>(bytes > 0 ? bytes : 1)));
+  });
+}
+int frehg_device_free(void* ptr) {
+  return guard([&] {
+    if (ptr) HIP_CHECK(hipFree(ptr));
+  });
+}
+int frehg_memcpy_htod(void* dst, const void* src, int64_t bytes) {
+  return guard([&] {
+    HIP_CHECK(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice));
+  });
+}
+int frehg_memcpy_dtoh(void* dst, const void* src, int64_t bytes) {
+  return guard([&] {
+    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost));
+  });
+}
 int frehg_gw_synchronize(frehg_gw_solver* s) {
   return guard([&] { HIP_CHECK(hipStreamSynchronize(frehg::S(s)->stream)); });
 }

@@ -72,27 +72,26 @@ class RichardsSolver:
                         vent_all=0):
         """GwCoupling: 2D (ny+2, nx+2) surface arrays, copied to device buffers
         that the step then mutates in place (RichardsSolver.hpp:138-163)."""
-        import torch  # device memory for the aliased coupling arrays
-
-        dev = torch.device("cuda")
         names = ("eta", "dept", "seep_accum", "avail", "gain", "bounce")
         vals = (eta, dept, seep_accum, avail, gain, bounce)
-        self._cpl = {}
-        for n, v in zip(names, vals):
-            if v is None:
-                self._cpl[n] = None
-                continue
-            t = torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64), device=dev).contiguous()
-            self._cpl[n] = t
-        torch.cuda.synchronize()
-        ptr = lambda n: C.c_void_p(self._cpl[n].data_ptr()) if self._cpl[n] is not None else None
+        self._cpl = {n: (abi.DeviceArray(v, self.lib) if v is not None else None)
+                     for n, v in zip(names, vals)}
+        ptr = lambda n: self._cpl[n].ptr if self._cpl[n] is not None else None
         check(self.lib, self.lib.frehg_gw_attach_coupling(
             self._h, ptr("eta"), ptr("dept"), ptr("seep_accum"), ptr("avail"), ptr("gain"),
             ptr("bounce"), float(min_depth), int(vent_all)))
 
+    def set_coupling_state(self, name, values) -> None:
+        """Overwrite one attached coupling array (e.g. after a surface update)."""
+        arr = self._cpl[name]
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(arr.shape)
+        check(self.lib, self.lib.frehg_gw_synchronize(self._h))
+        check(self.lib, self.lib.frehg_memcpy_htod(arr.ptr, a.ctypes.data_as(C.c_void_p),
+                                                   arr.nbytes))
+
     def coupling_state(self, name) -> np.ndarray:
         check(self.lib, self.lib.frehg_gw_synchronize(self._h))
-        return self._cpl[name].cpu().numpy().copy()
+        return self._cpl[name].numpy()
 
     def step(self, dt=None, t=None) -> float:
         if dt is None:
@@ -140,7 +139,7 @@ class RichardsSolver:
 
 class LinearSystem:
     """The COO solve seam (LinearSystem::setPattern / setValues / solve).
-    Values, rhs and solution are device arrays (torch tensors on cuda)."""
+    Values, rhs and solution are device arrays (abi.DeviceArray)."""
 
     def __init__(self, n_rows, rtol=1e-8, atol=1e-14, max_iterations=500, pc=abi.PC_JACOBI):
         self.lib = load_library()
@@ -164,12 +163,12 @@ class LinearSystem:
             c.ctypes.data_as(C.POINTER(C.c_int64))))
 
     def set_values(self, values_device):
-        check(self.lib, self.lib.frehg_ls_set_values(self._h, C.c_void_p(values_device.data_ptr())))
+        check(self.lib, self.lib.frehg_ls_set_values(self._h, values_device.ptr))
 
     def solve(self, rhs_device, sol_device) -> dict:
         st = abi.SolveStats()
-        check(self.lib, self.lib.frehg_ls_solve(self._h, C.c_void_p(rhs_device.data_ptr()),
-                                                C.c_void_p(sol_device.data_ptr()), C.byref(st)))
+        check(self.lib, self.lib.frehg_ls_solve(self._h, rhs_device.ptr, sol_device.ptr,
+                                                C.byref(st)))
         return {n: getattr(st, n) for n, _ in st._fields_}
 
     def is_symmetric(self, tol) -> bool:

@@ -230,6 +230,16 @@ int frehg_gw_get_timing(frehg_gw_solver* s, frehg_gw_timing* out);
 /* Block until all work on the solver's stream is done. */
 int frehg_gw_synchronize(frehg_gw_solver* s);
 
+/* ---- device-memory plumbing -------------------------------------------
+ * No counterpart in the reference (its Kokkos views own device memory).
+ * They let an FFI caller without its own HIP binding stage the device
+ * arrays the seams take (coupling state, COO values, rhs, solution) in the
+ * library's own HIP runtime. */
+int frehg_device_malloc(int64_t bytes, void** ptr);
+int frehg_device_free(void* ptr);
+int frehg_memcpy_htod(void* dst_device, const void* src_host, int64_t bytes);
+int frehg_memcpy_dtoh(void* dst_host, const void* src_device, int64_t bytes);
+
 /* ---- solve seam (replaces frehg::LinearSystem / PETSc KSPCG) ---------- */
 
 /* LinearSystem(comm, prefix, nLocalRows, nGlobalRows, settings)
