@@ -316,7 +316,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, Stencil<T> A, const 
   reduce_parts<2>(parts, gridDim.x, rs, red);
   const double rzv = red[0], rrv = red[1];
   if (rrv <= tol2) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // The first launch that sees convergence records it; the no-op launches
+    // queued behind it in the same polling chunk leave the count alone.
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !st->done) {
       st->done = 1;
       st->it = n;
       st->rr = rrv;
