@@ -19,6 +19,13 @@ struct Stencil {
   T *Ad = nullptr, *Axp = nullptr, *Ayp = nullptr, *Azp = nullptr;
 };
 
+// j-march launch shape: each wave walks S consecutive j rows of one i.
+struct March {
+  int S = 1;
+  int nslab = 1;
+  int blocks = 8;             // multiple of 8 (one share per XCD)
+};
+
 struct PcgStats {
   int iterations = 0;
   int refinements = 0;
@@ -34,7 +41,7 @@ struct PcgOptions {
   int max_iterations = 1000;
   bool zline = true;          // z-line block Jacobi, else point Jacobi
   bool mixed = false;         // FP32 inner PCG inside FP64 refinement
-  double inner_rtol = 1e-4;   // relative reduction of each inner FP32 solve
+  double inner_rtol = 1e-4;   // FP32 residual reduction between reliable updates
   int max_refinements = 40;
   bool timing = false;        // bracket each SpMV launch with HIP events
 };
@@ -63,12 +70,17 @@ class Pcg {
   template <class T>
   int innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T* z, T* q,
                       const T* invd, const T* w, bool zline, double tol2, int maxIt,
+                      const March& mj, int nprev0, int n0, bool restart,
                       const PcgOptions& opt, PcgStats* st, int* itDone);
-  void reduceHost(int nvals, double* out);
+  void reduceHost(int nparts, int nvals, double* out);
+
+  static March makeMarch(const Grid& g, int cus, int wavesPerCu);
 
   Grid g_;
   hipStream_t stream_;
   int blocks_ = 0;            // grid size of the column kernels (multiple of 8)
+  March march64_, march32_;   // stencil-kernel shapes, FP64 and FP32 operands
+  int pstride_ = 0;           // slot stride of parts_ (largest grid)
   // FP64 vectors (halo layout)
   double *r_ = nullptr, *z_ = nullptr, *p0_ = nullptr, *p1_ = nullptr, *q_ = nullptr;
   double *invd_ = nullptr, *w_ = nullptr;

@@ -10,9 +10,14 @@
 // l holding rows k = l + 64 m (m < R = ceil(nz / 64)), so every load and store
 // along k is coalesced, k +- 1 neighbours come from lane shuffles and the
 // z-line (tridiagonal) preconditioner is two wave-level affine scans in
-// registers.  Columns are split into 8 contiguous ranges, one per XCD, so the
-// i +- 1 / j +- 1 neighbour columns a wave reads were brought into the same L2
-// by the waves next to it.
+// registers.  Kernels without lateral neighbours split the columns into 8
+// contiguous ranges, one per XCD.  Stencil kernels use the j-march mapping: a
+// wave owns one i and a slab of S consecutive j, walks it in j and keeps the
+// j - 1 / j / j + 1 columns in registers, so each column is fetched once; the
+// i +- 1 columns belong to the waves next to it on the same XCD and come from
+// that XCD's L2.  (Fetching the j +- 1 columns of a row-ordered sweep from
+// cache instead costs ~1.7x in time at 512 x 512 x 100: three rows of the
+// operand and stencil arrays overflow the 4 MB L2 of an XCD.)
 //
 // One iteration is two kernels:
 //   dir:    p = z + beta p (beta from the previous update's partial sums,
@@ -74,10 +79,35 @@ __device__ __forceinline__ int64_t column_base(const Grid& g, int64_t col) {
   return g.c3(j, i, 0);
 }
 
+// j-march mapping: wave -> (i, rows j0..j1).  Blocks are remapped so each
+// XCD receives a contiguous range of waves (whole slabs when nx * kWpb
+// divides evenly).  Returns false for waves past the last slab.
+__device__ __forceinline__ bool march_span(const Grid& g, const March& mj, int& i, int& j0,
+                                           int& j1) {
+  const int64_t wg = xcd_block() * kWpb + threadIdx.x / kWave;
+  i = static_cast<int>(wg % g.nx) + 1;
+  const int64_t slab = wg / g.nx;
+  j0 = static_cast<int>(slab) * mj.S + 1;
+  j1 = min(g.ny, j0 + mj.S - 1);
+  return slab < mj.nslab && j0 <= g.ny;
+}
+
+template <class T, int R>
+__device__ __forceinline__ void load_col(const T* __restrict__ a, int64_t base, int lane, int nz,
+                                         T (&v)[R]) {
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int k = lane + kWave * m;
+    v[m] = k < nz ? a[base + k] : T(0);
+  }
+}
+
+// Partial sums live in parts[slot * pstride + block]; pstride is the largest
+// grid of any kernel, so kernels with different grids never overlap.
 // Redundant in-block reduction of the previous kernel's partial sums.
 template <int NV>
-__device__ __forceinline__ void reduce_parts(const double* parts, int nparts, const int (&slot)[NV],
-                                             double (&out)[NV]) {
+__device__ __forceinline__ void reduce_parts(const double* parts, int pstride, int nparts,
+                                             const int (&slot)[NV], double (&out)[NV]) {
   int op[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
@@ -86,13 +116,13 @@ __device__ __forceinline__ void reduce_parts(const double* parts, int nparts, co
   }
   for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) out[q] += parts[slot[q] * nparts + b];
+    for (int q = 0; q < NV; ++q) out[q] += parts[slot[q] * pstride + b];
   }
   block_reduce<NV>(out, op);
 }
 
 template <int NV>
-__device__ __forceinline__ void write_parts(double* parts, const int (&slot)[NV],
+__device__ __forceinline__ void write_parts(double* parts, int pstride, const int (&slot)[NV],
                                             double (&v)[NV]) {
   int op[NV];
 #pragma unroll
@@ -100,7 +130,7 @@ __device__ __forceinline__ void write_parts(double* parts, const int (&slot)[NV]
   block_reduce<NV>(v, op);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) parts[slot[q] * gridDim.x + blockIdx.x] = v[q];
+    for (int q = 0; q < NV; ++q) parts[slot[q] * pstride + blockIdx.x] = v[q];
   }
 }
 
@@ -247,73 +277,82 @@ __global__ __launch_bounds__(64) void k_factor(Grid g, const T* __restrict__ Ad,
   }
 }
 
-// r = b - A x, z = M^-1 r; partials rz (slot 0), rr (1), bb (3).
+// r = b - A x, z = M^-1 r; partials rz (slot 0), rr (1), bb (3).  j-march.
 template <int R, bool LINE>
-__global__ __launch_bounds__(kBlock) void k_residual64(Grid g, Stencil<double> A,
+__global__ __launch_bounds__(kBlock) void k_residual64(Grid g, March mj, Stencil<double> A,
                                                        const double* __restrict__ b,
                                                        const double* __restrict__ x,
                                                        double* __restrict__ r,
                                                        double* __restrict__ z,
                                                        const double* __restrict__ invd,
                                                        const double* __restrict__ w,
-                                                       double* __restrict__ parts) {
+                                                       double* __restrict__ parts, int pstride) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
+  const int64_t sx = g.nz;
   KSum rz, rr, bb;
-  int64_t col, end, stride;
-  column_loop(g.columns(), col, end, stride);
-  for (; col < end; col += stride) {
-    const int64_t base = column_base(g, col);
-    double xv[R], xdn[R], xup[R], az[R], azUp[R], dummy[R], rv[R], zv[R];
+  int i, j0, j1;
+  if (march_span(g, mj, i, j0, j1)) {
+    double xPrev[R], xCur[R], xNext[R], ayPrev[R];
+    load_col<double, R>(x, g.c3(j0 - 1, i, 0), lane, g.nz, xPrev);
+    load_col<double, R>(x, g.c3(j0, i, 0), lane, g.nz, xCur);
+    load_col<double, R>(A.Ayp, g.c3(j0 - 1, i, 0), lane, g.nz, ayPrev);
+    for (int j = j0; j <= j1; ++j) {
+      const int64_t base = g.c3(j, i, 0);
+      load_col<double, R>(x, base + static_cast<int64_t>(g.nx2) * g.nz, lane, g.nz, xNext);
+      double xdn[R], xup[R], az[R], azUp[R], ay[R], dummy[R], rv[R], zv[R];
+      load_col<double, R>(A.Azp, base, lane, g.nz, az);
+      load_col<double, R>(A.Ayp, base, lane, g.nz, ay);
+      shift_rows<double, R>(xCur, lane, xdn, xup);
+      shift_rows<double, R>(az, lane, dummy, azUp);
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int k = lane + kWave * m;
-      xv[m] = k < g.nz ? x[base + k] : 0.0;
-      az[m] = k < g.nz ? A.Azp[base + k] : 0.0;
-    }
-    shift_rows<double, R>(xv, lane, xdn, xup);
-    shift_rows<double, R>(az, lane, dummy, azUp);
-#pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int k = lane + kWave * m;
-      rv[m] = 0.0;
-      if (k < g.nz) {
-        const int64_t c = base + k;
-        const double q = A.Ad[c] * xv[m] + A.Axp[c] * x[c + sx] + A.Axp[c - sx] * x[c - sx] +
-                         A.Ayp[c] * x[c + sy] + A.Ayp[c - sy] * x[c - sy] + az[m] * xdn[m] +
-                         azUp[m] * xup[m];
-        const double bv = b[c];
-        rv[m] = bv - q;
-        bb.add(bv * bv);
+      for (int m = 0; m < R; ++m) {
+        const int k = lane + kWave * m;
+        rv[m] = 0.0;
+        if (k < g.nz) {
+          const int64_t c = base + k;
+          const double q = A.Ad[c] * xCur[m] + A.Axp[c] * x[c + sx] + A.Axp[c - sx] * x[c - sx] +
+                           ay[m] * xNext[m] + ayPrev[m] * xPrev[m] + az[m] * xdn[m] +
+                           azUp[m] * xup[m];
+          const double bv = b[c];
+          rv[m] = bv - q;
+          bb.add(bv * bv);
+        }
       }
-    }
-    apply_precond<double, R, LINE>(lane, g.nz, base, invd, w, rv, zv);
+      apply_precond<double, R, LINE>(lane, g.nz, base, invd, w, rv, zv);
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int k = lane + kWave * m;
-      if (k < g.nz) {
-        r[base + k] = rv[m];
-        z[base + k] = zv[m];
-        rz.add(rv[m] * zv[m]);
-        rr.add(rv[m] * rv[m]);
+      for (int m = 0; m < R; ++m) {
+        const int k = lane + kWave * m;
+        if (k < g.nz) {
+          r[base + k] = rv[m];
+          z[base + k] = zv[m];
+          rz.add(rv[m] * zv[m]);
+          rr.add(rv[m] * rv[m]);
+        }
+        xPrev[m] = xCur[m];
+        xCur[m] = xNext[m];
+        ayPrev[m] = ay[m];
       }
     }
   }
   double v[3] = {rz.value(), rr.value(), bb.value()};
   const int slots[3] = {0, 1, 3};
-  write_parts<3>(parts, slots, v);
+  write_parts<3>(parts, pstride, slots, v);
 }
 
 // Search direction and operator product: p = z + beta p_old, q = A p, p.q.
+// j-march: p at j - 1, j, j + 1 is formed once per column and kept in
+// registers; the i +- 1 values are formed from z and p_old at the neighbour.
 template <class T, int R>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, Stencil<T> A, const T* __restrict__ z,
-                                                    const T* __restrict__ pOld, T* __restrict__ pNew,
-                                                    T* __restrict__ q, double* __restrict__ parts,
-                                                    PcgState* __restrict__ st, int n, int restart,
-                                                    double tol2) {
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, March mj, Stencil<T> A,
+                                                    const T* __restrict__ z,
+                                                    const T* __restrict__ pOld,
+                                                    T* __restrict__ pNew, T* __restrict__ q,
+                                                    double* __restrict__ parts, int pstride,
+                                                    int nprev, PcgState* __restrict__ st, int n,
+                                                    int restart, double tol2) {
   double red[2];
   const int rs[2] = {0, 1};
-  reduce_parts<2>(parts, gridDim.x, rs, red);
+  reduce_parts<2>(parts, pstride, nprev, rs, red);
   const double rzv = red[0], rrv = red[1];
   if (rrv <= tol2) {
     // The first launch that sees convergence records it; the no-op launches
@@ -335,44 +374,50 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, Stencil<T> A, const 
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
   KSum pq;
-  int64_t col, end, stride;
-  column_loop(g.columns(), col, end, stride);
-  for (; col < end; col += stride) {
-    const int64_t base = column_base(g, col);
-    T pn[R], pdn[R], pup[R], az[R], azUp[R], dummy[R];
+  int i, j0, j1;
+  if (march_span(g, mj, i, j0, j1)) {
+    auto formp = [&](int64_t base, T (&pv)[R]) {
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int k = lane + kWave * m;
-      pn[m] = T(0);
-      az[m] = T(0);
-      if (k < g.nz) {
-        const int64_t c = base + k;
-        pn[m] = restart ? z[c] : z[c] + beta * pOld[c];
-        az[m] = A.Azp[c];
+      for (int m = 0; m < R; ++m) {
+        const int k = lane + kWave * m;
+        pv[m] = T(0);
+        if (k < g.nz) pv[m] = restart ? z[base + k] : z[base + k] + beta * pOld[base + k];
       }
-    }
-    shift_rows<T, R>(pn, lane, pdn, pup);
-    shift_rows<T, R>(az, lane, dummy, azUp);
+    };
+    T pPrev[R], pCur[R], pNext[R], ayPrev[R];
+    formp(g.c3(j0 - 1, i, 0), pPrev);
+    formp(g.c3(j0, i, 0), pCur);
+    load_col<T, R>(A.Ayp, g.c3(j0 - 1, i, 0), lane, g.nz, ayPrev);
+    for (int j = j0; j <= j1; ++j) {
+      const int64_t base = g.c3(j, i, 0);
+      formp(base + sy, pNext);
+      T pdn[R], pup[R], az[R], azUp[R], ay[R], dummy[R];
+      load_col<T, R>(A.Azp, base, lane, g.nz, az);
+      load_col<T, R>(A.Ayp, base, lane, g.nz, ay);
+      shift_rows<T, R>(pCur, lane, pdn, pup);
+      shift_rows<T, R>(az, lane, dummy, azUp);
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int k = lane + kWave * m;
-      if (k < g.nz) {
-        const int64_t c = base + k;
-        const T pxp = restart ? z[c + sx] : z[c + sx] + beta * pOld[c + sx];
-        const T pxm = restart ? z[c - sx] : z[c - sx] + beta * pOld[c - sx];
-        const T pyp = restart ? z[c + sy] : z[c + sy] + beta * pOld[c + sy];
-        const T pym = restart ? z[c - sy] : z[c - sy] + beta * pOld[c - sy];
-        const T qv = A.Ad[c] * pn[m] + A.Axp[c] * pxp + A.Axp[c - sx] * pxm + A.Ayp[c] * pyp +
-                     A.Ayp[c - sy] * pym + az[m] * pdn[m] + azUp[m] * pup[m];
-        pNew[c] = pn[m];
-        q[c] = qv;
-        pq.add(static_cast<double>(pn[m]) * static_cast<double>(qv));
+      for (int m = 0; m < R; ++m) {
+        const int k = lane + kWave * m;
+        if (k < g.nz) {
+          const int64_t c = base + k;
+          const T pxp = restart ? z[c + sx] : z[c + sx] + beta * pOld[c + sx];
+          const T pxm = restart ? z[c - sx] : z[c - sx] + beta * pOld[c - sx];
+          const T qv = A.Ad[c] * pCur[m] + A.Axp[c] * pxp + A.Axp[c - sx] * pxm + ay[m] * pNext[m] +
+                       ayPrev[m] * pPrev[m] + az[m] * pdn[m] + azUp[m] * pup[m];
+          pNew[c] = pCur[m];
+          q[c] = qv;
+          pq.add(static_cast<double>(pCur[m]) * static_cast<double>(qv));
+        }
+        pPrev[m] = pCur[m];
+        pCur[m] = pNext[m];
+        ayPrev[m] = ay[m];
       }
     }
   }
   double v[1] = {pq.value()};
   const int slots[1] = {2};
-  write_parts<1>(parts, slots, v);
+  write_parts<1>(parts, pstride, slots, v);
 }
 
 // alpha = rz / p.q; x += alpha p; r -= alpha q; z = M^-1 r; partials rz, rr.
@@ -382,12 +427,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Grid g, T* __restrict__ x
                                                        const T* __restrict__ q, T* __restrict__ z,
                                                        const T* __restrict__ invd,
                                                        const T* __restrict__ w,
-                                                       double* __restrict__ parts,
-                                                       PcgState* __restrict__ st, int n) {
+                                                       double* __restrict__ parts, int pstride,
+                                                       int nprev, PcgState* __restrict__ st,
+                                                       int n) {
   if (st->done) return;
   double red[1];
   const int rs[1] = {2};
-  reduce_parts<1>(parts, gridDim.x, rs, red);
+  reduce_parts<1>(parts, pstride, nprev, rs, red);
   const double alphaD = st->rz[n & 1] / red[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alphaD;
   const T alpha = static_cast<T>(alphaD);
@@ -422,56 +468,68 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Grid g, T* __restrict__ x
   }
   double v[2] = {rz.value(), rr.value()};
   const int slots[2] = {0, 1};
-  write_parts<2>(parts, slots, v);
+  write_parts<2>(parts, pstride, slots, v);
 }
 
 // Mixed form, outer sweep: xo = xi + sc d (all 7 stencil points), r = b - A xo
-// in FP64; partials rr (slot 1), bb (slot 3).  With hasD = 0, xo is xi.
+// in FP64; partials rr (slot 1), bb (slot 3).  With hasD = 0, xo is xi.  j-march.
 template <int R>
-__global__ __launch_bounds__(kBlock) void k_refine64(Grid g, Stencil<double> A,
+__global__ __launch_bounds__(kBlock) void k_refine64(Grid g, March mj, Stencil<double> A,
                                                      const double* __restrict__ b,
                                                      const double* __restrict__ xi,
                                                      double* __restrict__ xo,
                                                      const float* __restrict__ d, double sc,
                                                      int hasD, double* __restrict__ r,
-                                                     double* __restrict__ parts) {
+                                                     double* __restrict__ parts, int pstride) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
   KSum rr, bb;
-  int64_t col, end, stride;
-  column_loop(g.columns(), col, end, stride);
   auto xn = [&](int64_t c) { return hasD ? xi[c] + sc * static_cast<double>(d[c]) : xi[c]; };
-  for (; col < end; col += stride) {
-    const int64_t base = column_base(g, col);
-    double xv[R], xdn[R], xup[R], az[R], azUp[R], dummy[R];
+  auto formx = [&](int64_t base, double (&xv)[R]) {
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const int k = lane + kWave * m;
       xv[m] = k < g.nz ? xn(base + k) : 0.0;
-      az[m] = k < g.nz ? A.Azp[base + k] : 0.0;
     }
-    shift_rows<double, R>(xv, lane, xdn, xup);
-    shift_rows<double, R>(az, lane, dummy, azUp);
+  };
+  int i, j0, j1;
+  if (march_span(g, mj, i, j0, j1)) {
+    double xPrev[R], xCur[R], xNext[R], ayPrev[R];
+    formx(g.c3(j0 - 1, i, 0), xPrev);
+    formx(g.c3(j0, i, 0), xCur);
+    load_col<double, R>(A.Ayp, g.c3(j0 - 1, i, 0), lane, g.nz, ayPrev);
+    for (int j = j0; j <= j1; ++j) {
+      const int64_t base = g.c3(j, i, 0);
+      formx(base + sy, xNext);
+      double xdn[R], xup[R], az[R], azUp[R], ay[R], dummy[R];
+      load_col<double, R>(A.Azp, base, lane, g.nz, az);
+      load_col<double, R>(A.Ayp, base, lane, g.nz, ay);
+      shift_rows<double, R>(xCur, lane, xdn, xup);
+      shift_rows<double, R>(az, lane, dummy, azUp);
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int k = lane + kWave * m;
-      if (k < g.nz) {
-        const int64_t c = base + k;
-        const double qv = A.Ad[c] * xv[m] + A.Axp[c] * xn(c + sx) + A.Axp[c - sx] * xn(c - sx) +
-                          A.Ayp[c] * xn(c + sy) + A.Ayp[c - sy] * xn(c - sy) + az[m] * xdn[m] +
-                          azUp[m] * xup[m];
-        const double bv = b[c];
-        const double rv = bv - qv;
-        if (hasD) xo[c] = xv[m];
-        r[c] = rv;
-        rr.add(rv * rv);
-        bb.add(bv * bv);
+      for (int m = 0; m < R; ++m) {
+        const int k = lane + kWave * m;
+        if (k < g.nz) {
+          const int64_t c = base + k;
+          const double qv = A.Ad[c] * xCur[m] + A.Axp[c] * xn(c + sx) + A.Axp[c - sx] * xn(c - sx) +
+                            ay[m] * xNext[m] + ayPrev[m] * xPrev[m] + az[m] * xdn[m] +
+                            azUp[m] * xup[m];
+          const double bv = b[c];
+          const double rv = bv - qv;
+          if (hasD) xo[c] = xCur[m];
+          r[c] = rv;
+          rr.add(rv * rv);
+          bb.add(bv * bv);
+        }
+        xPrev[m] = xCur[m];
+        xCur[m] = xNext[m];
+        ayPrev[m] = ay[m];
       }
     }
   }
   double v[2] = {rr.value(), bb.value()};
   const int slots[2] = {1, 3};
-  write_parts<2>(parts, slots, v);
+  write_parts<2>(parts, pstride, slots, v);
 }
 
 // Mixed form, inner start: r32 = s r64, d = 0, z32 = M32^-1 r32; partials rz, rr.
@@ -482,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_inner_init(Grid g, const double* __r
                                                        float* __restrict__ d32,
                                                        const float* __restrict__ invd,
                                                        const float* __restrict__ w,
-                                                       double* __restrict__ parts) {
+                                                       double* __restrict__ parts, int pstride) {
   const int lane = threadIdx.x & (kWave - 1);
   KSum rz, rr;
   int64_t col, end, stride;
@@ -510,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_inner_init(Grid g, const double* __r
   }
   double v[2] = {rz.value(), rr.value()};
   const int slots[2] = {0, 1};
-  write_parts<2>(parts, slots, v);
+  write_parts<2>(parts, pstride, slots, v);
 }
 
 // FP64 stencil -> FP32 stencil over interior cells.
@@ -530,10 +588,11 @@ __global__ __launch_bounds__(kBlock) void k_cast_stencil(Grid g, Stencil<double>
 
 // Fixed-order reduction of nvals partial slots into out[0..nvals).
 __global__ __launch_bounds__(kBlock) void k_reduce_parts(const double* __restrict__ parts,
-                                                         int nparts, double* __restrict__ out) {
+                                                         int pstride, int nparts,
+                                                         double* __restrict__ out) {
   double v[4];
   const int slots[4] = {0, 1, 2, 3};
-  reduce_parts<4>(parts, nparts, slots, v);
+  reduce_parts<4>(parts, pstride, nparts, slots, v);
   if (threadIdx.x == 0) {
     for (int q = 0; q < 4; ++q) out[q] = v[q];
   }
@@ -580,6 +639,11 @@ Pcg::Pcg(const Grid& g, hipStream_t stream) : g_(g), stream_(stream) {
   const int64_t ncol = g.columns();
   const int64_t wavesWanted = std::max<int64_t>(8, std::min<int64_t>(ncol, int64_t(cus) * 16));
   blocks_ = static_cast<int>(std::max<int64_t>(8, ((wavesWanted + kWpb - 1) / kWpb + 7) / 8 * 8));
+  // Measured at 512 x 512 x 100: FP64 operands run best at ~16 waves per CU
+  // (S = 64), FP32 operands at ~64 (S = 16).
+  march64_ = makeMarch(g, cus, 16);
+  march32_ = makeMarch(g, cus, 64);
+  pstride_ = std::max({blocks_, march64_.blocks, march32_.blocks});
   const int64_t nH = g.cellsH();
   r_ = dalloc<double>(nH);
   z_ = dalloc<double>(nH);
@@ -588,7 +652,7 @@ Pcg::Pcg(const Grid& g, hipStream_t stream) : g_(g), stream_(stream) {
   q_ = dalloc<double>(nH);
   invd_ = dalloc<double>(nH);
   w_ = dalloc<double>(nH);
-  parts_ = dalloc<double>(4 * static_cast<int64_t>(blocks_));
+  parts_ = dalloc<double>(4 * static_cast<int64_t>(pstride_));
   red_ = dalloc<double>(4);
   state_ = dalloc<PcgState>(1);
   HIP_OK(hipHostMalloc(&redHost_, 4 * sizeof(double), hipHostMallocDefault));
@@ -615,8 +679,19 @@ double Pcg::spmvBytesPerLaunch(bool mixed) const {
   return per * static_cast<double>(g_.columns()) * g_.nz;
 }
 
-void Pcg::reduceHost(int nvals, double* out) {
-  k_reduce_parts<<<1, kBlock, 0, stream_>>>(parts_, blocks_, red_);
+March Pcg::makeMarch(const Grid& g, int cus, int wavesPerCu) {
+  March mj;
+  const int64_t target = static_cast<int64_t>(cus) * wavesPerCu;
+  mj.nslab = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(g.ny, target / g.nx)));
+  mj.S = (g.ny + mj.nslab - 1) / mj.nslab;
+  mj.nslab = (g.ny + mj.S - 1) / mj.S;  // every slab starts inside the grid
+  const int64_t waves = static_cast<int64_t>(g.nx) * mj.nslab;
+  mj.blocks = static_cast<int>(std::max<int64_t>(8, ((waves + kWpb - 1) / kWpb + 7) / 8 * 8));
+  return mj;
+}
+
+void Pcg::reduceHost(int nparts, int nvals, double* out) {
+  k_reduce_parts<<<1, kBlock, 0, stream_>>>(parts_, pstride_, nparts, red_);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(redHost_, red_, 4 * sizeof(double), hipMemcpyDeviceToHost, stream_));
   HIP_OK(hipStreamSynchronize(stream_));
@@ -632,12 +707,18 @@ int Pcg::solve(const Stencil<double>& A, const double* b, double* x, const PcgOp
 template <class T>
 int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T* z, T* q,
                          const T* invd, const T* w, bool zline, double tol2, int maxIt,
+                         const March& mj, int nprev0, int n0, bool restart,
                          const PcgOptions& opt, PcgStats* st, int* itDone) {
+  // Iterations are numbered on from n0 so that a resumed run finds the
+  // previous rz in the state slot the numbering expects.
   PcgState* dst = static_cast<PcgState*>(state_);
   PcgState* hst = static_cast<PcgState*>(stateHost_);
   k_state_reset<<<1, 1, 0, stream_>>>(dst);
   HIP_OK(hipGetLastError());
-  int n = 0;
+  int n = n0;
+  maxIt += n0;
+  T* const bufOld = pOld;  // dir launch n writes p into (n - n0 even ? bufNew : bufOld)
+  T* const bufNew = pNew;
   hipEvent_t evs[2 * kPoll];
   if (opt.timing)
     for (auto& e : evs) HIP_OK(hipEventCreate(&e));
@@ -649,19 +730,20 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
       if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c], stream_));
       dispatch_R(g_.nz, [&](auto RC) {
         constexpr int R = decltype(RC)::value;
-        k_pcg_dir<T, R><<<blocks_, kBlock, 0, stream_>>>(g_, A, z, pOld, pNew, q, parts_, dst, n,
-                                                         n == 0 ? 1 : 0, tol2);
+        k_pcg_dir<T, R><<<mj.blocks, kBlock, 0, stream_>>>(
+            g_, mj, A, z, pOld, pNew, q, parts_, pstride_, n == n0 ? nprev0 : blocks_, dst, n,
+            (n == n0 && restart) ? 1 : 0, tol2);
       });
       HIP_OK(hipGetLastError());
       if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c + 1], stream_));
       dispatch_R(g_.nz, [&](auto RC) {
         constexpr int R = decltype(RC)::value;
         if (zline)
-          k_pcg_update<T, R, true><<<blocks_, kBlock, 0, stream_>>>(g_, x, pNew, r, q, z, invd, w,
-                                                                    parts_, dst, n);
+          k_pcg_update<T, R, true><<<blocks_, kBlock, 0, stream_>>>(
+              g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
         else
-          k_pcg_update<T, R, false><<<blocks_, kBlock, 0, stream_>>>(g_, x, pNew, r, q, z, invd,
-                                                                     w, parts_, dst, n);
+          k_pcg_update<T, R, false><<<blocks_, kBlock, 0, stream_>>>(
+              g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
       });
       HIP_OK(hipGetLastError());
       std::swap(pOld, pNew);
@@ -685,7 +767,17 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
   }
   if (opt.timing)
     for (auto& e : evs) (void)hipEventDestroy(e);
-  *itDone = done ? hst->it : std::min(n, maxIt);
+  *itDone = (done ? hst->it : std::min(n, maxIt)) - n0;
+  // Launches queued after convergence are no-ops, yet the loop swapped the
+  // pointers for them too: point pOld at the last p actually formed.
+  if (*itDone > 0) {
+    const bool lastEven = ((*itDone - 1) & 1) == 0;
+    pOld = lastEven ? bufNew : bufOld;
+    pNew = lastEven ? bufOld : bufNew;
+  } else {
+    pOld = bufOld;
+    pNew = bufNew;
+  }
   return done;
 }
 
@@ -700,16 +792,17 @@ int Pcg::solveFp64(const Stencil<double>& A, const double* b, double* x, const P
   auto residual = [&]() {
     dispatch_R(g_.nz, [&](auto RC) {
       constexpr int R = decltype(RC)::value;
+      const March& mj = march64_;
       if (opt.zline)
-        k_residual64<R, true><<<blocks_, kBlock, 0, stream_>>>(g_, A, b, x, r_, z_, invd_, w_,
-                                                               parts_);
+        k_residual64<R, true><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, b, x, r_, z_, invd_,
+                                                                 w_, parts_, pstride_);
       else
-        k_residual64<R, false><<<blocks_, kBlock, 0, stream_>>>(g_, A, b, x, r_, z_, invd_, w_,
-                                                                parts_);
+        k_residual64<R, false><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, b, x, r_, z_, invd_,
+                                                                  w_, parts_, pstride_);
     });
     HIP_OK(hipGetLastError());
     double red[4];
-    reduceHost(4, red);
+    reduceHost(march64_.blocks, 4, red);
     return std::make_pair(red[1], red[3]);
   };
   auto [rr, bb] = residual();
@@ -725,14 +818,15 @@ int Pcg::solveFp64(const Stencil<double>& A, const double* b, double* x, const P
       st->residual_norm = std::sqrt(rr);
       break;
     }
-    if (total >= opt.max_iterations) {
+    if (!std::isfinite(rr) || total >= opt.max_iterations) {
       st->residual_norm = std::sqrt(rr);
       break;
     }
     int its = 0;
     Stencil<double> Ac = A;
     innerIterations<double>(Ac, x, pOld, pNew, r_, z_, q_, invd_, w_, opt.zline, tol2,
-                            opt.max_iterations - total, opt, st, &its);
+                            opt.max_iterations - total, march64_, march64_.blocks, 0, true, opt,
+                            st, &its);
     total += its;
     std::tie(rr, bb) = residual();  // true residual (restart point if not converged)
     if (its == 0 && ++stalls > 2 && !(rr <= tol2)) {
@@ -776,12 +870,12 @@ int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const 
   auto refine = [&](const double* xi, double* xo, double sc, int hasD) {
     dispatch_R(g_.nz, [&](auto RC) {
       constexpr int R = decltype(RC)::value;
-      k_refine64<R><<<blocks_, kBlock, 0, stream_>>>(g_, A, b, xi, xo, d32_, sc, hasD, r_,
-                                                     parts_);
+      k_refine64<R><<<march64_.blocks, kBlock, 0, stream_>>>(g_, march64_, A, b, xi, xo, d32_, sc,
+                                                             hasD, r_, parts_, pstride_);
     });
     HIP_OK(hipGetLastError());
     double red[4];
-    reduceHost(4, red);
+    reduceHost(march64_.blocks, 4, red);
     return std::make_pair(red[1], red[3]);
   };
   auto [rr, bb] = refine(xa, xb, 0.0, 0);
@@ -789,31 +883,45 @@ int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const 
   const double tol2 = tol * tol;
   st->rhs_norm = std::sqrt(bb);
   const double ncell = static_cast<double>(g_.columns()) * g_.nz;
-  int total = 0;
+  // Reliable updates: the FP32 CG runs on s r (one fixed scale, so the FP32
+  // iterates are O(1) at the start); whenever its recursive residual has
+  // fallen by inner_rtol, or to the final tolerance, the correction is added
+  // to x in FP64, r = b - A x is recomputed in FP64 and handed back to the
+  // FP32 recursion, which carries on with its search direction instead of
+  // restarting, so the Krylov space built so far is kept.
+  const double s = rr > 0.0 ? 1.0 / std::sqrt(rr / ncell) : 1.0;
+  float* pOld = p032_;
+  float* pNew = p132_;
+  int total = 0, n0 = 0;
+  bool restart = true;
   while (true) {
     if (rr <= tol2) {
       st->converged = 1;
       break;
     }
-    if (st->refinements >= opt.max_refinements || total >= opt.max_iterations) break;
-    const double s = 1.0 / std::sqrt(rr / ncell);
+    if (!std::isfinite(rr) || st->refinements >= opt.max_refinements ||
+        total >= opt.max_iterations)
+      break;
     dispatch_R(g_.nz, [&](auto RC) {
       constexpr int R = decltype(RC)::value;
       if (opt.zline)
         k_inner_init<R, true><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
-                                                               invd32_, w32_, parts_);
+                                                               invd32_, w32_, parts_, pstride_);
       else
         k_inner_init<R, false><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
-                                                                invd32_, w32_, parts_);
+                                                                invd32_, w32_, parts_, pstride_);
     });
     HIP_OK(hipGetLastError());
-    const double tolIn2 = opt.inner_rtol * opt.inner_rtol * rr * s * s;
+    const double rrS = rr * s * s;
+    const double tolIn2 =
+        std::max(opt.inner_rtol * opt.inner_rtol * rrS, 0.25 * tol2 * s * s);
     int its = 0;
-    float* pOld = p032_;
-    float* pNew = p132_;
     innerIterations<float>(A32, d32_, pOld, pNew, r32_, z32_, q32_, invd32_, w32_, opt.zline,
-                           tolIn2, std::min(opt.max_iterations - total, 400), opt, st, &its);
+                           tolIn2, std::min(opt.max_iterations - total, 400), march32_, blocks_,
+                           n0, restart, opt, st, &its);
     total += its;
+    n0 += its;
+    restart = false;
     std::tie(rr, bb) = refine(xa, xb, 1.0 / s, 1);
     std::swap(xa, xb);
     st->refinements += 1;
@@ -828,9 +936,11 @@ int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const 
 
 template int Pcg::innerIterations<double>(const Stencil<double>&, double*, double*&, double*&,
                                           double*, double*, double*, const double*, const double*,
-                                          bool, double, int, const PcgOptions&, PcgStats*, int*);
+                                          bool, double, int, const March&, int, int, bool,
+                                          const PcgOptions&, PcgStats*, int*);
 template int Pcg::innerIterations<float>(const Stencil<float>&, float*, float*&, float*&, float*,
                                          float*, float*, const float*, const float*, bool, double,
-                                         int, const PcgOptions&, PcgStats*, int*);
+                                         int, const March&, int, int, bool, const PcgOptions&,
+                                         PcgStats*, int*);
 
 }  // namespace frehg
