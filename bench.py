@@ -2,17 +2,23 @@
 
 Workload richards_hetero_512x512x100 (512 x 512 x 100 cells, terrain-following,
 log-normal Ks, 300 s steps): W untimed warmup steps, then K timed steps, for
-the plain FP64 form and the mixed form (FP32 inner PCG inside FP64
-refinement), each from the same initial state.  Inputs are resident in HBM
+the plain FP64 form and the mixed form (FP32 inner PCG with FP64 residual
+updates), each from the same initial state.  Inputs are resident in HBM
 before the timed region starts.
 
 metric      cell-iterations/s = active cells x PCG iterations / PCG wall time
             (the FP64 form is `value`; the mixed form is reported beside it)
-roofline    the dominant kernel is the PCG direction kernel (p = z + beta p,
-            q = A p): algorithmic bytes per launch / its mean launch time,
-            measured with HIP events on the solver's stream
-cpu_baseline  the FP64 oracle (C++/OpenMP, z-line PCG) on a bounded sample
-            of the same workload: a 128 x 128 x 100 slice, 2 steps
+accuracy    "@1e-8 rel err": each form runs at the tolerance that keeps its
+            head within 1e-8 (max-abs, relative to max |h|) of a converged
+            solution.  After the timed steps both forms take one step from the
+            same state and are compared with that step converged to rtol 1e-13
+            (FP64 residual, iterative refinement); the errors are reported.
+roofline    the dominant kernel is the PCG direction kernel k_pcg_dir (FP64):
+            algorithmic bytes per launch / its mean launch time, measured with
+            HIP events on the solver's own stream over the timed steps
+            (launches that exit at convergence are not counted)
+cpu_baseline  the FP64 oracle (C++/OpenMP, z-line PCG) on a bounded sample of
+            the same workload: a 256 x 256 x 100 slice, 3 steps
 
 One GPU only: --gpus N > 1 exits with a message.
 """
@@ -31,18 +37,26 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# Measured on this machine with plain streaming kernels over 211 MB arrays
+# (FP64): a copy reaches 5.7 TB/s, a 6-read/2-write stream 4.8 TB/s.
+STREAM_GBS = {"copy": 5700.0, "stream_6r2w": 4800.0}
+ERR_TARGET = 1e-8
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=100)
+    ap.add_argument("--rtol-fp64", type=float, default=1e-12,
+                    help="FP64 form tolerance (meets 1e-8 head error on this workload)")
+    ap.add_argument("--rtol-mixed", type=float, default=1e-12,
+                    help="mixed form tolerance (meets 1e-8 head error on this workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy check")
     ap.add_argument("--forms", default="fp64,mixed")
     return ap.parse_args()
 
@@ -51,22 +65,29 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_form(case, steps, warmup, precision):
+def make_case(args, precision, rtol):
+    from frehg2_amd import cases
+    c = cases.hetero_512x512x100(nx=args.nx, ny=args.ny, nz=args.nz, precision=precision)
+    c.rtol = rtol
+    return c
+
+
+def run_form(args, precision, rtol):
     from frehg2_amd.gw import RichardsSolver
-    case.precision = precision
+    case = make_case(args, precision, rtol)
     t0 = time.perf_counter()
     g = RichardsSolver(case)
     g.synchronize()
     log(f"{precision}: solver staged in {time.perf_counter() - t0:.1f} s "
-        f"({g.active_cells()} active cells)")
-    for _ in range(warmup):
+        f"({g.active_cells()} active cells), rtol {rtol:g}")
+    for _ in range(args.warmup):
         g.step(case.dt)
     g.synchronize()
     g.set_timing(True)
     t0 = time.perf_counter()
     its = 0
     refs = 0
-    for s in range(steps):
+    for _ in range(args.steps):
         g.step(case.dt)
         st = g.last_solve()
         its += st["iterations"]
@@ -74,56 +95,98 @@ def run_form(case, steps, warmup, precision):
     g.synchronize()
     wall = time.perf_counter() - t0
     tm = g.timing()
+    g.set_timing(False)
     n = g.active_cells()
     res = {
         "precision": precision,
+        "rtol": rtol,
         "wall_s": wall,
-        "ms_per_step": 1e3 * wall / steps,
+        "ms_per_step": 1e3 * wall / args.steps,
         "step_ms_events": tm["step_ms"] / max(tm["steps"], 1),
         "solve_ms_per_step": tm["solve_ms"] / max(tm["steps"], 1),
         "pcg_iterations": its,
-        "iterations_per_solve": its / steps,
-        "refinements_per_solve": refs / steps,
+        "iterations_per_solve": its / args.steps,
+        "refinements_per_solve": refs / args.steps,
         "cell_iters_per_s": n * its / (tm["solve_ms"] * 1e-3),
-        "cell_steps_per_s": n * steps / wall,
+        "cell_steps_per_s": n * args.steps / wall,
         "spmv_avg_ms": tm["spmv_ms"] / max(tm["spmv_launches"], 1),
         "spmv_launches": tm["spmv_launches"],
         "spmv_bytes": tm["spmv_bytes"],
         "active_cells": n,
     }
     res["spmv_gbs"] = res["spmv_bytes"] / (res["spmv_avg_ms"] * 1e-3) / 1e9
+    log(f"{precision}: {res['ms_per_step']:.2f} ms/step, solve {res['solve_ms_per_step']:.2f} ms, "
+        f"{res['iterations_per_solve']:.1f} it/solve, {res['cell_iters_per_s'] / 1e9:.2f} G cell-it/s, "
+        f"dir kernel {res['spmv_avg_ms']:.3f} ms ({res['spmv_gbs']:.0f} GB/s)")
+    return res, g, case
+
+
+def accuracy(args, solvers):
+    """One step of each form from the FP64 form's final state, against that
+    step converged to rtol 1e-13 with FP64 residuals."""
     from frehg2_amd import abi
-    head = g.get(abi.F_HEAD)[1:-1, 1:-1]
-    wc = g.get(abi.F_WC)[1:-1, 1:-1]
-    log(f"{precision}: {res['ms_per_step']:.2f} ms/step, {res['iterations_per_solve']:.1f} it/solve, "
-        f"{res['cell_iters_per_s'] / 1e9:.2f} G cell-it/s, spmv {res['spmv_avg_ms']:.3f} ms "
-        f"({res['spmv_gbs']:.0f} GB/s)")
-    g.close()
-    return res, head, wc
+    from frehg2_amd.gw import RichardsSolver
+    g64, case = solvers["fp64"]
+    state = (g64.get(abi.F_HEAD), g64.get(abi.F_WC))
+    tcase = make_case(args, "mixed", 1e-13)
+    tcase.inner_rtol = 1e-3
+    truth = RichardsSolver(tcase)
+    out = {}
+    heads = {}
+    for name, g in [("truth", truth)] + [(f, solvers[f][0]) for f in solvers]:
+        g.set(abi.F_HEAD, state[0])
+        g.set(abi.F_WC, state[1])
+        g.step(case.dt)
+        heads[name] = (g.get(abi.F_HEAD)[1:-1, 1:-1], g.get(abi.F_WC)[1:-1, 1:-1],
+                       g.last_solve())
+    truth.close()
+    hT, wT, sT = heads["truth"]
+    out["reference"] = ("one step from the FP64 form's final state, converged to rtol 1e-13 "
+                        f"(residual {sT['residual_norm']:.3e}, ||b|| {sT['rhs_norm']:.3e})")
+    for f in solvers:
+        h, w, st = heads[f]
+        out[f"{f}_head_rel_err"] = float(np.max(np.abs(h - hT)) / np.max(np.abs(hT)))
+        out[f"{f}_theta_rel_err"] = float(np.max(np.abs(w - wT)) / np.max(np.abs(wT)))
+        out[f"{f}_residual"] = st["residual_norm"]
+    out["target"] = ERR_TARGET
+    out["met"] = all(out[f"{f}_head_rel_err"] <= ERR_TARGET for f in solvers)
+    for f in solvers:
+        log(f"accuracy {f}: head {out[f'{f}_head_rel_err']:.3e}, "
+            f"theta {out[f'{f}_theta_rel_err']:.3e}")
+    return out
 
 
 def cpu_baseline():
-    """FP64 oracle on a 128 x 128 x 100 slice of the same workload, 2 steps."""
+    """FP64 oracle on a 256 x 256 x 100 slice of the same workload, 3 steps."""
     from frehg2_amd import cases
     from oracle import oracle
     threads = min(os.cpu_count() or 1, 16)
     oracle.set_threads(threads)
-    c = cases.hetero_512x512x100(nx=128, ny=128, nz=100)
+    c = cases.hetero_512x512x100(nx=256, ny=256, nz=100)
+    c.rtol = 1e-12
     o = oracle.OracleRichards(c)
-    steps = 2
+    steps = 3
     for _ in range(steps):
         o.step(c.dt)
     t = o.timing()
     n = o.active_cells()
+    cpu = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {
         "value": n * t["iterations"] / t["solve_s"],
         "unit": "cell-iterations/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"hetero workload generator at 128x128x100 ({n} cells), {steps} steps of "
-                  f"300 s, z-line PCG rtol 1e-10; solve {t['solve_s']:.2f} s, "
-                  f"{t['iterations']} iterations; whole steps {n * steps / t['step_s']:.3e} "
-                  f"cell-steps/s",
+        "sample": f"FP64 oracle (C++/OpenMP, z-line PCG, rtol 1e-12) on the same generator at "
+                  f"256x256x100 ({n} cells), {steps} steps of 300 s: solve {t['solve_s']:.2f} s, "
+                  f"{t['iterations']} iterations; whole steps {t['step_s']:.2f} s = "
+                  f"{n * steps / t['step_s']:.3e} cell-steps/s; {threads} threads on {cpu}",
     }
 
 
@@ -132,7 +195,7 @@ def load_traffic():
     if p.exists():
         try:
             return json.loads(p.read_text()).get("bytes_per_launch")
-        except Exception:
+        except (OSError, ValueError):
             return None
     return None
 
@@ -144,15 +207,13 @@ def main():
               f"sharding of this path is described in DESIGN.md and not measured here.",
               file=sys.stderr)
         sys.exit(2)
-    from frehg2_amd import cases
-    t0 = time.perf_counter()
-    case = cases.hetero_512x512x100(nx=args.nx, ny=args.ny, nz=args.nz)
-    log(f"synthetic inputs generated in {time.perf_counter() - t0:.1f} s")
-    forms = [f for f in args.forms.split(",") if f and not (args.no_mixed and f == "mixed")]
+    forms = [f for f in args.forms.split(",") if f]
     results = {}
-    heads = {}
+    solvers = {}
     for f in forms:
-        results[f], heads[f], _ = run_form(case, args.steps, args.warmup, f)
+        rtol = args.rtol_fp64 if f == "fp64" else args.rtol_mixed
+        results[f], g, case = run_form(args, f, rtol)
+        solvers[f] = (g, case)
     r64 = results.get("fp64") or results[forms[0]]
     out = {
         "metric": "Richards cell-iters/s @26M cells: FP64 vs mixed-prec @1e-8 rel err; % HBM peak",
@@ -166,26 +227,29 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if r64["precision"] == "fp64" else "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": case.name, "grid": [args.nx, args.ny, args.nz],
-                   "cells": r64["active_cells"], "dt_s": case.dt,
-                   "preconditioner": case.preconditioner, "rtol": case.rtol,
+        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": "richards_hetero_512x512x100", "grid": [args.nx, args.ny, args.nz],
+                   "cells": r64["active_cells"], "dt_s": 300.0, "preconditioner": "zline",
+                   "rtol_fp64": args.rtol_fp64, "rtol_mixed": args.rtol_mixed,
                    "parallelism": "single GPU"},
     }
     bw = r64["spmv_gbs"]
     out["roofline"] = {"bound": "hbm", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": bw / HBM_PEAK_GBS, "traffic": load_traffic(),
-                       "kernel": "k_pcg_dir (FP64)", "avg_launch_ms": r64["spmv_avg_ms"],
-                       "algorithmic_bytes_per_launch": r64["spmv_bytes"]}
+                       "kernel": "k_pcg_dir<double> (FP64 p = z + beta p, q = A p, p.q)",
+                       "avg_launch_ms": r64["spmv_avg_ms"],
+                       "algorithmic_bytes_per_launch": r64["spmv_bytes"],
+                       "frac_of_measured_stream": bw / STREAM_GBS["stream_6r2w"]}
     out["fp64"] = r64
     if "mixed" in results:
         rm = results["mixed"]
-        hm, h64 = heads["mixed"], heads.get("fp64")
-        rm["head_rel_err_vs_fp64"] = (float(np.max(np.abs(hm - h64)) / np.max(np.abs(h64)))
-                                      if h64 is not None else None)
         rm["solve_speedup_vs_fp64"] = r64["solve_ms_per_step"] / rm["solve_ms_per_step"]
         rm["step_speedup_vs_fp64"] = r64["ms_per_step"] / rm["ms_per_step"]
         out["mixed"] = rm
+    if not args.no_accuracy and "fp64" in solvers:
+        out["accuracy"] = accuracy(args, solvers)
+    for g, _ in solvers.values():
+        g.close()
     if not args.no_cpu:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline()

@@ -31,9 +31,18 @@ struct Grid {
 };
 
 // ---------------------------------------------------------------------------
-// van Genuchten-Mualem closures (src/gw/VanGenuchten.hpp:45-128).  The
-// expression order of each closure is the reference's; the device pow/log/exp
-// are ROCm's own implementations.
+// van Genuchten-Mualem closures (src/gw/VanGenuchten.hpp:45-128).
+//
+// Branches, clamps and limiters are the reference's.  The powers are formed
+// from shared logarithms instead of one pow() each: with t = (a|h|)^n,
+//   Se            = (1 + t)^-m              = exp(-m log1p t)
+//   Se^(1/m)      = 1 / (1 + t)             (exact identity)
+//   1 - (1 - Se^(1/m))^m = 1 - (t/(1+t))^m  = -expm1(m log(t/(1+t)))
+// with log(t/(1+t)) = n log(a|h|) - log1p t for t < 1 and -log1p(1/t)
+// otherwise, so no term cancels.  Mualem's K then costs two log-class and
+// three exp-class evaluations instead of five pow(); the results agree with
+// the pow() forms to rounding (and are better conditioned for dry cells,
+// where 1 - (1 - u)^m cancels).
 // ---------------------------------------------------------------------------
 struct VgSoil {
   double a, n, ts, tr, aev;
@@ -45,7 +54,8 @@ __device__ __forceinline__ double vg_theta(const VgSoil& s, double h) {
   if (h > s.aev) {
     wc = s.ts;
   } else {
-    const double se = pow(1.0 + pow(fabs(s.a * h), s.n), -m);
+    const double t = exp(s.n * log(fabs(s.a * h)));
+    const double se = exp(-m * log1p(t));
     wc = s.tr + (s.ts - s.tr) * se;
   }
   if (wc > s.ts) wc = s.ts;
@@ -68,8 +78,11 @@ __device__ __forceinline__ double vg_capacity(const VgSoil& s, double h) {
   if (h > 0.0) return 0.0;
   const double m = 1.0 - 1.0 / s.n;
   const double ah = fabs(s.a * h);
-  const double top = s.a * s.n * m * (s.ts - s.tr) * pow(ah, s.n - 1.0);
-  const double bot = pow(1.0 + pow(ah, s.n), m + 1.0);
+  const double lah = log(ah);
+  const double p1 = exp((s.n - 1.0) * lah);  // (a|h|)^(n-1)
+  const double t = exp(s.n * lah);           // (a|h|)^n
+  const double top = s.a * s.n * m * (s.ts - s.tr) * p1;
+  const double bot = exp((m + 1.0) * log1p(t));
   return top / bot;
 }
 
@@ -81,9 +94,13 @@ struct MualemFactors {
 };
 __device__ __forceinline__ MualemFactors vg_mualem(const VgSoil& s, double h) {
   const double m = 1.0 - 1.0 / s.n;
-  const double se = pow(1.0 + pow(fabs(s.a * h), s.n), -m);
-  const double br = 1.0 - pow(1.0 - pow(se, 1.0 / m), m);
-  return MualemFactors{pow(se, 0.5), br, h > 0.0};
+  const double nl = s.n * log(fabs(s.a * h));  // log t
+  const double t = exp(nl);
+  const double l1 = log1p(t);
+  const double se = exp(-m * l1);
+  const double lr = t < 1.0 ? nl - l1 : -log1p(exp(-nl));  // log(t / (1 + t))
+  const double br = -expm1(m * lr);
+  return MualemFactors{sqrt(se), br, h > 0.0};
 }
 __device__ __forceinline__ double vg_k_from(const MualemFactors& f, double ks) {
   double k = ks * f.sq * f.br * f.br;

@@ -104,7 +104,11 @@ __device__ __forceinline__ void load_col(const T* __restrict__ a, int64_t base, 
 
 // Partial sums live in parts[slot * pstride + block]; pstride is the largest
 // grid of any kernel, so kernels with different grids never overlap.
-// Redundant in-block reduction of the previous kernel's partial sums.
+// Redundant in-block reduction of the previous kernel's partial sums.  Every
+// block repeats it, which measured faster in the solver than both
+// alternatives tried: a one-block reduction kernel between the two (one more
+// launch and kernel boundary per reduction) and a last-block-done reduction
+// (its device-scope release per block writes back the XCD's L2).
 template <int NV>
 __device__ __forceinline__ void reduce_parts(const double* parts, int pstride, int nparts,
                                              const int (&slot)[NV], double (&out)[NV]) {
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_residual64(Grid g, March mj, Stencil
 // Search direction and operator product: p = z + beta p_old, q = A p, p.q.
 // j-march: p at j - 1, j, j + 1 is formed once per column and kept in
 // registers; the i +- 1 values are formed from z and p_old at the neighbour.
-template <class T, int R>
+template <class T, int R, bool RESTART>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, March mj, Stencil<T> A,
                                                     const T* __restrict__ z,
                                                     const T* __restrict__ pOld,
@@ -364,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, March mj, Stencil<T>
     }
     return;
   }
-  const double betaD = restart ? 0.0 : rzv / st->rz[(n + 1) & 1];
+  const double betaD = RESTART ? 0.0 : rzv / st->rz[(n + 1) & 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->rz[n & 1] = rzv;
     st->it = n;
@@ -381,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, March mj, Stencil<T>
       for (int m = 0; m < R; ++m) {
         const int k = lane + kWave * m;
         pv[m] = T(0);
-        if (k < g.nz) pv[m] = restart ? z[base + k] : z[base + k] + beta * pOld[base + k];
+        if (k < g.nz) pv[m] = RESTART ? z[base + k] : z[base + k] + beta * pOld[base + k];
       }
     };
     T pPrev[R], pCur[R], pNext[R], ayPrev[R];
@@ -401,8 +405,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, March mj, Stencil<T>
         const int k = lane + kWave * m;
         if (k < g.nz) {
           const int64_t c = base + k;
-          const T pxp = restart ? z[c + sx] : z[c + sx] + beta * pOld[c + sx];
-          const T pxm = restart ? z[c - sx] : z[c - sx] + beta * pOld[c - sx];
+          const T pxp = RESTART ? z[c + sx] : z[c + sx] + beta * pOld[c + sx];
+          const T pxm = RESTART ? z[c - sx] : z[c - sx] + beta * pOld[c - sx];
           const T qv = A.Ad[c] * pCur[m] + A.Axp[c] * pxp + A.Axp[c - sx] * pxm + ay[m] * pNext[m] +
                        ayPrev[m] * pPrev[m] + az[m] * pdn[m] + azUp[m] * pup[m];
           pNew[c] = pCur[m];
@@ -472,19 +476,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Grid g, T* __restrict__ x
 }
 
 // Mixed form, outer sweep: xo = xi + sc d (all 7 stencil points), r = b - A xo
-// in FP64; partials rr (slot 1), bb (slot 3).  With hasD = 0, xo is xi.  j-march.
-template <int R>
+// in FP64; partials rr (slot 1), bb (slot 3).  Without HASD, xo is xi.  j-march.
+template <int R, bool HASD>
 __global__ __launch_bounds__(kBlock) void k_refine64(Grid g, March mj, Stencil<double> A,
                                                      const double* __restrict__ b,
                                                      const double* __restrict__ xi,
                                                      double* __restrict__ xo,
                                                      const float* __restrict__ d, double sc,
-                                                     int hasD, double* __restrict__ r,
+                                                     double* __restrict__ r,
                                                      double* __restrict__ parts, int pstride) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
   KSum rr, bb;
-  auto xn = [&](int64_t c) { return hasD ? xi[c] + sc * static_cast<double>(d[c]) : xi[c]; };
+  auto xn = [&](int64_t c) { return HASD ? xi[c] + sc * static_cast<double>(d[c]) : xi[c]; };
   auto formx = [&](int64_t base, double (&xv)[R]) {
 #pragma unroll
     for (int m = 0; m < R; ++m) {
@@ -516,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_refine64(Grid g, March mj, Stencil<d
                             azUp[m] * xup[m];
           const double bv = b[c];
           const double rv = bv - qv;
-          if (hasD) xo[c] = xCur[m];
+          if (HASD) xo[c] = xCur[m];
           r[c] = rv;
           rr.add(rv * rv);
           bb.add(bv * bv);
@@ -730,7 +734,12 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
       if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c], stream_));
       dispatch_R(g_.nz, [&](auto RC) {
         constexpr int R = decltype(RC)::value;
-        k_pcg_dir<T, R><<<mj.blocks, kBlock, 0, stream_>>>(
+        if (n == n0 && restart)
+          k_pcg_dir<T, R, true><<<mj.blocks, kBlock, 0, stream_>>>(
+            g_, mj, A, z, pOld, pNew, q, parts_, pstride_, n == n0 ? nprev0 : blocks_, dst, n,
+            (n == n0 && restart) ? 1 : 0, tol2);
+        else
+          k_pcg_dir<T, R, false><<<mj.blocks, kBlock, 0, stream_>>>(
             g_, mj, A, z, pOld, pNew, q, parts_, pstride_, n == n0 ? nprev0 : blocks_, dst, n,
             (n == n0 && restart) ? 1 : 0, tol2);
       });
@@ -870,8 +879,12 @@ int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const 
   auto refine = [&](const double* xi, double* xo, double sc, int hasD) {
     dispatch_R(g_.nz, [&](auto RC) {
       constexpr int R = decltype(RC)::value;
-      k_refine64<R><<<march64_.blocks, kBlock, 0, stream_>>>(g_, march64_, A, b, xi, xo, d32_, sc,
-                                                             hasD, r_, parts_, pstride_);
+      if (hasD)
+        k_refine64<R, true><<<march64_.blocks, kBlock, 0, stream_>>>(
+            g_, march64_, A, b, xi, xo, d32_, sc, r_, parts_, pstride_);
+      else
+        k_refine64<R, false><<<march64_.blocks, kBlock, 0, stream_>>>(
+            g_, march64_, A, b, xi, xo, d32_, sc, r_, parts_, pstride_);
     });
     HIP_OK(hipGetLastError());
     double red[4];

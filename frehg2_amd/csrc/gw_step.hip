@@ -15,9 +15,9 @@
 //   k_coupled_top       Corrector.cpp:539-627
 //   k_update_wc         Corrector.cpp:629-672
 //   k_realloc_classify  Reallocate.cpp:181-391 (saturated_contact: head)
-//   k_realloc_walk      Reallocate.cpp:410-540
-//   k_finalize          Corrector.cpp:758-792
-//   k_step_end          RichardsSolver.cpp:961-1006, AdaptiveStep.cpp:44-98
+//   k_realloc_walk      Reallocate.cpp:410-540, Corrector.cpp:758-792,
+//                       RichardsSolver.cpp:961-1006, AdaptiveStep.cpp:59-98
+//                       (one wave per column)
 // Density / viscosity ratios are 1 (no transport module): their factors are
 // omitted, which is exact.
 #include <algorithm>
@@ -70,7 +70,11 @@ struct Dev {
   double *ksx, *ksy, *ksz, *vga, *vgn, *wcs, *wcr, *aev;
   double *Kx, *Ky, *Kz;
   double *kx, *ky, *kzF, *qx, *qy, *qzF;
-  double *vloss, *room, *sendUp, *sendDown, *wcSnap, *hSnap;
+  double *vloss;
+  double *hOut, *wcOut;  // second buffer pair written by reallocation
+  double *sendUp, *sendDown;
+  int* colFlag;
+  unsigned char* sat;     // wc >= theta_s || h > aev entering reallocation
   double *dz3d, *bot3d, *ax, *ay, *sinx, *cosx, *siny, *cosy;
   double* bath;
   int* kt;
@@ -82,7 +86,6 @@ struct Dev {
   double ss, courantMax;
   int full3d, harmonic, reallocOff, redistribute, terrain, guessPrev;
   double *Ad, *Axp, *Ayp, *Azp, *rhs, *x;
-  int* colFlag;
   double* parts;
   int rb;
 };
@@ -196,14 +199,17 @@ __global__ void k_classify_coupled(Dev d, double dtg) {
 }
 
 // ---- per-cell conductivities, halo cells included -------------------------
+template <bool ISO>
 __global__ void k_cell_k(Dev d) {
   const int64_t n = d.g.cellsH();
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const MualemFactors f = vg_mualem(soil_at(d, c), d.h[c]);
     d.Kx[c] = vg_k_from(f, d.ksx[c]);
-    d.Ky[c] = vg_k_from(f, d.ksy[c]);
-    d.Kz[c] = vg_k_from(f, d.ksz[c]);
+    if (!ISO) {
+      d.Ky[c] = vg_k_from(f, d.ksy[c]);
+      d.Kz[c] = vg_k_from(f, d.ksz[c]);
+    }
   }
 }
 
@@ -622,6 +628,7 @@ __global__ void k_update_wc(Dev d, double dtg) {
     const double dqz = dtg * (d.qzF[g.cF(j, i, k + 1)] - d.qzF[g.cF(j, i, k)]);
     const double upd = (d.wcn[c] * volume + dqx + dqy + dqz) / coeff;
     d.wc[c] = upd;
+    d.sat[c] = (upd >= d.wcs[c] || d.h[c] > d.aev[c]) ? 1 : 0;
     ss.add(upd * volume * d.ss * dh / d.wcs[c]);
   }
   double v[1] = {ss.value()};
@@ -635,7 +642,7 @@ struct Split {
 };
 __device__ Split gradient_split(const Dev& d, int j, int i, int k) {
   const Grid& g = d.g;
-  const double* hR = d.hSnap;
+  const double* hR = d.h;  // the state entering reallocation
   double d6[6];
   const int64_t c = g.c3(j, i, k);
   auto a = [&](int jj, int ii, int kk) { return kk >= 0 && kk < g.nz && act(d, jj, ii, kk); };
@@ -682,236 +689,445 @@ __device__ Split gradient_split(const Dev& d, int j, int i, int k) {
   return sp;
 }
 
-__global__ void k_realloc_classify(Dev d) {
-  const Grid& g = d.g;
-  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
-  KSum adj, disc;
-  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
-    const int k = static_cast<int>(r.t % g.nz);
-    const int col = static_cast<int>(r.t / g.nz);
-    const int j = col / g.nx + 1, i = col % g.nx + 1;
-    const int64_t c = g.c3(j, i, k);
-    if (!act(d, j, i, k)) {
-      d.room[c] = 0.0;
-      d.sendUp[c] = 0.0;
-      d.sendDown[c] = 0.0;
-      continue;
-    }
-    const VgSoil s = soil_at(d, c);
-    const double volume = g.az * d.dz3d[c];
-    const double before = d.wc[c];
-    d.sendUp[c] = 0.0;
-    d.sendDown[c] = 0.0;
-    if (before >= s.ts) {
-      if (d.reallocOff) {
-        d.room[c] = 0.0;
-        continue;
-      }
-      const double excess = (before - s.ts) * volume;
-      const Split sp = gradient_split(d, j, i, k);
-      const double su = excess * sp.up, sd = excess * sp.down;
-      d.sendUp[c] = su;
-      d.sendDown[c] = sd;
-      adj.add(-(excess - su - sd));
-      disc.add(excess - su - sd);
-      if (su > 0.0 || sd > 0.0) d.colFlag[col] = 1;
-      d.wc[c] = s.ts;
-      d.room[c] = 0.0;
-      continue;
-    }
-    d.room[c] = (s.ts - before) * volume;
-    auto sat = [&](int jj, int ii, int kk) {
-      const int64_t cc = g.c3(jj, ii, kk);
-      return d.wcSnap[cc] >= d.wcs[cc] || d.hSnap[cc] > d.aev[cc];
-    };
-    bool adjSat = false;
-    if (k + 1 < g.nz && act(d, j, i, k + 1) && sat(j, i, k + 1)) adjSat = true;
-    else if (d.kx[c] != 0.0 && sat(j, i + 1, k)) adjSat = true;
-    else if (d.kx[g.c3(j, i - 1, k)] != 0.0 && sat(j, i - 1, k)) adjSat = true;
-    else if (d.ky[c] != 0.0 && sat(j + 1, i, k)) adjSat = true;
-    else if (d.ky[g.c3(j - 1, i, k)] != 0.0 && sat(j - 1, i, k)) adjSat = true;
-    else {
-      const bool isTop = (k == 0) || !act(d, j, i, k - 1);
-      if (isTop) {
-        if (d.coupled) {
-          if (d.dept[g.c2(j, i)] > 0.0) adjSat = true;
-        } else if (d.topCode[g.c2(j, i)] == FREHG_BC_HEAD && d.topValue[g.c2(j, i)] >= 0.0) {
-          adjSat = true;
-        }
-      } else if (act(d, j, i, k - 1) && sat(j, i, k - 1)) {
-        adjSat = true;
-      }
-    }
-    if (!adjSat) {
-      if (before < 0.9999 * s.ts) d.h[c] = vg_head(s, before);
-    } else if (d.reallocOff) {
-      continue;
-    } else {
-      const double after = vg_theta(s, d.h[c]);
-      d.wc[c] = after;
-      d.room[c] = (s.ts - after) * volume;
-      if (d.redistribute && after < before) {
-        const double surplus = (before - after) * volume;
-        const Split sp = gradient_split(d, j, i, k);
-        const double su = surplus * sp.up, sd = surplus * sp.down;
-        d.sendUp[c] = su;
-        d.sendDown[c] = sd;
-        adj.add(-(surplus - su - sd));
-        disc.add(surplus - su - sd);
-        if (su > 0.0 || sd > 0.0) d.colFlag[col] = 1;
-      } else {
-        adj.add((after - before) * volume);
-        if (after < before) disc.add((before - after) * volume);
-      }
-    }
+// ---- reallocation, final clamp and step-end inputs: one wave per column ----
+// Reallocate.cpp:122-545 (saturated_contact: head), Corrector.cpp:758-792,
+// RichardsSolver.cpp:961-1006, AdaptiveStep.cpp:59-98.
+//
+// Lane l holds rows k = l + 64 m of the wave's column.  Phase 1 (the
+// per-cell classification) reads the state as it stood when the phase began
+// (h, wc and the neighbours' saturation flags) and writes its result into the
+// other buffer pair (hOut, wcOut), so no snapshot copy is taken.  Phase 2
+// (the send walks) is sequential within a column: the wave runs it as
+// wave-uniform control flow over the senders in k order, keeping the
+// column's room, theta and face fluxes in registers; the cells that still
+// have room are tracked as ballot masks, so a walk jumps straight to the next
+// receiving cell (the reference's loop visits the full cells in between and
+// leaves them untouched).  Every arithmetic operation and its order are the
+// reference's.
+
+// Value of lane l (wave-uniform) of a double.
+__device__ __forceinline__ double lane_read(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+// Row k (wave-uniform) of a column held as v[m] at row lane + 64 m.
+template <int N>
+__device__ __forceinline__ double col_get(const double (&v)[N], int k) {
+  const int m = k >> 6;
+  double s = v[0];
+#pragma unroll
+  for (int mm = 1; mm < N; ++mm)
+    if (m == mm) s = v[mm];
+  return lane_read(s, k & 63);
+}
+template <int N>
+__device__ __forceinline__ void col_set(double (&v)[N], int k, double x, int lane) {
+  const int m = k >> 6;
+#pragma unroll
+  for (int mm = 0; mm < N; ++mm)
+    if (m == mm && lane == (k & 63)) v[mm] = x;
+}
+// Highest set row below k, or -1.
+template <int R>
+__device__ __forceinline__ int mask_prev(const unsigned long long (&mk)[R], int k) {
+  if (k <= 0) return -1;
+  const int m = (k - 1) >> 6, l = (k - 1) & 63;
+  int res = -1;
+#pragma unroll
+  for (int mm = 0; mm < R; ++mm) {
+    unsigned long long bits = mk[mm];
+    if (mm > m) bits = 0ull;
+    else if (mm == m && l < 63) bits &= (1ull << (l + 1)) - 1ull;
+    if (bits) res = mm * 64 + 63 - __clzll(static_cast<long long>(bits));
   }
-  double v[2] = {adj.value(), disc.value()};
-  const int s[2] = {S_ADJ, S_DISC};
-  put_parts<2>(d, s, v);
+  return res;
+}
+// Lowest set row above k, or -1.
+template <int R>
+__device__ __forceinline__ int mask_next(const unsigned long long (&mk)[R], int k) {
+  const int m = (k + 1) >> 6, l = (k + 1) & 63;
+  int res = -1;
+#pragma unroll
+  for (int mm = R - 1; mm >= 0; --mm) {
+    unsigned long long bits = mk[mm];
+    if (mm < m) bits = 0ull;
+    else if (mm == m) bits &= ~((1ull << l) - 1ull);
+    if (bits) res = mm * 64 + __ffsll(static_cast<long long>(bits)) - 1;
+  }
+  return res;
+}
+template <int R>
+__device__ __forceinline__ void mask_clear(unsigned long long (&mk)[R], int k) {
+#pragma unroll
+  for (int mm = 0; mm < R; ++mm)
+    if ((k >> 6) == mm) mk[mm] &= ~(1ull << (k & 63));
 }
 
-// ---- post-allocation, phase 2: column send walks (Reallocate.cpp:410-540) --
-__global__ void k_realloc_walk(Dev d, double dtg) {
+// Saturation flag of the state entering reallocation (Reallocate.cpp:303-337
+// "sat": wc >= theta_s or h > aev), for the halo columns; interior cells get
+// theirs from k_update_wc.
+__global__ void k_sat_halo(Dev d) {
   const Grid& g = d.g;
-  KSum lost, vent, thin;
-  for (Range r = xcd_range(g.columns()); r.t < r.end; r.t += r.stride) {
-    const int col = static_cast<int>(r.t);
-    if (!d.colFlag[col]) continue;
-    d.colFlag[col] = 0;
+  const int nh = 2 * (g.nx2 + g.ny);
+  const int64_t n = static_cast<int64_t>(nh) * g.nz;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int k = static_cast<int>(t % g.nz);
+    const int q = static_cast<int>(t / g.nz);
+    int j, i;
+    if (q < g.nx2) { j = 0; i = q; }
+    else if (q < 2 * g.nx2) { j = g.ny + 1; i = q - g.nx2; }
+    else if (q < 2 * g.nx2 + g.ny) { j = q - 2 * g.nx2 + 1; i = 0; }
+    else { j = q - 2 * g.nx2 - g.ny + 1; i = g.nx + 1; }
+    const int64_t c = g.c3(j, i, k);
+    d.sat[c] = (d.wc[c] >= d.wcs[c] || d.h[c] > d.aev[c]) ? 1 : 0;
+  }
+}
+
+// Phase 1, per cell: the classification (Reallocate.cpp:181-391).  Writes
+// the cell's outcome into the second buffer pair (wcOut, hOut), its send
+// volumes, and flags the columns that have senders.
+__global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
+  const Grid& g = d.g;
+  const int nz = g.nz;
+  const int64_t n = static_cast<int64_t>(g.columns()) * nz;
+  const int64_t sy = static_cast<int64_t>(g.nx2) * nz;
+  KSum adj, disc;
+  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % nz);
+    const int col = static_cast<int>(r.t / nz);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    const int c2 = g.c2(j, i);
+    const int kt = d.kt[c2];
+    const int64_t c = g.c3(j, i, k);
+    const double before = d.wc[c];
+    const double hc = d.h[c];
+    // ghost heads next to edge columns are carried into the new buffer (the
+    // ghost theta is rebuilt from the interior afterwards)
+    if (i == 1) d.hOut[c - nz] = d.h[c - nz];
+    if (i == g.nx) d.hOut[c + nz] = d.h[c + nz];
+    if (j == 1) d.hOut[c - sy] = d.h[c - sy];
+    if (j == g.ny) d.hOut[c + sy] = d.h[c + sy];
+    double wOut = before, hOutV = hc, su = 0.0, sd = 0.0;
+    if (k >= kt) {
+      const VgSoil s = soil_at(d, c);
+      const double volume = g.az * d.dz3d[c];
+      if (before >= s.ts) {
+        if (!d.reallocOff) {
+          const double excess = (before - s.ts) * volume;
+          const Split sp = gradient_split(d, j, i, k);
+          su = excess * sp.up;
+          sd = excess * sp.down;
+          adj.add(-(excess - su - sd));
+          disc.add(excess - su - sd);
+          wOut = s.ts;
+        }
+      } else {
+        bool adjSat = false;
+        if (k + 1 < nz && d.sat[c + 1]) adjSat = true;
+        else if (d.kx[c] != 0.0 && d.sat[c + nz]) adjSat = true;
+        else if (d.kx[c - nz] != 0.0 && d.sat[c - nz]) adjSat = true;
+        else if (d.ky[c] != 0.0 && d.sat[c + sy]) adjSat = true;
+        else if (d.ky[c - sy] != 0.0 && d.sat[c - sy]) adjSat = true;
+        else if (k == 0 || k - 1 < kt) {
+          if (d.coupled) {
+            if (d.dept[c2] > 0.0) adjSat = true;
+          } else if (d.topCode[c2] == FREHG_BC_HEAD && d.topValue[c2] >= 0.0) {
+            adjSat = true;
+          }
+        } else if (d.sat[c - 1]) {
+          adjSat = true;
+        }
+        if (!adjSat) {
+          if (before < 0.9999 * s.ts) hOutV = vg_head(s, before);
+        } else if (!d.reallocOff) {
+          const double after = vg_theta(s, hc);
+          wOut = after;
+          if (d.redistribute && after < before) {
+            const double surplus = (before - after) * volume;
+            const Split sp = gradient_split(d, j, i, k);
+            su = surplus * sp.up;
+            sd = surplus * sp.down;
+            adj.add(-(surplus - su - sd));
+            disc.add(surplus - su - sd);
+          } else {
+            adj.add((after - before) * volume);
+            if (after < before) disc.add((before - after) * volume);
+          }
+        }
+      }
+    }
+    d.wcOut[c] = wOut;
+    d.hOut[c] = hOutV;
+    d.sendUp[c] = su;
+    d.sendDown[c] = sd;
+    if (su > 0.0 || sd > 0.0) d.colFlag[col] = 1;
+  }
+  double v[2] = {adj.value(), disc.value()};
+  const int sl[2] = {S_ADJ, S_DISC};
+  put_parts<2>(d, sl, v);
+}
+
+// Phase 2 (the send walks, Reallocate.cpp:410-540) and the final clamp
+// (Corrector.cpp:758-792): one wave per column.  The room of every cell is
+// what phase 1 left, (theta_s - theta) * volume, and is rebuilt here from
+// wcOut with the same operands.
+template <int R>
+__global__ __launch_bounds__(kThreads) void k_realloc_walk(Dev d, double dtg) {
+  constexpr int RF = R + 1;  // face rows 0..nz
+  const Grid& g = d.g;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nz = g.nz;
+  KSum lost, vent, thin, vl;
+  const int ncol = g.columns();
+  const int xcd = blockIdx.x % kXcds, bpx = gridDim.x / kXcds, local = blockIdx.x / kXcds;
+  constexpr int wpb = kThreads / kWave;
+  const int cBeg = static_cast<int>(static_cast<int64_t>(xcd) * ncol / kXcds);
+  const int cEnd = static_cast<int>(static_cast<int64_t>(xcd + 1) * ncol / kXcds);
+  const bool lead = lane == 0;
+  for (int col = cBeg + local * wpb + static_cast<int>(threadIdx.x / kWave); col < cEnd;
+       col += bpx * wpb) {
     const int j = col / g.nx + 1, i = col % g.nx + 1;
     const int c2 = g.c2(j, i);
     const int kt = d.kt[c2];
     const int64_t b0 = g.c3(j, i, 0);
     const int64_t f0 = g.cF(j, i, 0);
-    const bool topIsHead = d.topCode[c2] == FREHG_BC_HEAD;
-    for (int k = kt; k < g.nz; ++k) {
-      double dV = d.sendUp[b0 + k];
-      if (dV > 0.0) {
-        int ll = k;
-        while (ll > 0 && ll - 1 >= kt && dV > 0.0) {
-          --ll;
-          const double room = d.room[b0 + ll];
-          if (room > 0.0) {
-            const double take = (room > dV) ? dV : room;
-            d.wc[b0 + ll] += take / (g.az * d.dz3d[b0 + ll]);
-            d.room[b0 + ll] = room - take;
-            d.qzF[f0 + ll + 1] += take / dtg;
-            dV -= take;
-          }
+    const bool walk = d.colFlag[col] != 0 && !d.reallocOff;
+    double wcv[R], dzv[R], qz[RF];
+#pragma unroll
+    for (int m = 0; m < RF; ++m) {
+      const int k = lane + kWave * m;
+      qz[m] = k <= nz ? d.qzF[f0 + k] : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      wcv[m] = k < nz ? d.wcOut[b0 + k] : 0.0;
+      dzv[m] = k < nz ? d.dz3d[b0 + k] : 1.0;
+    }
+    if (walk) {
+      if (lead) d.colFlag[col] = 0;
+      double room[R], sUp[R], sDn[R];
+      unsigned long long sendMask[R], roomMask[R];
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const int k = lane + kWave * m;
+        room[m] = 0.0;
+        sUp[m] = 0.0;
+        sDn[m] = 0.0;
+        if (k < nz && k >= kt) {
+          room[m] = (d.wcs[b0 + k] - wcv[m]) * (g.az * dzv[m]);
+          sUp[m] = d.sendUp[b0 + k];
+          sDn[m] = d.sendDown[b0 + k];
         }
-        if (dV > 0.0) {
-          if (d.coupled) {
-            const double dd = dV / g.az;
-            if (d.ventAll || d.dept[c2] > 0.0 || dd > d.minDepth) {
-              if (!(d.dept[c2] > 0.0 || dd > d.minDepth)) thin.add(dV);
-              d.eta[c2] += dd;
-              d.dept[c2] += dd;
-              d.avail[c2] += dd;
-              d.gain[c2] += dd;
-              d.qzF[f0 + ll] += dV / dtg;
-              vent.add(dV);
-            } else {
-              lost.add(-dV);
+        sendMask[m] = __ballot(sUp[m] > 0.0 || sDn[m] > 0.0);
+        roomMask[m] = __ballot(room[m] > 0.0);
+      }
+      const bool topIsHead = d.topCode[c2] == FREHG_BC_HEAD;
+      double dept = d.coupled ? d.dept[c2] : 0.0;
+      // The receiving cell and the face a walk is working on are cached in
+      // wave-uniform registers and written back to the lane that owns them
+      // only when the walk moves on: most senders of a column fill the same
+      // first cell with room above the saturated zone.
+      int curRow = -1, curFace = -1;
+      double cRoom = 0.0, cW = 0.0, cDz = 1.0, cQ = 0.0;
+      auto flush_cell = [&]() {
+        if (curRow >= 0) {
+          col_set<R>(room, curRow, cRoom, lane);
+          col_set<R>(wcv, curRow, cW, lane);
+        }
+      };
+      auto use_cell = [&](int rr) {
+        if (rr != curRow) {
+          flush_cell();
+          curRow = rr;
+          cRoom = col_get<R>(room, rr);
+          cW = col_get<R>(wcv, rr);
+          cDz = col_get<R>(dzv, rr);
+        }
+      };
+      auto flush_face = [&]() {
+        if (curFace >= 0) col_set<RF>(qz, curFace, cQ, lane);
+      };
+      auto use_face = [&](int f) {
+        if (f != curFace) {
+          flush_face();
+          curFace = f;
+          cQ = col_get<RF>(qz, f);
+        }
+      };
+      // One receiving cell of a walk: room, theta and the mirror face flux.
+      auto take_at = [&](int ll, double& dV, int faceRow, bool faceAdd) {
+        use_cell(ll);
+        const double take = (cRoom > dV) ? dV : cRoom;
+        cW = cW + take / (g.az * cDz);
+        const double left = cRoom - take;
+        cRoom = left;
+        if (!(left > 0.0)) mask_clear<R>(roomMask, ll);
+        use_face(faceRow);
+        cQ = faceAdd ? cQ + take / dtg : cQ - take / dtg;
+        dV -= take;
+      };
+#pragma unroll 1
+      for (int ms = 0; ms < R; ++ms) {
+        unsigned long long pend = 0ull;
+#pragma unroll
+        for (int mm = 0; mm < R; ++mm)
+          if (mm == ms) pend = sendMask[mm];
+#pragma unroll 1
+        while (pend) {
+          const int k = ms * 64 + __ffsll(static_cast<long long>(pend)) - 1;
+          pend &= pend - 1ull;
+          double dV = col_get<R>(sUp, k);
+          if (dV > 0.0) {
+            int ll = k;
+            while (dV > 0.0) {
+              const int nxt = mask_prev<R>(roomMask, ll);
+              if (nxt < kt) {
+                ll = kt;
+                break;
+              }
+              ll = nxt;
+              take_at(ll, dV, ll + 1, true);
             }
-            dV = 0.0;
-          } else if (topIsHead) {
-            d.qzF[f0 + ll] += dV / dtg;
-            dV = 0.0;
-          } else {
-            while (ll + 1 < g.nz && dV > 0.0) {
-              ++ll;
-              const double room = d.room[b0 + ll];
-              if (room > 0.0) {
-                const double take = (room > dV) ? dV : room;
-                d.wc[b0 + ll] += take / (g.az * d.dz3d[b0 + ll]);
-                d.room[b0 + ll] = room - take;
-                d.qzF[f0 + ll] -= take / dtg;
-                dV -= take;
+            if (dV > 0.0) {
+              if (d.coupled) {
+                const double dd = dV / g.az;
+                if (d.ventAll || dept > 0.0 || dd > d.minDepth) {
+                  if (!(dept > 0.0 || dd > d.minDepth) && lead) thin.add(dV);
+                  if (lead) {
+                    d.eta[c2] += dd;
+                    d.dept[c2] += dd;
+                    d.avail[c2] += dd;
+                    d.gain[c2] += dd;
+                  }
+                  dept += dd;
+                  use_face(ll);
+                  cQ = cQ + dV / dtg;
+                  if (lead) vent.add(dV);
+                } else if (lead) {
+                  lost.add(-dV);
+                }
+                dV = 0.0;
+              } else if (topIsHead) {
+                use_face(ll);
+                cQ = cQ + dV / dtg;
+                dV = 0.0;
+              } else {
+                while (dV > 0.0) {
+                  const int nxt = mask_next<R>(roomMask, ll);
+                  if (nxt < 0) break;
+                  ll = nxt;
+                  take_at(ll, dV, ll, false);
+                }
+                if (lead) lost.add(-dV);
+                dV = 0.0;
               }
             }
-            lost.add(-dV);
-            dV = 0.0;
+          }
+          dV = col_get<R>(sDn, k);
+          if (dV > 0.0) {
+            int ll = k;
+            while (dV > 0.0) {
+              const int nxt = mask_next<R>(roomMask, ll);
+              if (nxt < 0) break;
+              ll = nxt;
+              take_at(ll, dV, ll, false);
+            }
+            if (lead) lost.add(-dV);
           }
         }
       }
-      dV = d.sendDown[b0 + k];
-      if (dV > 0.0) {
-        int ll = k;
-        while (ll + 1 < g.nz && dV > 0.0) {
-          ++ll;
-          const double room = d.room[b0 + ll];
-          if (room > 0.0) {
-            const double take = (room > dV) ? dV : room;
-            d.wc[b0 + ll] += take / (g.az * d.dz3d[b0 + ll]);
-            d.room[b0 + ll] = room - take;
-            d.qzF[f0 + ll] -= take / dtg;
-            dV -= take;
-          }
-        }
-        lost.add(-dV);
+      flush_cell();
+      flush_face();
+#pragma unroll
+      for (int m = 0; m < RF; ++m) {
+        const int k = lane + kWave * m;
+        if (k <= nz) d.qzF[f0 + k] = qz[m];
       }
     }
-  }
-  double v[3] = {lost.value(), vent.value(), thin.value()};
-  const int s[3] = {S_LOST, S_VENT, S_THIN};
-  put_parts<3>(d, s, v);
-}
-
-// ---- final clamp (Corrector.cpp:758-792) -----------------------------------
-__global__ void k_finalize(Dev d) {
-  const Grid& g = d.g;
-  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
-  KSum lost;
-  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
-    const int k = static_cast<int>(r.t % g.nz);
-    const int col = static_cast<int>(r.t / g.nz);
-    const int j = col / g.nx + 1, i = col % g.nx + 1;
-    if (!act(d, j, i, k)) continue;
-    const int64_t c = g.c3(j, i, k);
-    const double volume = g.az * d.dz3d[c];
-    const double w = d.wc[c];
-    if (w > d.wcs[c]) {
-      const double ex = (w - d.wcs[c]) * volume;
-      d.vloss[c] += ex;
-      lost.add(ex);
-      d.wc[c] = d.wcs[c];
-    } else if (w < d.wcr[c]) {
-      const double de = (d.wcr[c] - w) * volume;
-      d.vloss[c] -= de;
-      lost.add(-de);
-      d.wc[c] = d.wcr[c];
+    // ---- final clamp and the theta buffer out ----
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int k = lane + kWave * m;
+      if (k >= nz || k < kt) continue;
+      const int64_t c = b0 + k;
+      double w = wcv[m];
+      const double volume = g.az * dzv[m];
+      const double ts = d.wcs[c], tr = d.wcr[c];
+      if (w > ts) {
+        const double ex = (w - ts) * volume;
+        d.vloss[c] += ex;
+        vl.add(ex);
+        w = ts;
+      } else if (w < tr) {
+        const double de = (tr - w) * volume;
+        d.vloss[c] -= de;
+        vl.add(-de);
+        w = tr;
+      }
+      wcv[m] = w;
+      d.wcOut[c] = w;
     }
   }
-  double v[1] = {lost.value()};
-  const int s[1] = {S_VLOSS};
-  put_parts<1>(d, s, v);
+  double v[4] = {lost.value(), vent.value(), thin.value(), vl.value()};
+  const int sl[4] = {S_LOST, S_VENT, S_THIN, S_VLOSS};
+  put_parts<4>(d, sl, v);
 }
 
-// ---- boundary inflow audit + adaptive-step inputs --------------------------
-// (RichardsSolver.cpp:961-1006, AdaptiveStep.cpp:59-98)
-__global__ void k_step_end(Dev d, double dtg) {
+// Boundary inflow of the step (RichardsSolver.cpp:961-1006): the net volume
+// through the domain's faces, one thread per column (top and bottom faces;
+// edge columns add their lateral faces).
+__global__ __launch_bounds__(kThreads) void k_boundary_in(Dev d, double dtg) {
   const Grid& g = d.g;
-  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
-  const int64_t sy = static_cast<int64_t>(g.nx2) * g.nz;
+  const int nz = g.nz;
+  const int64_t sy = static_cast<int64_t>(g.nx2) * nz;
   KSum bin;
+  for (Range r = xcd_range(g.columns()); r.t < r.end; r.t += r.stride) {
+    const int col = static_cast<int>(r.t);
+    const int j = col / g.nx + 1, i = col % g.nx + 1;
+    const int kt = d.kt[g.c2(j, i)];
+    if (kt >= nz) continue;
+    const int64_t f0 = g.cF(j, i, 0);
+    bin.add(-d.qzF[f0 + kt] * dtg);
+    bin.add(d.qzF[f0 + nz] * dtg);
+    if (i == 1 || i == g.nx || j == 1 || j == g.ny) {
+      for (int k = kt; k < nz; ++k) {
+        const int64_t c = g.c3(j, i, k);
+        if (i == 1) bin.add(-d.qx[c - nz] * dtg);
+        if (i == g.nx) bin.add(d.qx[c] * dtg);
+        if (j == 1) bin.add(-d.qy[c - sy] * dtg);
+        if (j == g.ny) bin.add(d.qy[c] * dtg);
+      }
+    }
+  }
+  double v[1] = {bin.value()};
+  const int sl[1] = {S_BIN};
+  put_parts<1>(d, sl, v);
+}
+
+// Adaptive-step inputs (AdaptiveStep.cpp:59-98): the largest flux imbalance
+// and the Courant limit of the unsaturated cells, top cells excluded.  Only
+// launched when dt can change (dt_min < dt_max).
+__global__ __launch_bounds__(kThreads) void k_adapt_inputs(Dev d, double dtg) {
+  const Grid& g = d.g;
+  const int nz = g.nz;
+  const int64_t n = static_cast<int64_t>(g.columns()) * nz;
+  const int64_t sy = static_cast<int64_t>(g.nx2) * nz;
   double dqMax = 0.0, coMin = 1.0e8;
   for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
-    const int k = static_cast<int>(r.t % g.nz);
-    const int col = static_cast<int>(r.t / g.nz);
+    const int k = static_cast<int>(r.t % nz);
+    const int col = static_cast<int>(r.t / nz);
     const int j = col / g.nx + 1, i = col % g.nx + 1;
-    if (!act(d, j, i, k)) continue;
+    const int kt = d.kt[g.c2(j, i)];
+    if (k <= kt) continue;  // inactive, or the top cell
     const int64_t c = g.c3(j, i, k);
     const int64_t fk = g.cF(j, i, k);
-    const bool top = (k == 0) || !act(d, j, i, k - 1);
-    if (top) bin.add(-d.qzF[fk] * dtg);
-    if (k == g.nz - 1) bin.add(d.qzF[fk + 1] * dtg);
-    if (i == 1) bin.add(-d.qx[c - g.nz] * dtg);
-    if (i == g.nx) bin.add(d.qx[c] * dtg);
-    if (j == 1) bin.add(-d.qy[c - sy] * dtg);
-    if (j == g.ny) bin.add(d.qy[c] * dtg);
-    if (top) continue;
     const double qin = d.qx[c] / d.ax[c] + d.qy[c] / d.ay[c] + d.qzF[fk + 1] / g.az;
-    const double qou = d.qx[c - g.nz] / d.ax[c] + d.qy[c - sy] / d.ay[c] + d.qzF[fk] / g.az;
+    const double qou = d.qx[c - nz] / d.ax[c] + d.qy[c - sy] / d.ay[c] + d.qzF[fk] / g.az;
     const double dq = fabs(qin - qou) * dtg / d.dz3d[c];
     if (dq > dqMax) dqMax = dq;
     if (d.wc[c] < d.wcs[c]) {
@@ -920,9 +1136,9 @@ __global__ void k_step_end(Dev d, double dtg) {
       if (co < coMin) coMin = co;
     }
   }
-  double v[3] = {bin.value(), dqMax, coMin};
-  const int s[3] = {S_BIN, S_DQMAX, S_COMIN};
-  put_parts<3>(d, s, v);
+  double v[2] = {dqMax, coMin};
+  const int sl[2] = {S_DQMAX, S_COMIN};
+  put_parts<2>(d, sl, v);
 }
 
 __global__ void k_volume(Dev d) {
@@ -997,6 +1213,22 @@ __global__ void k_initial(Dev d, int form, const double* vals, double vconst, do
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
+// Row segments per column: R = ceil(nz / 64) lanes-rows per wave lane.
+template <class F>
+void dispatch_col(int nz, F&& f) {
+  switch ((nz + kWave - 1) / kWave) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    default: fail(FREHG_EUNSUPPORTED, "nz > 512 is not supported by the column kernels");
+  }
+}
+
 template <class T>
 T* dev_alloc(int64_t n) {
   void* p = nullptr;
@@ -1019,6 +1251,8 @@ struct GwSolver {
   double dtMin = 0, dtMax = 0, dqGrow = 0.01, dqShrink = 0.02;
   double dtgNext = 0, dtgCurrent = 0;
   int rb = 0, nb2d = 0;
+  int rbCol = 0;  // blocks of the column kernels (<= rb)
+  bool iso = false;  // ksx = ksy = ksz: one Ks and one per-cell K array
   int64_t nActive = 0;
   std::vector<int> codesHost[6];
   // results
@@ -1060,7 +1294,8 @@ struct GwSolver {
     HIP_CHECK(hipGetLastError());
   }
   void faces(int corr, double dtg) {
-    k_cell_k<<<rb, kThreads, 0, stream>>>(d);
+    if (iso) k_cell_k<true><<<rb, kThreads, 0, stream>>>(d);
+    else k_cell_k<false><<<rb, kThreads, 0, stream>>>(d);
     HIP_CHECK(hipGetLastError());
     k_face_lat<0><<<rb, kThreads, 0, stream>>>(d, corr);
     HIP_CHECK(hipGetLastError());
@@ -1247,6 +1482,9 @@ void GwSolver::create(const frehg_gw_desc& ds) {
                                                                  8, (ncell + kThreads - 1) / kThreads)));
   rb = (rb + 7) / 8 * 8;
   d.rb = rb;
+  rbCol = static_cast<int>(std::min<int64_t>(
+      rb, std::max<int64_t>(8, ((g.columns() + kThreads / kWave - 1) / (kThreads / kWave) + 7) /
+                                   8 * 8)));
 
   std::vector<double> bath, dz, bot;
   std::vector<int> ktop;
@@ -1330,7 +1568,16 @@ void GwSolver::create(const frehg_gw_desc& ds) {
     const double* src[8] = {ds.ksx, ds.ksy, ds.ksz, ds.vg_alpha, ds.vg_n, ds.theta_s, ds.theta_r,
                             ds.aev};
     double** dst[8] = {&d.ksx, &d.ksy, &d.ksz, &d.vga, &d.vgn, &d.wcs, &d.wcr, &d.aev};
+    // Isotropic soil (ksx = ksy = ksz everywhere) keeps one Ks array and one
+    // per-cell K: the three closures would be bit-identical.
+    const size_t nIn = static_cast<size_t>(nx) * ny * nz;
+    iso = (ds.ksx == ds.ksy || std::memcmp(ds.ksx, ds.ksy, nIn * sizeof(double)) == 0) &&
+          (ds.ksx == ds.ksz || std::memcmp(ds.ksx, ds.ksz, nIn * sizeof(double)) == 0);
     for (int a = 0; a < 8; ++a) {
+      if (iso && (a == 1 || a == 2)) {
+        *dst[a] = d.ksx;
+        continue;
+      }
       std::fill(tmp.begin(), tmp.end(), 0.0);
       for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx; ++i)
@@ -1355,8 +1602,8 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   d.wcn = field();
   d.ch = field();
   d.Kx = field();
-  d.Ky = field();
-  d.Kz = field();
+  d.Ky = iso ? d.Kx : field();
+  d.Kz = iso ? d.Kx : field();
   d.kx = field();
   d.ky = field();
   d.qx = field();
@@ -1364,18 +1611,18 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   d.kzF = alloc<double>(nF);
   d.qzF = alloc<double>(nF);
   d.vloss = field();
-  d.room = field();
+  d.hOut = field();
+  d.wcOut = field();
   d.sendUp = field();
   d.sendDown = field();
-  d.wcSnap = field();
-  d.hSnap = field();
+  d.colFlag = alloc<int>(g.columns());
+  d.sat = alloc<unsigned char>(nH);
   d.Ad = field();
   d.Axp = field();
   d.Ayp = field();
   d.Azp = field();
   d.rhs = field();
   d.x = field();
-  d.colFlag = alloc<int>(g.columns());
   d.parts = alloc<double>(static_cast<int64_t>(S_COUNT) * rb);
   slots = alloc<double>(S_COUNT);
   HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&slotsHost), S_COUNT * sizeof(double),
@@ -1510,20 +1757,32 @@ void GwSolver::step(double /*t*/, double dtg) {
   }
   k_update_wc<<<rb, kThreads, 0, stream>>>(d, dtg);
   HIP_CHECK(hipGetLastError());
-  // post-allocation and final clamp
-  copy(d.wcSnap, d.wc, nH);
-  copy(d.hSnap, d.h, nH);
-  k_realloc_classify<<<rb, kThreads, 0, stream>>>(d);
-  HIP_CHECK(hipGetLastError());
-  if (!d.reallocOff) {
-    k_realloc_walk<<<rb, kThreads, 0, stream>>>(d, dtg);
+  // post-allocation, final clamp, boundary inflow and dt inputs: one column
+  // kernel writing the second buffer pair, which then becomes the state
+  {
+    const int nh = 2 * (g.nx2 + g.ny);
+    const int64_t nhc = static_cast<int64_t>(nh) * g.nz;
+    k_sat_halo<<<static_cast<int>(std::min<int64_t>((nhc + kThreads - 1) / kThreads, 1024)),
+                 kThreads, 0, stream>>>(d);
     HIP_CHECK(hipGetLastError());
+    const int adapt = dtMin < dtMax ? 1 : 0;
+    k_realloc_classify<<<rb, kThreads, 0, stream>>>(d);
+    HIP_CHECK(hipGetLastError());
+    dispatch_col(g.nz, [&](auto RC) {
+      constexpr int R = decltype(RC)::value;
+      k_realloc_walk<R><<<rbCol, kThreads, 0, stream>>>(d, dtg);
+    });
+    HIP_CHECK(hipGetLastError());
+    std::swap(d.h, d.hOut);
+    std::swap(d.wc, d.wcOut);
+    ghosts(false, true);
+    k_boundary_in<<<std::min(rb, rbCol * 4), kThreads, 0, stream>>>(d, dtg);
+    HIP_CHECK(hipGetLastError());
+    if (adapt) {
+      k_adapt_inputs<<<rb, kThreads, 0, stream>>>(d, dtg);
+      HIP_CHECK(hipGetLastError());
+    }
   }
-  k_finalize<<<rb, kThreads, 0, stream>>>(d);
-  HIP_CHECK(hipGetLastError());
-  ghosts(false, true);
-  k_step_end<<<rb, kThreads, 0, stream>>>(d, dtg);
-  HIP_CHECK(hipGetLastError());
   if (timing) HIP_CHECK(hipEventRecord(ev1, stream));
   reduceSlots();
   const double* s = slotsHost;
