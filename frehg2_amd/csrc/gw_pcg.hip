@@ -734,14 +734,15 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
       if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c], stream_));
       dispatch_R(g_.nz, [&](auto RC) {
         constexpr int R = decltype(RC)::value;
-        if (n == n0 && restart)
+        const int np = n == n0 ? nprev0 : blocks_;
+        const int rst = (n == n0 && restart) ? 1 : 0;
+        if (rst) {
           k_pcg_dir<T, R, true><<<mj.blocks, kBlock, 0, stream_>>>(
-            g_, mj, A, z, pOld, pNew, q, parts_, pstride_, n == n0 ? nprev0 : blocks_, dst, n,
-            (n == n0 && restart) ? 1 : 0, tol2);
-        else
+              g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, rst, tol2);
+        } else {
           k_pcg_dir<T, R, false><<<mj.blocks, kBlock, 0, stream_>>>(
-            g_, mj, A, z, pOld, pNew, q, parts_, pstride_, n == n0 ? nprev0 : blocks_, dst, n,
-            (n == n0 && restart) ? 1 : 0, tol2);
+              g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, rst, tol2);
+        }
       });
       HIP_OK(hipGetLastError());
       if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c + 1], stream_));
