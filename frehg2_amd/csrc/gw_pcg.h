@@ -81,6 +81,7 @@ class Pcg {
   int blocks_ = 0;            // grid size of the column kernels (multiple of 8)
   March march64_, march32_;   // stencil-kernel shapes, FP64 and FP32 operands
   int pstride_ = 0;           // slot stride of parts_ (largest grid)
+  bool pairs_ = false;        // FP32 kernels in the row-pair layout (nz even)
   // FP64 vectors (halo layout)
   double *r_ = nullptr, *z_ = nullptr, *p0_ = nullptr, *p1_ = nullptr, *q_ = nullptr;
   double *invd_ = nullptr, *w_ = nullptr;

@@ -34,6 +34,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "../../include/frehg_gw.h"
 #include "gw_pcg.h"
@@ -575,6 +576,336 @@ __global__ __launch_bounds__(kBlock) void k_inner_init(Grid g, const double* __r
   write_parts<2>(parts, pstride, slots, v);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 operands in the row-pair layout: lane l of row segment m holds rows
+// k0 = 128 m + 2 l and k1 = k0 + 1 (R2 = ceil(nz / 128) segments), so every
+// load and store moves 8 bytes a lane, and the z-line recurrences first
+// compose each lane's two steps and then scan over half as many lanes-rows.
+// Used when nz is even (the pairs are then 8-byte aligned).
+// ---------------------------------------------------------------------------
+template <int R2>
+__device__ __forceinline__ void load_pair(const float* __restrict__ a, int64_t base, int lane,
+                                          int nz, float (&v0)[R2], float (&v1)[R2]) {
+#pragma unroll
+  for (int m = 0; m < R2; ++m) {
+    const int k0 = 128 * m + 2 * lane;
+    v0[m] = 0.0f;
+    v1[m] = 0.0f;
+    if (k0 < nz) {
+      const float2 t = *reinterpret_cast<const float2*>(a + base + k0);
+      v0[m] = t.x;
+      v1[m] = t.y;
+    }
+  }
+}
+
+template <int R2>
+__device__ __forceinline__ void store_pair(float* __restrict__ a, int64_t base, int lane, int nz,
+                                           const float (&v0)[R2], const float (&v1)[R2]) {
+#pragma unroll
+  for (int m = 0; m < R2; ++m) {
+    const int k0 = 128 * m + 2 * lane;
+    if (k0 < nz) *reinterpret_cast<float2*>(a + base + k0) = make_float2(v0[m], v1[m]);
+  }
+}
+
+// Row k1 + 1 (dn) and row k0 - 1 (up) of a column in the pair layout.
+template <int R2>
+__device__ __forceinline__ void pair_shift(const float (&v0)[R2], const float (&v1)[R2], int lane,
+                                           float (&dn)[R2], float (&up)[R2]) {
+#pragma unroll
+  for (int m = 0; m < R2; ++m) {
+    const float nx0 = __shfl_down(v0[m], 1, kWave);
+    const float seg = (m + 1 < R2) ? __shfl(v0[m + 1 < R2 ? m + 1 : m], 0, kWave) : 0.0f;
+    dn[m] = lane < kWave - 1 ? nx0 : seg;
+    const float pv1 = __shfl_up(v1[m], 1, kWave);
+    const float segp = (m > 0) ? __shfl(v1[m > 0 ? m - 1 : 0], kWave - 1, kWave) : 0.0f;
+    up[m] = lane > 0 ? pv1 : segp;
+  }
+}
+
+// z = M^-1 r in the pair layout (the recurrences of apply_precond).
+template <int R2, bool LINE>
+__device__ __forceinline__ void precond_pair(int lane, int nz, int64_t base,
+                                             const float* __restrict__ invd,
+                                             const float* __restrict__ w, const float (&r0)[R2],
+                                             const float (&r1)[R2], float (&z0)[R2],
+                                             float (&z1)[R2]) {
+  float iv0[R2], iv1[R2];
+  load_pair<R2>(invd, base, lane, nz, iv0, iv1);
+  if constexpr (!LINE) {
+#pragma unroll
+    for (int m = 0; m < R2; ++m) {
+      z0[m] = r0[m] * iv0[m];
+      z1[m] = r1[m] * iv1[m];
+    }
+    return;
+  } else {
+    float w0[R2], w1[R2], wDn[R2], wUp[R2];
+    load_pair<R2>(w, base, lane, nz, w0, w1);
+    pair_shift<R2>(w0, w1, lane, wDn, wUp);
+    float y0[R2], y1[R2];
+    float carry = 0.0f;
+#pragma unroll
+    for (int m = 0; m < R2; ++m) {
+      // y_k = r_k - w_{k-1} y_{k-1}: rows k0 then k1, composed per lane
+      const float a0 = r0[m], b0 = -wUp[m];
+      const float a1 = r1[m], b1 = -w0[m];
+      float A = a1 + b1 * a0, B = b1 * b0;
+#pragma unroll
+      for (int sft = 1; sft < kWave; sft <<= 1) {
+        const float Ap = __shfl_up(A, sft, kWave);
+        const float Bp = __shfl_up(B, sft, kWave);
+        if (lane >= sft) {
+          A = A + B * Ap;
+          B = B * Bp;
+        }
+      }
+      const float Y1 = A + B * carry;
+      float P = __shfl_up(Y1, 1, kWave);
+      if (lane == 0) P = carry;
+      y0[m] = a0 + b0 * P;
+      y1[m] = Y1;
+      carry = __shfl(Y1, kWave - 1, kWave);
+    }
+    carry = 0.0f;
+#pragma unroll
+    for (int m = R2 - 1; m >= 0; --m) {
+      // z_k = y_k invd_k - w_k z_{k+1}: rows k1 then k0, composed per lane
+      const float a0 = y0[m] * iv0[m], b0 = -w0[m];
+      const float a1 = y1[m] * iv1[m], b1 = -w1[m];
+      float A = a0 + b0 * a1, B = b0 * b1;
+#pragma unroll
+      for (int sft = 1; sft < kWave; sft <<= 1) {
+        const float An = __shfl_down(A, sft, kWave);
+        const float Bn = __shfl_down(B, sft, kWave);
+        if (lane + sft < kWave) {
+          A = A + B * An;
+          B = B * Bn;
+        }
+      }
+      const float Z0 = A + B * carry;
+      float Nx = __shfl_down(Z0, 1, kWave);
+      if (lane == kWave - 1) Nx = carry;
+      z0[m] = Z0;
+      z1[m] = a1 + b1 * Nx;
+      carry = __shfl(Z0, 0, kWave);
+    }
+  }
+}
+
+// k_pcg_dir for FP32 operands in the pair layout.
+template <int R2, bool RESTART>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir_p(Grid g, March mj, Stencil<float> A,
+                                                      const float* __restrict__ z,
+                                                      const float* __restrict__ pOld,
+                                                      float* __restrict__ pNew,
+                                                      float* __restrict__ q,
+                                                      double* __restrict__ parts, int pstride,
+                                                      int nprev, PcgState* __restrict__ st, int n,
+                                                      double tol2) {
+  double red[2];
+  const int rs[2] = {0, 1};
+  reduce_parts<2>(parts, pstride, nprev, rs, red);
+  const double rzv = red[0], rrv = red[1];
+  if (rrv <= tol2) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !st->done) {
+      st->done = 1;
+      st->it = n;
+      st->rr = rrv;
+    }
+    return;
+  }
+  const double betaD = RESTART ? 0.0 : rzv / st->rz[(n + 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rz[n & 1] = rzv;
+    st->it = n;
+    st->rr = rrv;
+  }
+  const float beta = static_cast<float>(betaD);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nz = g.nz;
+  const int64_t sx = nz, sy = static_cast<int64_t>(g.nx2) * nz;
+  KSum pq;
+  int i, j0, j1;
+  if (march_span(g, mj, i, j0, j1)) {
+    auto formp = [&](int64_t base, float (&p0)[R2], float (&p1)[R2]) {
+      float z0[R2], z1[R2];
+      load_pair<R2>(z, base, lane, nz, z0, z1);
+      if (RESTART) {
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          p0[m] = z0[m];
+          p1[m] = z1[m];
+        }
+      } else {
+        float o0[R2], o1[R2];
+        load_pair<R2>(pOld, base, lane, nz, o0, o1);
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          p0[m] = z0[m] + beta * o0[m];
+          p1[m] = z1[m] + beta * o1[m];
+        }
+      }
+    };
+    float pP0[R2], pP1[R2], pC0[R2], pC1[R2], ayP0[R2], ayP1[R2];
+    formp(g.c3(j0 - 1, i, 0), pP0, pP1);
+    formp(g.c3(j0, i, 0), pC0, pC1);
+    load_pair<R2>(A.Ayp, g.c3(j0 - 1, i, 0), lane, nz, ayP0, ayP1);
+    for (int j = j0; j <= j1; ++j) {
+      const int64_t base = g.c3(j, i, 0);
+      float pN0[R2], pN1[R2], px0[R2], px1[R2], pm0[R2], pm1[R2];
+      formp(base + sy, pN0, pN1);
+      formp(base + sx, px0, px1);
+      formp(base - sx, pm0, pm1);
+      float ad0[R2], ad1[R2], ax0[R2], ax1[R2], am0[R2], am1[R2], ay0[R2], ay1[R2], az0[R2],
+          az1[R2];
+      load_pair<R2>(A.Ad, base, lane, nz, ad0, ad1);
+      load_pair<R2>(A.Axp, base, lane, nz, ax0, ax1);
+      load_pair<R2>(A.Axp, base - sx, lane, nz, am0, am1);
+      load_pair<R2>(A.Ayp, base, lane, nz, ay0, ay1);
+      load_pair<R2>(A.Azp, base, lane, nz, az0, az1);
+      float pDn[R2], pUp[R2], azDn[R2], azUp[R2];
+      pair_shift<R2>(pC0, pC1, lane, pDn, pUp);
+      pair_shift<R2>(az0, az1, lane, azDn, azUp);
+      float q0[R2], q1[R2];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        q0[m] = ad0[m] * pC0[m] + ax0[m] * px0[m] + am0[m] * pm0[m] + ay0[m] * pN0[m] +
+                ayP0[m] * pP0[m] + az0[m] * pC1[m] + azUp[m] * pUp[m];
+        q1[m] = ad1[m] * pC1[m] + ax1[m] * px1[m] + am1[m] * pm1[m] + ay1[m] * pN1[m] +
+                ayP1[m] * pP1[m] + az1[m] * pDn[m] + az0[m] * pC0[m];
+        if (128 * m + 2 * lane < nz) {
+          pq.add(static_cast<double>(pC0[m]) * static_cast<double>(q0[m]));
+          pq.add(static_cast<double>(pC1[m]) * static_cast<double>(q1[m]));
+        }
+      }
+      store_pair<R2>(pNew, base, lane, nz, pC0, pC1);
+      store_pair<R2>(q, base, lane, nz, q0, q1);
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        pP0[m] = pC0[m];
+        pP1[m] = pC1[m];
+        pC0[m] = pN0[m];
+        pC1[m] = pN1[m];
+        ayP0[m] = ay0[m];
+        ayP1[m] = ay1[m];
+      }
+    }
+  }
+  double v[1] = {pq.value()};
+  const int slots[1] = {2};
+  write_parts<1>(parts, pstride, slots, v);
+}
+
+// k_pcg_update for FP32 operands in the pair layout.
+template <int R2, bool LINE>
+__global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, float* __restrict__ x,
+                                                         const float* __restrict__ p,
+                                                         float* __restrict__ r,
+                                                         const float* __restrict__ q,
+                                                         float* __restrict__ z,
+                                                         const float* __restrict__ invd,
+                                                         const float* __restrict__ w,
+                                                         double* __restrict__ parts, int pstride,
+                                                         int nprev, PcgState* __restrict__ st,
+                                                         int n) {
+  if (st->done) return;
+  double red[1];
+  const int rs[1] = {2};
+  reduce_parts<1>(parts, pstride, nprev, rs, red);
+  const double alphaD = st->rz[n & 1] / red[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alphaD;
+  const float alpha = static_cast<float>(alphaD);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nz = g.nz;
+  KSum rz, rr;
+  int64_t col, end, stride;
+  column_loop(g.columns(), col, end, stride);
+  for (; col < end; col += stride) {
+    const int64_t base = column_base(g, col);
+    float x0[R2], x1[R2], p0[R2], p1[R2], r0[R2], r1[R2], q0[R2], q1[R2];
+    load_pair<R2>(x, base, lane, nz, x0, x1);
+    load_pair<R2>(p, base, lane, nz, p0, p1);
+    load_pair<R2>(r, base, lane, nz, r0, r1);
+    load_pair<R2>(q, base, lane, nz, q0, q1);
+#pragma unroll
+    for (int m = 0; m < R2; ++m) {
+      x0[m] = x0[m] + alpha * p0[m];
+      x1[m] = x1[m] + alpha * p1[m];
+      r0[m] = r0[m] - alpha * q0[m];
+      r1[m] = r1[m] - alpha * q1[m];
+    }
+    store_pair<R2>(x, base, lane, nz, x0, x1);
+    float z0[R2], z1[R2];
+    precond_pair<R2, LINE>(lane, nz, base, invd, w, r0, r1, z0, z1);
+    store_pair<R2>(r, base, lane, nz, r0, r1);
+    store_pair<R2>(z, base, lane, nz, z0, z1);
+#pragma unroll
+    for (int m = 0; m < R2; ++m) {
+      if (128 * m + 2 * lane < nz) {
+        rz.add(static_cast<double>(r0[m]) * static_cast<double>(z0[m]));
+        rz.add(static_cast<double>(r1[m]) * static_cast<double>(z1[m]));
+        rr.add(static_cast<double>(r0[m]) * static_cast<double>(r0[m]));
+        rr.add(static_cast<double>(r1[m]) * static_cast<double>(r1[m]));
+      }
+    }
+  }
+  double v[2] = {rz.value(), rr.value()};
+  const int slots[2] = {0, 1};
+  write_parts<2>(parts, pstride, slots, v);
+}
+
+// k_inner_init in the pair layout: r32 = s r64, d = 0, z32 = M32^-1 r32.
+template <int R2, bool LINE>
+__global__ __launch_bounds__(kBlock) void k_inner_init_p(Grid g, const double* __restrict__ r64,
+                                                         double s, float* __restrict__ r32,
+                                                         float* __restrict__ z32,
+                                                         float* __restrict__ d32,
+                                                         const float* __restrict__ invd,
+                                                         const float* __restrict__ w,
+                                                         double* __restrict__ parts,
+                                                         int pstride) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nz = g.nz;
+  KSum rz, rr;
+  int64_t col, end, stride;
+  column_loop(g.columns(), col, end, stride);
+  for (; col < end; col += stride) {
+    const int64_t base = column_base(g, col);
+    float r0[R2], r1[R2], zero[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) {
+      const int k0 = 128 * m + 2 * lane;
+      r0[m] = 0.0f;
+      r1[m] = 0.0f;
+      zero[m] = 0.0f;
+      if (k0 < nz) {
+        const double2 t = *reinterpret_cast<const double2*>(r64 + base + k0);
+        r0[m] = static_cast<float>(t.x * s);
+        r1[m] = static_cast<float>(t.y * s);
+      }
+    }
+    float z0[R2], z1[R2];
+    precond_pair<R2, LINE>(lane, nz, base, invd, w, r0, r1, z0, z1);
+    store_pair<R2>(r32, base, lane, nz, r0, r1);
+    store_pair<R2>(z32, base, lane, nz, z0, z1);
+    store_pair<R2>(d32, base, lane, nz, zero, zero);
+#pragma unroll
+    for (int m = 0; m < R2; ++m) {
+      if (128 * m + 2 * lane < nz) {
+        rz.add(static_cast<double>(r0[m]) * static_cast<double>(z0[m]));
+        rz.add(static_cast<double>(r1[m]) * static_cast<double>(z1[m]));
+        rr.add(static_cast<double>(r0[m]) * static_cast<double>(r0[m]));
+        rr.add(static_cast<double>(r1[m]) * static_cast<double>(r1[m]));
+      }
+    }
+  }
+  double v[2] = {rz.value(), rr.value()};
+  const int slots[2] = {0, 1};
+  write_parts<2>(parts, pstride, slots, v);
+}
+
 // FP64 stencil -> FP32 stencil over interior cells.
 __global__ __launch_bounds__(kBlock) void k_cast_stencil(Grid g, Stencil<double> A,
                                                          Stencil<float> B) {
@@ -623,6 +954,17 @@ void dispatch_R(int nz, F&& f) {
   }
 }
 
+template <class F>
+void dispatch_R2(int nz, F&& f) {
+  switch ((nz + 127) / 128) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    default: throw std::runtime_error("nz > 512 is not supported by the column kernels");
+  }
+}
+
 template <class T>
 T* dalloc(int64_t n) {
   void* p = nullptr;
@@ -646,6 +988,7 @@ Pcg::Pcg(const Grid& g, hipStream_t stream) : g_(g), stream_(stream) {
   // Measured at 512 x 512 x 100: FP64 operands run best at ~16 waves per CU
   // (S = 64), FP32 operands at ~64 (S = 16).
   march64_ = makeMarch(g, cus, 16);
+  pairs_ = g.nz % 2 == 0;
   march32_ = makeMarch(g, cus, 64);
   pstride_ = std::max({blocks_, march64_.blocks, march32_.blocks});
   const int64_t nH = g.cellsH();
@@ -732,29 +1075,59 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
     if (chunk <= 0) break;
     for (int c = 0; c < chunk; ++c, ++n) {
       if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c], stream_));
-      dispatch_R(g_.nz, [&](auto RC) {
-        constexpr int R = decltype(RC)::value;
-        const int np = n == n0 ? nprev0 : blocks_;
-        const int rst = (n == n0 && restart) ? 1 : 0;
-        if (rst) {
-          k_pcg_dir<T, R, true><<<mj.blocks, kBlock, 0, stream_>>>(
-              g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, rst, tol2);
-        } else {
-          k_pcg_dir<T, R, false><<<mj.blocks, kBlock, 0, stream_>>>(
-              g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, rst, tol2);
+      const int np = n == n0 ? nprev0 : blocks_;
+      const int rst = (n == n0 && restart) ? 1 : 0;
+      if constexpr (std::is_same<T, float>::value) {
+        if (pairs_) {
+          dispatch_R2(g_.nz, [&](auto RC) {
+            constexpr int R2 = decltype(RC)::value;
+            if (rst)
+              k_pcg_dir_p<R2, true><<<mj.blocks, kBlock, 0, stream_>>>(
+                  g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, tol2);
+            else
+              k_pcg_dir_p<R2, false><<<mj.blocks, kBlock, 0, stream_>>>(
+                  g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, tol2);
+          });
         }
-      });
+      }
+      if (!(std::is_same<T, float>::value && pairs_)) {
+        dispatch_R(g_.nz, [&](auto RC) {
+          constexpr int R = decltype(RC)::value;
+          if (rst) {
+            k_pcg_dir<T, R, true><<<mj.blocks, kBlock, 0, stream_>>>(
+                g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, rst, tol2);
+          } else {
+            k_pcg_dir<T, R, false><<<mj.blocks, kBlock, 0, stream_>>>(
+                g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, rst, tol2);
+          }
+        });
+      }
       HIP_OK(hipGetLastError());
       if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c + 1], stream_));
-      dispatch_R(g_.nz, [&](auto RC) {
-        constexpr int R = decltype(RC)::value;
-        if (zline)
-          k_pcg_update<T, R, true><<<blocks_, kBlock, 0, stream_>>>(
-              g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
-        else
-          k_pcg_update<T, R, false><<<blocks_, kBlock, 0, stream_>>>(
-              g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
-      });
+      if constexpr (std::is_same<T, float>::value) {
+        if (pairs_) {
+          dispatch_R2(g_.nz, [&](auto RC) {
+            constexpr int R2 = decltype(RC)::value;
+            if (zline)
+              k_pcg_update_p<R2, true><<<blocks_, kBlock, 0, stream_>>>(
+                  g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+            else
+              k_pcg_update_p<R2, false><<<blocks_, kBlock, 0, stream_>>>(
+                  g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+          });
+        }
+      }
+      if (!(std::is_same<T, float>::value && pairs_)) {
+        dispatch_R(g_.nz, [&](auto RC) {
+          constexpr int R = decltype(RC)::value;
+          if (zline)
+            k_pcg_update<T, R, true><<<blocks_, kBlock, 0, stream_>>>(
+                g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+          else
+            k_pcg_update<T, R, false><<<blocks_, kBlock, 0, stream_>>>(
+                g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+        });
+      }
       HIP_OK(hipGetLastError());
       std::swap(pOld, pNew);
     }
@@ -916,15 +1289,27 @@ int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const 
     if (!std::isfinite(rr) || st->refinements >= opt.max_refinements ||
         total >= opt.max_iterations)
       break;
-    dispatch_R(g_.nz, [&](auto RC) {
-      constexpr int R = decltype(RC)::value;
-      if (opt.zline)
-        k_inner_init<R, true><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
-                                                               invd32_, w32_, parts_, pstride_);
-      else
-        k_inner_init<R, false><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
-                                                                invd32_, w32_, parts_, pstride_);
-    });
+    if (pairs_) {
+      dispatch_R2(g_.nz, [&](auto RC) {
+        constexpr int R2 = decltype(RC)::value;
+        if (opt.zline)
+          k_inner_init_p<R2, true><<<blocks_, kBlock, 0, stream_>>>(
+              g_, r_, s, r32_, z32_, d32_, invd32_, w32_, parts_, pstride_);
+        else
+          k_inner_init_p<R2, false><<<blocks_, kBlock, 0, stream_>>>(
+              g_, r_, s, r32_, z32_, d32_, invd32_, w32_, parts_, pstride_);
+      });
+    } else {
+      dispatch_R(g_.nz, [&](auto RC) {
+        constexpr int R = decltype(RC)::value;
+        if (opt.zline)
+          k_inner_init<R, true><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
+                                                                 invd32_, w32_, parts_, pstride_);
+        else
+          k_inner_init<R, false><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
+                                                                  invd32_, w32_, parts_, pstride_);
+      });
+    }
     HIP_OK(hipGetLastError());
     const double rrS = rr * s * s;
     const double tolIn2 =
