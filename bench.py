@@ -51,10 +51,15 @@ def parse():
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=100)
-    ap.add_argument("--rtol-fp64", type=float, default=1e-12,
+    # Matched accuracy: on this workload FP64 at rtol 1.5e-12 and the mixed
+    # form at 3e-12 (FP32 residual reduction 1e-5 between FP64 updates) both
+    # land near 5e-9 head error; FP64 at 3e-12 does not meet 1e-8.
+    ap.add_argument("--rtol-fp64", type=float, default=1.5e-12,
                     help="FP64 form tolerance (meets 1e-8 head error on this workload)")
-    ap.add_argument("--rtol-mixed", type=float, default=1e-12,
+    ap.add_argument("--rtol-mixed", type=float, default=3e-12,
                     help="mixed form tolerance (meets 1e-8 head error on this workload)")
+    ap.add_argument("--inner-rtol", type=float, default=1e-5,
+                    help="mixed form: FP32 residual reduction between FP64 updates (0: library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy check")
     ap.add_argument("--forms", default="fp64,mixed")
@@ -69,6 +74,8 @@ def make_case(args, precision, rtol):
     from frehg2_amd import cases
     c = cases.hetero_512x512x100(nx=args.nx, ny=args.ny, nz=args.nz, precision=precision)
     c.rtol = rtol
+    if precision == "mixed" and args.inner_rtol > 0:
+        c.inner_rtol = args.inner_rtol
     return c
 
 
