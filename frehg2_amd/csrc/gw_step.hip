@@ -430,6 +430,78 @@ __global__ void k_face_z(Dev d, int corr, double dtg) {
   }
 }
 
+// ---- predictor face conductivities evaluated where the assembly needs them --
+// The rules of k_face_lat<DIR> / k_face_z with corr = 0 (Predictor.cpp:127-385),
+// as functions of one face: the assembly calls them for the six faces of its
+// cell instead of reading face arrays written by separate kernels.
+template <int DIR>
+__device__ __forceinline__ double face_k_lat(const Dev& d, int j, int i, int k) {
+  // DIR 0: face between (j, i) and (j, i + 1), s = i in [0, nx];
+  // DIR 1: face between (j, i) and (j + 1, i), s = j in [0, ny].
+  const Grid& g = d.g;
+  const double* K = DIR == 0 ? d.Kx : d.Ky;
+  const double* ks = DIR == 0 ? d.ksx : d.ksy;
+  const int jn = DIR == 0 ? j : j + 1, in = DIR == 0 ? i + 1 : i;
+  const int s = DIR == 0 ? i : j;
+  const int nlast = DIR == 0 ? g.nx : g.ny;
+  const int64_t c = g.c3(j, i, k), cn = g.c3(jn, in, k);
+  double v = d.harmonic ? face_k_harmonic(K[c], ks[c], 1.0, K[cn], ks[cn], 1.0)
+                        : 0.5 * (K[c] + K[cn]);
+  if (s == 0) {
+    const int jj = DIR == 0 ? j : 1, ii = DIR == 0 ? 1 : i;
+    const int code = DIR == 0 ? d.sxm[g.c2(jj, ii)] : d.sym[g.c2(jj, ii)];
+    if (code == FREHG_BC_NOFLUX) v = 0.0;
+    else if (is_head(code)) v = K[g.c3(jj, ii, k)];
+  }
+  if (s == nlast) {
+    const int code = DIR == 0 ? d.sxp[g.c2(j, i)] : d.syp[g.c2(j, i)];
+    if (code == FREHG_BC_NOFLUX) v = 0.0;
+    else if (is_head(code)) v = K[c];
+  }
+  if (!d.full3d && (unsat(d, j, i, k) || unsat(d, jn, in, k))) v = 0.0;
+  const bool inA = s >= 1 && s <= nlast, inB = s + 1 >= 1 && s + 1 <= nlast;
+  if ((inA && !act(d, j, i, k)) || (inB && !act(d, jn, in, k))) v = 0.0;
+  return v;
+}
+
+__device__ __forceinline__ double face_k_z(const Dev& d, int j, int i, int k) {
+  // face above cell k of column (j, i), k in [0, nz]
+  const Grid& g = d.g;
+  const int c2 = g.c2(j, i);
+  const int kt = d.kt[c2];
+  const bool actK = k < g.nz && k >= kt;
+  const bool actM = k >= 1 && k - 1 >= kt;
+  double v;
+  if (k == 0) {
+    v = 0.0;
+  } else if (k == g.nz) {
+    v = d.botCode[c2] == FREHG_BC_NOFLUX ? 0.0 : d.Kz[g.c3(j, i, g.nz - 1)];
+  } else if (actK && !actM) {
+    v = d.Kz[g.c3(j, i, k)];
+  } else if (!actM) {
+    v = 0.0;
+  } else {
+    const int64_t ca = g.c3(j, i, k - 1), cb = g.c3(j, i, k);
+    v = d.harmonic ? face_k_harmonic(d.Kz[ca], d.ksz[ca], d.dz3d[ca], d.Kz[cb], d.ksz[cb],
+                                     d.dz3d[cb])
+                   : 0.5 * (d.Kz[ca] + d.Kz[cb]);
+  }
+  if (actK && (k == 0 || !actM)) {
+    const int64_t c = g.c3(j, i, k);
+    const double own = d.Kz[c];
+    if (d.coupled) {
+      v = d.dept[c2] > 0.0 ? d.ksz[c] : own;
+    } else {
+      const int code = d.topCode[c2];
+      if (code == FREHG_BC_HEAD) v = d.ksz[c];
+      else if (code == FREHG_BC_NOFLUX) v = 0.0;
+      else v = own;
+    }
+  }
+  if ((k >= 1 && !actM) || (k < g.nz && !actK)) v = 0.0;
+  return v;
+}
+
 // ---- assembly (Predictor.cpp:400-698, PC = true) into stencil storage -----
 __global__ void k_assemble(Dev d, double dtg) {
   const Grid& g = d.g;
@@ -458,10 +530,12 @@ __global__ void k_assemble(Dev d, double dtg) {
     const bool isTop = (k == 0) || !act(d, j, i, k - 1);
     const double dzc = d.dz3d[c];
     const double volume = az * dzc;
-    double gxp = -d.kx[c] * dtg * d.cosx[c] / (dx * dx) * (d.ax[c] * dx);
-    double gxm = -d.kx[cxm] * dtg * d.cosx[cxm] / (dx * dx) * (d.ax[cxm] * dx);
-    double gyp = -d.ky[c] * dtg * d.cosy[c] / (dy * dy) * (d.ay[c] * dy);
-    double gym = -d.ky[cym] * dtg * d.cosy[cym] / (dy * dy) * (d.ay[cym] * dy);
+    const double kxp = face_k_lat<0>(d, j, i, k), kxm = face_k_lat<0>(d, j, i - 1, k);
+    const double kyp = face_k_lat<1>(d, j, i, k), kym = face_k_lat<1>(d, j - 1, i, k);
+    double gxp = -kxp * dtg * d.cosx[c] / (dx * dx) * (d.ax[c] * dx);
+    double gxm = -kxm * dtg * d.cosx[cxm] / (dx * dx) * (d.ax[cxm] * dx);
+    double gyp = -kyp * dtg * d.cosy[c] / (dy * dy) * (d.ay[c] * dy);
+    double gym = -kym * dtg * d.cosy[cym] / (dy * dy) * (d.ay[cym] * dy);
     const bool east = i == g.nx, west = i == 1, north = j == g.ny, south = j == 1;
     const int cE = d.sxp[c2], cW = d.sxm[c2], cN = d.syp[c2], cS = d.sym[c2];
     if (east && is_head(cE)) gxp *= 2.0;
@@ -469,7 +543,7 @@ __global__ void k_assemble(Dev d, double dtg) {
     if (north && is_head(cN)) gyp *= 2.0;
     if (south && is_head(cS)) gym *= 2.0;
     double dzf = (k == g.nz - 1) ? 0.5 * dzc : 0.5 * (dzc + d.dz3d[c + 1]);
-    const double kzTop = d.kzF[g.cF(j, i, k)], kzBot = d.kzF[g.cF(j, i, k + 1)];
+    const double kzTop = face_k_z(d, j, i, k), kzBot = face_k_z(d, j, i, k + 1);
     const double gzp = -kzBot * dtg / (dzc * dzf) * volume;
     dzf = isTop ? 0.5 * dzc : 0.5 * (dzc + d.dz3d[c - 1]);
     const double gzm = -kzTop * dtg / (dzc * dzf) * volume;
@@ -496,13 +570,13 @@ __global__ void k_assemble(Dev d, double dtg) {
     b += dtg * volume * kzTop / dzc;
     if (d.terrain) {
       double sg = (d.bot3d[cxp] > d.bot3d[c]) ? 1.0 : -1.0;
-      b += sg * dtg * d.ax[c] * d.kx[c] * d.sinx[c];
+      b += sg * dtg * d.ax[c] * kxp * d.sinx[c];
       sg = (d.bot3d[c] > d.bot3d[cxm]) ? -1.0 : 1.0;
-      b += sg * dtg * d.ax[cxm] * d.kx[cxm] * d.sinx[cxm];
+      b += sg * dtg * d.ax[cxm] * kxm * d.sinx[cxm];
       sg = (d.bot3d[cyp] > d.bot3d[c]) ? 1.0 : -1.0;
-      b += sg * dtg * d.ay[c] * d.ky[c] * d.siny[c];
+      b += sg * dtg * d.ay[c] * kyp * d.siny[c];
       sg = (d.bot3d[c] > d.bot3d[cym]) ? -1.0 : 1.0;
-      b += sg * dtg * d.ay[cym] * d.ky[cym] * d.siny[cym];
+      b += sg * dtg * d.ay[cym] * kym * d.siny[cym];
     }
     if (east) {
       if (cE == FREHG_BC_FLUX) b += d.ax[c] * d.vxp[c2] * dtg;
@@ -1297,6 +1371,8 @@ struct GwSolver {
     if (iso) k_cell_k<true><<<rb, kThreads, 0, stream>>>(d);
     else k_cell_k<false><<<rb, kThreads, 0, stream>>>(d);
     HIP_CHECK(hipGetLastError());
+    // the predictor's face K are formed inside the assembly
+    if (!corr) return;
     k_face_lat<0><<<rb, kThreads, 0, stream>>>(d, corr);
     HIP_CHECK(hipGetLastError());
     k_face_lat<1><<<rb, kThreads, 0, stream>>>(d, corr);
