@@ -102,6 +102,27 @@ __device__ __forceinline__ MualemFactors vg_mualem(const VgSoil& s, double h) {
   const double br = -expm1(m * lr);
   return MualemFactors{sqrt(se), br, h > 0.0};
 }
+// Mualem factors and C(h) together: both start from log(a|h|) and
+// log1p((a|h|)^n), so C costs two more exponentials (the same operands, in
+// the same order, as vg_capacity).
+__device__ __forceinline__ MualemFactors vg_mualem_capacity(const VgSoil& s, double h,
+                                                            double& cap) {
+  const double m = 1.0 - 1.0 / s.n;
+  const double lah = log(fabs(s.a * h));
+  const double nl = s.n * lah;
+  const double t = exp(nl);
+  const double l1 = log1p(t);
+  const double se = exp(-m * l1);
+  const double lr = t < 1.0 ? nl - l1 : -log1p(exp(-nl));
+  const double br = -expm1(m * lr);
+  if (h > 0.0) {
+    cap = 0.0;
+  } else {
+    const double top = s.a * s.n * m * (s.ts - s.tr) * exp((s.n - 1.0) * lah);
+    cap = top / exp((m + 1.0) * l1);
+  }
+  return MualemFactors{sqrt(se), br, h > 0.0};
+}
 __device__ __forceinline__ double vg_k_from(const MualemFactors& f, double ks) {
   double k = ks * f.sq * f.br * f.br;
   if (k > ks) k = ks;

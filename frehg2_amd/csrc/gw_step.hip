@@ -199,12 +199,21 @@ __global__ void k_classify_coupled(Dev d, double dtg) {
 }
 
 // ---- per-cell conductivities, halo cells included -------------------------
-template <bool ISO>
+// With CAP (the predictor, where h is h^n) it also stores C(h^n) for the
+// assembly.
+template <bool ISO, bool CAP>
 __global__ void k_cell_k(Dev d) {
   const int64_t n = d.g.cellsH();
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const MualemFactors f = vg_mualem(soil_at(d, c), d.h[c]);
+    MualemFactors f;
+    if (CAP) {
+      double cap;
+      f = vg_mualem_capacity(soil_at(d, c), d.h[c], cap);
+      d.ch[c] = cap;
+    } else {
+      f = vg_mualem(soil_at(d, c), d.h[c]);
+    }
     d.Kx[c] = vg_k_from(f, d.ksx[c]);
     if (!ISO) {
       d.Ky[c] = vg_k_from(f, d.ksy[c]);
@@ -513,9 +522,7 @@ __global__ void k_assemble(Dev d, double dtg) {
     const int j = col / g.nx + 1, i = col % g.nx + 1;
     const int c2 = g.c2(j, i);
     const int64_t c = g.c3(j, i, k);
-    const VgSoil s = soil_at(d, c);
-    const double chv = vg_capacity(s, d.hn[c]);
-    d.ch[c] = chv;
+    const double chv = d.ch[c];  // C(h^n), stored by the predictor's k_cell_k
     if (!act(d, j, i, k)) {
       d.Ad[c] = 1.0;
       d.Axp[c] = 0.0;
@@ -1368,8 +1375,10 @@ struct GwSolver {
     HIP_CHECK(hipGetLastError());
   }
   void faces(int corr, double dtg) {
-    if (iso) k_cell_k<true><<<rb, kThreads, 0, stream>>>(d);
-    else k_cell_k<false><<<rb, kThreads, 0, stream>>>(d);
+    if (iso && !corr) k_cell_k<true, true><<<rb, kThreads, 0, stream>>>(d);
+    else if (iso) k_cell_k<true, false><<<rb, kThreads, 0, stream>>>(d);
+    else if (!corr) k_cell_k<false, true><<<rb, kThreads, 0, stream>>>(d);
+    else k_cell_k<false, false><<<rb, kThreads, 0, stream>>>(d);
     HIP_CHECK(hipGetLastError());
     // the predictor's face K are formed inside the assembly
     if (!corr) return;
