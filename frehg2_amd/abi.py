@@ -76,7 +76,8 @@ class GwAudit(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("converged", C.c_int),
-                ("residual_norm", C.c_double), ("rhs_norm", C.c_double)]
+                ("residual_norm", C.c_double), ("rhs_norm", C.c_double),
+                ("fp64_iterations", C.c_int)]
 
 
 class GwTiming(C.Structure):

@@ -32,6 +32,7 @@ struct PcgStats {
   int converged = 0;
   double residual_norm = 0.0;
   double rhs_norm = 0.0;
+  int fp64_iterations = 0;    // mixed: iterations finished in FP64 after an FP32 stall
   double spmv_ms = 0.0;       // accumulated SpMV-kernel time (timing mode)
   int64_t spmv_launches = 0;
 };

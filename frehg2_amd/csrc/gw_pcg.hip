@@ -1321,9 +1321,32 @@ int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const 
     total += its;
     n0 += its;
     restart = false;
+    const double rrPrev = rr;
     std::tie(rr, bb) = refine(xa, xb, 1.0 / s, 1);
-    std::swap(xa, xb);
     st->refinements += 1;
+    if (!std::isfinite(rr) || rr > 0.25 * rrPrev) {
+      // The FP32 cycle did not halve the FP64 residual (an operator too
+      // ill-conditioned for FP32, or a breakdown): finish in FP64 from the
+      // better of the two iterates.
+      const bool keepNew = std::isfinite(rr) && rr < rrPrev;
+      double* best = keepNew ? xb : xa;
+      if (best != x)
+        HIP_OK(hipMemcpyAsync(x, best, static_cast<size_t>(nH) * sizeof(double),
+                              hipMemcpyDeviceToDevice, stream_));
+      PcgOptions o64 = opt;
+      o64.mixed = false;
+      o64.max_iterations = std::max(1, opt.max_iterations - total);
+      PcgStats s64;
+      const int rc = solveFp64(A, b, x, o64, &s64);
+      st->fp64_iterations = s64.iterations;
+      st->iterations = total + s64.iterations;
+      st->residual_norm = s64.residual_norm;
+      st->converged = s64.converged;
+      st->spmv_ms += s64.spmv_ms;
+      st->spmv_launches += s64.spmv_launches;
+      return rc;
+    }
+    std::swap(xa, xb);
   }
   if (xa != x)
     HIP_OK(hipMemcpyAsync(x, xa, static_cast<size_t>(nH) * sizeof(double),

@@ -1825,6 +1825,7 @@ void GwSolver::step(double /*t*/, double dtg) {
   last.iterations = ps.iterations;
   last.refinements = ps.refinements;
   last.converged = ps.converged;
+  last.fp64_iterations = ps.fp64_iterations;
   last.residual_norm = ps.residual_norm;
   last.rhs_norm = ps.rhs_norm;
   if (rc != FREHG_OK)

@@ -139,6 +139,9 @@ typedef struct frehg_gw_solve_stats {
   int converged;                   /* 1 converged, 0 not */
   double residual_norm;            /* final true residual 2-norm */
   double rhs_norm;                 /* ||b||_2 */
+  int fp64_iterations;             /* mixed mode: iterations the solve finished in FP64
+                                      after an FP32 cycle stopped reducing the FP64
+                                      residual (0 when it did not) */
 } frehg_gw_solve_stats;
 
 /* Field identifiers for frehg_gw_get_field / frehg_gw_set_field. */

@@ -270,6 +270,48 @@ def test_mixed_precision_stepwise_stiff():
         assert eh <= 1e-8 and ew <= 1e-8, (s, eh, ew)
 
 
+def _mixed_stepwise(c, steps):
+    o, g = _pair(c)
+    fp64_its = 0
+    for s in range(steps):
+        _schedule(c, o, g, s)
+        _sync(o, g)
+        dt = o.next_dt() if c.adaptive else c.dt
+        o.step(dt)
+        g.step(dt)
+        eh = _nerr(interior(g.get(abi.F_HEAD)), interior(o.get(abi.F_HEAD)))
+        ew = _nerr(interior(g.get(abi.F_WC)), interior(o.get(abi.F_WC)))
+        assert eh <= 1e-8 and ew <= 1e-8, (c.name, s, eh, ew)
+        st = g.last_solve()
+        assert st["converged"] == 1
+        fp64_its += st["fp64_iterations"]
+    print(f"[parity] {c.name} mixed: iterations finished in FP64 {fp64_its}", flush=True)
+    return fp64_its
+
+
+def test_mixed_precision_odd_nz():
+    """nz = 51: the FP32 kernels take one row per lane (no row pairs)."""
+    c = cases.loam_32x32x50(nx=10, ny=9, nz=51)
+    c.precision = "mixed"
+    assert _mixed_stepwise(c, 6) == 0
+
+
+def test_mixed_precision_two_row_pair_segments():
+    """nz = 200 terrain-following: two row-pair segments per column."""
+    c = cases.hetero_512x512x100(nx=12, ny=10, nz=200, precision="mixed")
+    assert _mixed_stepwise(c, 5) == 0
+
+
+def test_mixed_precision_deep_columns_finish_in_fp64():
+    """nz = 400 saturated columns at dz = 1 cm and Ss = 1e-5: the column
+    blocks are too ill-conditioned for FP32, the FP32 cycles stop reducing
+    the FP64 residual and the solve finishes in FP64; the heads still match
+    the FP64 oracle."""
+    c = cases.columns_4096(ncol_side=6, lmax=400)
+    c.precision = "mixed"
+    _mixed_stepwise(c, 6)
+
+
 # ---- the solve seam (LinearSystem) on the GPU -------------------------------
 def _poisson(split):
     n = 101
