@@ -160,6 +160,22 @@ def test_stepwise_loam():
     _stepwise(cases.loam_32x32x50(), 10)
 
 
+def test_stepwise_deepest_columns():
+    """nz = 512, the deepest column the column kernels take (8 rows a lane)."""
+    _stepwise(cases.loam_32x32x50(nx=4, ny=3, nz=512), 3)
+
+
+def test_stepwise_single_layer():
+    _stepwise(cases.loam_32x32x50(nx=9, ny=7, nz=1), 4)
+
+
+def test_deeper_columns_are_rejected():
+    from frehg2_amd.gw import RichardsSolver
+    with pytest.raises(abi.FrehgError) as e:
+        RichardsSolver(cases.loam_32x32x50(nx=2, ny=2, nz=513))
+    assert e.value.code == abi.FREHG_EUNSUPPORTED
+
+
 def test_stepwise_hetero_terrain():
     _stepwise(cases.hetero_512x512x100(nx=32, ny=24, nz=100), 4)
 
