@@ -63,6 +63,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy check")
     ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="run the coupled-ensemble workload instead (64 members of 128x128x64)")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--member-shape", default="128,128,64")
+    ap.add_argument("--ens-rtol", type=float, default=0.0,
+                    help="ensemble: PCG rtol of every leg (0: the workload's 1e-10)")
     return ap.parse_args()
 
 
@@ -197,6 +203,220 @@ def cpu_baseline():
     }
 
 
+def ensemble_accuracy(ens, gm, g64, solos, dt):
+    """One step from the same state (the stacked mixed run's final state) by
+    the stacked mixed solve, the stacked FP64 solve and each member's own FP64
+    solve, against that step converged to rtol 1e-13 (stacked, FP64).  Per
+    member: max |h - h_truth| / max |h_truth| and the error of the member's
+    volume change over the step relative to that change."""
+    from frehg2_amd import abi
+    from frehg2_amd.gw import RichardsSolver
+    names = ("eta", "dept", "seep_accum", "avail", "gain")
+    state = {"h": gm.get(abi.F_HEAD), "w": gm.get(abi.F_WC)}
+    cpl = {n: gm.coupling_state(n) for n in names}
+    c0 = ens.members[0]
+    cellv = c0.dx * c0.dy * c0.dz
+
+    def load(g, h, w, cp):
+        g.set(abi.F_HEAD, h)
+        g.set(abi.F_WC, w)
+        for n in names:
+            g.set_coupling_state(n, cp[n])
+
+    def member_state(m):
+        h = np.zeros((ens.ny + 2, c0.nx + 2, c0.nz))
+        w = np.zeros_like(h)
+        h[1:-1, 1:-1] = ens.member_view(state["h"], m)
+        w[1:-1, 1:-1] = ens.member_view(state["w"], m)
+        return h, w, {n: ens.member_coupling(cpl[n], m) for n in names}
+
+    ens.case.precision = "fp64"
+    ens.case.rtol = 1e-13
+    truth = RichardsSolver(ens.case)
+    heads = {}
+    closure = {}
+
+    def closed_step(g):
+        v0 = g.volume()
+        g.step(dt)
+        a = g.audit()
+        dv = g.volume() - v0
+        return a, (dv - (a["boundary_in"] - a["ss_storage"] + a["realloc_adjust"] - a["vloss"])) / abs(v0)
+
+    for name, g in (("truth", truth), ("stacked_mixed", gm), ("stacked_fp64", g64)):
+        load(g, state["h"], state["w"], cpl)
+        a, closure[name] = closed_step(g)
+        heads[name] = (g.get(abi.F_HEAD), g.get(abi.F_WC))
+        if name == "stacked_mixed":
+            audit = a
+    truth.close()
+    hT, wT = heads["truth"]
+    out = {"reference": "one step of dt from the stacked mixed run's final state, stacked FP64 "
+                        "converged to rtol 1e-13",
+           "rtol_workload": c0.rtol}
+    err = {k: {"head": [], "volume": []} for k in ("stacked_mixed", "stacked_fp64",
+                                                    "sequential_fp64")}
+    solo_audit = {k: 0.0 for k in audit}
+    solo_closure = 0.0
+    for m, r in enumerate(solos):
+        h, w, cp = member_state(m)
+        load(r, h, w, cp)
+        a, res = closed_step(r)
+        solo_closure = max(solo_closure, abs(res))
+        for k in solo_audit:
+            solo_audit[k] += a[k]
+        v0 = float(np.sum(ens.member_view(state["w"], m))) * cellv
+        dvT = float(np.sum(ens.member_view(wT, m))) * cellv - v0
+        ht = ens.member_view(hT, m)
+        sc = float(np.max(np.abs(ht)))
+        got = {"stacked_mixed": tuple(ens.member_view(a, m) for a in heads["stacked_mixed"]),
+               "stacked_fp64": tuple(ens.member_view(a, m) for a in heads["stacked_fp64"]),
+               "sequential_fp64": (r.get(abi.F_HEAD)[1:-1, 1:-1], r.get(abi.F_WC)[1:-1, 1:-1])}
+        for k, (hh, ww) in got.items():
+            err[k]["head"].append(float(np.max(np.abs(hh - ht))) / sc)
+            dv = float(np.sum(ww)) * cellv - v0
+            err[k]["volume"].append(abs(dv - dvT) / max(abs(dvT), 1e-300))
+    for k, e in err.items():
+        out[k] = {"member_head_rel_err_max": max(e["head"]),
+                  "member_head_rel_err_median": float(np.median(e["head"])),
+                  "member_volume_change_rel_err_max": max(e["volume"]),
+                  "member_volume_change_rel_err_median": float(np.median(e["volume"]))}
+        log(f"ensemble accuracy {k}: head max {max(e['head']):.2e}, "
+            f"volume change max {max(e['volume']):.2e}")
+    out["audit_closure_rel"] = {k: abs(v) for k, v in closure.items()}
+    out["audit_closure_rel"]["sequential_fp64_max_member"] = solo_closure
+    scale = max(abs(v) for v in audit.values()) + 1e-300
+    out["stacked_mixed_audit"] = audit
+    out["stacked_mixed_audit_vs_sum_of_members_rel"] = max(
+        abs(audit[k] - solo_audit[k]) for k in audit) / scale
+    log(f"ensemble audit closure (relative to volume): " +
+        ", ".join(f"{k} {v:.1e}" for k, v in out["audit_closure_rel"].items()))
+    return out
+
+
+def ensemble_bench(args):
+    """Coupled ensemble: members stacked along y behind inactive separator
+    rows, advanced by one solver (FP64 and mixed), against the same members
+    run one after another by their own FP64 solvers at the same dt sequence."""
+    from frehg2_amd import abi, cases
+    from frehg2_amd.ensemble import Ensemble
+    from frehg2_amd.gw import RichardsSolver
+    nx, ny, nz = (int(v) for v in args.member_shape.split(","))
+    M = args.members
+
+    def members(precision):
+        out = []
+        for m in range(M):
+            c = cases.coupled_128x128x64(member=m, nx=nx, ny=ny, nz=nz, precision=precision)
+            if args.ens_rtol > 0:
+                c.rtol = args.ens_rtol
+            if precision == "mixed" and args.inner_rtol > 0:
+                c.inner_rtol = args.inner_rtol
+            out.append(c)
+        return out
+
+    t0 = time.perf_counter()
+    ens = Ensemble(members("mixed"))
+    log(f"ensemble: {M} members of {nx}x{ny}x{nz} stacked to {ens.case.nx}x{ens.case.ny}x"
+        f"{ens.case.nz} in {time.perf_counter() - t0:.1f} s")
+    legs = {}
+    # mixed stacked leg: the adaptive controller picks the dt sequence that
+    # every other leg replays
+    ens.case.precision = "mixed"
+    g = RichardsSolver(ens.case)
+    warm_dts, dts = [], []
+    for _ in range(args.warmup):
+        warm_dts.append(g.next_dt())
+        g.step(warm_dts[-1])
+    g.synchronize()
+    g.set_timing(True)
+    its = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dts.append(g.next_dt())
+        g.step(dts[-1])
+        its += g.last_solve()["iterations"]
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    tm = g.timing()
+    legs["stacked_mixed"] = {"ms_per_substep": 1e3 * wall / args.steps,
+                             "solve_ms_per_substep": tm["solve_ms"] / max(tm["steps"], 1),
+                             "iterations_per_solve": its / args.steps,
+                             "active_cells": g.active_cells(),
+                             "cell_steps_per_s": g.active_cells() * args.steps / wall}
+    gm = g
+    log(f"ensemble stacked mixed: {legs['stacked_mixed']['ms_per_substep']:.2f} ms/substep, "
+        f"{its / args.steps:.1f} it/solve")
+
+    ens.case.precision = "fp64"
+    g = RichardsSolver(ens.case)
+    for dt in warm_dts:
+        g.step(dt)
+    g.synchronize()
+    g.set_timing(True)
+    its = 0
+    t0 = time.perf_counter()
+    for dt in dts:
+        g.step(dt)
+        its += g.last_solve()["iterations"]
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    tm = g.timing()
+    legs["stacked_fp64"] = {"ms_per_substep": 1e3 * wall / args.steps,
+                            "solve_ms_per_substep": tm["solve_ms"] / max(tm["steps"], 1),
+                            "iterations_per_solve": its / args.steps,
+                            "cell_steps_per_s": g.active_cells() * args.steps / wall}
+    g64 = g
+    log(f"ensemble stacked fp64: {legs['stacked_fp64']['ms_per_substep']:.2f} ms/substep")
+
+    solos = [RichardsSolver(c) for c in members("fp64")]
+    for dt in warm_dts:
+        for r in solos:
+            r.step(dt)
+    for r in solos:
+        r.synchronize()
+    t0 = time.perf_counter()
+    for dt in dts:
+        for r in solos:
+            r.step(dt)
+    for r in solos:
+        r.synchronize()
+    wall = time.perf_counter() - t0
+    n = sum(r.active_cells() for r in solos)
+    legs["sequential_fp64"] = {"ms_per_substep": 1e3 * wall / args.steps,
+                               "cell_steps_per_s": n * args.steps / wall}
+    log(f"ensemble sequential fp64: {legs['sequential_fp64']['ms_per_substep']:.2f} ms/substep")
+    accuracy = ensemble_accuracy(ens, gm, g64, solos, dts[-1])
+    for r in solos:
+        r.close()
+    gm.close()
+    g64.close()
+    seq = legs["sequential_fp64"]["ms_per_substep"]
+    out = {
+        "metric": "coupled ensemble substep throughput: stacked mixed vs sequential FP64 solves",
+        "value": legs["stacked_mixed"]["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": legs["stacked_mixed"]["ms_per_substep"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": "richards_coupled_128x128x64 ensemble", "members": M,
+                   "member_grid": [nx, ny, nz], "stacked_grid": [ens.case.nx, ens.case.ny, nz],
+                   "dt": "adaptive (common to all members)", "parallelism": "single GPU"},
+        "legs": legs,
+        "speedup_stacked_mixed_vs_sequential_fp64": seq / legs["stacked_mixed"]["ms_per_substep"],
+        "speedup_stacked_fp64_vs_sequential_fp64": seq / legs["stacked_fp64"]["ms_per_substep"],
+        "accuracy": accuracy,
+        "dt_sequence": dts,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic():
     p = ROOT / "profiles" / "traffic_pcg_dir_hetero_512x512x100.json"
     if p.exists():
@@ -214,6 +434,9 @@ def main():
               f"sharding of this path is described in DESIGN.md and not measured here.",
               file=sys.stderr)
         sys.exit(2)
+    if args.ensemble:
+        ensemble_bench(args)
+        return
     forms = [f for f in args.forms.split(",") if f]
     results = {}
     solvers = {}
