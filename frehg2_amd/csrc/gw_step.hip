@@ -129,6 +129,20 @@ __device__ __forceinline__ void put_parts(const Dev& d, const int (&slot)[NV], d
   }
 }
 
+// Adds to the partials put_parts wrote in an earlier kernel of the same grid
+// (sum slots only).
+template <int NV>
+__device__ __forceinline__ void add_parts(const Dev& d, const int (&slot)[NV], double (&v)[NV]) {
+  int op[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) op[q] = kSlotOp[slot[q]];
+  block_reduce<NV>(v, op);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) d.parts[static_cast<int64_t>(slot[q]) * d.rb + blockIdx.x] += v[q];
+  }
+}
+
 // ---- ghosts (Predictor.cpp:66-87, :757-817; Corrector.cpp:709-756) --------
 __device__ __forceinline__ double ghost_head(int code, double value, double interior, double zc,
                                              bool coupled, double bed, double depth) {
@@ -847,6 +861,183 @@ __device__ __forceinline__ void mask_clear(unsigned long long (&mk)[R], int k) {
     if ((k >> 6) == mm) mk[mm] &= ~(1ull << (k & 63));
 }
 
+// Rows [a, b) of mask word w as bits.
+__device__ __forceinline__ unsigned long long row_bits(int w, int a, int b) {
+  const int lo = max(a - 64 * w, 0), hi = min(b - 64 * w, 64);
+  if (hi <= lo) return 0ull;
+  const unsigned long long top = hi == 64 ? ~0ull : ((1ull << hi) - 1ull);
+  return top & ~((1ull << lo) - 1ull);
+}
+
+// The send walks of one column done by all lanes at once, for the common
+// case; returns false, having changed nothing, when the case does not hold.
+//
+// The serial walk (k_realloc_walk_serial) visits the senders in k order; an
+// up-send fills the nearest cells with room above the sender, a down-send
+// those below.  When (1) no sender holds room itself, (2) every up-send has a
+// cell with room above it, and (3) no receiving cell runs out of room, every
+// send lands whole in the nearest cell with room as the column stood before
+// the walk.  A receiving cell r then gets, in the serial order, the
+// down-sends of the senders between the previous cell with room and r, then
+// the up-sends of the senders between r and the next cell with room.  Face r
+// takes the down-sends and face r+1 the up-sends, and by (1) no face is
+// written from two receiving cells.  Each sender lane forms its sends'
+// quotients (volume / receiving cell volume, volume / dt) with the serial
+// walk's operands, and each receiving lane adds them in the serial walk's
+// order: the result is bit-identical.  Down-sends with no cell with room
+// below are dropped (the serial walk's "lost" volume).
+// sh holds 7 * 64 R doubles: dz, then (volume, theta increment, flux
+// increment) of the up-send and of the down-send of every row.
+template <int R>
+__device__ __forceinline__ bool walk_parallel(const Grid& g, int lane, int kt, double dtg,
+                                              double (&wcv)[R], const double (&dzv)[R],
+                                              double (&qz)[R + 1], const double (&room)[R],
+                                              const double (&sUp)[R], const double (&sDn)[R],
+                                              const unsigned long long (&sendMask)[R],
+                                              const unsigned long long (&roomMask)[R],
+                                              double* sh, KSum& lost) {
+  const int nz = g.nz;
+  unsigned long long both = 0ull;
+#pragma unroll
+  for (int m = 0; m < R; ++m) both |= sendMask[m] & roomMask[m];
+  if (both) return false;                                   // (1)
+  const int firstRoom = mask_next<R>(roomMask, kt - 1);
+  const int topEnd = firstRoom < 0 ? nz : firstRoom;
+  unsigned long long topUp = 0ull;
+#pragma unroll
+  for (int m = 0; m < R; ++m) topUp |= __ballot(sUp[m] > 0.0) & row_bits(m, 0, topEnd);
+  if (topUp) return false;                                  // (2)
+  double* shDz = sh;
+  double* shU = sh + kWave * R;
+  double* shD = sh + 4 * kWave * R;
+#pragma unroll
+  for (int m = 0; m < R; ++m) shDz[lane + kWave * m] = dzv[m];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  bool dropped[R];
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int k = lane + kWave * m;
+    double uW = 0.0, uQ = 0.0, dW = 0.0, dQ = 0.0;
+    dropped[m] = false;
+    if (sUp[m] > 0.0) {  // by (2) a cell with room lies above
+      const int t = mask_prev<R>(roomMask, k);
+      uW = sUp[m] / (g.az * shDz[t]);
+      uQ = sUp[m] / dtg;
+    }
+    if (sDn[m] > 0.0) {
+      const int t = mask_next<R>(roomMask, k);
+      if (t >= 0) {
+        dW = sDn[m] / (g.az * shDz[t]);
+        dQ = sDn[m] / dtg;
+      } else {
+        dropped[m] = true;
+      }
+    }
+    shU[3 * k] = sUp[m];
+    shU[3 * k + 1] = uW;
+    shU[3 * k + 2] = uQ;
+    shD[3 * k] = sDn[m];
+    shD[3 * k + 1] = dW;
+    shD[3 * k + 2] = dQ;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // face r+1 of row r = lane + 64 m
+  double qNext[R];
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const double down = __shfl_down(qz[m], 1, kWave);
+    const double wrap = __shfl(qz[m + 1], 0, kWave);
+    qNext[m] = lane == kWave - 1 ? wrap : down;
+  }
+  bool over = false;
+  // The sends of the senders in rows [a, b) into one receiving cell, four
+  // table loads in flight at a time, applied in row order.
+  auto apply = [&](const double* e, int a, int b, double& w, double& rm, double& q, int& has,
+                   bool add) {
+#pragma unroll
+    for (int wd = 0; wd < R; ++wd) {
+      unsigned long long bits = sendMask[wd] & row_bits(wd, a, b);
+      while (bits) {
+        int ks[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          ks[u] = bits ? wd * 64 + __ffsll(static_cast<long long>(bits)) - 1 : -1;
+          bits &= bits - 1ull;
+        }
+        double v[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int kk = ks[u] < 0 ? 0 : ks[u];
+          v[u][0] = e[3 * kk];
+          v[u][1] = e[3 * kk + 1];
+          v[u][2] = e[3 * kk + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (ks[u] >= 0 && v[u][0] > 0.0) {
+            if (!(rm > v[u][0])) over = true;               // (3)
+            w = w + v[u][1];
+            rm = rm - v[u][0];
+            q = add ? q + v[u][2] : q - v[u][2];
+            has = 1;
+          }
+        }
+      }
+    }
+  };
+  double nW[R], qDn[R], qUp[R];
+  int hasDn[R], hasUp[R];
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int r = lane + kWave * m;
+    nW[m] = wcv[m];
+    qDn[m] = qz[m];
+    qUp[m] = qNext[m];
+    hasDn[m] = 0;
+    hasUp[m] = 0;
+    if (r >= nz || !((roomMask[m] >> lane) & 1ull)) continue;
+    double cRoom = room[m];
+    const int prev = mask_prev<R>(roomMask, r);
+    apply(shD, prev + 1, r, nW[m], cRoom, qDn[m], hasDn[m], false);
+    const int next = mask_next<R>(roomMask, r);
+    apply(shU, r + 1, next < 0 ? nz : next, nW[m], cRoom, qUp[m], hasUp[m], true);
+  }
+  if (__ballot(over)) return false;
+  // commit
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    if (dropped[m]) lost.add(-sDn[m]);
+    wcv[m] = nW[m];
+    if (hasDn[m]) qz[m] = qDn[m];
+  }
+#pragma unroll
+  for (int m = 0; m <= R; ++m) {
+    // face lane + 64 m takes the up-sends of row lane + 64 m - 1
+    double v = 0.0;
+    int h = 0;
+    if (m < R) {
+      v = __shfl_up(qUp[m], 1, kWave);
+      h = __shfl_up(hasUp[m], 1, kWave);
+    }
+    if (m > 0) {
+      const double vw = __shfl(qUp[m - 1], kWave - 1, kWave);
+      const int hw = __shfl(hasUp[m - 1], kWave - 1, kWave);
+      if (lane == 0) {
+        v = vw;
+        h = hw;
+      }
+    } else if (lane == 0) {
+      h = 0;
+    }
+    if (h) qz[m] = v;
+  }
+  return true;
+}
+
 // Saturation flag of the state entering reallocation (Reallocate.cpp:303-337
 // "sat": wc >= theta_s or h > aev), for the halo columns; interior cells get
 // theirs from k_update_wc.
@@ -955,34 +1146,85 @@ __global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
 // Phase 2 (the send walks, Reallocate.cpp:410-540) and the final clamp
 // (Corrector.cpp:758-792): one wave per column.  The room of every cell is
 // what phase 1 left, (theta_s - theta) * volume, and is rebuilt here from
-// wcOut with the same operands.
+// wcOut with the same operands.  k_realloc_walk finishes the columns without
+// senders and those walk_parallel can do (nz <= 256); the columns it leaves
+// flagged are finished by k_realloc_walk_serial.
+constexpr int kParallelWalkRows = 4;  // R limit of walk_parallel (LDS)
+
+// The final clamp of a column and its theta out (Corrector.cpp:758-792).
+template <int R>
+__device__ __forceinline__ void clamp_column(const Dev& d, int lane, int kt, int64_t b0,
+                                             double (&wcv)[R], const double (&dzv)[R], KSum& vl) {
+  const Grid& g = d.g;
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int k = lane + kWave * m;
+    if (k >= g.nz || k < kt) continue;
+    const int64_t c = b0 + k;
+    double w = wcv[m];
+    const double volume = g.az * dzv[m];
+    const double ts = d.wcs[c], tr = d.wcr[c];
+    if (w > ts) {
+      const double ex = (w - ts) * volume;
+      d.vloss[c] += ex;
+      vl.add(ex);
+      w = ts;
+    } else if (w < tr) {
+      const double de = (tr - w) * volume;
+      d.vloss[c] -= de;
+      vl.add(-de);
+      w = tr;
+    }
+    wcv[m] = w;
+    d.wcOut[c] = w;
+  }
+}
+
+// Room, sends and their ballot masks of a walking column.
+template <int R>
+__device__ __forceinline__ void walk_inputs(const Dev& d, int lane, int kt, int64_t b0,
+                                            const double (&wcv)[R], const double (&dzv)[R],
+                                            double (&room)[R], double (&sUp)[R], double (&sDn)[R],
+                                            unsigned long long (&sendMask)[R],
+                                            unsigned long long (&roomMask)[R]) {
+  const Grid& g = d.g;
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int k = lane + kWave * m;
+    room[m] = 0.0;
+    sUp[m] = 0.0;
+    sDn[m] = 0.0;
+    if (k < g.nz && k >= kt) {
+      room[m] = (d.wcs[b0 + k] - wcv[m]) * (g.az * dzv[m]);
+      sUp[m] = d.sendUp[b0 + k];
+      sDn[m] = d.sendDown[b0 + k];
+    }
+    sendMask[m] = __ballot(sUp[m] > 0.0 || sDn[m] > 0.0);
+    roomMask[m] = __ballot(room[m] > 0.0);
+  }
+}
+
 template <int R>
 __global__ __launch_bounds__(kThreads) void k_realloc_walk(Dev d, double dtg) {
   constexpr int RF = R + 1;  // face rows 0..nz
+  constexpr int RS = R <= kParallelWalkRows ? R : 1;
+  __shared__ double shWalk[kThreads / kWave][7 * kWave * RS];
   const Grid& g = d.g;
   const int lane = threadIdx.x & (kWave - 1);
+  const int wv = static_cast<int>(threadIdx.x / kWave);
   const int nz = g.nz;
-  KSum lost, vent, thin, vl;
+  KSum lost, vl;
   const int ncol = g.columns();
   const int xcd = blockIdx.x % kXcds, bpx = gridDim.x / kXcds, local = blockIdx.x / kXcds;
   constexpr int wpb = kThreads / kWave;
   const int cBeg = static_cast<int>(static_cast<int64_t>(xcd) * ncol / kXcds);
   const int cEnd = static_cast<int>(static_cast<int64_t>(xcd + 1) * ncol / kXcds);
-  const bool lead = lane == 0;
-  for (int col = cBeg + local * wpb + static_cast<int>(threadIdx.x / kWave); col < cEnd;
-       col += bpx * wpb) {
+  for (int col = cBeg + local * wpb + wv; col < cEnd; col += bpx * wpb) {
     const int j = col / g.nx + 1, i = col % g.nx + 1;
-    const int c2 = g.c2(j, i);
-    const int kt = d.kt[c2];
+    const int kt = d.kt[g.c2(j, i)];
     const int64_t b0 = g.c3(j, i, 0);
-    const int64_t f0 = g.cF(j, i, 0);
     const bool walk = d.colFlag[col] != 0 && !d.reallocOff;
-    double wcv[R], dzv[R], qz[RF];
-#pragma unroll
-    for (int m = 0; m < RF; ++m) {
-      const int k = lane + kWave * m;
-      qz[m] = k <= nz ? d.qzF[f0 + k] : 0.0;
-    }
+    double wcv[R], dzv[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const int k = lane + kWave * m;
@@ -990,23 +1232,80 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk(Dev d, double dtg) {
       dzv[m] = k < nz ? d.dz3d[b0 + k] : 1.0;
     }
     if (walk) {
+      if constexpr (R > kParallelWalkRows) {
+        continue;  // k_realloc_walk_serial
+      } else {
+        const int64_t f0 = g.cF(j, i, 0);
+        double room[R], sUp[R], sDn[R], qz[RF];
+        unsigned long long sendMask[R], roomMask[R];
+        walk_inputs<R>(d, lane, kt, b0, wcv, dzv, room, sUp, sDn, sendMask, roomMask);
+#pragma unroll
+        for (int m = 0; m < RF; ++m) {
+          const int k = lane + kWave * m;
+          qz[m] = k <= nz ? d.qzF[f0 + k] : 0.0;
+        }
+        if (!walk_parallel<R>(g, lane, kt, dtg, wcv, dzv, qz, room, sUp, sDn, sendMask, roomMask,
+                              shWalk[wv], lost))
+          continue;  // k_realloc_walk_serial
+        if (lane == 0) d.colFlag[col] = 0;
+#pragma unroll
+        for (int m = 0; m < RF; ++m) {
+          const int k = lane + kWave * m;
+          if (k <= nz) d.qzF[f0 + k] = qz[m];
+        }
+      }
+    }
+    clamp_column<R>(d, lane, kt, b0, wcv, dzv, vl);
+  }
+  double v[4] = {lost.value(), 0.0, 0.0, vl.value()};
+  const int sl[4] = {S_LOST, S_VENT, S_THIN, S_VLOSS};
+  put_parts<4>(d, sl, v);
+}
+
+// The columns k_realloc_walk left flagged: the serial walk, sender by sender
+// in k order, as wave-uniform control flow.  The column's room, theta and
+// face fluxes sit in registers; the cells that still have room are ballot
+// masks, so a walk jumps straight to the next receiving cell (the
+// reference's loop visits the full cells in between and leaves them
+// untouched).  Every arithmetic operation and its order are the reference's.
+// Same grid as k_realloc_walk: its audit partials are added to that kernel's.
+template <int R>
+__global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double dtg) {
+  constexpr int RF = R + 1;
+  const Grid& g = d.g;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nz = g.nz;
+  KSum lost, vent, thin, vl;
+  const int ncol = g.columns();
+  const bool lead = lane == 0;
+  const int gw = static_cast<int>(blockIdx.x * (kThreads / kWave) + threadIdx.x / kWave);
+  const int nw = static_cast<int>(gridDim.x * (kThreads / kWave));
+  for (int base = gw * kWave; base < ncol; base += nw * kWave) {
+    const bool flagged = base + lane < ncol && d.colFlag[base + lane] != 0;
+    unsigned long long pend = __ballot(flagged);
+    while (pend) {
+      const int col = base + __ffsll(static_cast<long long>(pend)) - 1;
+      pend &= pend - 1ull;
       if (lead) d.colFlag[col] = 0;
-      double room[R], sUp[R], sDn[R];
+      const int j = col / g.nx + 1, i = col % g.nx + 1;
+      const int c2 = g.c2(j, i);
+      const int kt = d.kt[c2];
+      const int64_t b0 = g.c3(j, i, 0);
+      const int64_t f0 = g.cF(j, i, 0);
+      double wcv[R], dzv[R], qz[RF], room[R], sUp[R], sDn[R];
       unsigned long long sendMask[R], roomMask[R];
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         const int k = lane + kWave * m;
-        room[m] = 0.0;
-        sUp[m] = 0.0;
-        sDn[m] = 0.0;
-        if (k < nz && k >= kt) {
-          room[m] = (d.wcs[b0 + k] - wcv[m]) * (g.az * dzv[m]);
-          sUp[m] = d.sendUp[b0 + k];
-          sDn[m] = d.sendDown[b0 + k];
-        }
-        sendMask[m] = __ballot(sUp[m] > 0.0 || sDn[m] > 0.0);
-        roomMask[m] = __ballot(room[m] > 0.0);
+        wcv[m] = k < nz ? d.wcOut[b0 + k] : 0.0;
+        dzv[m] = k < nz ? d.dz3d[b0 + k] : 1.0;
       }
+#pragma unroll
+      for (int m = 0; m < RF; ++m) {
+        const int k = lane + kWave * m;
+        qz[m] = k <= nz ? d.qzF[f0 + k] : 0.0;
+      }
+      walk_inputs<R>(d, lane, kt, b0, wcv, dzv, room, sUp, sDn, sendMask, roomMask);
       const bool topIsHead = d.topCode[c2] == FREHG_BC_HEAD;
       double dept = d.coupled ? d.dept[c2] : 0.0;
       // The receiving cell and the face a walk is working on are cached in
@@ -1129,34 +1428,12 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk(Dev d, double dtg) {
         const int k = lane + kWave * m;
         if (k <= nz) d.qzF[f0 + k] = qz[m];
       }
-    }
-    // ---- final clamp and the theta buffer out ----
-#pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int k = lane + kWave * m;
-      if (k >= nz || k < kt) continue;
-      const int64_t c = b0 + k;
-      double w = wcv[m];
-      const double volume = g.az * dzv[m];
-      const double ts = d.wcs[c], tr = d.wcr[c];
-      if (w > ts) {
-        const double ex = (w - ts) * volume;
-        d.vloss[c] += ex;
-        vl.add(ex);
-        w = ts;
-      } else if (w < tr) {
-        const double de = (tr - w) * volume;
-        d.vloss[c] -= de;
-        vl.add(-de);
-        w = tr;
-      }
-      wcv[m] = w;
-      d.wcOut[c] = w;
+      clamp_column<R>(d, lane, kt, b0, wcv, dzv, vl);
     }
   }
   double v[4] = {lost.value(), vent.value(), thin.value(), vl.value()};
   const int sl[4] = {S_LOST, S_VENT, S_THIN, S_VLOSS};
-  put_parts<4>(d, sl, v);
+  add_parts<4>(d, sl, v);
 }
 
 // Boundary inflow of the step (RichardsSolver.cpp:961-1006): the net volume
@@ -1857,6 +2134,7 @@ void GwSolver::step(double /*t*/, double dtg) {
     dispatch_col(g.nz, [&](auto RC) {
       constexpr int R = decltype(RC)::value;
       k_realloc_walk<R><<<rbCol, kThreads, 0, stream>>>(d, dtg);
+      k_realloc_walk_serial<R><<<rbCol, kThreads, 0, stream>>>(d, dtg);
     });
     HIP_CHECK(hipGetLastError());
     std::swap(d.h, d.hOut);
