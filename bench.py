@@ -15,10 +15,10 @@ accuracy    "@1e-8 rel err": each form runs at the tolerance that keeps its
             (FP64 residual, iterative refinement); the errors are reported.
 roofline    the dominant kernel is the PCG direction kernel k_pcg_dir (FP64):
             algorithmic bytes per launch / its mean launch time, measured with
-            HIP events on the solver's own stream over the timed steps
-            (launches that exit at convergence are not counted)
+            HIP events on the solver's own stream over the timed steps, one
+            launch in 8 (launches that exit at convergence are not counted)
 cpu_baseline  the FP64 oracle (C++/OpenMP, z-line PCG) on a bounded sample of
-            the same workload: a 256 x 256 x 100 slice, 3 steps
+            the same workload: a 256 x 256 x 100 slice, 5 steps
 
 One GPU only: --gpus N > 1 exits with a message.
 """
@@ -41,6 +41,10 @@ HBM_PEAK_GBS = 8000.0
 # (FP64): a copy reaches 5.7 TB/s, a 6-read/2-write stream 4.8 TB/s.
 STREAM_GBS = {"copy": 5700.0, "stream_6r2w": 4800.0}
 ERR_TARGET = 1e-8
+# One SpMV launch in SPMV_SAMPLE is bracketed by HIP events: an event pair
+# delays the next dispatch by ~5 us, which over every launch would add ~0.45 ms
+# to a step.
+SPMV_SAMPLE = 8
 
 
 def parse():
@@ -96,7 +100,7 @@ def run_form(args, precision, rtol):
     for _ in range(args.warmup):
         g.step(case.dt)
     g.synchronize()
-    g.set_timing(True)
+    g.set_timing(SPMV_SAMPLE)
     t0 = time.perf_counter()
     its = 0
     refs = 0
@@ -170,7 +174,7 @@ def accuracy(args, solvers):
 
 
 def cpu_baseline():
-    """FP64 oracle on a 256 x 256 x 100 slice of the same workload, 3 steps."""
+    """FP64 oracle on a 256 x 256 x 100 slice of the same workload, 5 steps."""
     from frehg2_amd import cases
     from oracle import oracle
     threads = min(os.cpu_count() or 1, 16)
@@ -178,7 +182,7 @@ def cpu_baseline():
     c = cases.hetero_512x512x100(nx=256, ny=256, nz=100)
     c.rtol = 1e-12
     o = oracle.OracleRichards(c)
-    steps = 3
+    steps = 5
     for _ in range(steps):
         o.step(c.dt)
     t = o.timing()
@@ -329,7 +333,7 @@ def ensemble_bench(args):
         warm_dts.append(g.next_dt())
         g.step(warm_dts[-1])
     g.synchronize()
-    g.set_timing(True)
+    g.set_timing(SPMV_SAMPLE)
     its = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -353,7 +357,7 @@ def ensemble_bench(args):
     for dt in warm_dts:
         g.step(dt)
     g.synchronize()
-    g.set_timing(True)
+    g.set_timing(SPMV_SAMPLE)
     its = 0
     t0 = time.perf_counter()
     for dt in dts:

@@ -44,7 +44,9 @@ struct PcgOptions {
   bool mixed = false;         // FP32 inner PCG inside FP64 refinement
   double inner_rtol = 1e-4;   // FP32 residual reduction between reliable updates
   int max_refinements = 40;
-  bool timing = false;        // bracket each SpMV launch with HIP events
+  bool timing = false;        // bracket SpMV launches with HIP events
+  int timing_every = 1;       // ... one launch in timing_every
+  bool stencil32_ready = false;  // mixed: the caller wrote the FP32 stencil (stencil32())
 };
 
 // Device-resident PCG workspace for one grid (allocated once).
@@ -59,6 +61,11 @@ class Pcg {
   // rows) and the solution on exit.  Returns 0, or FREHG_EDIVERGED.
   int solve(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
             PcgStats* stats);
+
+  // The FP32 stencil of the mixed form (allocated on the first call).  A
+  // caller that writes it as the rounded FP64 stencil sets
+  // PcgOptions::stencil32_ready, and the solve skips its own cast.
+  Stencil<float> stencil32();
 
   // Algorithmic bytes of one launch of the stencil kernel (the dominant one).
   double spmvBytesPerLaunch(bool mixed) const;

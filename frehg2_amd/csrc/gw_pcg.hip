@@ -1067,6 +1067,8 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
   T* const bufOld = pOld;  // dir launch n writes p into (n - n0 even ? bufNew : bufOld)
   T* const bufNew = pNew;
   hipEvent_t evs[2 * kPoll];
+  bool sampled[kPoll] = {};
+  const int every = std::max(1, opt.timing_every);
   if (opt.timing)
     for (auto& e : evs) HIP_OK(hipEventCreate(&e));
   int done = 0;
@@ -1074,7 +1076,10 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
     const int chunk = std::min(kPoll, maxIt - n);
     if (chunk <= 0) break;
     for (int c = 0; c < chunk; ++c, ++n) {
-      if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c], stream_));
+      // an event pair around a launch delays the next dispatch by a few
+      // microseconds, so only one launch in `every` is bracketed
+      sampled[c] = opt.timing && (n - n0) % every == 0;
+      if (sampled[c]) HIP_OK(hipEventRecord(evs[2 * c], stream_));
       const int np = n == n0 ? nprev0 : blocks_;
       const int rst = (n == n0 && restart) ? 1 : 0;
       if constexpr (std::is_same<T, float>::value) {
@@ -1103,7 +1108,7 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
         });
       }
       HIP_OK(hipGetLastError());
-      if (opt.timing) HIP_OK(hipEventRecord(evs[2 * c + 1], stream_));
+      if (sampled[c]) HIP_OK(hipEventRecord(evs[2 * c + 1], stream_));
       if constexpr (std::is_same<T, float>::value) {
         if (pairs_) {
           dispatch_R2(g_.nz, [&](auto RC) {
@@ -1137,6 +1142,7 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
       int active = hst->done ? hst->it - (n - chunk) : chunk;
       active = std::max(0, std::min(active, chunk));
       for (int c = 0; c < active; ++c) {
+        if (!sampled[c]) continue;
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, evs[2 * c], evs[2 * c + 1]));
         st->spmv_ms += ms;
@@ -1221,8 +1227,7 @@ int Pcg::solveFp64(const Stencil<double>& A, const double* b, double* x, const P
   return st->converged ? FREHG_OK : FREHG_EDIVERGED;
 }
 
-int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
-                    PcgStats* st) {
+Stencil<float> Pcg::stencil32() {
   const int64_t nH = g_.cellsH();
   if (!Ad32_) {
     Ad32_ = dalloc<float>(nH);
@@ -1238,9 +1243,17 @@ int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const 
     invd32_ = dalloc<float>(nH);
     w32_ = dalloc<float>(nH);
   }
-  Stencil<float> A32{Ad32_, Axp32_, Ayp32_, Azp32_};
-  k_cast_stencil<<<blocks_ * 4, kBlock, 0, stream_>>>(g_, A, A32);
-  HIP_OK(hipGetLastError());
+  return Stencil<float>{Ad32_, Axp32_, Ayp32_, Azp32_};
+}
+
+int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                    PcgStats* st) {
+  const int64_t nH = g_.cellsH();
+  const Stencil<float> A32 = stencil32();
+  if (!opt.stencil32_ready) {
+    k_cast_stencil<<<blocks_ * 4, kBlock, 0, stream_>>>(g_, A, A32);
+    HIP_OK(hipGetLastError());
+  }
   const int nFactorBlocks = (g_.columns() + kWave - 1) / kWave;
   if (opt.zline)
     k_factor<float, true><<<nFactorBlocks, kWave, 0, stream_>>>(g_, Ad32_, Azp32_, invd32_, w32_);

@@ -86,6 +86,7 @@ struct Dev {
   double ss, courantMax;
   int full3d, harmonic, reallocOff, redistribute, terrain, guessPrev;
   double *Ad, *Axp, *Ayp, *Azp, *rhs, *x;
+  float *Ad32, *Axp32, *Ayp32, *Azp32;  // the stencil rounded to FP32 (mixed form), or null
   double* parts;
   int rb;
 };
@@ -544,6 +545,12 @@ __global__ void k_assemble(Dev d, double dtg) {
       d.Azp[c] = 0.0;
       d.rhs[c] = 0.0;
       d.x[c] = 0.0;
+      if (d.Ad32) {
+        d.Ad32[c] = 1.0f;
+        d.Axp32[c] = 0.0f;
+        d.Ayp32[c] = 0.0f;
+        d.Azp32[c] = 0.0f;
+      }
       continue;
     }
     const int64_t cxm = g.c3(j, i - 1, k), cxp = g.c3(j, i + 1, k);
@@ -644,11 +651,20 @@ __global__ void k_assemble(Dev d, double dtg) {
         b += -dtg * volume * (d.topValue[c2] + kzTop) / dzc;
       }
     }
+    const double axp = (!east && act(d, j, i + 1, k)) ? gxp : 0.0;
+    const double ayp = (!north && act(d, j + 1, i, k)) ? gyp : 0.0;
+    const double azp = (k + 1 < g.nz) ? gzp : 0.0;
     d.Ad[c] = gct;
-    d.Axp[c] = (!east && act(d, j, i + 1, k)) ? gxp : 0.0;
-    d.Ayp[c] = (!north && act(d, j + 1, i, k)) ? gyp : 0.0;
-    d.Azp[c] = (k + 1 < g.nz) ? gzp : 0.0;
+    d.Axp[c] = axp;
+    d.Ayp[c] = ayp;
+    d.Azp[c] = azp;
     d.rhs[c] = b;
+    if (d.Ad32) {
+      d.Ad32[c] = static_cast<float>(gct);
+      d.Axp32[c] = static_cast<float>(axp);
+      d.Ayp32[c] = static_cast<float>(ayp);
+      d.Azp32[c] = static_cast<float>(azp);
+    }
     d.x[c] = d.guessPrev ? d.hn[c] : 0.0;
   }
 }
@@ -1620,6 +1636,7 @@ struct GwSolver {
   double* slotsHost = nullptr;  // pinned
   // timing
   bool timing = false;
+  int timingEvery = 1;  // SpMV launches bracketed: one in timingEvery
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evs0 = nullptr, evs1 = nullptr;
   frehg_gw_timing tm{};
 
@@ -2064,6 +2081,13 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   HIP_CHECK(hipStreamSynchronize(stream));
   if (icDev) (void)hipFree(icDev);
   pcg = std::make_unique<Pcg>(g, stream);
+  if (popt.mixed) {  // the assembly writes the FP32 stencil beside the FP64 one
+    const Stencil<float> a32 = pcg->stencil32();
+    d.Ad32 = a32.Ad;
+    d.Axp32 = a32.Axp;
+    d.Ayp32 = a32.Ayp;
+    d.Azp32 = a32.Azp;
+  }
   HIP_CHECK(hipEventCreate(&ev0));
   HIP_CHECK(hipEventCreate(&ev1));
   HIP_CHECK(hipEventCreate(&evs0));
@@ -2097,6 +2121,8 @@ void GwSolver::step(double /*t*/, double dtg) {
   PcgStats ps;
   PcgOptions o = popt;
   o.timing = timing;
+  o.timing_every = timingEvery;
+  o.stencil32_ready = d.Ad32 != nullptr;
   const int rc = pcg->solve(A, d.rhs, d.x, o, &ps);
   if (timing) HIP_CHECK(hipEventRecord(evs1, stream));
   last.iterations = ps.iterations;
@@ -2353,7 +2379,8 @@ int frehg_gw_attach_coupling(frehg_gw_solver* s, double* eta, double* dept, doub
 int frehg_gw_set_timing(frehg_gw_solver* s, int enable) {
   return guard([&] {
     GwSolver* g = frehg::S(s);
-    g->timing = enable != 0;
+    g->timing = enable > 0;
+    g->timingEvery = enable > 0 ? enable : 1;
     const double bytes = g->tm.spmv_bytes;
     g->tm = frehg_gw_timing{};
     g->tm.spmv_bytes = bytes;

@@ -125,7 +125,9 @@ class RichardsSolver:
         check(self.lib, self.lib.frehg_gw_active_cells(self._h, C.byref(v)))
         return v.value
 
-    def set_timing(self, enable: bool) -> None:
+    def set_timing(self, enable) -> None:
+        """0/False off; n > 0 (True = 1): step and solve events every step and
+        one SpMV launch in n bracketed by events."""
         check(self.lib, self.lib.frehg_gw_set_timing(self._h, int(enable)))
 
     def timing(self) -> dict:

@@ -226,8 +226,9 @@ typedef struct frehg_gw_timing {
   int64_t steps;
   int64_t pcg_iterations;
 } frehg_gw_timing;
-/* Enable (1) or disable (0) per-launch event timing of the SpMV kernel and
- * reset the accumulators. */
+/* Event timing and reset of the accumulators: 0 off; n > 0 times every step
+ * and every solve, and brackets one SpMV launch in n (an event pair around a
+ * launch delays the next dispatch by a few microseconds). */
 int frehg_gw_set_timing(frehg_gw_solver* s, int enable);
 int frehg_gw_get_timing(frehg_gw_solver* s, frehg_gw_timing* out);
 /* Block until all work on the solver's stream is done. */
