@@ -754,24 +754,34 @@ struct Split {
 __device__ Split gradient_split(const Dev& d, int j, int i, int k) {
   const Grid& g = d.g;
   const double* hR = d.h;  // the state entering reallocation
-  double d6[6];
   const int64_t c = g.c3(j, i, k);
-  auto a = [&](int jj, int ii, int kk) { return kk >= 0 && kk < g.nz && act(d, jj, ii, kk); };
-  d6[0] = (a(j, i + 1, k) && d.kx[c] > 0.0) ? (hR[g.c3(j, i + 1, k)] - hR[c]) / g.dx : 0.0;
-  d6[1] = (a(j, i - 1, k) && d.kx[g.c3(j, i - 1, k)] > 0.0) ? (hR[c] - hR[g.c3(j, i - 1, k)]) / g.dx
-                                                            : 0.0;
-  d6[2] = (a(j + 1, i, k) && d.ky[c] > 0.0) ? (hR[g.c3(j + 1, i, k)] - hR[c]) / g.dy : 0.0;
-  d6[3] = (a(j - 1, i, k) && d.ky[g.c3(j - 1, i, k)] > 0.0) ? (hR[c] - hR[g.c3(j - 1, i, k)]) / g.dy
-                                                            : 0.0;
-  if (a(j, i, k + 1) && d.kzF[g.cF(j, i, k + 1)] > 0.0) {
-    const double dzp = 0.5 * (d.dz3d[c] + d.dz3d[c + 1]);
-    d6[4] = (hR[c + 1] - hR[c]) / dzp - 1.0;
+  const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
+  const int64_t f = g.cF(j, i, k);
+  // Every operand is loaded up front (the halo and the face planes keep all
+  // of these addresses in bounds), so the loads are in flight together
+  // instead of one behind each test.
+  const double h0 = hR[c], hE = hR[c + sx], hW = hR[c - sx], hN = hR[c + sy], hS = hR[c - sy];
+  const double hU = hR[c + 1], hD = hR[c - 1];
+  const double kxE = d.kx[c], kxW = d.kx[c - sx], kyN = d.ky[c], kyS = d.ky[c - sy];
+  const double kzUp = d.kzF[f + 1], kzDn = d.kzF[f];
+  const double dz0 = d.dz3d[c], dzU = d.dz3d[c + 1], dzD = d.dz3d[c - 1];
+  const int ktE = d.kt[g.c2(j, i + 1)], ktW = d.kt[g.c2(j, i - 1)];
+  const int ktN = d.kt[g.c2(j + 1, i)], ktS = d.kt[g.c2(j - 1, i)];
+  const int kt0 = d.kt[g.c2(j, i)];
+  double d6[6];
+  d6[0] = (k >= ktE && kxE > 0.0) ? (hE - h0) / g.dx : 0.0;
+  d6[1] = (k >= ktW && kxW > 0.0) ? (h0 - hW) / g.dx : 0.0;
+  d6[2] = (k >= ktN && kyN > 0.0) ? (hN - h0) / g.dy : 0.0;
+  d6[3] = (k >= ktS && kyS > 0.0) ? (h0 - hS) / g.dy : 0.0;
+  if (k + 1 < g.nz && k + 1 >= kt0 && kzUp > 0.0) {
+    const double dzp = 0.5 * (dz0 + dzU);
+    d6[4] = (hU - h0) / dzp - 1.0;
   } else {
     d6[4] = 0.0;
   }
-  if (a(j, i, k - 1) && d.kzF[g.cF(j, i, k)] > 0.0) {
-    const double dzm = 0.5 * (d.dz3d[c] + d.dz3d[c - 1]);
-    d6[5] = (hR[c] - hR[c - 1]) / dzm - 1.0;
+  if (k - 1 >= 0 && k - 1 >= kt0 && kzDn > 0.0) {
+    const double dzm = 0.5 * (dz0 + dzD);
+    d6[5] = (h0 - hD) / dzm - 1.0;
   } else {
     d6[5] = 0.0;
   }
@@ -1103,6 +1113,10 @@ __global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
     if (k >= kt) {
       const VgSoil s = soil_at(d, c);
       const double volume = g.az * d.dz3d[c];
+      // neighbour saturation operands, loaded together (in bounds: halo)
+      const bool satU = d.sat[c + 1], satE = d.sat[c + nz], satW = d.sat[c - nz];
+      const bool satN = d.sat[c + sy], satS = d.sat[c - sy], satD = d.sat[c - 1];
+      const double kxE = d.kx[c], kxW = d.kx[c - nz], kyN = d.ky[c], kyS = d.ky[c - sy];
       if (before >= s.ts) {
         if (!d.reallocOff) {
           const double excess = (before - s.ts) * volume;
@@ -1114,20 +1128,19 @@ __global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
           wOut = s.ts;
         }
       } else {
-        bool adjSat = false;
-        if (k + 1 < nz && d.sat[c + 1]) adjSat = true;
-        else if (d.kx[c] != 0.0 && d.sat[c + nz]) adjSat = true;
-        else if (d.kx[c - nz] != 0.0 && d.sat[c - nz]) adjSat = true;
-        else if (d.ky[c] != 0.0 && d.sat[c + sy]) adjSat = true;
-        else if (d.ky[c - sy] != 0.0 && d.sat[c - sy]) adjSat = true;
-        else if (k == 0 || k - 1 < kt) {
-          if (d.coupled) {
-            if (d.dept[c2] > 0.0) adjSat = true;
-          } else if (d.topCode[c2] == FREHG_BC_HEAD && d.topValue[c2] >= 0.0) {
+        // the reference's test chain (first true wins) is an OR of its tests
+        bool adjSat = (k + 1 < nz && satU) || (kxE != 0.0 && satE) || (kxW != 0.0 && satW) ||
+                      (kyN != 0.0 && satN) || (kyS != 0.0 && satS);
+        if (!adjSat) {
+          if (k == 0 || k - 1 < kt) {
+            if (d.coupled) {
+              if (d.dept[c2] > 0.0) adjSat = true;
+            } else if (d.topCode[c2] == FREHG_BC_HEAD && d.topValue[c2] >= 0.0) {
+              adjSat = true;
+            }
+          } else if (satD) {
             adjSat = true;
           }
-        } else if (d.sat[c - 1]) {
-          adjSat = true;
         }
         if (!adjSat) {
           if (before < 0.9999 * s.ts) hOutV = vg_head(s, before);
