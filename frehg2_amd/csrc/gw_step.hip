@@ -527,7 +527,7 @@ __device__ __forceinline__ double face_k_z(const Dev& d, int j, int i, int k) {
 }
 
 // ---- assembly (Predictor.cpp:400-698, PC = true) into stencil storage -----
-__global__ void k_assemble(Dev d, double dtg) {
+__global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
   const Grid& g = d.g;
   const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
   const double dx = g.dx, dy = g.dy, az = g.az;
@@ -537,8 +537,8 @@ __global__ void k_assemble(Dev d, double dtg) {
     const int j = col / g.nx + 1, i = col % g.nx + 1;
     const int c2 = g.c2(j, i);
     const int64_t c = g.c3(j, i, k);
-    const double chv = d.ch[c];  // C(h^n), stored by the predictor's k_cell_k
-    if (!act(d, j, i, k)) {
+    const int kt = d.kt[c2];
+    if (k < kt) {
       d.Ad[c] = 1.0;
       d.Axp[c] = 0.0;
       d.Ayp[c] = 0.0;
@@ -553,36 +553,112 @@ __global__ void k_assemble(Dev d, double dtg) {
       }
       continue;
     }
-    const int64_t cxm = g.c3(j, i - 1, k), cxp = g.c3(j, i + 1, k);
-    const int64_t cym = g.c3(j - 1, i, k), cyp = g.c3(j + 1, i, k);
-    const bool isTop = (k == 0) || !act(d, j, i, k - 1);
-    const double dzc = d.dz3d[c];
-    const double volume = az * dzc;
-    const double kxp = face_k_lat<0>(d, j, i, k), kxm = face_k_lat<0>(d, j, i - 1, k);
-    const double kyp = face_k_lat<1>(d, j, i, k), kym = face_k_lat<1>(d, j - 1, i, k);
-    double gxp = -kxp * dtg * d.cosx[c] / (dx * dx) * (d.ax[c] * dx);
-    double gxm = -kxm * dtg * d.cosx[cxm] / (dx * dx) * (d.ax[cxm] * dx);
-    double gyp = -kyp * dtg * d.cosy[c] / (dy * dy) * (d.ay[c] * dy);
-    double gym = -kym * dtg * d.cosy[cym] / (dy * dy) * (d.ay[cym] * dy);
-    const bool east = i == g.nx, west = i == 1, north = j == g.ny, south = j == 1;
+    // Operands, loaded together ahead of the rules that use them (the halo
+    // keeps every neighbour address in bounds).  The six face conductivities
+    // are face_k_lat / face_k_z specialised to an active cell.
+    const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
+    const int64_t cxm = c - sx, cxp = c + sx, cym = c - sy, cyp = c + sy;
+    const int ktE = d.kt[c2 + 1], ktW = d.kt[c2 - 1];
+    const int ktN = d.kt[c2 + g.nx2], ktS = d.kt[c2 - g.nx2];
+    const double chv = d.ch[c];  // C(h^n), stored by the predictor's k_cell_k
+    const double dzc = d.dz3d[c], dzU = d.dz3d[c + 1], dzD = d.dz3d[c - 1];
+    const double Kxc = d.Kx[c], KxE = d.Kx[cxp], KxW = d.Kx[cxm];
+    const double Kyc = d.Ky[c], KyN = d.Ky[cyp], KyS = d.Ky[cym];
+    const double Kzc = d.Kz[c], KzU = d.Kz[c + 1], KzD = d.Kz[c - 1];
+    const double cosxc = d.cosx[c], cosxW = d.cosx[cxm], axc = d.ax[c], axW = d.ax[cxm];
+    const double cosyc = d.cosy[c], cosyS = d.cosy[cym], ayc = d.ay[c], ayS = d.ay[cym];
     const int cE = d.sxp[c2], cW = d.sxm[c2], cN = d.syp[c2], cS = d.sym[c2];
+    const int bot = d.botCode[c2];
+    const double wcnc = d.wcn[c], wcsc = d.wcs[c], hnc = d.hn[c];
+    double b3c = 0.0, b3E = 0.0, b3W = 0.0, b3N = 0.0, b3S = 0.0;
+    double sinxc = 0.0, sinxW = 0.0, sinyc = 0.0, sinyS = 0.0;
+    if (d.terrain) {
+      b3c = d.bot3d[c];
+      b3E = d.bot3d[cxp];
+      b3W = d.bot3d[cxm];
+      b3N = d.bot3d[cyp];
+      b3S = d.bot3d[cym];
+      sinxc = d.sinx[c];
+      sinxW = d.sinx[cxm];
+      sinyc = d.siny[c];
+      sinyS = d.siny[cym];
+    }
+    const bool east = i == g.nx, west = i == 1, north = j == g.ny, south = j == 1;
+    const bool isTop = (k == 0) || (k - 1 < kt);
+    const double volume = az * dzc;
+    const bool unsatC = !d.full3d && unsat(d, j, i, k);
+    double kxp = d.harmonic ? face_k_harmonic(Kxc, d.ksx[c], 1.0, KxE, d.ksx[cxp], 1.0)
+                            : 0.5 * (Kxc + KxE);
+    if (east) {
+      if (cE == FREHG_BC_NOFLUX) kxp = 0.0;
+      else if (is_head(cE)) kxp = Kxc;
+    }
+    if (!d.full3d && (unsatC || unsat(d, j, i + 1, k))) kxp = 0.0;
+    if (!east && k < ktE) kxp = 0.0;
+    double kxm = d.harmonic ? face_k_harmonic(KxW, d.ksx[cxm], 1.0, Kxc, d.ksx[c], 1.0)
+                            : 0.5 * (KxW + Kxc);
+    if (west) {
+      if (cW == FREHG_BC_NOFLUX) kxm = 0.0;
+      else if (is_head(cW)) kxm = Kxc;
+    }
+    if (!d.full3d && (unsat(d, j, i - 1, k) || unsatC)) kxm = 0.0;
+    if (!west && k < ktW) kxm = 0.0;
+    double kyp = d.harmonic ? face_k_harmonic(Kyc, d.ksy[c], 1.0, KyN, d.ksy[cyp], 1.0)
+                            : 0.5 * (Kyc + KyN);
+    if (north) {
+      if (cN == FREHG_BC_NOFLUX) kyp = 0.0;
+      else if (is_head(cN)) kyp = Kyc;
+    }
+    if (!d.full3d && (unsatC || unsat(d, j + 1, i, k))) kyp = 0.0;
+    if (!north && k < ktN) kyp = 0.0;
+    double kym = d.harmonic ? face_k_harmonic(KyS, d.ksy[cym], 1.0, Kyc, d.ksy[c], 1.0)
+                            : 0.5 * (KyS + Kyc);
+    if (south) {
+      if (cS == FREHG_BC_NOFLUX) kym = 0.0;
+      else if (is_head(cS)) kym = Kyc;
+    }
+    if (!d.full3d && (unsat(d, j - 1, i, k) || unsatC)) kym = 0.0;
+    if (!south && k < ktS) kym = 0.0;
+    double gxp = -kxp * dtg * cosxc / (dx * dx) * (axc * dx);
+    double gxm = -kxm * dtg * cosxW / (dx * dx) * (axW * dx);
+    double gyp = -kyp * dtg * cosyc / (dy * dy) * (ayc * dy);
+    double gym = -kym * dtg * cosyS / (dy * dy) * (ayS * dy);
     if (east && is_head(cE)) gxp *= 2.0;
     if (west && is_head(cW)) gxm *= 2.0;
     if (north && is_head(cN)) gyp *= 2.0;
     if (south && is_head(cS)) gym *= 2.0;
-    double dzf = (k == g.nz - 1) ? 0.5 * dzc : 0.5 * (dzc + d.dz3d[c + 1]);
-    const double kzTop = face_k_z(d, j, i, k), kzBot = face_k_z(d, j, i, k + 1);
+    double dzf = (k == g.nz - 1) ? 0.5 * dzc : 0.5 * (dzc + dzU);
+    // face above the cell (k) and below it (k + 1)
+    double kzTop;
+    if (k == 0) kzTop = 0.0;
+    else if (isTop) kzTop = Kzc;
+    else kzTop = d.harmonic ? face_k_harmonic(KzD, d.ksz[c - 1], dzD, Kzc, d.ksz[c], dzc)
+                            : 0.5 * (KzD + Kzc);
+    if (isTop) {
+      if (d.coupled) {
+        kzTop = d.dept[c2] > 0.0 ? d.ksz[c] : Kzc;
+      } else {
+        const int code = d.topCode[c2];
+        if (code == FREHG_BC_HEAD) kzTop = d.ksz[c];
+        else if (code == FREHG_BC_NOFLUX) kzTop = 0.0;
+        else kzTop = Kzc;
+      }
+      if (k >= 1) kzTop = 0.0;
+    }
+    double kzBot;
+    if (k == g.nz - 1) kzBot = bot == FREHG_BC_NOFLUX ? 0.0 : Kzc;
+    else kzBot = d.harmonic ? face_k_harmonic(Kzc, d.ksz[c], dzc, KzU, d.ksz[c + 1], dzU)
+                            : 0.5 * (Kzc + KzU);
     const double gzp = -kzBot * dtg / (dzc * dzf) * volume;
-    dzf = isTop ? 0.5 * dzc : 0.5 * (dzc + d.dz3d[c - 1]);
+    dzf = isTop ? 0.5 * dzc : 0.5 * (dzc + dzD);
     const double gzm = -kzTop * dtg / (dzc * dzf) * volume;
 
-    const double storage = (chv + d.ss * d.wcn[c] / d.wcs[c]);
+    const double storage = (chv + d.ss * wcnc / wcsc);
     double gct = storage * volume - (gxp + gxm + gyp + gym);
     if (east && cE == FREHG_BC_FLUX) gct += gxp;
     if (west && cW == FREHG_BC_FLUX) gct += gxm;
     if (north && cN == FREHG_BC_FLUX) gct += gyp;
     if (south && cS == FREHG_BC_FLUX) gct += gym;
-    const int bot = d.botCode[c2];
     if (!(k == g.nz - 1 && bot != FREHG_BC_HEAD)) gct -= gzp;
     const int mode = d.coupled ? d.cplMode[c2] : 0;
     if (isTop) {
@@ -593,33 +669,33 @@ __global__ void k_assemble(Dev d, double dtg) {
       gct -= gzm;
     }
 
-    double b = storage * d.hn[c] * volume;
+    double b = storage * hnc * volume;
     b -= dtg * volume * kzBot / dzc;
     b += dtg * volume * kzTop / dzc;
     if (d.terrain) {
-      double sg = (d.bot3d[cxp] > d.bot3d[c]) ? 1.0 : -1.0;
-      b += sg * dtg * d.ax[c] * kxp * d.sinx[c];
-      sg = (d.bot3d[c] > d.bot3d[cxm]) ? -1.0 : 1.0;
-      b += sg * dtg * d.ax[cxm] * kxm * d.sinx[cxm];
-      sg = (d.bot3d[cyp] > d.bot3d[c]) ? 1.0 : -1.0;
-      b += sg * dtg * d.ay[c] * kyp * d.siny[c];
-      sg = (d.bot3d[c] > d.bot3d[cym]) ? -1.0 : 1.0;
-      b += sg * dtg * d.ay[cym] * kym * d.siny[cym];
+      double sg = (b3E > b3c) ? 1.0 : -1.0;
+      b += sg * dtg * axc * kxp * sinxc;
+      sg = (b3c > b3W) ? -1.0 : 1.0;
+      b += sg * dtg * axW * kxm * sinxW;
+      sg = (b3N > b3c) ? 1.0 : -1.0;
+      b += sg * dtg * ayc * kyp * sinyc;
+      sg = (b3c > b3S) ? -1.0 : 1.0;
+      b += sg * dtg * ayS * kym * sinyS;
     }
     if (east) {
-      if (cE == FREHG_BC_FLUX) b += d.ax[c] * d.vxp[c2] * dtg;
+      if (cE == FREHG_BC_FLUX) b += axc * d.vxp[c2] * dtg;
       else b -= gxp * d.hn[cxp];
     }
     if (west) {
-      if (cW == FREHG_BC_FLUX) b += d.ax[cxm] * d.vxm[c2] * dtg;
+      if (cW == FREHG_BC_FLUX) b += axW * d.vxm[c2] * dtg;
       else b -= gxm * d.hn[cxm];
     }
     if (north) {
-      if (cN == FREHG_BC_FLUX) b += d.ay[c] * d.vyp[c2] * dtg;
+      if (cN == FREHG_BC_FLUX) b += ayc * d.vyp[c2] * dtg;
       else b -= gyp * d.hn[cyp];
     }
     if (south) {
-      if (cS == FREHG_BC_FLUX) b += d.ay[cym] * d.vym[c2] * dtg;
+      if (cS == FREHG_BC_FLUX) b += ayS * d.vym[c2] * dtg;
       else b -= gym * d.hn[cym];
     }
     if (k == g.nz - 1) {
@@ -633,7 +709,7 @@ __global__ void k_assemble(Dev d, double dtg) {
         } else if (mode == 3) {
           // Dirichlet ghost head 0: the leg sits in the diagonal only.
         } else if (mode == 2) {
-          const double roomTop = fmax(0.0, (0.9999 * d.wcs[c] - d.wcn[c]) * dzc);
+          const double roomTop = fmax(0.0, (0.9999 * wcsc - wcnc) * dzc);
           const double take = fmin(d.avail[c2], roomTop);
           const double qSupply = -take / dtg;
           b += -dtg * volume * (qSupply + kzTop) / dzc;
@@ -641,7 +717,7 @@ __global__ void k_assemble(Dev d, double dtg) {
           double qtop = 0.0;
           if (d.topCode[c2] == FREHG_BC_FLUX) {
             const double tv = d.topValue[c2];
-            if ((tv > 0.0 && d.wcn[c] < d.wcs[c]) || (tv < 0.0 && d.wcn[c] > d.wcr[c])) qtop = tv;
+            if ((tv > 0.0 && wcnc < wcsc) || (tv < 0.0 && wcnc > d.wcr[c])) qtop = tv;
           }
           b += -dtg * volume * (qtop + kzTop) / dzc;
         }
@@ -651,8 +727,8 @@ __global__ void k_assemble(Dev d, double dtg) {
         b += -dtg * volume * (d.topValue[c2] + kzTop) / dzc;
       }
     }
-    const double axp = (!east && act(d, j, i + 1, k)) ? gxp : 0.0;
-    const double ayp = (!north && act(d, j + 1, i, k)) ? gyp : 0.0;
+    const double axp = (!east && k >= ktE) ? gxp : 0.0;
+    const double ayp = (!north && k >= ktN) ? gyp : 0.0;
     const double azp = (k + 1 < g.nz) ? gzp : 0.0;
     d.Ad[c] = gct;
     d.Axp[c] = axp;
@@ -665,7 +741,7 @@ __global__ void k_assemble(Dev d, double dtg) {
       d.Ayp32[c] = static_cast<float>(ayp);
       d.Azp32[c] = static_cast<float>(azp);
     }
-    d.x[c] = d.guessPrev ? d.hn[c] : 0.0;
+    d.x[c] = d.guessPrev ? hnc : 0.0;
   }
 }
 
