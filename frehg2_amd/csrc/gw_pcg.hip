@@ -1077,8 +1077,9 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
     if (chunk <= 0) break;
     for (int c = 0; c < chunk; ++c, ++n) {
       // an event pair around a launch delays the next dispatch by a few
-      // microseconds, so only one launch in `every` is bracketed
-      sampled[c] = opt.timing && (n - n0) % every == 0;
+      // microseconds, so only one launch in `every` is bracketed, mid-way
+      // through each group (the first launch of a solve is the restart form)
+      sampled[c] = opt.timing && (n - n0) % every == every / 2;
       if (sampled[c]) HIP_OK(hipEventRecord(evs[2 * c], stream_));
       const int np = n == n0 ? nprev0 : blocks_;
       const int rst = (n == n0 && restart) ? 1 : 0;
