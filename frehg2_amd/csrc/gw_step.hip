@@ -245,7 +245,7 @@ __device__ __forceinline__ bool unsat(const Dev& d, int j, int i, int k) {
 // ---- lateral faces: K (Predictor.cpp:127-180, 236-268, 318-374) and, with
 // corr, the Darcy flux (Corrector.cpp:176-366).  dir 0 = x, 1 = y.
 template <int DIR>
-__global__ void k_face_lat(Dev d, int corr) {
+__global__ __launch_bounds__(kThreads) void k_face_lat(Dev d, int corr) {
   const Grid& g = d.g;
   // faces: x: j in [1,ny], i in [0,nx]; y: j in [0,ny], i in [1,nx]
   const int na = DIR == 0 ? g.nx + 1 : g.nx;
@@ -278,20 +278,24 @@ __global__ void k_face_lat(Dev d, int corr) {
       s = j;
     }
     const int64_t c = g.c3(j, i, k), cn = g.c3(jn, in, k);
-    const double base = d.harmonic ? face_k_harmonic(K[c], ks[c], 1.0, K[cn], ks[cn], 1.0)
-                                   : 0.5 * (K[c] + K[cn]);
+    // every operand of the face, loaded once up front (c and cn lie in the halo)
+    const double Kc = K[c], Kn = K[cn];
+    const double A = area[c], cv = cosv[c], sv = sinv[c];
+    const double hc = d.h[c], hn = d.h[cn], b3c = d.bot3d[c], b3n = d.bot3d[cn];
+    const double base = d.harmonic ? face_k_harmonic(Kc, ks[c], 1.0, Kn, ks[cn], 1.0)
+                                   : 0.5 * (Kc + Kn);
     double v = base;
     // domain-edge override (Predictor.cpp:236-268)
     if (s == 0) {
       const int jj = DIR == 0 ? j : 1, ii = DIR == 0 ? 1 : i;
       const int code = DIR == 0 ? d.sxm[g.c2(jj, ii)] : d.sym[g.c2(jj, ii)];
       if (code == FREHG_BC_NOFLUX) v = 0.0;
-      else if (is_head(code)) v = K[g.c3(jj, ii, k)];
+      else if (is_head(code)) v = Kn;  // cell (jj, ii) is the face's inner cell cn
     }
     if (s == nlast) {
       const int code = DIR == 0 ? d.sxp[g.c2(j, i)] : d.syp[g.c2(j, i)];
       if (code == FREHG_BC_NOFLUX) v = 0.0;
-      else if (is_head(code)) v = K[c];
+      else if (is_head(code)) v = Kc;
     }
     const bool uns = !d.full3d && (unsat(d, j, i, k) || unsat(d, jn, in, k));
     if (uns) v = 0.0;
@@ -301,29 +305,26 @@ __global__ void k_face_lat(Dev d, int corr) {
     kf[c] = v;
     if (!corr) continue;
     // Darcy flux (Corrector.cpp:176-366), positive toward -axis
-    const double A = area[c];
     double q;
     if (s == 0) {
-      const int64_t c1 = DIR == 0 ? g.c3(j, 1, k) : g.c3(1, i, k);
-      const int64_t c0 = DIR == 0 ? g.c3(j, 0, k) : g.c3(0, i, k);
-      const double kface = K[c1];
-      q = A * kface * cosv[c0] * (d.h[c1] - d.h[c0]) / (0.5 * spacing);
+      // the inner cell (j, 1) / (1, i) is cn and the ghost cell is c
+      const double kface = Kn;
+      q = A * kface * cv * (hn - hc) / (0.5 * spacing);
       const int code = DIR == 0 ? d.sxm[g.c2(j, 1)] : d.sym[g.c2(1, i)];
       const double val = DIR == 0 ? d.vxm[g.c2(j, 1)] : d.vym[g.c2(1, i)];
       if (code == FREHG_BC_NOFLUX) q = 0.0;
       else if (code == FREHG_BC_FLUX) q = -val * A;
     } else {
       double kface, delta = spacing;
-      const int sgn = (d.bot3d[cn] > d.bot3d[c]) ? 1 : -1;
+      const int sgn = (b3n > b3c) ? 1 : -1;
       if (s == nlast) {
-        kface = K[c];
+        kface = Kc;
         delta = 0.5 * spacing;
       } else {
         kface = base;
         if (!actA || !actB) kface = 0.0;
       }
-      q = A * (kface * cosv[c] * (d.h[cn] - d.h[c]) / delta +
-               static_cast<double>(sgn) * kface * sinv[c]);
+      q = A * (kface * cv * (hn - hc) / delta + static_cast<double>(sgn) * kface * sv);
       if (uns) q = 0.0;
       if (s == nlast) {
         const int code = DIR == 0 ? d.sxp[g.c2(j, i)] : d.syp[g.c2(j, i)];
@@ -338,7 +339,7 @@ __global__ void k_face_lat(Dev d, int corr) {
 
 // ---- vertical faces: K (Predictor.cpp:181-232, 280-313, 375-385) and, with
 // corr, the Darcy flux (Corrector.cpp:373-527) with the coupled top modes.
-__global__ void k_face_z(Dev d, int corr, double dtg) {
+__global__ __launch_bounds__(kThreads) void k_face_z(Dev d, int corr, double dtg) {
   const Grid& g = d.g;
   const int nzf = g.nz + 1;
   const int64_t n = static_cast<int64_t>(g.columns()) * nzf;
@@ -351,24 +352,28 @@ __global__ void k_face_z(Dev d, int corr, double dtg) {
     const int kt = d.kt[c2];
     const bool actK = k < g.nz && k >= kt;
     const bool actM = k >= 1 && k - 1 >= kt;
+    // the cells above (ca) and below (cb) the face, loaded once up front; at
+    // k = 0 and k = nz they are the neighbouring columns' end rows, in bounds
+    const int64_t cb = g.c3(j, i, 0) + k, ca = cb - 1;
+    const double Kza = d.Kz[ca], Kzb = d.Kz[cb], ha = d.h[ca], hb = d.h[cb];
+    const double dza = d.dz3d[ca], dzb = d.dz3d[cb];
+    const double base = d.harmonic ? face_k_harmonic(Kza, d.ksz[ca], dza, Kzb, d.ksz[cb], dzb)
+                                   : 0.5 * (Kza + Kzb);
     double v;
     if (k == 0) {
       v = 0.0;
     } else if (k == g.nz) {
-      v = d.botCode[c2] == FREHG_BC_NOFLUX ? 0.0 : d.Kz[g.c3(j, i, g.nz - 1)];
+      v = d.botCode[c2] == FREHG_BC_NOFLUX ? 0.0 : Kza;
     } else if (actK && !actM) {
-      v = d.Kz[g.c3(j, i, k)];
+      v = Kzb;
     } else if (!actM) {
       v = 0.0;
     } else {
-      const int64_t ca = g.c3(j, i, k - 1), cb = g.c3(j, i, k);
-      v = d.harmonic ? face_k_harmonic(d.Kz[ca], d.ksz[ca], d.dz3d[ca], d.Kz[cb], d.ksz[cb],
-                                       d.dz3d[cb])
-                     : 0.5 * (d.Kz[ca] + d.Kz[cb]);
+      v = base;
     }
     if (actK && (k == 0 || !actM)) {  // top-face rule
-      const int64_t c = g.c3(j, i, k);
-      const double own = d.Kz[c];
+      const int64_t c = cb;
+      const double own = Kzb;
       if (d.coupled) {
         v = d.dept[c2] > 0.0 ? d.ksz[c] : own;
       } else {
@@ -383,19 +388,19 @@ __global__ void k_face_z(Dev d, int corr, double dtg) {
     if (!corr) continue;
     double q = 0.0;
     if (actK && !actM) {
-      const int64_t c = g.c3(j, i, k);
+      const int64_t c = cb;
       if (d.coupled) {
         const double ks = d.ksz[c];
         const double depth = d.dept[c2];
         const int mode = d.cplMode[c2];
-        const double delta = 0.5 * d.dz3d[c];
+        const double delta = 0.5 * dzb;
         if (mode == 2) {
-          const double roomTop = fmax(0.0, (0.9999 * d.wcs[c] - d.wc[c]) * d.dz3d[c]);
+          const double roomTop = fmax(0.0, (0.9999 * d.wcs[c] - d.wc[c]) * dzb);
           const double take = fmin(d.avail[c2], roomTop);
           q = -take * g.az / dtg;
           d.avail[c2] -= take;
         } else if (mode == 1) {
-          q = g.az * ks * ((d.h[c] - depth) / delta - 1.0);
+          q = g.az * ks * ((hb - depth) / delta - 1.0);
           if (q < 0.0) {
             const double need = -q * dtg / g.az;
             if (need > d.avail[c2]) {
@@ -406,15 +411,15 @@ __global__ void k_face_z(Dev d, int corr, double dtg) {
             }
           }
         } else if (mode == 3) {
-          q = g.az * ks * ((d.h[c] - depth) / delta - 1.0);
+          q = g.az * ks * ((hb - depth) / delta - 1.0);
           if (q <= 0.0) {
             clamp += -q * dtg;
             q = 0.0;
           }
         } else {
           const double kG = vg_conductivity(soil_at(d, c), depth, ks);
-          const double kface = 0.5 * (kG + d.Kz[c]);
-          q = g.az * kface * ((d.h[c] - depth) / delta - 1.0);
+          const double kface = 0.5 * (kG + Kzb);
+          q = g.az * kface * ((hb - depth) / delta - 1.0);
           if (q <= 0.0) q = 0.0;
           if (d.topCode[c2] == FREHG_BC_FLUX) {
             const double tv = d.topValue[c2];
@@ -426,24 +431,18 @@ __global__ void k_face_z(Dev d, int corr, double dtg) {
         if (code == FREHG_BC_FLUX) {
           q = d.topValue[c2] * g.az;
         } else if (code == FREHG_BC_HEAD) {
-          q = g.az * d.ksz[c] * ((d.h[c] - d.topValue[c2]) / (0.5 * d.dz3d[c]) - 1.0);
+          q = g.az * d.ksz[c] * ((hb - d.topValue[c2]) / (0.5 * dzb) - 1.0);
         }
       }
     } else if (k == g.nz && actM) {
-      const int64_t c = g.c3(j, i, k - 1);
-      const double kface = d.Kz[c];
+      const double kface = Kza;
       const int code = d.botCode[c2];
       if (code == FREHG_BC_FLUX) q = d.botValue[c2] * g.az;
       else if (code == FREHG_BC_GRAVITY) q = -kface * g.az;
       else if (code == FREHG_BC_HEAD)
-        q = g.az * kface * ((d.botValue[c2] - d.h[c]) / (0.5 * d.dz3d[c]) - 1.0);
+        q = g.az * kface * ((d.botValue[c2] - ha) / (0.5 * dza) - 1.0);
     } else if (actM && actK) {
-      const int64_t ca = g.c3(j, i, k - 1), cb = g.c3(j, i, k);
-      const double kface =
-          d.harmonic ? face_k_harmonic(d.Kz[ca], d.ksz[ca], d.dz3d[ca], d.Kz[cb], d.ksz[cb],
-                                       d.dz3d[cb])
-                     : 0.5 * (d.Kz[ca] + d.Kz[cb]);
-      q = g.az * kface * ((d.h[cb] - d.h[ca]) / (0.5 * (d.dz3d[ca] + d.dz3d[cb])) - 1.0);
+      q = g.az * base * ((hb - ha) / (0.5 * (dza + dzb)) - 1.0);
     }
     d.qzF[g.cF(j, i, k)] = q;
   }
