@@ -68,6 +68,9 @@ struct Dev {
   Grid g;
   double *h, *hn, *wc, *wcn, *ch;
   double *ksx, *ksy, *ksz, *vga, *vgn, *wcs, *wcr, *aev;
+  // a van Genuchten field that is the same in every cell is kept as one value
+  // (its pointer is then null): no kernel streams it
+  double vgaC, vgnC, wcsC, wcrC, aevC;
   double *Kx, *Ky, *Kz;
   double *kx, *ky, *kzF, *qx, *qy, *qzF;
   double *vloss;
@@ -94,8 +97,18 @@ struct Dev {
 __device__ __forceinline__ bool act(const Dev& d, int j, int i, int k) {
   return k >= d.kt[d.g.c2(j, i)];
 }
+__device__ __forceinline__ double wcs_at(const Dev& d, int64_t c) {
+  return d.wcs ? d.wcs[c] : d.wcsC;
+}
+__device__ __forceinline__ double wcr_at(const Dev& d, int64_t c) {
+  return d.wcr ? d.wcr[c] : d.wcrC;
+}
+__device__ __forceinline__ double aev_at(const Dev& d, int64_t c) {
+  return d.aev ? d.aev[c] : d.aevC;
+}
 __device__ __forceinline__ VgSoil soil_at(const Dev& d, int64_t c) {
-  return VgSoil{d.vga[c], d.vgn[c], d.wcs[c], d.wcr[c], d.aev[c]};
+  return VgSoil{d.vga ? d.vga[c] : d.vgaC, d.vgn ? d.vgn[c] : d.vgnC, wcs_at(d, c), wcr_at(d, c),
+                aev_at(d, c)};
 }
 __device__ __forceinline__ bool is_head(int code) {
   return code == FREHG_BC_HEAD || code == FREHG_BC_HEAD_HYDROSTATIC;
@@ -187,7 +200,7 @@ __global__ void k_ghosts(Dev d, int doH, int doWc) {
         const double dep = d.coupled ? d.dept[g.c2(ji, ii)] : 0.0;
         d.h[cg] = ghost_head(code, value, d.h[cin], zc, d.coupled != 0, d.bath[g.c2(ji, ii)], dep);
       }
-      if (doWc) d.wc[cg] = is_head(code) ? d.wcs[cin] : d.wc[cin];
+      if (doWc) d.wc[cg] = is_head(code) ? wcs_at(d, cin) : d.wc[cin];
     }
   }
 }
@@ -239,7 +252,7 @@ __global__ void k_cell_k(Dev d) {
 
 __device__ __forceinline__ bool unsat(const Dev& d, int j, int i, int k) {
   const int64_t c = d.g.c3(j, i, k);
-  return act(d, j, i, k) && d.wc[c] < d.wcs[c];
+  return act(d, j, i, k) && d.wc[c] < wcs_at(d, c);
 }
 
 // ---- lateral faces: K (Predictor.cpp:127-180, 236-268, 318-374) and, with
@@ -395,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void k_face_z(Dev d, int corr, double dtg
         const int mode = d.cplMode[c2];
         const double delta = 0.5 * dzb;
         if (mode == 2) {
-          const double roomTop = fmax(0.0, (0.9999 * d.wcs[c] - d.wc[c]) * dzb);
+          const double roomTop = fmax(0.0, (0.9999 * wcs_at(d, c) - d.wc[c]) * dzb);
           const double take = fmin(d.avail[c2], roomTop);
           q = -take * g.az / dtg;
           d.avail[c2] -= take;
@@ -423,7 +436,7 @@ __global__ __launch_bounds__(kThreads) void k_face_z(Dev d, int corr, double dtg
           if (q <= 0.0) q = 0.0;
           if (d.topCode[c2] == FREHG_BC_FLUX) {
             const double tv = d.topValue[c2];
-            if ((tv > 0.0 && d.wc[c] > d.wcr[c]) || tv < 0.0) q += tv * g.az;
+            if ((tv > 0.0 && d.wc[c] > wcr_at(d, c)) || tv < 0.0) q += tv * g.az;
           }
         }
       } else {
@@ -568,7 +581,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
     const double cosyc = d.cosy[c], cosyS = d.cosy[cym], ayc = d.ay[c], ayS = d.ay[cym];
     const int cE = d.sxp[c2], cW = d.sxm[c2], cN = d.syp[c2], cS = d.sym[c2];
     const int bot = d.botCode[c2];
-    const double wcnc = d.wcn[c], wcsc = d.wcs[c], hnc = d.hn[c];
+    const double wcnc = d.wcn[c], wcsc = wcs_at(d, c), hnc = d.hn[c];
     double b3c = 0.0, b3E = 0.0, b3W = 0.0, b3N = 0.0, b3S = 0.0;
     double sinxc = 0.0, sinxW = 0.0, sinyc = 0.0, sinyS = 0.0;
     if (d.terrain) {
@@ -716,7 +729,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
           double qtop = 0.0;
           if (d.topCode[c2] == FREHG_BC_FLUX) {
             const double tv = d.topValue[c2];
-            if ((tv > 0.0 && wcnc < wcsc) || (tv < 0.0 && wcnc > d.wcr[c])) qtop = tv;
+            if ((tv > 0.0 && wcnc < wcsc) || (tv < 0.0 && wcnc > wcr_at(d, c))) qtop = tv;
           }
           b += -dtg * volume * (qtop + kzTop) / dzc;
         }
@@ -772,7 +785,7 @@ __global__ void k_coupled_top(Dev d, double dtg) {
     ex.add(q * dtg);
     if (q < 0.0 && d.dept[c2] <= d.minDepth) {
       const double volume = g.az * d.dz3d[c];
-      if ((d.wcs[c] - d.wc[c]) < -q * dtg / volume) {
+      if ((wcs_at(d, c) - d.wc[c]) < -q * dtg / volume) {
         const double dd = -q * dtg / g.az;
         d.eta[c2] += dd;
         d.dept[c2] += dd;
@@ -783,7 +796,7 @@ __global__ void k_coupled_top(Dev d, double dtg) {
       }
     }
     d.seep[c2] += q * dtg / g.az;
-    if (d.topCode[c2] == FREHG_BC_FLUX && d.topValue[c2] > 0.0 && d.wc[c] > d.wcr[c] &&
+    if (d.topCode[c2] == FREHG_BC_FLUX && d.topValue[c2] > 0.0 && d.wc[c] > wcr_at(d, c) &&
         d.dept[c2] <= d.minDepth) {
       d.seep[c2] -= d.topValue[c2] * dtg;
       ev.add(d.topValue[c2] * dtg * g.az);
@@ -808,14 +821,14 @@ __global__ void k_update_wc(Dev d, double dtg) {
     const int64_t c = g.c3(j, i, k);
     const double volume = g.az * d.dz3d[c];
     const double dh = d.h[c] - d.hn[c];
-    const double coeff = volume * (1.0 + d.ss * dh / d.wcs[c]);
+    const double coeff = volume * (1.0 + d.ss * dh / wcs_at(d, c));
     const double dqx = dtg * (d.qx[c] - d.qx[c - g.nz]);
     const double dqy = dtg * (d.qy[c] - d.qy[c - sy]);
     const double dqz = dtg * (d.qzF[g.cF(j, i, k + 1)] - d.qzF[g.cF(j, i, k)]);
     const double upd = (d.wcn[c] * volume + dqx + dqy + dqz) / coeff;
     d.wc[c] = upd;
-    d.sat[c] = (upd >= d.wcs[c] || d.h[c] > d.aev[c]) ? 1 : 0;
-    ss.add(upd * volume * d.ss * dh / d.wcs[c]);
+    d.sat[c] = (upd >= wcs_at(d, c) || d.h[c] > aev_at(d, c)) ? 1 : 0;
+    ss.add(upd * volume * d.ss * dh / wcs_at(d, c));
   }
   double v[1] = {ss.value()};
   const int s[1] = {S_SS};
@@ -1156,7 +1169,7 @@ __global__ void k_sat_halo(Dev d) {
     else if (q < 2 * g.nx2 + g.ny) { j = q - 2 * g.nx2 + 1; i = 0; }
     else { j = q - 2 * g.nx2 - g.ny + 1; i = g.nx + 1; }
     const int64_t c = g.c3(j, i, k);
-    d.sat[c] = (d.wc[c] >= d.wcs[c] || d.h[c] > d.aev[c]) ? 1 : 0;
+    d.sat[c] = (d.wc[c] >= wcs_at(d, c) || d.h[c] > aev_at(d, c)) ? 1 : 0;
   }
 }
 
@@ -1267,7 +1280,7 @@ __device__ __forceinline__ void clamp_column(const Dev& d, int lane, int kt, int
     const int64_t c = b0 + k;
     double w = wcv[m];
     const double volume = g.az * dzv[m];
-    const double ts = d.wcs[c], tr = d.wcr[c];
+    const double ts = wcs_at(d, c), tr = wcr_at(d, c);
     if (w > ts) {
       const double ex = (w - ts) * volume;
       d.vloss[c] += ex;
@@ -1299,7 +1312,7 @@ __device__ __forceinline__ void walk_inputs(const Dev& d, int lane, int kt, int6
     sUp[m] = 0.0;
     sDn[m] = 0.0;
     if (k < g.nz && k >= kt) {
-      room[m] = (d.wcs[b0 + k] - wcv[m]) * (g.az * dzv[m]);
+      room[m] = (wcs_at(d, b0 + k) - wcv[m]) * (g.az * dzv[m]);
       sUp[m] = d.sendUp[b0 + k];
       sDn[m] = d.sendDown[b0 + k];
     }
@@ -1592,7 +1605,7 @@ __global__ __launch_bounds__(kThreads) void k_adapt_inputs(Dev d, double dtg) {
     const double qou = d.qx[c - nz] / d.ax[c] + d.qy[c - sy] / d.ay[c] + d.qzF[fk] / g.az;
     const double dq = fabs(qin - qou) * dtg / d.dz3d[c];
     if (dq > dqMax) dqMax = dq;
-    if (d.wc[c] < d.wcs[c]) {
+    if (d.wc[c] < wcs_at(d, c)) {
       const double dk = vg_dkdtheta(soil_at(d, c), d.wc[c], d.ksz[c]);
       const double co = d.courantMax * d.dz3d[c] / dk;
       if (co < coMin) coMin = co;
@@ -2057,6 +2070,16 @@ void GwSolver::create(const frehg_gw_desc& ds) {
       for (int i = 0; i <= nx + 1; ++i) {
         std::memcpy(&tmp[I3(0, i, 0)], &tmp[I3(1, i, 0)], nz * sizeof(double));
         std::memcpy(&tmp[I3(ny + 1, i, 0)], &tmp[I3(ny, i, 0)], nz * sizeof(double));
+      }
+      if (a >= 3) {  // van Genuchten fields: one value when uniform
+        bool uniform = true;
+        for (int64_t q = 1; q < nH && uniform; ++q) uniform = tmp[q] == tmp[0];
+        if (uniform) {
+          double* cst[5] = {&d.vgaC, &d.vgnC, &d.wcsC, &d.wcrC, &d.aevC};
+          *cst[a - 3] = tmp[0];
+          *dst[a] = nullptr;
+          continue;
+        }
       }
       *dst[a] = field();
       up(*dst[a], tmp.data(), nH, 8);
