@@ -46,6 +46,7 @@ struct PcgState {
   double rz[2];
   double rr;
   double alpha;
+  double alphaH[2];  // alpha of iteration n in slot n & 1 (deferred x updates)
   int it;
   int done;
 };
@@ -426,8 +427,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, March mj, Stencil<T>
 }
 
 // alpha = rz / p.q; x += alpha p; r -= alpha q; z = M^-1 r; partials rz, rr.
-template <class T, int R, bool LINE>
+// The x update is deferred by one iteration (XM): an even iteration of a
+// sweep leaves x alone (XM = 0) and the odd one after it adds both terms,
+// x = (x + alpha_prev p_prev) + alpha p (XM = 2), the same two roundings as
+// two single updates.  p_prev is still intact then (the next direction
+// kernel overwrites it), so x is read and written every other iteration.
+template <class T, int R, bool LINE, int XM>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(Grid g, T* __restrict__ x,
+                                                       const T* __restrict__ pPrev,
                                                        const T* __restrict__ p, T* __restrict__ r,
                                                        const T* __restrict__ q, T* __restrict__ z,
                                                        const T* __restrict__ invd,
@@ -440,7 +447,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Grid g, T* __restrict__ x
   const int rs[1] = {2};
   reduce_parts<1>(parts, pstride, nprev, rs, red);
   const double alphaD = st->rz[n & 1] / red[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alphaD;
+  const T alphaPrev = XM == 2 ? static_cast<T>(st->alphaH[(n - 1) & 1]) : T(0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha = alphaD;
+    st->alphaH[n & 1] = alphaD;
+  }
   const T alpha = static_cast<T>(alphaD);
   const int lane = threadIdx.x & (kWave - 1);
   KSum rz, rr;
@@ -455,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Grid g, T* __restrict__ x
       rv[m] = T(0);
       if (k < g.nz) {
         const int64_t c = base + k;
-        x[c] = x[c] + alpha * p[c];
+        if (XM == 2) x[c] = (x[c] + alphaPrev * pPrev[c]) + alpha * p[c];
         rv[m] = r[c] - alpha * q[c];
       }
     }
@@ -474,6 +485,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Grid g, T* __restrict__ x
   double v[2] = {rz.value(), rr.value()};
   const int slots[2] = {0, 1};
   write_parts<2>(parts, pstride, slots, v);
+}
+
+// The x term an even last iteration deferred: x += alpha_n p_n (interior).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_flush_x(Grid g, T* __restrict__ x,
+                                                    const T* __restrict__ p,
+                                                    const PcgState* __restrict__ st, int slot) {
+  const T alpha = static_cast<T>(st->alphaH[slot]);
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t c = column_base(g, t / g.nz) + t % g.nz;
+    x[c] = x[c] + alpha * p[c];
+  }
 }
 
 // Mixed form, outer sweep: xo = xi + sc d (all 7 stencil points), r = b - A xo
@@ -1124,14 +1149,21 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
         }
       }
       if (!(std::is_same<T, float>::value && pairs_)) {
+        const bool two = ((n - n0) & 1) != 0;  // x takes this and the previous term
         dispatch_R(g_.nz, [&](auto RC) {
           constexpr int R = decltype(RC)::value;
-          if (zline)
-            k_pcg_update<T, R, true><<<blocks_, kBlock, 0, stream_>>>(
-                g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+          if (zline && two)
+            k_pcg_update<T, R, true, 2><<<blocks_, kBlock, 0, stream_>>>(
+                g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+          else if (zline)
+            k_pcg_update<T, R, true, 0><<<blocks_, kBlock, 0, stream_>>>(
+                g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+          else if (two)
+            k_pcg_update<T, R, false, 2><<<blocks_, kBlock, 0, stream_>>>(
+                g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
           else
-            k_pcg_update<T, R, false><<<blocks_, kBlock, 0, stream_>>>(
-                g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+            k_pcg_update<T, R, false, 0><<<blocks_, kBlock, 0, stream_>>>(
+                g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
         });
       }
       HIP_OK(hipGetLastError());
@@ -1167,6 +1199,12 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
   } else {
     pOld = bufOld;
     pNew = bufNew;
+  }
+  // an odd count ends on an even iteration, whose x term was deferred
+  const bool deferred = !(std::is_same<T, float>::value && pairs_);
+  if (deferred && (*itDone & 1)) {
+    k_flush_x<T><<<blocks_ * 4, kBlock, 0, stream_>>>(g_, x, pOld, dst, (n0 + *itDone - 1) & 1);
+    HIP_OK(hipGetLastError());
   }
   return done;
 }
