@@ -823,9 +823,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p(Grid g, March mj, Stencil<
   write_parts<1>(parts, pstride, slots, v);
 }
 
-// k_pcg_update for FP32 operands in the pair layout.
-template <int R2, bool LINE>
+// k_pcg_update for FP32 operands in the pair layout (x deferred the same way).
+template <int R2, bool LINE, int XM>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, float* __restrict__ x,
+                                                         const float* __restrict__ pPrev,
                                                          const float* __restrict__ p,
                                                          float* __restrict__ r,
                                                          const float* __restrict__ q,
@@ -840,7 +841,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, float* __restri
   const int rs[1] = {2};
   reduce_parts<1>(parts, pstride, nprev, rs, red);
   const double alphaD = st->rz[n & 1] / red[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alphaD;
+  const float alphaPrev = XM == 2 ? static_cast<float>(st->alphaH[(n - 1) & 1]) : 0.0f;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha = alphaD;
+    st->alphaH[n & 1] = alphaD;
+  }
   const float alpha = static_cast<float>(alphaD);
   const int lane = threadIdx.x & (kWave - 1);
   const int nz = g.nz;
@@ -849,19 +854,26 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, float* __restri
   column_loop(g.columns(), col, end, stride);
   for (; col < end; col += stride) {
     const int64_t base = column_base(g, col);
-    float x0[R2], x1[R2], p0[R2], p1[R2], r0[R2], r1[R2], q0[R2], q1[R2];
-    load_pair<R2>(x, base, lane, nz, x0, x1);
-    load_pair<R2>(p, base, lane, nz, p0, p1);
+    float r0[R2], r1[R2], q0[R2], q1[R2];
+    if (XM == 2) {
+      float x0[R2], x1[R2], a0[R2], a1[R2], p0[R2], p1[R2];
+      load_pair<R2>(x, base, lane, nz, x0, x1);
+      load_pair<R2>(pPrev, base, lane, nz, a0, a1);
+      load_pair<R2>(p, base, lane, nz, p0, p1);
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        x0[m] = (x0[m] + alphaPrev * a0[m]) + alpha * p0[m];
+        x1[m] = (x1[m] + alphaPrev * a1[m]) + alpha * p1[m];
+      }
+      store_pair<R2>(x, base, lane, nz, x0, x1);
+    }
     load_pair<R2>(r, base, lane, nz, r0, r1);
     load_pair<R2>(q, base, lane, nz, q0, q1);
 #pragma unroll
     for (int m = 0; m < R2; ++m) {
-      x0[m] = x0[m] + alpha * p0[m];
-      x1[m] = x1[m] + alpha * p1[m];
       r0[m] = r0[m] - alpha * q0[m];
       r1[m] = r1[m] - alpha * q1[m];
     }
-    store_pair<R2>(x, base, lane, nz, x0, x1);
     float z0[R2], z1[R2];
     precond_pair<R2, LINE>(lane, nz, base, invd, w, r0, r1, z0, z1);
     store_pair<R2>(r, base, lane, nz, r0, r1);
@@ -1139,12 +1151,19 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
         if (pairs_) {
           dispatch_R2(g_.nz, [&](auto RC) {
             constexpr int R2 = decltype(RC)::value;
-            if (zline)
-              k_pcg_update_p<R2, true><<<blocks_, kBlock, 0, stream_>>>(
-                  g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+            const bool two = ((n - n0) & 1) != 0;  // x takes this and the previous term
+            if (zline && two)
+              k_pcg_update_p<R2, true, 2><<<blocks_, kBlock, 0, stream_>>>(
+                  g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+            else if (zline)
+              k_pcg_update_p<R2, true, 0><<<blocks_, kBlock, 0, stream_>>>(
+                  g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+            else if (two)
+              k_pcg_update_p<R2, false, 2><<<blocks_, kBlock, 0, stream_>>>(
+                  g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
             else
-              k_pcg_update_p<R2, false><<<blocks_, kBlock, 0, stream_>>>(
-                  g_, x, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+              k_pcg_update_p<R2, false, 0><<<blocks_, kBlock, 0, stream_>>>(
+                  g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
           });
         }
       }
@@ -1201,8 +1220,7 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
     pNew = bufNew;
   }
   // an odd count ends on an even iteration, whose x term was deferred
-  const bool deferred = !(std::is_same<T, float>::value && pairs_);
-  if (deferred && (*itDone & 1)) {
+  if (*itDone & 1) {
     k_flush_x<T><<<blocks_ * 4, kBlock, 0, stream_>>>(g_, x, pOld, dst, (n0 + *itDone - 1) & 1);
     HIP_OK(hipGetLastError());
   }
