@@ -241,18 +241,36 @@ __global__ __launch_bounds__(64) void k_factor(Grid g, const T* __restrict__ Ad,
   const int64_t mycol = col0 + lane;
   sbase[lane] = mycol < ncol ? column_base(g, mycol) : -1;
   __syncthreads();
+  // The next tile is fetched into registers while the current one is
+  // factored, so its loads are in flight during the sequential recurrence.
+  T ra[kChunk], re[kChunk];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int m = 0; m < kChunk; ++m) {
+      const int e = m * kWave + lane;
+      const int cc = e / kChunk, kk = e % kChunk, k = k0 + kk;
+      ra[m] = T(0);
+      re[m] = T(0);
+      if (sbase[cc] >= 0 && k < g.nz) {
+        ra[m] = Ad[sbase[cc] + k];
+        if (LINE) re[m] = Azp[sbase[cc] + k];
+      }
+    }
+  };
+  fetch(0);
   T ePrev = T(0), wPrev = T(0);
   for (int k0 = 0; k0 < g.nz; k0 += kChunk) {
-#pragma unroll 4
+#pragma unroll
     for (int m = 0; m < kChunk; ++m) {
       const int e = m * kWave + lane;
       const int cc = e / kChunk, kk = e % kChunk, k = k0 + kk;
       if (sbase[cc] >= 0 && k < g.nz) {
-        sA[cc][kk] = Ad[sbase[cc] + k];
-        if (LINE) sE[cc][kk] = Azp[sbase[cc] + k];
+        sA[cc][kk] = ra[m];
+        if (LINE) sE[cc][kk] = re[m];
       }
     }
     __syncthreads();
+    if (k0 + kChunk < g.nz) fetch(k0 + kChunk);
     if (mycol < ncol) {
       for (int kk = 0; kk < kChunk && k0 + kk < g.nz; ++kk) {
         if (LINE) {
