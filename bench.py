@@ -17,6 +17,9 @@ roofline    the dominant kernel is the PCG direction kernel k_pcg_dir (FP64):
             algorithmic bytes per launch / its mean launch time, measured with
             HIP events on the solver's own stream over the timed steps, one
             launch in 8 (launches that exit at convergence are not counted)
+stream      frehg_hbm_probe, run first: the GB/s of an FP64 copy and of a
+            6-read/2-write FP64 stream over eight fields of the workload's size
+            (roofline.measured_stream_gbs, roofline.frac_of_measured_stream)
 cpu_baseline  the FP64 oracle (C++/OpenMP, z-line PCG) on a bounded sample of
             the same workload: a 256 x 256 x 100 slice, 5 steps
 
@@ -37,9 +40,6 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# Measured on this machine with plain streaming kernels over 211 MB arrays
-# (FP64): a copy reaches 5.7 TB/s, a 6-read/2-write stream 4.8 TB/s.
-STREAM_GBS = {"copy": 5700.0, "stream_6r2w": 4800.0}
 ERR_TARGET = 1e-8
 # One SpMV launch in SPMV_SAMPLE is bracketed by HIP events: an event pair
 # delays the next dispatch by ~5 us, which over every launch would add ~0.45 ms
@@ -441,6 +441,12 @@ def main():
     if args.ensemble:
         ensemble_bench(args)
         return
+    # the achievable streaming rate on this device, measured live: a copy and
+    # a 6-read/2-write FP64 stream over eight fields of this workload's size
+    from frehg2_amd import hbm
+    stream = hbm.measure((args.nx + 2) * (args.ny + 2) * args.nz)
+    log(f"hbm probe: copy {stream['copy_gbs']:.0f} GB/s, 6r2w stream "
+        f"{stream['stream_6r2w_gbs']:.0f} GB/s")
     forms = [f for f in args.forms.split(",") if f]
     results = {}
     solvers = {}
@@ -473,7 +479,9 @@ def main():
                        "kernel": "k_pcg_dir<double> (FP64 p = z + beta p, q = A p, p.q)",
                        "avg_launch_ms": r64["spmv_avg_ms"],
                        "algorithmic_bytes_per_launch": r64["spmv_bytes"],
-                       "frac_of_measured_stream": bw / STREAM_GBS["stream_6r2w"]}
+                       "measured_stream_gbs": {"copy": stream["copy_gbs"],
+                                               "stream_6r2w": stream["stream_6r2w_gbs"]},
+                       "frac_of_measured_stream": bw / stream["stream_6r2w_gbs"]}
     out["fp64"] = r64
     if "mixed" in results:
         rm = results["mixed"]

@@ -123,6 +123,13 @@ SYMBOLS = {
 }
 
 
+# The achievable-bandwidth probe (include/frehg_probe.h; no reference counterpart).
+PROBE_SYMBOLS = {
+    "frehg_hbm_probe": ([C.c_int64, C.c_int, _dp, _dp], C.c_int),
+    "frehg_probe_last_error": ([], C.c_char_p),
+}
+
+
 class FrehgError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"[frehg error {code}] {message}")
@@ -142,7 +149,7 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
         raise FrehgError(FREHG_EDEVICE,
                          f"HIP library {p} is not built; run __graft_entry__.build() first")
     lib = C.CDLL(str(p))
-    for name, (args, res) in SYMBOLS.items():
+    for name, (args, res) in {**SYMBOLS, **PROBE_SYMBOLS}.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res

@@ -17,7 +17,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libfrehg_gw.so"
-SOURCES = ["gw_pcg.hip", "gw_step.hip", "gw_ls.hip"]
+SOURCES = ["gw_pcg.hip", "gw_step.hip", "gw_ls.hip", "hbm_probe.hip"]
 ARCH = os.environ.get("FREHG_OFFLOAD_ARCH", "gfx950")
 FLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -13,8 +13,11 @@ ROOT = Path(__file__).resolve().parents[1]
 HEADER = ROOT / "include" / "frehg_gw.h"
 
 
-def declared_functions():
-    text = HEADER.read_text()
+PROBE_HEADER = ROOT / "include" / "frehg_probe.h"
+
+
+def declared_functions(header=HEADER):
+    text = header.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(frehg_\w+)\s*\(", text, re.M)))
 
@@ -29,9 +32,20 @@ def test_header_declares_the_seams():
 
 def test_library_exports_every_declared_symbol():
     lib = abi.load_library()
-    for name in declared_functions():
+    for name in declared_functions() + declared_functions(PROBE_HEADER):
         assert hasattr(lib, name), name
     assert lib.frehg_abi_version() == abi.ABI_VERSION
+
+
+def test_probe_header_matches_its_ctypes_table():
+    assert set(declared_functions(PROBE_HEADER)) == set(abi.PROBE_SYMBOLS)
+
+
+def test_probe_rejects_bad_arguments_without_a_device():
+    lib = abi.load_library()
+    a, b = C.c_double(), C.c_double()
+    assert lib.frehg_hbm_probe(0, 5, C.byref(a), C.byref(b)) != 0
+    assert b"positive" in lib.frehg_probe_last_error()
 
 
 def test_ctypes_structs_match_the_c_layout(tmp_path):
