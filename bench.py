@@ -195,6 +195,17 @@ def cpu_baseline():
                 break
     except OSError:
         pass
+    # The reference's default algebra (CG + ICC(0), triangular solves serial)
+    # on a smaller slice, reported beside the port's z-line lane.
+    ci = cases.hetero_512x512x100(nx=128, ny=128, nz=100)
+    ci.rtol = 1e-12
+    ci.preconditioner = "icc0"
+    oi = oracle.OracleRichards(ci)
+    steps_i = 2
+    for _ in range(steps_i):
+        oi.step(ci.dt)
+    ti = oi.timing()
+    ni = oi.active_cells()
     return {
         "value": n * t["iterations"] / t["solve_s"],
         "unit": "cell-iterations/s",
@@ -204,6 +215,14 @@ def cpu_baseline():
                   f"256x256x100 ({n} cells), {steps} steps of 300 s: solve {t['solve_s']:.2f} s, "
                   f"{t['iterations']} iterations; whole steps {t['step_s']:.2f} s = "
                   f"{n * steps / t['step_s']:.3e} cell-steps/s; {threads} threads on {cpu}",
+        "reference_algebra": {
+            "value": ni * ti["iterations"] / ti["solve_s"],
+            "unit": "cell-iterations/s",
+            "sample": f"the reference's default algebra, CG + ICC(0) (triangular solves serial, the "
+                      f"rest on {threads} threads), at 128x128x100 ({ni} cells), {steps_i} steps: "
+                      f"solve {ti['solve_s']:.2f} s, {ti['iterations']} iterations; whole steps "
+                      f"{ti['step_s']:.2f} s = {ni * steps_i / ti['step_s']:.3e} cell-steps/s",
+        },
     }
 
 
