@@ -94,13 +94,20 @@ __device__ __forceinline__ bool march_span(const Grid& g, const March& mj, int& 
   return slab < mj.nslab && j0 <= g.ny;
 }
 
+// Row k of a column for a lane past the column's end: such a lane reads the
+// last row (in bounds) and its caller zeroes the value.  All lanes' loads then
+// issue in one block; a load under `if (k < nz)` compiles to a branch that
+// waits for it before the next load issues.
+__device__ __forceinline__ int row_in(int k, int nz) { return k < nz ? k : nz - 1; }
+
 template <class T, int R>
 __device__ __forceinline__ void load_col(const T* __restrict__ a, int64_t base, int lane, int nz,
                                          T (&v)[R]) {
 #pragma unroll
   for (int m = 0; m < R; ++m) {
     const int k = lane + kWave * m;
-    v[m] = k < nz ? a[base + k] : T(0);
+    const T t = a[base + row_in(k, nz)];
+    v[m] = k < nz ? t : T(0);
   }
 }
 
@@ -169,7 +176,8 @@ __device__ __forceinline__ void apply_precond(int lane, int nz, int64_t base, co
 #pragma unroll
   for (int m = 0; m < R; ++m) {
     const int k = lane + kWave * m;
-    iv[m] = k < nz ? invd[base + k] : T(0);
+    const T t = invd[base + row_in(k, nz)];
+    iv[m] = k < nz ? t : T(0);
   }
   if constexpr (!LINE) {
 #pragma unroll
@@ -180,7 +188,8 @@ __device__ __forceinline__ void apply_precond(int lane, int nz, int64_t base, co
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const int k = lane + kWave * m;
-      wv[m] = k < nz ? w[base + k] : T(0);
+      const T t = w[base + row_in(k, nz)];
+      wv[m] = k < nz ? t : T(0);
     }
     shift_rows<T, R>(wv, lane, dummy, wPrev);
     T y[R];
@@ -404,8 +413,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, March mj, Stencil<T>
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         const int k = lane + kWave * m;
-        pv[m] = T(0);
-        if (k < g.nz) pv[m] = RESTART ? z[base + k] : z[base + k] + beta * pOld[base + k];
+        const int64_t cc = base + row_in(k, g.nz);
+        const T t = RESTART ? z[cc] : z[cc] + beta * pOld[cc];
+        pv[m] = k < g.nz ? t : T(0);
       }
     };
     T pPrev[R], pCur[R], pNext[R], ayPrev[R];
@@ -423,12 +433,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, March mj, Stencil<T>
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         const int k = lane + kWave * m;
+        const int64_t c = base + row_in(k, g.nz);
+        const T pxp = RESTART ? z[c + sx] : z[c + sx] + beta * pOld[c + sx];
+        const T pxm = RESTART ? z[c - sx] : z[c - sx] + beta * pOld[c - sx];
+        const T qv = A.Ad[c] * pCur[m] + A.Axp[c] * pxp + A.Axp[c - sx] * pxm + ay[m] * pNext[m] +
+                     ayPrev[m] * pPrev[m] + az[m] * pdn[m] + azUp[m] * pup[m];
         if (k < g.nz) {
-          const int64_t c = base + k;
-          const T pxp = RESTART ? z[c + sx] : z[c + sx] + beta * pOld[c + sx];
-          const T pxm = RESTART ? z[c - sx] : z[c - sx] + beta * pOld[c - sx];
-          const T qv = A.Ad[c] * pCur[m] + A.Axp[c] * pxp + A.Axp[c - sx] * pxm + ay[m] * pNext[m] +
-                       ayPrev[m] * pPrev[m] + az[m] * pdn[m] + azUp[m] * pup[m];
           pNew[c] = pCur[m];
           q[c] = qv;
           pq.add(static_cast<double>(pCur[m]) * static_cast<double>(qv));
@@ -481,12 +491,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Grid g, T* __restrict__ x
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const int k = lane + kWave * m;
-      rv[m] = T(0);
-      if (k < g.nz) {
-        const int64_t c = base + k;
-        if (XM == 2) x[c] = (x[c] + alphaPrev * pPrev[c]) + alpha * p[c];
-        rv[m] = r[c] - alpha * q[c];
-      }
+      const int64_t c = base + row_in(k, g.nz);
+      T xNew = T(0);
+      if (XM == 2) xNew = (x[c] + alphaPrev * pPrev[c]) + alpha * p[c];
+      const T rNew = r[c] - alpha * q[c];
+      rv[m] = k < g.nz ? rNew : T(0);
+      if (XM == 2 && k < g.nz) x[c] = xNew;
     }
     apply_precond<T, R, LINE>(lane, g.nz, base, invd, w, rv, zv);
 #pragma unroll
@@ -599,7 +609,8 @@ __global__ __launch_bounds__(kBlock) void k_inner_init(Grid g, const double* __r
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const int k = lane + kWave * m;
-      rv[m] = k < g.nz ? static_cast<float>(r64[base + k] * s) : 0.0f;
+      const double t = r64[base + row_in(k, g.nz)];
+      rv[m] = k < g.nz ? static_cast<float>(t * s) : 0.0f;
     }
     apply_precond<float, R, LINE>(lane, g.nz, base, invd, w, rv, zv);
 #pragma unroll
@@ -632,13 +643,10 @@ __device__ __forceinline__ void load_pair(const float* __restrict__ a, int64_t b
 #pragma unroll
   for (int m = 0; m < R2; ++m) {
     const int k0 = 128 * m + 2 * lane;
-    v0[m] = 0.0f;
-    v1[m] = 0.0f;
-    if (k0 < nz) {
-      const float2 t = *reinterpret_cast<const float2*>(a + base + k0);
-      v0[m] = t.x;
-      v1[m] = t.y;
-    }
+    // nz is even in this layout: the last pair (nz - 2, nz - 1) stands in
+    const float2 t = *reinterpret_cast<const float2*>(a + base + (k0 < nz ? k0 : nz - 2));
+    v0[m] = k0 < nz ? t.x : 0.0f;
+    v1[m] = k0 < nz ? t.y : 0.0f;
   }
 }
 
@@ -932,14 +940,10 @@ __global__ __launch_bounds__(kBlock) void k_inner_init_p(Grid g, const double* _
 #pragma unroll
     for (int m = 0; m < R2; ++m) {
       const int k0 = 128 * m + 2 * lane;
-      r0[m] = 0.0f;
-      r1[m] = 0.0f;
       zero[m] = 0.0f;
-      if (k0 < nz) {
-        const double2 t = *reinterpret_cast<const double2*>(r64 + base + k0);
-        r0[m] = static_cast<float>(t.x * s);
-        r1[m] = static_cast<float>(t.y * s);
-      }
+      const double2 t = *reinterpret_cast<const double2*>(r64 + base + (k0 < nz ? k0 : nz - 2));
+      r0[m] = k0 < nz ? static_cast<float>(t.x * s) : 0.0f;
+      r1[m] = k0 < nz ? static_cast<float>(t.y * s) : 0.0f;
     }
     float z0[R2], z1[R2];
     precond_pair<R2, LINE>(lane, nz, base, invd, w, r0, r1, z0, z1);
