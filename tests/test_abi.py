@@ -83,3 +83,12 @@ def test_missing_device_is_an_error_not_a_fallback():
         assert e.code == abi.FREHG_EDEVICE
     else:
         pytest.skip("a GPU is visible")
+
+
+@pytest.mark.gpu
+def test_hbm_probe_measures_both_streams():
+    from frehg2_amd import hbm
+    print("[abi] hbm probe n=4M", flush=True)
+    r = hbm.measure(4 * 1024 * 1024, 5)
+    assert r["copy_gbs"] > 100.0 and r["stream_6r2w_gbs"] > 100.0, r
+    assert r["copy_gbs"] < 2 * r["peak_gbs"] and r["stream_6r2w_gbs"] < 2 * r["peak_gbs"], r
