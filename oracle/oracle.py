@@ -149,6 +149,12 @@ class OracleRichards:
     def coupling_state(self, name) -> np.ndarray:
         return self._coupling[name].copy()
 
+    def set_coupling_state(self, name, values) -> None:
+        """Overwrite one attached coupling array in place (the oracle reads it
+        through the pointer it was given)."""
+        self._coupling[name][...] = np.asarray(values, dtype=np.float64).reshape(
+            self._coupling[name].shape)
+
     def step(self, dt=None, t=None):
         if dt is None:
             dt = self.next_dt()
