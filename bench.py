@@ -71,6 +71,8 @@ def parse():
                     help="run the coupled-ensemble workload instead (64 members of 128x128x64)")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--member-shape", default="128,128,64")
+    ap.add_argument("--columns", action="store_true",
+                    help="run the uneven-columns workload instead (4096 columns of 50-400 cells)")
     ap.add_argument("--ens-rtol", type=float, default=0.0,
                     help="ensemble: PCG rtol of every leg (0: the workload's 1e-10)")
     return ap.parse_args()
@@ -440,6 +442,73 @@ def ensemble_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def columns_bench(args):
+    """4096 independent columns of 50-400 cells (lateral Ks = 0, beds lowered
+    per column): FP64 and mixed forms from the same state, the adaptive
+    controller's dt sequence of the FP64 run replayed by the mixed one."""
+    from frehg2_amd import cases
+    from frehg2_amd.gw import RichardsSolver
+    legs = {}
+    dts = []
+    for form in ("fp64", "mixed"):
+        c = cases.columns_4096()
+        c.precision = form
+        c.rtol = 1e-10
+        g = RichardsSolver(c)
+        n = g.active_cells()
+
+        def one(s, timed):
+            for which, vals in c.bc_schedule(s).items():
+                g.set_bc_values(which, vals)
+            if form == "fp64":
+                dt = g.next_dt()
+                if timed:
+                    dts.append(dt)
+            else:
+                dt = dts[s - args.warmup] if timed else warm[s]
+            g.step(dt)
+            return g.last_solve()["iterations"]
+
+        if form == "fp64":
+            warm = []
+            for s in range(args.warmup):
+                warm.append(g.next_dt())
+                for which, vals in c.bc_schedule(s).items():
+                    g.set_bc_values(which, vals)
+                g.step(warm[-1])
+        else:
+            for s in range(args.warmup):
+                one(s, False)
+        g.synchronize()
+        its = 0
+        t0 = time.perf_counter()
+        for s in range(args.warmup, args.warmup + args.steps):
+            its += one(s, True)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        legs[form] = {"ms_per_step": 1e3 * wall / args.steps, "active_cells": n,
+                      "iterations_per_solve": its / args.steps,
+                      "cell_steps_per_s": n * args.steps / wall,
+                      "column_steps_per_s": 4096 * args.steps / wall}
+        log(f"columns {form}: {legs[form]['ms_per_step']:.3f} ms/step, "
+            f"{legs[form]['iterations_per_solve']:.2f} it/solve, {n} active cells")
+        g.close()
+    out = {
+        "metric": "uneven-columns step throughput (4096 columns of 50-400 cells)",
+        "value": legs["fp64"]["column_steps_per_s"],
+        "unit": "column-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": legs["fp64"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": "richards_columns_4096", "grid": [64, 64, 400],
+                   "dt": "adaptive (FP64 run's sequence replayed)", "parallelism": "single GPU"},
+        "legs": legs,
+        "mixed_speedup": legs["fp64"]["ms_per_step"] / legs["mixed"]["ms_per_step"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic():
     p = ROOT / "profiles" / "traffic_pcg_dir_hetero_512x512x100.json"
     if p.exists():
@@ -459,6 +528,9 @@ def main():
         sys.exit(2)
     if args.ensemble:
         ensemble_bench(args)
+        return
+    if args.columns:
+        columns_bench(args)
         return
     # the achievable streaming rate on this device, measured live: a copy and
     # a 6-read/2-write FP64 stream over eight fields of this workload's size

@@ -1554,8 +1554,8 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double 
 }
 
 // Boundary inflow of the step (RichardsSolver.cpp:961-1006): the net volume
-// through the domain's faces, one thread per column (top and bottom faces;
-// edge columns add their lateral faces).
+// through the domain's faces, one thread per column for the top and bottom
+// faces and one per face for the sides.
 __global__ __launch_bounds__(kThreads) void k_boundary_in(Dev d, double dtg) {
   const Grid& g = d.g;
   const int nz = g.nz;
@@ -1569,15 +1569,29 @@ __global__ __launch_bounds__(kThreads) void k_boundary_in(Dev d, double dtg) {
     const int64_t f0 = g.cF(j, i, 0);
     bin.add(-d.qzF[f0 + kt] * dtg);
     bin.add(d.qzF[f0 + nz] * dtg);
-    if (i == 1 || i == g.nx || j == 1 || j == g.ny) {
-      for (int k = kt; k < nz; ++k) {
-        const int64_t c = g.c3(j, i, k);
-        if (i == 1) bin.add(-d.qx[c - nz] * dtg);
-        if (i == g.nx) bin.add(d.qx[c] * dtg);
-        if (j == 1) bin.add(-d.qy[c - sy] * dtg);
-        if (j == g.ny) bin.add(d.qy[c] * dtg);
-      }
+  }
+  // the domain's side faces, one thread per face: west, east, south, north
+  // (a corner column takes both of its sides)
+  const int64_t nSide = (2 * static_cast<int64_t>(g.ny) + 2 * static_cast<int64_t>(g.nx)) * nz;
+  for (Range r = xcd_range(nSide); r.t < r.end; r.t += r.stride) {
+    const int k = static_cast<int>(r.t % nz);
+    const int q = static_cast<int>(r.t / nz);
+    int j, i, side;
+    if (q < g.ny) {
+      side = 0, j = q + 1, i = 1;
+    } else if (q < 2 * g.ny) {
+      side = 1, j = q - g.ny + 1, i = g.nx;
+    } else if (q < 2 * g.ny + g.nx) {
+      side = 2, j = 1, i = q - 2 * g.ny + 1;
+    } else {
+      side = 3, j = g.ny, i = q - 2 * g.ny - g.nx + 1;
     }
+    if (k < d.kt[g.c2(j, i)]) continue;
+    const int64_t c = g.c3(j, i, k);
+    if (side == 0) bin.add(-d.qx[c - nz] * dtg);
+    else if (side == 1) bin.add(d.qx[c] * dtg);
+    else if (side == 2) bin.add(-d.qy[c - sy] * dtg);
+    else bin.add(d.qy[c] * dtg);
   }
   double v[1] = {bin.value()};
   const int sl[1] = {S_BIN};
