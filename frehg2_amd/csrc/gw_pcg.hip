@@ -22,8 +22,9 @@
 // One iteration is two kernels:
 //   dir:    p = z + beta p (beta from the previous update's partial sums,
 //           recomputed at the 4 lateral neighbours), q = A p, partial p.q
-//   update: alpha = rz / p.q, x += alpha p, r -= alpha q, z = M^-1 r,
-//           partial r.z and r.r
+//   update: alpha = rz / p.q, r -= alpha q, z = M^-1 r, partial r.z and
+//           r.r; x takes alpha p one iteration late, two terms at a time
+//           (k_flush_x adds a last deferred term)
 // Each kernel reduces the other's per-block partials redundantly in every
 // block, in a fixed order, so the scalars are deterministic and no host
 // round trip sits inside the iteration.  Block 0 mirrors the scalars to a

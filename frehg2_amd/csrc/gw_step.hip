@@ -7,17 +7,22 @@
 //   k_ghosts            Predictor.cpp:757-817, Corrector.cpp:709-756
 //   k_classify_coupled  Corrector.cpp:55-118 <PC=true>
 //   k_cell_k            VanGenuchten.hpp:99-111, once per cell and phase
-//   k_face_x / _y / _z  Predictor.cpp:105-386 <PC=true,HARM>; with corr = 1
-//                       also the Darcy fluxes of Corrector.cpp:135-533
-//   k_assemble          Predictor.cpp:400-698 <PC=true> into stencil storage
+//   k_assemble          Predictor.cpp:105-698 <PC=true>: the predictor's face
+//                       K (face_k_lat / face_k_z rules) and the assembly into
+//                       stencil storage (FP64, and FP32 for the mixed form)
 //   (PCG)               gw_pcg.hip, replacing LinearSystem::solve
 //   k_scatter           Predictor.cpp:745-754
+//   k_face_lat / _z     Predictor.cpp:127-386 and the Darcy fluxes of
+//                       Corrector.cpp:135-533 (corrector)
 //   k_coupled_top       Corrector.cpp:539-627
 //   k_update_wc         Corrector.cpp:629-672
 //   k_realloc_classify  Reallocate.cpp:181-391 (saturated_contact: head)
-//   k_realloc_walk      Reallocate.cpp:410-540, Corrector.cpp:758-792,
-//                       RichardsSolver.cpp:961-1006, AdaptiveStep.cpp:59-98
-//                       (one wave per column)
+//   k_realloc_walk      Reallocate.cpp:410-540 and Corrector.cpp:758-792, one
+//                       wave per column, the walks done in parallel where the
+//                       result is the serial walk's (walk_parallel)
+//   k_realloc_walk_serial  the serial walk for the columns left over
+//   k_boundary_in       RichardsSolver.cpp:961-1006
+//   k_adapt_inputs      AdaptiveStep.cpp:59-98
 // Density / viscosity ratios are 1 (no transport module): their factors are
 // omitted, which is exact.
 #include <algorithm>
