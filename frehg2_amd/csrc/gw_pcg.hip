@@ -10,7 +10,7 @@
 // l holding rows k = l + 64 m (m < R = ceil(nz / 64)), so every load and store
 // along k is coalesced, k +- 1 neighbours come from lane shuffles and the
 // z-line (tridiagonal) preconditioner is two wave-level affine scans in
-// registers.  Kernels without lateral neighbours split the columns into 8
+// registers (DPP lane moves).  Kernels without lateral neighbours split the columns into 8
 // contiguous ranges, one per XCD.  Stencil kernels use the j-march mapping: a
 // wave owns one i and a slab of S consecutive j, walks it in j and keeps the
 // j - 1 / j / j + 1 columns in registers, so each column is fetched once; the
@@ -168,8 +168,8 @@ __device__ __forceinline__ void shift_rows(const T (&v)[R], int lane, T (&below)
 // invd_k = 1 / delta_k and w_k = Azp_k / delta_k:
 //   forward  y_k = r_k - w_{k-1} y_{k-1}
 //   backward z_k = y_k invd_k - w_k z_{k+1}
-// Both recurrences are affine, y_k = a_k + b_k y_{k-1}, and run as
-// Hillis-Steele scans over the 64 lanes of each row segment.
+// Both recurrences are affine, y_k = a_k + b_k y_{k-1}, and run as DPP
+// scans over the 64 lanes of each row segment.
 template <class T, int R, bool LINE>
 __device__ __forceinline__ void apply_precond(int lane, int nz, int64_t base, const T* __restrict__ invd,
                                               const T* __restrict__ w, const T (&r)[R], T (&z)[R]) {
@@ -196,38 +196,12 @@ __device__ __forceinline__ void apply_precond(int lane, int nz, int64_t base, co
     T y[R];
     T carry = T(0);
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-      T a = r[m];
-      T b = -wPrev[m];
-#pragma unroll
-      for (int s = 1; s < kWave; s <<= 1) {
-        const T ap = __shfl_up(a, s, kWave);
-        const T bp = __shfl_up(b, s, kWave);
-        if (lane >= s) {
-          a = a + b * ap;
-          b = b * bp;
-        }
-      }
-      y[m] = a + b * carry;
-      carry = __shfl(y[m], kWave - 1, kWave);
-    }
+    for (int m = 0; m < R; ++m) y[m] = scan_affine_up(r[m], -wPrev[m], carry, lane);
     carry = T(0);
 #pragma unroll
     for (int m = R - 1; m >= 0; --m) {
       const int k = lane + kWave * m;
-      T a = k < nz ? y[m] * iv[m] : T(0);
-      T b = k < nz ? -wv[m] : T(0);
-#pragma unroll
-      for (int s = 1; s < kWave; s <<= 1) {
-        const T an = __shfl_down(a, s, kWave);
-        const T bn = __shfl_down(b, s, kWave);
-        if (lane + s < kWave) {
-          a = a + b * an;
-          b = b * bn;
-        }
-      }
-      z[m] = a + b * carry;
-      carry = __shfl(z[m], 0, kWave);
+      z[m] = scan_affine_down(k < nz ? y[m] * iv[m] : T(0), k < nz ? -wv[m] : T(0), carry, lane);
     }
   }
 }
@@ -703,22 +677,11 @@ __device__ __forceinline__ void precond_pair(int lane, int nz, int64_t base,
       // y_k = r_k - w_{k-1} y_{k-1}: rows k0 then k1, composed per lane
       const float a0 = r0[m], b0 = -wUp[m];
       const float a1 = r1[m], b1 = -w0[m];
-      float A = a1 + b1 * a0, B = b1 * b0;
-#pragma unroll
-      for (int sft = 1; sft < kWave; sft <<= 1) {
-        const float Ap = __shfl_up(A, sft, kWave);
-        const float Bp = __shfl_up(B, sft, kWave);
-        if (lane >= sft) {
-          A = A + B * Ap;
-          B = B * Bp;
-        }
-      }
-      const float Y1 = A + B * carry;
-      float P = __shfl_up(Y1, 1, kWave);
-      if (lane == 0) P = carry;
+      const float cin = carry;
+      const float Y1 = scan_affine_up(a1 + b1 * a0, b1 * b0, carry, lane);
+      const float P = dpp_move<kDppWaveShr1>(cin, Y1);  // row k0 - 1
       y0[m] = a0 + b0 * P;
       y1[m] = Y1;
-      carry = __shfl(Y1, kWave - 1, kWave);
     }
     carry = 0.0f;
 #pragma unroll
@@ -726,22 +689,11 @@ __device__ __forceinline__ void precond_pair(int lane, int nz, int64_t base,
       // z_k = y_k invd_k - w_k z_{k+1}: rows k1 then k0, composed per lane
       const float a0 = y0[m] * iv0[m], b0 = -w0[m];
       const float a1 = y1[m] * iv1[m], b1 = -w1[m];
-      float A = a0 + b0 * a1, B = b0 * b1;
-#pragma unroll
-      for (int sft = 1; sft < kWave; sft <<= 1) {
-        const float An = __shfl_down(A, sft, kWave);
-        const float Bn = __shfl_down(B, sft, kWave);
-        if (lane + sft < kWave) {
-          A = A + B * An;
-          B = B * Bn;
-        }
-      }
-      const float Z0 = A + B * carry;
-      float Nx = __shfl_down(Z0, 1, kWave);
-      if (lane == kWave - 1) Nx = carry;
+      const float cin = carry;
+      const float Z0 = scan_affine_down(a0 + b0 * a1, b0 * b1, carry, lane);
+      const float Nx = dpp_move<kDppWaveShl1>(cin, Z0);  // row k1 + 1
       z0[m] = Z0;
       z1[m] = a1 + b1 * Nx;
-      carry = __shfl(Z0, 0, kWave);
     }
   }
 }
