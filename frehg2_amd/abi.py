@@ -123,6 +123,15 @@ SYMBOLS = {
 }
 
 
+# The coupling window (include/frehg_coupler.h; src/coupling/Exchange.cpp).
+COUPLER_SYMBOLS = {
+    "frehg_cpl_begin_window": ([_vp], C.c_int),
+    "frehg_cpl_apply_seepage": ([_vp, _vp, C.c_double, _vp, _dp, _dp], C.c_int),
+    "frehg_cpl_reduce_audit": ([_vp, _vp, _dp, _dp, _dp], C.c_int),
+    "frehg_cpl_refresh_depth": ([_vp, _vp], C.c_int),
+}
+
+
 # The achievable-bandwidth probe (include/frehg_probe.h; no reference counterpart).
 PROBE_SYMBOLS = {
     "frehg_hbm_probe": ([C.c_int64, C.c_int, _dp, _dp], C.c_int),
@@ -149,7 +158,7 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
         raise FrehgError(FREHG_EDEVICE,
                          f"HIP library {p} is not built; run __graft_entry__.build() first")
     lib = C.CDLL(str(p))
-    for name, (args, res) in {**SYMBOLS, **PROBE_SYMBOLS}.items():
+    for name, (args, res) in {**SYMBOLS, **COUPLER_SYMBOLS, **PROBE_SYMBOLS}.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res

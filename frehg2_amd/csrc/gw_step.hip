@@ -23,6 +23,9 @@
 //   k_realloc_walk_serial  the serial walk for the columns left over
 //   k_boundary_in       RichardsSolver.cpp:961-1006
 //   k_adapt_inputs      AdaptiveStep.cpp:59-98
+//   k_cpl_*             the coupling window around the step (include/
+//                       frehg_coupler.h): src/coupling/Exchange.cpp:21-128
+//                       and the surface depth rule src/swe/WetDry.cpp:24-43
 // Density / viscosity ratios are 1 (no transport module): their factors are
 // omitted, which is exact.
 #include <algorithm>
@@ -34,13 +37,14 @@
 #include <string>
 #include <vector>
 
+#include "../../include/frehg_coupler.h"
 #include "../../include/frehg_gw.h"
 #include "gw_device.h"
 #include "gw_pcg.h"
 
 namespace frehg {
 
-// The last error message of this thread, shared by both seams.
+// The last error message of this thread, shared by every entry point.
 thread_local std::string g_error;
 void set_error_message(const char* m) { g_error = m; }
 
@@ -65,9 +69,12 @@ constexpr int kThreads = 256;
 // Reduction slots of one step (each filled by kRb per-block partials).
 enum Slot {
   S_SS = 0, S_ADJ, S_DISC, S_LOST, S_VENT, S_THIN, S_VLOSS, S_BIN, S_DQMAX, S_COMIN,
-  S_VOL, S_CEX, S_CBO, S_CEV, S_CLAMP, S_COUNT
+  S_VOL, S_CEX, S_CBO, S_CEV, S_CLAMP,
+  S_XAPP, S_XDISC, S_XGAIN, S_XHELD, S_XSURF,  // coupling window (Exchange.cpp)
+  S_COUNT
 };
-__constant__ int kSlotOp[S_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 2, 0, 0, 0, 0, 0};
+__constant__ int kSlotOp[S_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 2, 0, 0, 0, 0, 0,
+                                     0, 0, 0, 0, 0};
 
 struct Dev {
   Grid g;
@@ -809,6 +816,95 @@ __global__ void k_coupled_top(Dev d, double dtg) {
   }
   double v[3] = {ex.value(), bo.value(), ev.value()};
   const int s[3] = {S_CEX, S_CBO, S_CEV};
+  put_parts<3>(d, s, v);
+}
+
+// ---- coupling window (src/coupling/Exchange.cpp) ---------------------------
+// The gw side of Coupler::step around the groundwater window, on the attached
+// 2D coupling arrays.  (The held-seepage salt snapshot of Exchange.cpp:27-39
+// and :75-79 belongs to transport and is not built.)
+
+// Coupler::beginWindow (Exchange.cpp:21-52): avail = dept and gain = 0 on the
+// interior, the bounce share cleared over the whole plane.
+__global__ void k_cpl_begin_window(Dev d) {
+  const Grid& g = d.g;
+  const int64_t n = static_cast<int64_t>(g.ny2) * g.nx2;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int j = static_cast<int>(t / g.nx2), i = static_cast<int>(t % g.nx2);
+    if (j >= 1 && j <= g.ny && i >= 1 && i <= g.nx) {
+      d.avail[t] = d.dept[t];
+      d.gain[t] = 0.0;
+    }
+    if (d.bounce) d.bounce[t] = 0.0;
+  }
+}
+
+// SurfaceSolver::updateDepth's cell-centre depth (src/swe/WetDry.cpp:24-43)
+// over the whole plane: dept = eta - bottom, 0 where that is <= min_depth.
+__global__ void k_cpl_refresh_depth(Dev d, const double* __restrict__ bottom) {
+  const int64_t n = static_cast<int64_t>(d.g.ny2) * d.g.nx2;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double v = d.eta[t] - bottom[t];
+    if (v <= d.minDepth) v = 0.0;
+    d.dept[t] = v;
+  }
+}
+
+// Coupler::applySeepage (Exchange.cpp:54-110): the accumulated exchange depth
+// a = seep_accum goes onto eta.  Infiltration (a < 0) applies in full down to
+// the bottom (the clamp's remainder is discarded and booked); seepage applies
+// onto a wet surface or once it exceeds min_depth, else it is held.  qss
+// (may be null) = a / dt.  Partials: applied and discarded depths.
+__global__ void k_cpl_apply_seepage(Dev d, const double* __restrict__ bottom, double dt,
+                                    double* __restrict__ qss) {
+  const Grid& g = d.g;
+  KSum ap, dc;
+  for (Range r = xcd_range(g.columns()); r.t < r.end; r.t += r.stride) {
+    const int j = static_cast<int>(r.t / g.nx) + 1, i = static_cast<int>(r.t % g.nx) + 1;
+    const int c2 = g.c2(j, i);
+    const double a = d.seep[c2];
+    if (qss) qss[c2] = a / dt;
+    if (a < 0.0) {
+      const double before = d.eta[c2];
+      double e = before + a;
+      if (e < bottom[c2]) e = bottom[c2];
+      d.eta[c2] = e;
+      const double change = e - before;
+      d.gain[c2] += change;
+      ap.add(change);
+      dc.add(change - a);
+      d.seep[c2] = 0.0;
+    } else if (d.dept[c2] > d.minDepth || a > d.minDepth) {
+      d.eta[c2] += a;
+      d.gain[c2] += a;
+      ap.add(a);
+      d.seep[c2] = 0.0;
+    }
+  }
+  double v[2] = {ap.value(), dc.value()};
+  const int s[2] = {S_XAPP, S_XDISC};
+  put_parts<2>(d, s, v);
+}
+
+// Coupler::reduceAudit (Exchange.cpp:112-128): sums of gain and of the held
+// accumulator over the interior; with bottom, also the depth of a surface
+// at rest, sum(eta - bottom) (SurfaceSolver::ownedVolume,
+// src/swe/SurfaceSolver.cpp:831-844, counted from eta so that films below
+// min_depth are included).
+__global__ void k_cpl_reduce_audit(Dev d, const double* __restrict__ bottom) {
+  const Grid& g = d.g;
+  KSum ga, he, sv;
+  for (Range r = xcd_range(g.columns()); r.t < r.end; r.t += r.stride) {
+    const int j = static_cast<int>(r.t / g.nx) + 1, i = static_cast<int>(r.t % g.nx) + 1;
+    const int c2 = g.c2(j, i);
+    ga.add(d.gain[c2]);
+    he.add(d.seep[c2]);
+    if (bottom) sv.add(d.eta[c2] - bottom[c2]);
+  }
+  double v[3] = {ga.value(), he.value(), sv.value()};
+  const int s[3] = {S_XGAIN, S_XHELD, S_XSURF};
   put_parts<3>(d, s, v);
 }
 
@@ -1822,6 +1918,49 @@ struct GwSolver {
     reduceSlots();
     return slotsHost[S_VOL];
   }
+  // Coupling window (Exchange.cpp), on the attached arrays.
+  void needCoupling(const char* who) const {
+    if (!d.coupled) fail(FREHG_EINVAL, std::string(who) + ": no coupling attached");
+  }
+  int planeBlocks() const {
+    const int64_t n = static_cast<int64_t>(d.g.ny2) * d.g.nx2;
+    return static_cast<int>(std::min<int64_t>(1024, (n + kThreads - 1) / kThreads));
+  }
+  void beginWindow() {
+    needCoupling("frehg_cpl_begin_window");
+    k_cpl_begin_window<<<planeBlocks(), kThreads, 0, stream>>>(d);
+    HIP_CHECK(hipGetLastError());
+  }
+  void refreshDepth(const double* bottom) {
+    needCoupling("frehg_cpl_refresh_depth");
+    if (!bottom) fail(FREHG_EINVAL, "frehg_cpl_refresh_depth: bottom is required");
+    k_cpl_refresh_depth<<<planeBlocks(), kThreads, 0, stream>>>(d, bottom);
+    HIP_CHECK(hipGetLastError());
+  }
+  void applySeepage(const double* bottom, double dt, double* qss, double* applied,
+                    double* discarded) {
+    needCoupling("frehg_cpl_apply_seepage");
+    if (!bottom || !(dt > 0.0) || !applied || !discarded)
+      fail(FREHG_EINVAL, "frehg_cpl_apply_seepage: bottom, dt > 0 and outputs are required");
+    k_reset_parts<<<64, kThreads, 0, stream>>>(d.parts, rb);
+    k_cpl_apply_seepage<<<rb, kThreads, 0, stream>>>(d, bottom, dt, qss);
+    HIP_CHECK(hipGetLastError());
+    reduceSlots();
+    const double cell = d.g.dx * d.g.dy;
+    *applied = slotsHost[S_XAPP] * cell;
+    *discarded = slotsHost[S_XDISC] * cell;
+  }
+  void reduceAudit(const double* bottom, double* gain, double* held, double* surface) {
+    needCoupling("frehg_cpl_reduce_audit");
+    k_reset_parts<<<64, kThreads, 0, stream>>>(d.parts, rb);
+    k_cpl_reduce_audit<<<rb, kThreads, 0, stream>>>(d, bottom);
+    HIP_CHECK(hipGetLastError());
+    reduceSlots();
+    const double cell = d.g.dx * d.g.dy;
+    if (gain) *gain = slotsHost[S_XGAIN] * cell;
+    if (held) *held = slotsHost[S_XHELD] * cell;
+    if (surface) *surface = slotsHost[S_XSURF] * cell;
+  }
   double* fieldPtr(int f, int64_t* n);
 };
 
@@ -2504,6 +2643,25 @@ int frehg_gw_attach_coupling(frehg_gw_solver* s, double* eta, double* dept, doub
     g->d.minDepth = min_depth;
     g->d.ventAll = vent_all ? 1 : 0;
     g->d.coupled = 1;
+  });
+}
+int frehg_cpl_begin_window(frehg_gw_solver* s) {
+  return guard([&] { frehg::S(s)->beginWindow(); });
+}
+int frehg_cpl_refresh_depth(frehg_gw_solver* s, const double* bottom) {
+  return guard([&] { frehg::S(s)->refreshDepth(bottom); });
+}
+int frehg_cpl_apply_seepage(frehg_gw_solver* s, const double* bottom, double dt, double* qss,
+                            double* applied, double* discarded) {
+  return guard([&] { frehg::S(s)->applySeepage(bottom, dt, qss, applied, discarded); });
+}
+int frehg_cpl_reduce_audit(frehg_gw_solver* s, const double* bottom, double* surface_gain,
+                           double* accum_volume, double* surface_volume) {
+  return guard([&] {
+    if (!surface_gain || !accum_volume || (surface_volume && !bottom))
+      frehg::fail(FREHG_EINVAL, "frehg_cpl_reduce_audit: outputs (and bottom for the volume) "
+                                "are required");
+    frehg::S(s)->reduceAudit(bottom, surface_gain, accum_volume, surface_volume);
   });
 }
 int frehg_gw_set_timing(frehg_gw_solver* s, int enable) {

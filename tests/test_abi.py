@@ -14,6 +14,7 @@ HEADER = ROOT / "include" / "frehg_gw.h"
 
 
 PROBE_HEADER = ROOT / "include" / "frehg_probe.h"
+COUPLER_HEADER = ROOT / "include" / "frehg_coupler.h"
 
 
 def declared_functions(header=HEADER):
@@ -32,13 +33,31 @@ def test_header_declares_the_seams():
 
 def test_library_exports_every_declared_symbol():
     lib = abi.load_library()
-    for name in declared_functions() + declared_functions(PROBE_HEADER):
+    for name in (declared_functions() + declared_functions(PROBE_HEADER) +
+                 declared_functions(COUPLER_HEADER)):
         assert hasattr(lib, name), name
     assert lib.frehg_abi_version() == abi.ABI_VERSION
 
 
 def test_probe_header_matches_its_ctypes_table():
     assert set(declared_functions(PROBE_HEADER)) == set(abi.PROBE_SYMBOLS)
+
+
+def test_coupler_header_matches_its_ctypes_table():
+    assert set(declared_functions(COUPLER_HEADER)) == set(abi.COUPLER_SYMBOLS)
+
+
+def test_coupler_header_compiles_as_c(tmp_path):
+    prog = tmp_path / "cpl.c"
+    prog.write_text(f'#include "{COUPLER_HEADER}"\nint main(void){{return 0;}}\n')
+    subprocess.run(["gcc", "-Wall", "-Werror", "-o", str(tmp_path / "cpl"), str(prog)], check=True)
+
+
+def test_coupler_calls_without_coupling_are_rejected():
+    """A null solver handle or a solver without coupling is an error code
+    (no device work is launched)."""
+    lib = abi.load_library()
+    assert lib.frehg_cpl_begin_window(None) != 0
 
 
 def test_probe_rejects_bad_arguments_without_a_device():

@@ -1,0 +1,299 @@
+"""The coupled step around the groundwater window (frehg2_amd.coupler, the
+kernels of include/frehg_coupler.h) against the reference's coupler tests,
+tests/unit/test_coupling.cpp:240-440, whose miniature configurations have a
+quiescent surface (flat bed, uniform eta, no rain): there a surface at rest
+is the reference's surface.
+
+CPU tests drive the oracle with the numpy restatement of the window
+(oracle/coupler.py) and check the reference's assertions: the per-step
+coupled closure at 1e-10 of the total volume, the exchange identities, the
+dry-cell seepage hold, bounce-back, the reallocation vent and the
+evaporation correction.  GPU tests run the same sequences through the HIP
+library: every scenario against the reference's assertions, and the end
+state (surface arrays, theta, h, window audit) against the oracle's on the
+scenarios whose columns stay below theta_s.  (The held-seepage salt path and
+the stage-boundary cases, :462-520, need transport or a live surface and are
+not covered.)
+"""
+import numpy as np
+import pytest
+
+from frehg2_amd import abi
+from frehg2_amd.coupler import Coupler, closure_residual
+from helpers import gw_base_case
+
+
+def coupled_case(nx, nz, dz, dt, eta=0.0, min_depth=1.0e-6, ks=1.0e-5, **kw):
+    """coupledBaseConfig (test_coupling.cpp:160-195): nx columns of nz cells
+    under a flat bed at 0, fixed lockstep dt, the soft test soil, moisture
+    0.2, a surface at eta."""
+    c = gw_base_case(nx, 1, nz, dz, dt, ks=ks, **kw)
+    shape = (c.ny + 2, c.nx + 2)
+    e = np.full(shape, float(eta))
+    dept = np.maximum(e, 0.0)
+    dept[dept <= min_depth] = 0.0
+    c.coupling = dict(eta=e, dept=dept, seep_accum=np.zeros(shape), avail=np.zeros(shape),
+                      gain=np.zeros(shape), bounce=None, min_depth=min_depth)
+    return c
+
+
+class Mini:
+    """MiniCoupled (test_coupling.cpp:36-151) on one backend."""
+
+    def __init__(self, case, backend, mode="sync", dt_init=None):
+        bottom = np.zeros((case.ny + 2, case.nx + 2))
+        if backend == "oracle":
+            from oracle.coupler import OracleExchange
+            from oracle.oracle import OracleRichards
+            self.gw = OracleRichards(case)
+            ex = OracleExchange(self.gw, bottom, case.dx, case.dy, case.coupling["min_depth"])
+        else:
+            from frehg2_amd.gw import RichardsSolver
+            self.gw = RichardsSolver(case)
+            ex = None
+        self.cpl = Coupler(self.gw, bottom, dt=case.dt, mode=mode, dt_init=dt_init, exchange=ex)
+        self.t = 0.0
+
+    def step(self):
+        dt = self.cpl.next_dt()
+        self.t += dt
+        self.cpl.step(self.t, dt)
+        return dt
+
+    def surface_volume(self):
+        return self.cpl.surface_volume()
+
+    def gw_volume(self):
+        return self.gw.volume()
+
+    def state(self):
+        out = {n: self.gw.coupling_state(n) for n in ("eta", "dept", "seep_accum", "avail",
+                                                      "gain")}
+        out["wc"] = self.gw.get(abi.F_WC)
+        out["h"] = self.gw.get(abi.F_HEAD)
+        return out
+
+
+def run_closure(sim, steps, check=None):
+    """Step and assert the closure identity each step; returns the audits."""
+    v_total = sim.surface_volume() + sim.gw_volume()
+    vs, vg, acc = sim.surface_volume(), sim.gw_volume(), 0.0
+    audits = []
+    for n in range(steps):
+        sim.step()
+        a = sim.cpl.audit()
+        res = closure_residual(a, sim.surface_volume() - vs, sim.gw_volume() - vg,
+                               a.accum_volume - acc)
+        assert abs(res) <= 1.0e-10 * v_total, (n, res)
+        if check is not None:
+            check(n, sim, a, vs, acc)
+        audits.append(a)
+        vs, vg, acc = sim.surface_volume(), sim.gw_volume(), a.accum_volume
+    return audits
+
+
+# ---- the scenarios (one function per reference test) ------------------------
+def ponded_infiltration(backend, rtol=1e-8):
+    """PondedInfiltrationClosesMassBudget (:240-272)."""
+    sim = Mini(coupled_case(3, 10, 0.05, 0.5, eta=0.02, rtol=rtol), backend)
+    v_total = sim.surface_volume() + sim.gw_volume()
+    assert sim.surface_volume() > 0.0
+
+    def check(n, sim, a, vs, acc):
+        assert a.gw["cpl_bounce"] == 0.0 and a.gw["cpl_vent"] == 0.0
+        assert abs(a.applied - a.gw["cpl_exchanged"]) <= 1e-12 * v_total
+
+    run_closure(sim, 60, check)
+    assert 0.0 < sim.surface_volume() < 0.06
+    assert sim.gw.get(abi.F_WC)[1, 2, 0] > 0.2
+    return sim
+
+
+def subcycled_window(backend, rtol=1e-8):
+    """SubcycledWindowClosesMassBudget (:274-298)."""
+    c = coupled_case(3, 10, 0.05, 1.0, eta=0.02, rtol=rtol)
+    c.dt_init = c.dt_min = c.dt_max = 0.2
+    sim = Mini(c, backend, mode="subcycled", dt_init=0.2)
+    steps = []
+
+    def check(n, sim, a, vs, acc):
+        assert a.substeps == 5
+        steps.append(sim.cpl.next_dt())
+
+    run_closure(sim, 30, check)
+    assert all(s == 1.0 for s in steps)
+    assert sim.gw.get(abi.F_WC)[1, 2, 0] > 0.2
+    return sim
+
+
+def infiltration_limited(backend, rtol=1e-8):
+    """InfiltrationLimitedToAvailableWater (:300-329)."""
+    sim = Mini(coupled_case(1, 10, 0.05, 0.5, eta=1.0e-3, ks=1.0e-2, rtol=rtol), backend)
+    pond = sim.surface_volume()
+    assert abs(pond - 1.0e-3) <= 1e-12
+    a = run_closure(sim, 1)[0]
+    assert abs(sim.surface_volume()) <= 1e-12
+    assert abs(a.applied + pond) <= 1e-12
+    assert abs(a.gw["cpl_exchanged"] + pond) <= 1e-12
+    assert abs(a.discarded) <= 1e-12
+    return sim
+
+
+def seepage_held(backend, rtol=1e-8):
+    """SeepageHeldOnDrySurfaceUntilMinDepth (:331-372)."""
+    c = coupled_case(1, 5, 0.05, 0.5, min_depth=2.0e-4, ic_const=0.459, rtol=rtol)
+    c.set_bc("bot", abi.BC_HEAD, 2.0)
+    sim = Mini(c, backend)
+    seen = {"held": False, "released": False}
+
+    def check(n, sim, a, vs, acc):
+        if a.accum_volume > 0.0 and sim.surface_volume() == 0.0:
+            seen["held"] = True
+        if sim.surface_volume() > 0.0 and not seen["released"]:
+            seen["released"] = True
+            assert vs + acc > 0.0
+            assert sim.surface_volume() > 2.0e-4
+            raise StopIteration
+
+    try:
+        run_closure(sim, 200, check)
+    except StopIteration:
+        pass
+    assert seen["held"] and seen["released"]
+    return sim
+
+
+def bounce_back(backend, rtol=1e-8):
+    """BounceBackReturnsInjectionOnFullColumn (:374-401)."""
+    c = coupled_case(1, 5, 0.05, 0.5, ic_const=0.4599, rtol=rtol)
+    c.set_bc("top", abi.BC_FLUX, -1.0e-4)
+    sim = Mini(c, backend)
+    audits = run_closure(sim, 5)
+    assert sum(a.gw["cpl_bounce"] for a in audits) > 0.0
+    assert sim.gw.coupling_state("eta")[1, 1] > 0.0
+    return sim
+
+
+def reallocation_vent(backend, rtol=1e-8):
+    """ReallocationVentDepositsOnWetSurface (:403-434)."""
+    c = coupled_case(1, 5, 0.05, 0.5, eta=0.01, ks=1.0e-7, ic_const=0.46, rtol=rtol,
+                     specific_storage=1.0e-4)
+    c.set_bc("bot", abi.BC_FLUX, 1.0e-4)
+    sim = Mini(c, backend)
+    eta0 = sim.gw.coupling_state("eta")[1, 1]
+    audits = run_closure(sim, 5)
+    assert sum(a.gw["cpl_vent"] for a in audits) > 0.0
+    assert sim.gw.coupling_state("eta")[1, 1] > eta0
+    return sim
+
+
+def evaporation_correction(backend, rtol=1e-8):
+    """EvaporationCorrectionKeepsSubsurfaceLossOffTheSurface (:436-460)."""
+    c = coupled_case(1, 5, 0.05, 0.5, ic_const=0.3, rtol=rtol)
+    c.set_bc("top", abi.BC_FLUX, 5.0e-6)
+    sim = Mini(c, backend)
+    vg0 = sim.gw_volume()
+
+    def check(n, sim, a, vs, acc):
+        assert a.gw["cpl_evap"] > 0.0
+        assert sim.surface_volume() == 0.0
+
+    run_closure(sim, 10, check)
+    assert sim.gw_volume() < vg0 + 1e-15
+    return sim
+
+
+SCENARIOS = [ponded_infiltration, subcycled_window, infiltration_limited, seepage_held,
+             bounce_back, reallocation_vent, evaporation_correction]
+# Bounce-back and the vent run columns held at theta = theta_s, where
+# reallocation's over-saturation branch (theta >= theta_s,
+# Reallocate.cpp:205-274) is decided by the last bit of theta: GPU and oracle
+# round the theta update differently and may take different branches from
+# some step on (both books then close).  Those two are compared as far as
+# the reference's assertions go; the others state by state.
+TRAJECTORY = [ponded_infiltration, subcycled_window, infiltration_limited, seepage_held,
+              evaporation_correction]
+
+
+@pytest.mark.parametrize("scenario", SCENARIOS, ids=lambda f: f.__name__)
+def test_oracle_meets_the_reference_coupling_tests(scenario):
+    scenario("oracle")
+
+
+def test_oracle_exchange_rules():
+    """applySeepage's branches on a hand-made accumulator (Exchange.cpp:80-104):
+    infiltration clamped at the bottom (remainder discarded), seepage onto a
+    wet cell, seepage held on a dry cell below min_depth, released above."""
+    from oracle.coupler import OracleExchange
+
+    class Fake:
+        pass
+    g = Fake()
+    z = np.zeros((3, 6))
+    g._coupling = dict(eta=np.array([[0, 0, 0, 0, 0, 0], [0, 0.01, 0.001, 0.0, 0.0, 0], [0] * 6],
+                                    dtype=float),
+                       dept=np.zeros((3, 6)), seep_accum=np.array(
+                           [[0] * 6, [0, -0.02, 0.002, 1e-7, 3e-6, 0], [0] * 6], dtype=float),
+                       avail=z.copy(), gain=z.copy(), bounce=None)
+    ex = OracleExchange(g, np.zeros((3, 6)), 2.0, 3.0, 1.0e-6)
+    ex.refresh_depth()
+    ex.begin_window()
+    applied, discarded = ex.apply_seepage(10.0)
+    c = g._coupling
+    assert c["eta"][1, 1] == 0.0                     # 0.01 - 0.02 clamped at the bed
+    assert c["eta"][1, 2] == pytest.approx(0.003)    # wet: seepage applied
+    assert c["eta"][1, 3] == 0.0 and c["seep_accum"][1, 3] == 1e-7   # dry, held
+    assert c["eta"][1, 4] == pytest.approx(3e-6)     # dry but above min_depth: applied
+    assert applied == pytest.approx((-0.01 + 0.002 + 3e-6) * 6.0)
+    assert discarded == pytest.approx(0.01 * 6.0)
+    assert ex.qss[1, 1] == pytest.approx(-0.002)
+    gain, held, vol = ex.reduce_audit()
+    assert held == pytest.approx(1e-7 * 6.0)
+    assert vol == pytest.approx((0.003 + 3e-6) * 6.0)
+    assert gain == pytest.approx(applied)
+
+
+def _compare(a, b, what, rtol=1e-9):
+    for k in a:
+        x, y = a[k], b[k]
+        scale = max(np.max(np.abs(y)), 1e-300)
+        assert np.max(np.abs(x - y)) <= rtol * scale + 1e-18, (what, k, np.max(np.abs(x - y)),
+                                                               scale)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scenario", SCENARIOS, ids=lambda f: f.__name__)
+def test_gpu_meets_the_reference_coupling_tests(scenario):
+    scenario("gpu")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scenario", TRAJECTORY, ids=lambda f: f.__name__)
+def test_gpu_coupled_step_matches_the_oracle(scenario):
+    """The HIP window and step meet the reference's assertions, and their
+    state after the run matches the oracle's (solves at rtol 1e-12)."""
+    g = scenario("gpu", rtol=1e-12)
+    o = scenario("oracle", rtol=1e-12)
+    assert g.t == o.t
+    _compare(g.state(), o.state(), scenario.__name__)
+    ag, ao = g.cpl.audit(), o.cpl.audit()
+    for k in ("applied", "discarded", "surface_gain", "accum_volume"):
+        assert getattr(ag, k) == pytest.approx(getattr(ao, k), rel=1e-9, abs=1e-15), k
+    assert ag.substeps == ao.substeps
+    assert g.surface_volume() == pytest.approx(o.surface_volume(), rel=1e-9, abs=1e-15)
+
+
+@pytest.mark.gpu
+def test_gpu_seepage_rate_and_restore():
+    """qss = the applied accumulator over dt (Coupler.hpp:111), and restore()
+    sets the subcycled clock (Coupler.cpp:64-67)."""
+    c = coupled_case(3, 10, 0.05, 1.0, eta=0.02)
+    c.dt_init = c.dt_min = c.dt_max = 0.2
+    sim = Mini(c, "gpu", mode="subcycled", dt_init=0.2)
+    sim.step()
+    a = sim.cpl.audit()
+    qss = sim.cpl.seepage_rate()
+    assert np.sum(qss[1:-1, 1:-1]) * 1.0 == pytest.approx(a.applied, rel=1e-12)
+    sim.cpl.restore(5.0, 0.2, 0.1)
+    assert sim.cpl.gw_lag(5.0) == pytest.approx(0.1)
+    assert sim.cpl.current_dtg() == 0.2
