@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gw_math.h"
+
 namespace frehg {
 
 constexpr int kWave = 64;
@@ -39,10 +41,20 @@ struct Grid {
 //   Se^(1/m)      = 1 / (1 + t)             (exact identity)
 //   1 - (1 - Se^(1/m))^m = 1 - (t/(1+t))^m  = -expm1(m log(t/(1+t)))
 // with log(t/(1+t)) = n log(a|h|) - log1p t for t < 1 and -log1p(1/t)
-// otherwise, so no term cancels.  Mualem's K then costs two log-class and
-// three exp-class evaluations instead of five pow(); the results agree with
-// the pow() forms to rounding (and are better conditioned for dry cells,
-// where 1 - (1 - u)^m cancels).
+// otherwise (1/t from t itself), so no term cancels.  Mualem's K then costs
+// two log-class and two exp-class evaluations and a square root instead of
+// five pow(), and C(h) none more:
+//   C = a n m (ts - tr) (a|h|)^(n-1) / (1 + t)^(m+1)
+//     = a n m (ts - tr) (t / (a|h|)) Se / (1 + t).
+// h(theta) = -(1/a) (R^(1/m) - 1)^(1/n), R = (ts - tr)/(theta - tr), is
+// formed as -(1/a) exp(log(expm1(log(R)/m))/n) (no cancellation in
+// R^(1/m) - 1 near saturation).  Logarithms are gw_log / gw_log1p
+// (gw_math.h, a few ulp).  The results agree with the pow() forms to ~1e-14
+// relative (and are better conditioned for dry cells, where 1 - (1 - u)^m
+// cancels).  The K/C pass is bound by its vector instructions on the GPU
+// (~580 a cell and 85 % VALU busy with pow-class library calls): dropping
+// C's two exponentials and one of K's, and the cheaper logarithms, took it
+// from 0.52 to 0.38 ms at 512 x 512 x 100 (K alone 0.43 -> 0.32 ms).
 // ---------------------------------------------------------------------------
 struct VgSoil {
   double a, n, ts, tr, aev;
@@ -54,8 +66,8 @@ __device__ __forceinline__ double vg_theta(const VgSoil& s, double h) {
   if (h > s.aev) {
     wc = s.ts;
   } else {
-    const double t = exp(s.n * log(fabs(s.a * h)));
-    const double se = exp(-m * log1p(t));
+    const double t = exp(s.n * gw_log(fabs(s.a * h)));
+    const double se = exp(-m * gw_log1p(t));
     wc = s.tr + (s.ts - s.tr) * se;
   }
   if (wc > s.ts) wc = s.ts;
@@ -69,21 +81,10 @@ __device__ __forceinline__ double vg_head(const VgSoil& s, double wc) {
   if (wc - s.tr < eps) wc = s.tr + eps;
   if (wc < s.ts) {
     const double ratio = (s.ts - s.tr) / (wc - s.tr);
-    return -(1.0 / s.a) * pow(pow(ratio, 1.0 / m) - 1.0, 1.0 / s.n);
+    const double x = expm1(gw_log(ratio) * (1.0 / m));  // ratio^(1/m) - 1
+    return -(1.0 / s.a) * exp(gw_log(x) * (1.0 / s.n));
   }
   return 0.0;
-}
-
-__device__ __forceinline__ double vg_capacity(const VgSoil& s, double h) {
-  if (h > 0.0) return 0.0;
-  const double m = 1.0 - 1.0 / s.n;
-  const double ah = fabs(s.a * h);
-  const double lah = log(ah);
-  const double p1 = exp((s.n - 1.0) * lah);  // (a|h|)^(n-1)
-  const double t = exp(s.n * lah);           // (a|h|)^n
-  const double top = s.a * s.n * m * (s.ts - s.tr) * p1;
-  const double bot = exp((m + 1.0) * log1p(t));
-  return top / bot;
 }
 
 // The Mualem factors shared by every Ks the cell carries: K(h, ks) =
@@ -94,32 +95,30 @@ struct MualemFactors {
 };
 __device__ __forceinline__ MualemFactors vg_mualem(const VgSoil& s, double h) {
   const double m = 1.0 - 1.0 / s.n;
-  const double nl = s.n * log(fabs(s.a * h));  // log t
+  const double nl = s.n * gw_log(fabs(s.a * h));  // log t
   const double t = exp(nl);
-  const double l1 = log1p(t);
+  const double l1 = gw_log1p(t);
   const double se = exp(-m * l1);
-  const double lr = t < 1.0 ? nl - l1 : -log1p(exp(-nl));  // log(t / (1 + t))
+  const double lr = t < 1.0 ? nl - l1 : -gw_log1p(1.0 / t);  // log(t / (1 + t))
   const double br = -expm1(m * lr);
   return MualemFactors{sqrt(se), br, h > 0.0};
 }
-// Mualem factors and C(h) together: both start from log(a|h|) and
-// log1p((a|h|)^n), so C costs two more exponentials (the same operands, in
-// the same order, as vg_capacity).
+// Mualem factors and C(h) together: C reuses t and Se (no further
+// transcendental).  At h = 0, (a|h|)^(n-1) = 0 (n > 1) and so is C.
 __device__ __forceinline__ MualemFactors vg_mualem_capacity(const VgSoil& s, double h,
                                                             double& cap) {
   const double m = 1.0 - 1.0 / s.n;
-  const double lah = log(fabs(s.a * h));
-  const double nl = s.n * lah;
+  const double ah = fabs(s.a * h);
+  const double nl = s.n * gw_log(ah);
   const double t = exp(nl);
-  const double l1 = log1p(t);
+  const double l1 = gw_log1p(t);
   const double se = exp(-m * l1);
-  const double lr = t < 1.0 ? nl - l1 : -log1p(exp(-nl));
+  const double lr = t < 1.0 ? nl - l1 : -gw_log1p(1.0 / t);
   const double br = -expm1(m * lr);
-  if (h > 0.0) {
+  if (h > 0.0 || !(ah > 0.0)) {
     cap = 0.0;
   } else {
-    const double top = s.a * s.n * m * (s.ts - s.tr) * exp((s.n - 1.0) * lah);
-    cap = top / exp((m + 1.0) * l1);
+    cap = s.a * s.n * m * (s.ts - s.tr) * (t / ah) * (se / (1.0 + t));
   }
   return MualemFactors{sqrt(se), br, h > 0.0};
 }
