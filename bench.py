@@ -322,12 +322,19 @@ def ensemble_accuracy(ens, gm, g64, solos, dt):
 def ensemble_bench(args):
     """Coupled ensemble: members stacked along y behind inactive separator
     rows, advanced by one solver (FP64 and mixed), against the same members
-    run one after another by their own FP64 solvers at the same dt sequence."""
+    run one after another by their own FP64 solvers at the same dt sequence.
+    Every substep is a coupled step in lockstep (frehg2_amd.coupler.Coupler,
+    surface at rest): depth refresh, window start, the groundwater step and
+    the seepage application."""
     from frehg2_amd import abi, cases
+    from frehg2_amd.coupler import Coupler, closure_residual
     from frehg2_amd.ensemble import Ensemble
     from frehg2_amd.gw import RichardsSolver
     nx, ny, nz = (int(v) for v in args.member_shape.split(","))
     M = args.members
+
+    def bottom_of(c):
+        return np.pad(Ensemble._bath(c), 1, mode="edge")
 
     def members(precision):
         out = []
@@ -345,21 +352,27 @@ def ensemble_bench(args):
     log(f"ensemble: {M} members of {nx}x{ny}x{nz} stacked to {ens.case.nx}x{ens.case.ny}x"
         f"{ens.case.nz} in {time.perf_counter() - t0:.1f} s")
     legs = {}
+    bottom = ens.stack_plane([bottom_of(c) for c in ens.members])
+    dt0 = ens.case.dt_init
     # mixed stacked leg: the adaptive controller picks the dt sequence that
     # every other leg replays
     ens.case.precision = "mixed"
     g = RichardsSolver(ens.case)
+    cm = Coupler(g, bottom, dt=dt0, mode="sync")
     warm_dts, dts = [], []
+    t = 0.0
     for _ in range(args.warmup):
-        warm_dts.append(g.next_dt())
-        g.step(warm_dts[-1])
+        warm_dts.append(cm.next_dt())
+        t += warm_dts[-1]
+        cm.step(t, warm_dts[-1])
     g.synchronize()
     g.set_timing(SPMV_SAMPLE)
     its = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        dts.append(g.next_dt())
-        g.step(dts[-1])
+        dts.append(cm.next_dt())
+        t += dts[-1]
+        cm.step(t, dts[-1])
         its += g.last_solve()["iterations"]
     g.synchronize()
     wall = time.perf_counter() - t0
@@ -372,17 +385,36 @@ def ensemble_bench(args):
     gm = g
     log(f"ensemble stacked mixed: {legs['stacked_mixed']['ms_per_substep']:.2f} ms/substep, "
         f"{its / args.steps:.1f} it/solve")
+    # the coupled closure identity (Coupler.hpp:45-56) over three more
+    # substeps of this leg, outside the timed region
+    vs, vg, acc = cm.surface_volume(), g.volume(), cm.audit().accum_volume
+    vtot = vs + vg
+    cpl_closure = 0.0
+    for _ in range(3):
+        dt = cm.next_dt()
+        t += dt
+        cm.step(t, dt)
+        a = cm.audit()
+        res = closure_residual(a, cm.surface_volume() - vs, g.volume() - vg,
+                               a.accum_volume - acc)
+        cpl_closure = max(cpl_closure, abs(res) / vtot)
+        vs, vg, acc = cm.surface_volume(), g.volume(), a.accum_volume
+    log(f"ensemble coupled closure (stacked mixed, relative to volume): {cpl_closure:.1e}")
 
     ens.case.precision = "fp64"
     g = RichardsSolver(ens.case)
+    c64 = Coupler(g, bottom, dt=dt0, mode="sync")
+    t = 0.0
     for dt in warm_dts:
-        g.step(dt)
+        t += dt
+        c64.step(t, dt)
     g.synchronize()
     g.set_timing(SPMV_SAMPLE)
     its = 0
     t0 = time.perf_counter()
     for dt in dts:
-        g.step(dt)
+        t += dt
+        c64.step(t, dt)
         its += g.last_solve()["iterations"]
     g.synchronize()
     wall = time.perf_counter() - t0
@@ -394,16 +426,21 @@ def ensemble_bench(args):
     g64 = g
     log(f"ensemble stacked fp64: {legs['stacked_fp64']['ms_per_substep']:.2f} ms/substep")
 
-    solos = [RichardsSolver(c) for c in members("fp64")]
+    solo_cases = members("fp64")
+    solos = [RichardsSolver(c) for c in solo_cases]
+    couplers = [Coupler(r, bottom_of(c), dt=dt0, mode="sync") for r, c in zip(solos, solo_cases)]
+    t = 0.0
     for dt in warm_dts:
-        for r in solos:
-            r.step(dt)
+        t += dt
+        for cp in couplers:
+            cp.step(t, dt)
     for r in solos:
         r.synchronize()
     t0 = time.perf_counter()
     for dt in dts:
-        for r in solos:
-            r.step(dt)
+        t += dt
+        for cp in couplers:
+            cp.step(t, dt)
     for r in solos:
         r.synchronize()
     wall = time.perf_counter() - t0
@@ -432,11 +469,14 @@ def ensemble_bench(args):
         "data": "synthetic (seeded generator, inputs resident in HBM)",
         "config": {"workload": "richards_coupled_128x128x64 ensemble", "members": M,
                    "member_grid": [nx, ny, nz], "stacked_grid": [ens.case.nx, ens.case.ny, nz],
-                   "dt": "adaptive (common to all members)", "parallelism": "single GPU"},
+                   "dt": "adaptive (common to all members)",
+                   "coupling": "sync coupled step per substep, surface at rest",
+                   "parallelism": "single GPU"},
         "legs": legs,
         "speedup_stacked_mixed_vs_sequential_fp64": seq / legs["stacked_mixed"]["ms_per_substep"],
         "speedup_stacked_fp64_vs_sequential_fp64": seq / legs["stacked_fp64"]["ms_per_substep"],
         "accuracy": accuracy,
+        "coupled_closure_rel_max": cpl_closure,
         "dt_sequence": dts,
     }
     print(json.dumps(out), flush=True)

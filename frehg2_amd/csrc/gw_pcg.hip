@@ -54,7 +54,7 @@ struct PcgState {
 
 constexpr int kBlock = 256;
 constexpr int kWpb = kBlock / kWave;
-constexpr int kChunk = 16;   // rows per LDS tile in the factorization
+constexpr int kChunk = 32;   // rows per LDS tile in the factorization (16: +20 % time)
 constexpr int kPoll = 8;     // iterations between host polls
 
 #define HIP_OK(x)                                                                     \

@@ -155,22 +155,28 @@ class Ensemble:
                 out[which][self.rows(k)] = np.broadcast_to(vals, (self.ny, self.case.nx))
         return out
 
-    def stack_coupling(self, states: List[dict]) -> dict:
-        """Stack members' 2D halo-layout coupling arrays ((ny+2, nx+2) each)."""
+    def stack_plane(self, planes: List[np.ndarray], fill: float = 0.0) -> np.ndarray:
+        """Stack members' 2D halo-layout planes ((ny+2, nx+2) each) into the
+        stacked domain's plane; separator rows take ``fill``, the outer ghost
+        rows the first and last members' ghost rows."""
         nx = self.members[0].nx
         nys = self.count * self.ny + (self.count - 1)
+        a = np.full((nys + 2, nx + 2), float(fill))
+        for k, pl in enumerate(planes):
+            r0 = 1 + k * self.stride
+            a[r0:r0 + self.ny] = np.asarray(pl)[1:-1]
+        a[0] = np.asarray(planes[0])[0]
+        a[-1] = np.asarray(planes[-1])[-1]
+        return a
+
+    def stack_coupling(self, states: List[dict]) -> dict:
+        """Stack members' 2D halo-layout coupling arrays ((ny+2, nx+2) each)."""
         out = {}
         for key in ("eta", "dept", "seep_accum", "avail", "gain", "bounce"):
             if states[0].get(key) is None:
                 out[key] = None
                 continue
-            a = np.zeros((nys + 2, nx + 2))
-            for k, s in enumerate(states):
-                r0 = 1 + k * self.stride
-                a[r0:r0 + self.ny] = s[key][1:-1]
-            a[0] = states[0][key][0]
-            a[-1] = states[-1][key][-1]
-            out[key] = a
+            out[key] = self.stack_plane([s[key] for s in states])
         out["min_depth"] = states[0]["min_depth"]
         out["vent_all"] = states[0]["vent_all"]
         return out

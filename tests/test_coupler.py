@@ -33,7 +33,7 @@ def coupled_case(nx, nz, dz, dt, eta=0.0, min_depth=1.0e-6, ks=1.0e-5, **kw):
     dept = np.maximum(e, 0.0)
     dept[dept <= min_depth] = 0.0
     c.coupling = dict(eta=e, dept=dept, seep_accum=np.zeros(shape), avail=np.zeros(shape),
-                      gain=np.zeros(shape), bounce=None, min_depth=min_depth)
+                      gain=np.zeros(shape), bounce=None, min_depth=min_depth, vent_all=0)
     return c
 
 
@@ -297,3 +297,51 @@ def test_gpu_seepage_rate_and_restore():
     sim.cpl.restore(5.0, 0.2, 0.1)
     assert sim.cpl.gw_lag(5.0) == pytest.approx(0.1)
     assert sim.cpl.current_dtg() == 0.2
+
+
+def _stacked_vs_members(backend):
+    """Two coupled members stacked into one domain (frehg2_amd.ensemble) and
+    advanced by one Coupler match the members advanced by their own
+    Couplers: the window acts column by column, and the separator rows are
+    sealed walls with a dry surface."""
+    from frehg2_amd.ensemble import Ensemble
+    ms = [coupled_case(3, 10, 0.05, 0.5, eta=0.02, rtol=1e-12),
+          coupled_case(3, 10, 0.05, 0.5, eta=0.01, ks=2.0e-5, rtol=1e-12)]
+    ens = Ensemble(ms)
+    bottom = ens.stack_plane([np.zeros((m.ny + 2, m.nx + 2)) for m in ms])
+    if backend == "oracle":
+        from oracle.coupler import OracleExchange
+        from oracle.oracle import OracleRichards
+        make = lambda c, b: (lambda g: (g, Coupler(g, b, dt=0.5, exchange=OracleExchange(
+            g, b, c.dx, c.dy, c.coupling["min_depth"]))))(OracleRichards(c))
+    else:
+        from frehg2_amd.gw import RichardsSolver
+        make = lambda c, b: (lambda g: (g, Coupler(g, b, dt=0.5)))(RichardsSolver(c))
+    gs, cs = make(ens.case, bottom)
+    solo = [make(m, np.zeros((m.ny + 2, m.nx + 2))) for m in ms]
+    t = 0.0
+    for _ in range(8):
+        t += 0.5
+        cs.step(t, 0.5)
+        for _, c in solo:
+            c.step(t, 0.5)
+    for k, (g, c) in enumerate(solo):
+        for fid in (abi.F_HEAD, abi.F_WC):
+            a = ens.member_view(gs.get(fid), k)
+            b = g.get(fid)[1:-1, 1:-1]
+            assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b)), (k, fid)
+        for n in ("eta", "dept", "seep_accum", "gain"):
+            a = ens.member_coupling(gs.coupling_state(n), k)[1:-1, 1:-1]
+            b = g.coupling_state(n)[1:-1, 1:-1]
+            assert np.max(np.abs(a - b)) <= 1e-9 * max(np.max(np.abs(b)), 1e-12), (k, n)
+    total = sum(c.audit().applied for _, c in solo)
+    assert cs.audit().applied == pytest.approx(total, rel=1e-9)
+
+
+def test_oracle_stacked_members_match_standalone_members():
+    _stacked_vs_members("oracle")
+
+
+@pytest.mark.gpu
+def test_gpu_stacked_members_match_standalone_members():
+    _stacked_vs_members("gpu")
