@@ -67,6 +67,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy check")
     ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the PCG launch shapes by timing trial solves before warmup")
+    ap.add_argument("--pcg-shapes", default="",
+                    help="PCG launch shapes as waves per CU: MARCH64,MARCH32,COLUMN (0 keeps)")
     ap.add_argument("--ensemble", action="store_true",
                     help="run the coupled-ensemble workload instead (64 members of 128x128x64)")
     ap.add_argument("--members", type=int, default=64)
@@ -99,6 +103,15 @@ def run_form(args, precision, rtol):
     g.synchronize()
     log(f"{precision}: solver staged in {time.perf_counter() - t0:.1f} s "
         f"({g.active_cells()} active cells), rtol {rtol:g}")
+    tuned = None
+    if args.pcg_shapes:
+        g.set_pcg_shapes(*(int(v) for v in args.pcg_shapes.split(",")))
+    if args.autotune:
+        from frehg2_amd import autotune
+        t1 = time.perf_counter()
+        tuned = autotune.tune(g, dt=case.dt)
+        log(f"{precision}: autotuned PCG shapes {tuned['shapes']} "
+            f"({len(tuned['trials_ms'])} trials, {time.perf_counter() - t1:.1f} s)")
     for _ in range(args.warmup):
         g.step(case.dt)
     g.synchronize()
@@ -132,7 +145,10 @@ def run_form(args, precision, rtol):
         "spmv_launches": tm["spmv_launches"],
         "spmv_bytes": tm["spmv_bytes"],
         "active_cells": n,
+        "pcg_shapes_waves_per_cu": g.pcg_shapes(),
     }
+    if tuned is not None:
+        res["autotune_trials_solve_ms"] = tuned["trials_ms"]
     res["spmv_gbs"] = res["spmv_bytes"] / (res["spmv_avg_ms"] * 1e-3) / 1e9
     log(f"{precision}: {res['ms_per_step']:.2f} ms/step, solve {res['solve_ms_per_step']:.2f} ms, "
         f"{res['iterations_per_solve']:.1f} it/solve, {res['cell_iters_per_s'] / 1e9:.2f} G cell-it/s, "

@@ -108,6 +108,8 @@ SYMBOLS = {
     "frehg_gw_set_timing": ([_vp, C.c_int], C.c_int),
     "frehg_gw_get_timing": ([_vp, C.POINTER(GwTiming)], C.c_int),
     "frehg_gw_synchronize": ([_vp], C.c_int),
+    "frehg_gw_set_pcg_shapes": ([_vp, C.c_int, C.c_int, C.c_int], C.c_int),
+    "frehg_gw_get_pcg_shapes": ([_vp, _ip, _ip, _ip], C.c_int),
     "frehg_device_malloc": ([C.c_int64, C.POINTER(_vp)], C.c_int),
     "frehg_device_free": ([_vp], C.c_int),
     "frehg_memcpy_htod": ([_vp, _vp, C.c_int64], C.c_int),

@@ -70,6 +70,18 @@ class Pcg {
   // Algorithmic bytes of one launch of the stencil kernel (the dominant one).
   double spmvBytesPerLaunch(bool mixed) const;
 
+  // Launch shapes, as waves per CU: the FP64 and FP32 stencil (j-march)
+  // kernels and the column kernels (0 keeps a value).  Defaults 16 / 32 / 16
+  // (the fastest of a sweep at 512 x 512 x 100).  Results depend on the
+  // shapes at rounding level (the order of the partial sums) and are
+  // deterministic for given shapes.
+  void setShapes(int march64WavesPerCu, int march32WavesPerCu, int columnWavesPerCu);
+  void shapes(int* march64WavesPerCu, int* march32WavesPerCu, int* columnWavesPerCu) const {
+    *march64WavesPerCu = wpc64_;
+    *march32WavesPerCu = wpc32_;
+    *columnWavesPerCu = wpcCol_;
+  }
+
  private:
   int solveFp64(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
                 PcgStats* st);
@@ -81,6 +93,7 @@ class Pcg {
                       const March& mj, int nprev0, int n0, bool restart,
                       const PcgOptions& opt, PcgStats* st, int* itDone);
   void reduceHost(int nparts, int nvals, double* out);
+  void applyShapes();
 
   static March makeMarch(const Grid& g, int cus, int wavesPerCu);
 
@@ -89,6 +102,9 @@ class Pcg {
   int blocks_ = 0;            // grid size of the column kernels (multiple of 8)
   March march64_, march32_;   // stencil-kernel shapes, FP64 and FP32 operands
   int pstride_ = 0;           // slot stride of parts_ (largest grid)
+  int partsCap_ = 0;          // slot stride parts_ was allocated for
+  int cus_ = 256;
+  int wpc64_ = 16, wpc32_ = 32, wpcCol_ = 16;  // waves per CU (setShapes)
   bool pairs_ = false;        // FP32 kernels in the row-pair layout (nz even)
   // FP64 vectors (halo layout)
   double *r_ = nullptr, *z_ = nullptr, *p0_ = nullptr, *p1_ = nullptr, *q_ = nullptr;

@@ -994,15 +994,9 @@ Pcg::Pcg(const Grid& g, hipStream_t stream) : g_(g), stream_(stream) {
   int dev = 0;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
     cus = prop.multiProcessorCount;
-  const int64_t ncol = g.columns();
-  const int64_t wavesWanted = std::max<int64_t>(8, std::min<int64_t>(ncol, int64_t(cus) * 16));
-  blocks_ = static_cast<int>(std::max<int64_t>(8, ((wavesWanted + kWpb - 1) / kWpb + 7) / 8 * 8));
-  // Measured at 512 x 512 x 100: FP64 operands run best at ~16 waves per CU
-  // (S = 64), FP32 operands at ~64 (S = 16).
-  march64_ = makeMarch(g, cus, 16);
+  cus_ = cus;
   pairs_ = g.nz % 2 == 0;
-  march32_ = makeMarch(g, cus, 64);
-  pstride_ = std::max({blocks_, march64_.blocks, march32_.blocks});
+  applyShapes();
   const int64_t nH = g.cellsH();
   r_ = dalloc<double>(nH);
   z_ = dalloc<double>(nH);
@@ -1012,6 +1006,7 @@ Pcg::Pcg(const Grid& g, hipStream_t stream) : g_(g), stream_(stream) {
   invd_ = dalloc<double>(nH);
   w_ = dalloc<double>(nH);
   parts_ = dalloc<double>(4 * static_cast<int64_t>(pstride_));
+  partsCap_ = pstride_;
   red_ = dalloc<double>(4);
   state_ = dalloc<PcgState>(1);
   HIP_OK(hipHostMalloc(&redHost_, 4 * sizeof(double), hipHostMallocDefault));
@@ -1036,6 +1031,30 @@ double Pcg::spmvBytesPerLaunch(bool mixed) const {
   // dir kernel per interior cell: z, p_old, Ad, Axp, Ayp, Azp in; p, q out.
   const double per = mixed ? 8.0 * 4.0 : 8.0 * 8.0;
   return per * static_cast<double>(g_.columns()) * g_.nz;
+}
+
+void Pcg::applyShapes() {
+  const int64_t ncol = g_.columns();
+  const int64_t wavesWanted =
+      std::max<int64_t>(8, std::min<int64_t>(ncol, int64_t(cus_) * wpcCol_));
+  blocks_ = static_cast<int>(std::max<int64_t>(8, ((wavesWanted + kWpb - 1) / kWpb + 7) / 8 * 8));
+  march64_ = makeMarch(g_, cus_, wpc64_);
+  march32_ = makeMarch(g_, cus_, wpc32_);
+  pstride_ = std::max({blocks_, march64_.blocks, march32_.blocks});
+  if (parts_ && pstride_ > partsCap_) {
+    HIP_OK(hipStreamSynchronize(stream_));
+    HIP_OK(hipFree(parts_));
+    parts_ = nullptr;
+    parts_ = dalloc<double>(4 * static_cast<int64_t>(pstride_));
+    partsCap_ = pstride_;
+  }
+}
+
+void Pcg::setShapes(int march64WavesPerCu, int march32WavesPerCu, int columnWavesPerCu) {
+  if (march64WavesPerCu > 0) wpc64_ = march64WavesPerCu;
+  if (march32WavesPerCu > 0) wpc32_ = march32WavesPerCu;
+  if (columnWavesPerCu > 0) wpcCol_ = columnWavesPerCu;
+  applyShapes();
 }
 
 March Pcg::makeMarch(const Grid& g, int cus, int wavesPerCu) {

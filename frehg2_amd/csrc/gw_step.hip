@@ -2660,6 +2660,25 @@ int frehg_cpl_reduce_audit(frehg_gw_solver* s, const double* bottom, double* sur
     frehg::S(s)->reduceAudit(bottom, surface_gain, accum_volume, surface_volume);
   });
 }
+int frehg_gw_set_pcg_shapes(frehg_gw_solver* s, int march64_waves_per_cu,
+                            int march32_waves_per_cu, int column_waves_per_cu) {
+  return guard([&] {
+    for (int v : {march64_waves_per_cu, march32_waves_per_cu, column_waves_per_cu})
+      if (v < 0 || v > 256)
+        frehg::fail(FREHG_EINVAL, "frehg_gw_set_pcg_shapes: waves per CU must be in [1, 256] "
+                                  "(0 keeps a value)");
+    GwSolver* g = frehg::S(s);
+    g->pcg->setShapes(march64_waves_per_cu, march32_waves_per_cu, column_waves_per_cu);
+  });
+}
+int frehg_gw_get_pcg_shapes(const frehg_gw_solver* s, int* march64_waves_per_cu,
+                            int* march32_waves_per_cu, int* column_waves_per_cu) {
+  return guard([&] {
+    if (!march64_waves_per_cu || !march32_waves_per_cu || !column_waves_per_cu)
+      frehg::fail(FREHG_EINVAL, "frehg_gw_get_pcg_shapes: outputs are required");
+    frehg::S(s)->pcg->shapes(march64_waves_per_cu, march32_waves_per_cu, column_waves_per_cu);
+  });
+}
 int frehg_gw_set_timing(frehg_gw_solver* s, int enable) {
   return guard([&] {
     GwSolver* g = frehg::S(s);

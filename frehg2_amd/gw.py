@@ -138,6 +138,18 @@ class RichardsSolver:
     def synchronize(self) -> None:
         check(self.lib, self.lib.frehg_gw_synchronize(self._h))
 
+    def set_pcg_shapes(self, march64=0, march32=0, column=0) -> None:
+        """PCG launch shapes in waves per CU (0 keeps a value); see
+        frehg2_amd.autotune."""
+        check(self.lib, self.lib.frehg_gw_set_pcg_shapes(self._h, int(march64), int(march32),
+                                                         int(column)))
+
+    def pcg_shapes(self) -> dict:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(self.lib, self.lib.frehg_gw_get_pcg_shapes(self._h, C.byref(a), C.byref(b),
+                                                         C.byref(c)))
+        return {"march64": a.value, "march32": b.value, "column": c.value}
+
 
 class LinearSystem:
     """The COO solve seam (LinearSystem::setPattern / setValues / solve).

@@ -234,6 +234,17 @@ int frehg_gw_get_timing(frehg_gw_solver* s, frehg_gw_timing* out);
 /* Block until all work on the solver's stream is done. */
 int frehg_gw_synchronize(frehg_gw_solver* s);
 
+/* Launch shapes of the PCG (no counterpart in the reference, where PETSc
+ * makes these choices): waves per CU of the FP64 and FP32 stencil kernels
+ * and of the column kernels, each in [1, 256]; 0 keeps a value.  Defaults
+ * 16 / 32 / 16.  Results depend on the shapes at rounding level (the order
+ * of the partial sums) and are deterministic for given shapes.  The tuner
+ * frehg2_amd.autotune picks them by timing solves. */
+int frehg_gw_set_pcg_shapes(frehg_gw_solver* s, int march64_waves_per_cu,
+                            int march32_waves_per_cu, int column_waves_per_cu);
+int frehg_gw_get_pcg_shapes(const frehg_gw_solver* s, int* march64_waves_per_cu,
+                            int* march32_waves_per_cu, int* column_waves_per_cu);
+
 /* ---- device-memory plumbing -------------------------------------------
  * No counterpart in the reference (its Kokkos views own device memory).
  * They let an FFI caller without its own HIP binding stage the device
