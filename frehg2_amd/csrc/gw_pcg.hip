@@ -208,7 +208,7 @@ __device__ __forceinline__ void apply_precond(int lane, int nz, int64_t base, co
 
 // ---------------------------------------------------------------------------
 // Factorization of the column blocks (once per solve).  One wave owns 64
-// consecutive columns; rows are staged through LDS in 16-row tiles so the
+// consecutive columns; rows are staged through LDS in 32-row tiles so the
 // global loads and stores stay coalesced while each lane runs its column's
 // sequential LDL^T recurrence  delta_k = Ad_k - Azp_{k-1} w_{k-1}.
 // ---------------------------------------------------------------------------
