@@ -80,6 +80,7 @@ class GwCase:
     specific_storage: float = 0.0
     use_full3d: bool = True
     face_conductivity: str = "arithmetic"
+    predictor_corrector: str = "consistent"
     reallocation: str = "legacy"
     reallocation_surplus: str = "drop"
     dt_init: float = 1.0
@@ -154,7 +155,7 @@ class GwCase:
             setattr(d, n, dptr(self.soil_array(n)))
         d.specific_storage = self.specific_storage
         d.use_full3d = int(self.use_full3d)
-        d.predictor_corrector = 1
+        d.predictor_corrector = 0 if self.predictor_corrector == "legacy" else 1
         d.face_conductivity = 1 if self.face_conductivity == "harmonic" else 0
         d.saturated_contact = 1
         d.reallocation = 1 if self.reallocation == "disabled" else 0

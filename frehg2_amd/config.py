@@ -167,8 +167,9 @@ def load_config(src, base_dir=None):
         case.bath_const = bath
     else:
         case.bath = bath
-    if str(gw.get("predictor_corrector", "consistent")) != "consistent":
-        raise ValueError("groundwater.predictor_corrector: only 'consistent' is supported")
+    case.predictor_corrector = str(gw.get("predictor_corrector", "consistent"))
+    if case.predictor_corrector not in ("consistent", "legacy"):
+        raise ValueError("groundwater.predictor_corrector: consistent or legacy")
     if str(gw.get("saturated_contact", "head")) != "head":
         raise ValueError("groundwater.saturated_contact: only 'head' is supported")
     case.use_full3d = bool(gw.get("use_full3d", True))

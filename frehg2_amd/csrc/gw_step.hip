@@ -5,11 +5,12 @@
 //
 // Kernel map (reference file:line each kernel restates):
 //   k_ghosts            Predictor.cpp:757-817, Corrector.cpp:709-756
-//   k_classify_coupled  Corrector.cpp:55-118 <PC=true>
+//   k_classify_coupled  Corrector.cpp:55-118 <PC>
 //   k_cell_k            VanGenuchten.hpp:99-111, once per cell and phase
-//   k_assemble          Predictor.cpp:105-698 <PC=true>: the predictor's face
-//                       K (face_k_lat / face_k_z rules) and the assembly into
-//                       stencil storage (FP64, and FP32 for the mixed form)
+//   k_assemble          Predictor.cpp:105-698 <PC>: the predictor's face K
+//                       (the rules of face_lat_at / face_z_at, specialised
+//                       to an active cell) and the assembly into stencil
+//                       storage (FP64, and FP32 for the mixed form)
 //   (PCG)               gw_pcg.hip, replacing LinearSystem::solve
 //   k_scatter           Predictor.cpp:745-754
 //   k_faces             Predictor.cpp:127-386 and the Darcy fluxes of
@@ -101,6 +102,11 @@ struct Dev {
   int coupled, ventAll;
   double ss, courantMax;
   int full3d, harmonic, reallocOff, redistribute, terrain, guessPrev;
+  // predictor_corrector: legacy, the reference's <PC = false, HARM = false>
+  // pair: a wet coupled top at 0.5 (Ksz + K), no pressurized seepage face
+  // (mode 3), and corrector faces that give both flanks the minus (lateral)
+  // or lower (vertical) cell's Ks
+  int pcLegacy;
   double *Ad, *Axp, *Ayp, *Azp, *rhs, *x;
   float *Ad32, *Axp32, *Ayp32, *Azp32;  // the stencil rounded to FP32 (mixed form), or null
   double* parts;
@@ -218,7 +224,7 @@ __global__ void k_ghosts(Dev d, int doH, int doWc) {
   }
 }
 
-// ---- coupled-top classification (Corrector.cpp:55-118, PC = true) --------
+// ---- coupled-top classification (Corrector.cpp:55-118) -------------------
 __global__ void k_classify_coupled(Dev d, double dtg) {
   const Grid& g = d.g;
   const int n = g.columns();
@@ -228,8 +234,10 @@ __global__ void k_classify_coupled(Dev d, double dtg) {
     const int kt = d.kt[c2] < g.nz ? d.kt[c2] : -1;
     const double dep = d.dept[c2];
     if (kt < 0 || dep <= 0.0) {
-      d.cplMode[c2] =
-          (kt >= 0 && d.h[g.c3(j, i, kt)] > 0.5 * d.dz3d[g.c3(j, i, kt)]) ? 3 : 0;
+      // a pressurized dry top is a seepage face (mode 3), not under legacy
+      d.cplMode[c2] = (!d.pcLegacy && kt >= 0 && d.h[g.c3(j, i, kt)] > 0.5 * d.dz3d[g.c3(j, i, kt)])
+                          ? 3
+                          : 0;
       continue;
     }
     const int64_t c = g.c3(j, i, kt);
@@ -330,6 +338,8 @@ __device__ __forceinline__ void face_lat_at(const Dev& d, int corr, int j, int i
         delta = 0.5 * spacing;
       } else {
         kface = base;
+        if (d.pcLegacy && ks[cn] != ks[c])  // legacy: the minus cell's Ks on both flanks
+          kface = 0.5 * (Kc + vg_conductivity(soil_at(d, cn), hn, ks[c]));
         if (!actA || !actB) kface = 0.0;
       }
       q = A * (kface * cv * (hn - hc) / delta + static_cast<double>(sgn) * kface * sv);
@@ -378,7 +388,8 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
       const int64_t c = cb;
       const double own = Kzb;
       if (d.coupled) {
-        v = d.dept[c2] > 0.0 ? d.ksz[c] : own;
+        const double wet = d.pcLegacy ? 0.5 * (d.ksz[c] + own) : d.ksz[c];
+        v = d.dept[c2] > 0.0 ? wet : own;
       } else {
         const int code = d.topCode[c2];
         if (code == FREHG_BC_HEAD) v = d.ksz[c];
@@ -445,7 +456,10 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
       else if (code == FREHG_BC_HEAD)
         q = g.az * kface * ((d.botValue[c2] - ha) / (0.5 * dza) - 1.0);
     } else if (actM && actK) {
-      q = g.az * base * ((hb - ha) / (0.5 * (dza + dzb)) - 1.0);
+      double kface = base;
+      if (d.pcLegacy && d.ksz[ca] != d.ksz[cb])  // legacy: the lower cell's Ks on both flanks
+        kface = 0.5 * (vg_conductivity(soil_at(d, ca), ha, d.ksz[cb]) + Kzb);
+      q = g.az * kface * ((hb - ha) / (0.5 * (dza + dzb)) - 1.0);
     }
     d.qzF[g.cF(j, i, k)] = q;
   }
@@ -478,79 +492,7 @@ __global__ __launch_bounds__(kThreads) void k_faces(Dev d, int corr, double dtg)
   }
 }
 
-// ---- predictor face conductivities evaluated where the assembly needs them --
-// The rules of face_lat_at<DIR> / face_z_at with corr = 0 (Predictor.cpp:127-385),
-// as functions of one face: the assembly calls them for the six faces of its
-// cell instead of reading face arrays written by separate kernels.
-template <int DIR>
-__device__ __forceinline__ double face_k_lat(const Dev& d, int j, int i, int k) {
-  // DIR 0: face between (j, i) and (j, i + 1), s = i in [0, nx];
-  // DIR 1: face between (j, i) and (j + 1, i), s = j in [0, ny].
-  const Grid& g = d.g;
-  const double* K = DIR == 0 ? d.Kx : d.Ky;
-  const double* ks = DIR == 0 ? d.ksx : d.ksy;
-  const int jn = DIR == 0 ? j : j + 1, in = DIR == 0 ? i + 1 : i;
-  const int s = DIR == 0 ? i : j;
-  const int nlast = DIR == 0 ? g.nx : g.ny;
-  const int64_t c = g.c3(j, i, k), cn = g.c3(jn, in, k);
-  double v = d.harmonic ? face_k_harmonic(K[c], ks[c], 1.0, K[cn], ks[cn], 1.0)
-                        : 0.5 * (K[c] + K[cn]);
-  if (s == 0) {
-    const int jj = DIR == 0 ? j : 1, ii = DIR == 0 ? 1 : i;
-    const int code = DIR == 0 ? d.sxm[g.c2(jj, ii)] : d.sym[g.c2(jj, ii)];
-    if (code == FREHG_BC_NOFLUX) v = 0.0;
-    else if (is_head(code)) v = K[g.c3(jj, ii, k)];
-  }
-  if (s == nlast) {
-    const int code = DIR == 0 ? d.sxp[g.c2(j, i)] : d.syp[g.c2(j, i)];
-    if (code == FREHG_BC_NOFLUX) v = 0.0;
-    else if (is_head(code)) v = K[c];
-  }
-  if (!d.full3d && (unsat(d, j, i, k) || unsat(d, jn, in, k))) v = 0.0;
-  const bool inA = s >= 1 && s <= nlast, inB = s + 1 >= 1 && s + 1 <= nlast;
-  if ((inA && !act(d, j, i, k)) || (inB && !act(d, jn, in, k))) v = 0.0;
-  return v;
-}
-
-__device__ __forceinline__ double face_k_z(const Dev& d, int j, int i, int k) {
-  // face above cell k of column (j, i), k in [0, nz]
-  const Grid& g = d.g;
-  const int c2 = g.c2(j, i);
-  const int kt = d.kt[c2];
-  const bool actK = k < g.nz && k >= kt;
-  const bool actM = k >= 1 && k - 1 >= kt;
-  double v;
-  if (k == 0) {
-    v = 0.0;
-  } else if (k == g.nz) {
-    v = d.botCode[c2] == FREHG_BC_NOFLUX ? 0.0 : d.Kz[g.c3(j, i, g.nz - 1)];
-  } else if (actK && !actM) {
-    v = d.Kz[g.c3(j, i, k)];
-  } else if (!actM) {
-    v = 0.0;
-  } else {
-    const int64_t ca = g.c3(j, i, k - 1), cb = g.c3(j, i, k);
-    v = d.harmonic ? face_k_harmonic(d.Kz[ca], d.ksz[ca], d.dz3d[ca], d.Kz[cb], d.ksz[cb],
-                                     d.dz3d[cb])
-                   : 0.5 * (d.Kz[ca] + d.Kz[cb]);
-  }
-  if (actK && (k == 0 || !actM)) {
-    const int64_t c = g.c3(j, i, k);
-    const double own = d.Kz[c];
-    if (d.coupled) {
-      v = d.dept[c2] > 0.0 ? d.ksz[c] : own;
-    } else {
-      const int code = d.topCode[c2];
-      if (code == FREHG_BC_HEAD) v = d.ksz[c];
-      else if (code == FREHG_BC_NOFLUX) v = 0.0;
-      else v = own;
-    }
-  }
-  if ((k >= 1 && !actM) || (k < g.nz && !actK)) v = 0.0;
-  return v;
-}
-
-// ---- assembly (Predictor.cpp:400-698, PC = true) into stencil storage -----
+// ---- assembly (Predictor.cpp:400-698) into stencil storage ---------------
 __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
   const Grid& g = d.g;
   const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
@@ -579,7 +521,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
     }
     // Operands, loaded together ahead of the rules that use them (the halo
     // keeps every neighbour address in bounds).  The six face conductivities
-    // are face_k_lat / face_k_z specialised to an active cell.
+    // follow face_lat_at / face_z_at (corr = 0) specialised to an active cell.
     const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
     const int64_t cxm = c - sx, cxp = c + sx, cym = c - sy, cyp = c + sy;
     const int ktE = d.kt[c2 + 1], ktW = d.kt[c2 - 1];
@@ -660,7 +602,8 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
                             : 0.5 * (KzD + Kzc);
     if (isTop) {
       if (d.coupled) {
-        kzTop = d.dept[c2] > 0.0 ? d.ksz[c] : Kzc;
+        const double wet = d.pcLegacy ? 0.5 * (d.ksz[c] + Kzc) : d.ksz[c];
+        kzTop = d.dept[c2] > 0.0 ? wet : Kzc;
       } else {
         const int code = d.topCode[c2];
         if (code == FREHG_BC_HEAD) kzTop = d.ksz[c];
@@ -687,7 +630,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
     const int mode = d.coupled ? d.cplMode[c2] : 0;
     if (isTop) {
       bool dir = d.coupled ? (mode == 1) : (d.topCode[c2] == FREHG_BC_HEAD);
-      dir = dir || (d.coupled && mode == 3);
+      dir = dir || (d.coupled && mode == 3);  // never 3 under legacy
       if (dir) gct -= gzm;
     } else {
       gct -= gzm;
@@ -2083,8 +2026,8 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   if (ds.nx < 1 || ds.ny < 1 || ds.nz < 1) fail(FREHG_EINVAL, "extents must be positive");
   if (!(ds.dx > 0) || !(ds.dy > 0) || !(ds.dz > 0)) fail(FREHG_EINVAL, "spacings must be positive");
   if (ds.nz > 512) fail(FREHG_EUNSUPPORTED, "nz > 512 is not supported");
-  if (ds.predictor_corrector != 1)
-    fail(FREHG_EUNSUPPORTED, "predictor_corrector: legacy is not supported (consistent only)");
+  if (ds.predictor_corrector != 0 && ds.predictor_corrector != 1)
+    fail(FREHG_EINVAL, "predictor_corrector: 1 consistent or 0 legacy");
   if (ds.saturated_contact != 1)
     fail(FREHG_EUNSUPPORTED, "saturated_contact: legacy is order-dependent; head only");
   if (ds.preconditioner == FREHG_PC_ICC0)
@@ -2312,7 +2255,9 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   // options
   d.ss = ds.specific_storage;
   d.full3d = ds.use_full3d ? 1 : 0;
-  d.harmonic = ds.face_conductivity == 1 ? 1 : 0;
+  d.pcLegacy = ds.predictor_corrector == 0 ? 1 : 0;
+  // the legacy pair has no harmonic form (Predictor.cpp:94-102, Corrector.cpp:124-132)
+  d.harmonic = ds.face_conductivity == 1 && !d.pcLegacy ? 1 : 0;
   d.reallocOff = ds.reallocation == 1 ? 1 : 0;
   d.redistribute = ds.reallocation_surplus == 1 ? 1 : 0;
   d.terrain = ds.follow_terrain ? 1 : 0;

@@ -89,7 +89,7 @@ typedef struct frehg_gw_desc {
   /* groundwater.* */
   double specific_storage;
   int use_full3d;
-  int predictor_corrector;         /* 1 consistent (default); 0 legacy is rejected */
+  int predictor_corrector;         /* 1 consistent (default), 0 legacy */
   int face_conductivity;           /* 0 arithmetic (default), 1 harmonic */
   int saturated_contact;           /* 1 head (default); 0 legacy is rejected */
   int reallocation;                /* 0 legacy (default), 1 disabled */

@@ -9,11 +9,13 @@
 // FMA contraction (x86-64 baseline ISA), the reference's algorithm:
 //   VanGenuchten.hpp:45-128        van Genuchten-Mualem closures
 //   FaceConductivity.hpp:37-74     arithmetic / harmonic face rules
-//   Predictor.cpp:91-386           face conductivities  <PC=true, HARM>
-//   Predictor.cpp:392-698          capacity + 7-point assembly <PC=true>
+//   Predictor.cpp:91-386           face conductivities  <PC, HARM>
+//   Predictor.cpp:392-698          capacity + 7-point assembly <PC>
 //   Predictor.cpp:732-817          solve hand-off, scatter, ghost heads
-//   Corrector.cpp:47-118           coupled-top classification <PC=true>
-//   Corrector.cpp:123-533          Darcy fluxes <PC=true, HARM>
+//   Corrector.cpp:47-118           coupled-top classification <PC>
+//   Corrector.cpp:123-533          Darcy fluxes <PC, HARM>
+// PC = true is predictor_corrector: consistent (the default); the legacy pair
+// <false, false> (pcLegacy) differs in four rules, each marked "legacy".
 //   Corrector.cpp:539-627          coupled-top bookkeeping
 //   Corrector.cpp:629-672          flux-divergence theta update
 //   Corrector.cpp:709-792          moisture ghosts, final clamp
@@ -190,6 +192,7 @@ struct Oracle {
   // options
   double ss = 0;
   bool full3d = true, harmonic = false, reallocOff = false, redistribute = false;
+  bool pcLegacy = false;  // predictor_corrector: legacy, the <false, false> pair
   double dtMin = 0, dtMax = 0, dqGrow = 0.01, dqShrink = 0.02, courantMax = 2.0;
   double dtgNext = 0, dtgCurrent = 0;
   double rtol = 1e-8, atol = 1e-14;
@@ -497,8 +500,8 @@ void Oracle::applyIC(const frehg_gw_desc& d) {
 void Oracle::create(const frehg_gw_desc& d) {
   if (d.nx < 1 || d.ny < 1 || d.nz < 1) fail(FREHG_EINVAL, "extents must be positive");
   if (!(d.dx > 0) || !(d.dy > 0) || !(d.dz > 0)) fail(FREHG_EINVAL, "spacings must be positive");
-  if (d.predictor_corrector != 1)
-    fail(FREHG_EUNSUPPORTED, "predictor_corrector: legacy is not supported (consistent only)");
+  if (d.predictor_corrector != 0 && d.predictor_corrector != 1)
+    fail(FREHG_EINVAL, "predictor_corrector: 1 consistent or 0 legacy");
   if (d.saturated_contact != 1)
     fail(FREHG_EUNSUPPORTED, "saturated_contact: legacy is order-dependent; head only");
   if (!d.ksx || !d.ksy || !d.ksz || !d.vg_alpha || !d.vg_n || !d.theta_s || !d.theta_r || !d.aev)
@@ -520,7 +523,10 @@ void Oracle::create(const frehg_gw_desc& d) {
   uniformLayers = d.terrain_layers == 1;
   ss = d.specific_storage;
   full3d = d.use_full3d != 0;
-  harmonic = d.face_conductivity == 1;
+  pcLegacy = d.predictor_corrector == 0;
+  // the legacy pair has no harmonic instantiation (Predictor.cpp:94-102,
+  // Corrector.cpp:124-132)
+  harmonic = d.face_conductivity == 1 && !pcLegacy;
   reallocOff = d.reallocation == 1;
   redistribute = d.reallocation_surplus == 1;
   dtMin = d.dt_min;
@@ -637,7 +643,7 @@ void Oracle::enforceMoistureBc() {
     }
 }
 
-// --- Corrector.cpp:55-118 <PC=true> -------------------------------------
+// --- Corrector.cpp:55-118 <PC> --------------------------------------------
 void Oracle::classifyCoupledTop(double dtg) {
   for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i) {
@@ -649,7 +655,8 @@ void Oracle::classifyCoupledTop(double dtg) {
         }
       const size_t c2 = static_cast<size_t>(j) * (nx + 2) + i;
       if (kt < 0 || dept[c2] <= 0.0) {
-        if (kt >= 0 && h(j, i, kt) > 0.5 * dz3d(j, i, kt)) {
+        // legacy: no pressurized seepage face (mode 3), Corrector.cpp:96-106
+        if (!pcLegacy && kt >= 0 && h(j, i, kt) > 0.5 * dz3d(j, i, kt)) {
           cplMode(j, i) = 3;
           continue;
         }
@@ -662,7 +669,7 @@ void Oracle::classifyCoupledTop(double dtg) {
     }
 }
 
-// --- Predictor.cpp:105-386 <PC=true, HARM> ---------------------------------
+// --- Predictor.cpp:105-386 <PC, HARM> --------------------------------------
 void Oracle::faceConductivity() {
 #pragma omp parallel for schedule(static) num_threads(g_threads)
   for (int j = 1; j <= ny; ++j)
@@ -726,7 +733,9 @@ void Oracle::faceConductivity() {
         if (!isTop) continue;
         const double own = kh(j, i, k, ksz);
         if (coupled) {
-          kzF(j, i, k) = (dept[static_cast<size_t>(j) * (nx + 2) + i] > 0.0) ? ksz(j, i, k) : own;
+          // a wet top at Ksz; legacy at 0.5 (Ksz + K(h)) (Predictor.cpp:293-302)
+          const double wet = pcLegacy ? 0.5 * (ksz(j, i, k) + own) : ksz(j, i, k);
+          kzF(j, i, k) = (dept[static_cast<size_t>(j) * (nx + 2) + i] > 0.0) ? wet : own;
           continue;
         }
         const int code = topCode(j, i);
@@ -806,7 +815,7 @@ void Oracle::assemble(double dtg) {
         if (!(k == nz - 1 && botCode(j, i) != FREHG_BC_HEAD)) gct -= gzp;
         if (isTop) {
           bool dirichletTop = coupled ? (cplMode(j, i) == 1) : (topCode(j, i) == FREHG_BC_HEAD);
-          dirichletTop = dirichletTop || (coupled && cplMode(j, i) == 3);
+          if (!pcLegacy) dirichletTop = dirichletTop || (coupled && cplMode(j, i) == 3);
           if (dirichletTop) gct -= gzm;
         } else {
           gct -= gzm;
@@ -1130,10 +1139,13 @@ void Oracle::solve() {
   for (int64_t r = 0; r < nActive; ++r) h(rowJ[r], rowI[r], rowK[r]) = sol[r];
 }
 
-// --- Corrector.cpp:135-533 <PC=true, HARM> --------------------------------
+// --- Corrector.cpp:135-533 <PC, HARM> -------------------------------------
 void Oracle::fluxes(double dtg) {
   const auto faceKx = [&](int j, int i, int k, int jn, int in, const F3& ks, double la,
                           double lb) {
+    // legacy: the minus cell's Ks enters both flanks (Corrector.cpp:224-236)
+    if (pcLegacy)
+      return 0.5 * (kh(j, i, k, ks) + k_of_h(soil(jn, in, k), h(jn, in, k), ks(j, i, k)));
     if (harmonic)
       return face_k_harmonic(soil(j, i, k), h(j, i, k), ks(j, i, k), la, soil(jn, in, k),
                              h(jn, in, k), ks(jn, in, k), lb);
@@ -1289,7 +1301,9 @@ void Oracle::fluxes(double dtg) {
           } else q = 0.0;
         } else if (above && below) {
           double kf;
-          if (harmonic)
+          if (pcLegacy)  // the lower cell's Ks in both flanks (Corrector.cpp:512-521)
+            kf = 0.5 * (k_of_h(soil(j, i, k - 1), h(j, i, k - 1), ksz(j, i, k)) + kh(j, i, k, ksz));
+          else if (harmonic)
             kf = face_k_harmonic(soil(j, i, k - 1), h(j, i, k - 1), ksz(j, i, k - 1), dz3d(j, i, k - 1),
                                  soil(j, i, k), h(j, i, k), ksz(j, i, k), dz3d(j, i, k));
           else

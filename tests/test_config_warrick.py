@@ -56,6 +56,20 @@ def test_deck_maps_to_case():
     assert np.all(case.soil_array("vg_n") == 1.56)
     assert case.reallocation_surplus == "redistribute"
     assert not sched.time_dependent
+    assert case.predictor_corrector == "consistent"
+
+
+def test_scheme_keys():
+    import copy
+    deck = copy.deepcopy(DECK)
+    deck["groundwater"]["predictor_corrector"] = "legacy"
+    case, _ = load_config(deck)
+    assert case.predictor_corrector == "legacy" and case.to_desc()[0].predictor_corrector == 0
+    # saturated_contact: legacy reads cells its own sweep is rewriting
+    # (Reallocate.cpp:177): its result depends on the loop order
+    deck["groundwater"]["saturated_contact"] = "legacy"
+    with pytest.raises(ValueError):
+        load_config(deck)
 
 
 def test_polygon_and_series_semantics(tmp_path):

@@ -192,13 +192,28 @@ def test_stepwise_coupled():
     _stepwise(cases.coupled_128x128x64(nx=16, ny=12, nz=20, precision="fp64"), 10)
 
 
+def test_stepwise_coupled_legacy_pair():
+    # predictor_corrector: legacy on the coupled member: the wet top at
+    # 0.5 (Ksz + K), no pressurized seepage face, one-sided corrector Ks
+    c = cases.coupled_128x128x64(nx=16, ny=12, nz=20, precision="fp64")
+    c.predictor_corrector = "legacy"
+    _stepwise(c, 10)
+
+
+def test_stepwise_hetero_terrain_legacy_pair():
+    c = cases.hetero_512x512x100(nx=32, ny=24, nz=100)
+    c.predictor_corrector = "legacy"
+    _stepwise(c, 4)
+
+
 def test_stepwise_jacobi():
     c = cases.loam_32x32x50(nx=8, ny=6)
     c.preconditioner = "jacobi"
     _stepwise(c, 5)
 
 
-@pytest.mark.parametrize("variant", ["harmonic", "redistribute", "disabled", "full3d_off"])
+@pytest.mark.parametrize("variant", ["harmonic", "redistribute", "disabled", "full3d_off",
+                                     "legacy_pair"])
 def test_stepwise_scheme_variants(variant):
     c = cases.stiff_32x32x50(precision="fp64")
     c.nx = c.ny = 10
@@ -210,6 +225,8 @@ def test_stepwise_scheme_variants(variant):
         c.reallocation_surplus = "redistribute"
     elif variant == "disabled":
         c.reallocation = "disabled"
+    elif variant == "legacy_pair":
+        c.predictor_corrector = "legacy"
     else:
         c.use_full3d = False
     _stepwise(c, 10)

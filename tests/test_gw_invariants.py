@@ -1,16 +1,14 @@
 """The reference's groundwater invariants (tests/unit/test_gw.cpp), run on the
 oracle (CPU) and on the HIP path (gpu marker).
 
-Each test cites the reference test it ports.  Where the reference test needs
-a configuration this path does not support (predictor_corrector: legacy), the
-port keeps the assertion that holds under the default consistent pair and
-says so.
+Each test cites the reference test it ports, with the configuration it names
+(predictor_corrector: legacy where the reference asks for the legacy pair).
 """
 import numpy as np
 import pytest
 
 from frehg2_amd import abi
-from helpers import gw_base_case, interior, run_steps
+from helpers import TEST_SOIL, gw_base_case, interior, run_steps
 
 BACKENDS = ["oracle", pytest.param("gpu", marks=pytest.mark.gpu)]
 
@@ -165,19 +163,29 @@ def _sand_over_clay(**kw):
 
 
 @pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("pc", ["legacy", "consistent"])
 @pytest.mark.parametrize("surplus", ["drop", "redistribute"])
-def test_reallocation_surplus_modes_close_the_audit(backend, surplus):
-    # test_gw.cpp:375-434, audit-closure half.  The reference runs it under
-    # predictor_corrector: legacy to create the clay-front surplus; under the
-    # default consistent pair the closure identity must hold in both modes.
-    sim = make(backend, _sand_over_clay(reallocation_surplus=surplus))
+def test_reallocation_surplus_modes_close_the_audit(backend, pc, surplus):
+    # test_gw.cpp:375-434.  The reference runs it under predictor_corrector:
+    # legacy, whose one-sided clay Ks on the sand/clay faces leaves corrector
+    # moisture above theta(h) behind the front: drop discards it (more than
+    # 1e-5 m^3 over 200 steps), redistribute keeps it in the column.  Under
+    # the consistent pair only the closure identity is asserted.
+    sim = make(backend, _sand_over_clay(reallocation_surplus=surplus, predictor_corrector=pc))
     v0 = prev = sim.volume()
+    dropped = 0.0
     for n in range(200):
         sim.step()
         v, res = closure(sim, prev)
         assert abs(res) <= 1e-12, n
+        dropped += sim.audit()["realloc_dropped"]
         prev = v
     assert sim.volume() > v0
+    if pc == "legacy":
+        if surplus == "drop":
+            assert dropped > 1e-5
+        else:
+            assert dropped <= 1e-9
 
 
 @pytest.mark.parametrize("backend", BACKENDS)
@@ -291,3 +299,105 @@ def test_saturated_heterogeneous_column_restore_books_nothing(backend, harmonic)
         churn += abs(a["realloc_adjust"]) + 2.0 * a["realloc_dropped"]
     assert through > 1e-5
     assert churn <= 1e-10 * through
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_saturated_heterogeneous_column_legacy_churns(backend):
+    # test_gw.cpp:885-927: the same stack under predictor_corrector: legacy
+    # books the corrector/predictor gap (the red control of the test above)
+    ids = [0, 3, 1, 4, 2, 5]
+    ks = np.array([10.0 ** (-6.5 + 0.6 * n) for n in ids]).reshape(1, 1, 6)
+    c = gw_base_case(1, 1, 6, 0.25, 10.0, ks=ks, ic_form="water_table", ic_const=5.0,
+                     predictor_corrector="legacy")
+    c.set_bc("top", abi.BC_HEAD, 2.0)
+    c.set_bc("bot", abi.BC_HEAD, 3.3)
+    sim = make(backend, c)
+    through = churn = 0.0
+    for _ in range(20):
+        dtg = sim.step()
+        a = sim.audit()
+        through += abs(sim.get(abi.F_QZF)[1, 1, 3]) * dtg
+        churn += abs(a["realloc_adjust"]) + 2.0 * a["realloc_dropped"]
+    assert through > 1e-5
+    assert churn > 1e-4 * through
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_homogeneous_consistent_is_bitwise_legacy(backend):
+    # test_gw.cpp:929-955 (K3): homogeneous soil, uncoupled: every face rule
+    # of the two pairs coincides, so the heads agree bitwise
+    heads = []
+    for pc in ("legacy", "consistent"):
+        c = gw_base_case(2, 1, 4, 0.25, 10.0, predictor_corrector=pc)
+        c.set_bc("top", abi.BC_HEAD, 0.1)
+        sim = make(backend, c)
+        for _ in range(30):
+            sim.step()
+        heads.append(sim.get(abi.F_HEAD)[1, 1, :].copy())
+    assert np.array_equal(heads[0], heads[1])
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_ponded_top_face_consistent_is_bitwise_legacy(backend):
+    # test_gw.cpp:957-987 (K2): the uncoupled head top was Ksz in both pairs
+    thetas = []
+    for pc in ("legacy", "consistent"):
+        c = gw_base_case(1, 1, 6, 0.2, 5.0, ic_const=0.15, predictor_corrector=pc)
+        c.set_bc("top", abi.BC_HEAD, 0.3)
+        sim = make(backend, c)
+        for _ in range(50):
+            sim.step()
+        thetas.append(sim.get(abi.F_WC)[1, 1, :].copy())
+    assert np.array_equal(thetas[0], thetas[1])
+
+
+def _saturated_pair(pc):
+    # two saturated cells in x (Ks 1e-5 and 4e-5) between head sides, no
+    # storage, nothing after the fluxes that moves h (reallocation disabled)
+    ks = np.array([1e-5, 4e-5]).reshape(1, 2, 1)
+    c = gw_base_case(2, 1, 1, 0.5, 10.0, ks=ks, ic_form="water_table", ic_const=5.0,
+                     reallocation="disabled", predictor_corrector=pc)
+    c.set_bc("xm", abi.BC_HEAD, 2.0)
+    c.set_bc("xp", abi.BC_HEAD, 1.0)
+    return c
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_legacy_lateral_face_takes_the_minus_cells_ks(backend):
+    # Corrector.cpp:212-236: with both cells saturated (K = Ks) the legacy
+    # interior face conducts at the minus cell's Ks, the consistent one at
+    # the flanks' mean; the face flux over the head difference then differs
+    # by exactly that ratio (area and spacing cancel)
+    rate = {}
+    for pc in ("legacy", "consistent"):
+        sim = make(backend, _saturated_pair(pc))
+        sim.step()
+        h = sim.get(abi.F_HEAD)[1, 1:3, 0]
+        assert np.all(h > 0) and h[0] != h[1]
+        rate[pc] = sim.get(abi.F_QX)[1, 1, 0] / (h[1] - h[0])
+    assert rate["legacy"] / rate["consistent"] == pytest.approx(1e-5 / (0.5 * (1e-5 + 4e-5)),
+                                                                rel=1e-12)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("pc", ["legacy", "consistent"])
+def test_coupled_wet_top_face_conductivity(backend, pc):
+    # Predictor.cpp:293-302: a wet coupled top conducts at Ksz (consistent)
+    # or 0.5 (Ksz + K(h_top)) (legacy); the corrector's face array holds it
+    from oracle.oracle import vg
+    ks = 2e-5
+    c = gw_base_case(1, 1, 8, 0.1, 30.0, ks=ks, ic_const=0.15, reallocation="disabled",
+                     predictor_corrector=pc)
+    dept = np.zeros((3, 3))
+    dept[1, 1] = 0.05
+    c.coupling = dict(eta=dept.copy(), dept=dept.copy(), seep_accum=np.zeros_like(dept),
+                      avail=dept.copy(), gain=np.zeros_like(dept), bounce=np.zeros_like(dept),
+                      min_depth=1e-4, vent_all=0)
+    sim = make(backend, c)
+    sim.step()
+    h_top = sim.get(abi.F_HEAD)[1, 1, 0]
+    assert h_top < 0  # unsaturated, so the two rules differ
+    soil = (TEST_SOIL["vg_alpha"], TEST_SOIL["vg_n"], TEST_SOIL["theta_s"],
+            TEST_SOIL["theta_r"], TEST_SOIL["aev"])
+    want = 0.5 * (ks + vg("conductivity", soil, h_top, ks)) if pc == "legacy" else ks
+    assert sim.get(abi.F_KZF)[1, 1, 0] == pytest.approx(want, rel=1e-14)
