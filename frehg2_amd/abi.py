@@ -32,6 +32,7 @@ F_QX, F_QY, F_QZF = 7, 8, 9
 F_VLOSS, F_DZ3D, F_BOT3D = 10, 11, 12
 F_AREA_X, F_AREA_Y, F_SIN_X, F_COS_X, F_SIN_Y, F_COS_Y = 13, 14, 15, 16, 17, 18
 F_CAPACITY = 19
+F_SINK = 20
 
 FACE_FIELDS = (F_KZF, F_QZF)
 
@@ -68,7 +69,8 @@ class GwDesc(C.Structure):
 class GwAudit(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "boundary_in", "ss_storage", "realloc_adjust", "realloc_dropped", "vloss",
-        "cpl_exchanged", "cpl_bounce", "cpl_vent", "cpl_evap", "cpl_vent_thin", "pc_clamp")]
+        "cpl_exchanged", "cpl_bounce", "cpl_vent", "cpl_evap", "cpl_vent_thin", "pc_clamp",
+        "tr_applied")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

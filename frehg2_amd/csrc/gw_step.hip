@@ -17,7 +17,7 @@
 //                       Corrector.cpp:135-533 (corrector; face_lat_at<DIR>
 //                       and face_z_at for one face each)
 //   k_coupled_top       Corrector.cpp:539-627
-//   k_update_wc         Corrector.cpp:629-672
+//   k_update_wc         Corrector.cpp:629-705 (with the root-uptake sink)
 //   k_realloc_classify  Reallocate.cpp:181-391 (saturated_contact: head)
 //   k_realloc_walk      Reallocate.cpp:410-540 and Corrector.cpp:758-792, one
 //                       wave per column, the walks done in parallel where the
@@ -71,11 +71,11 @@ constexpr int kThreads = 256;
 // Reduction slots of one step (each filled by kRb per-block partials).
 enum Slot {
   S_SS = 0, S_ADJ, S_DISC, S_LOST, S_VENT, S_THIN, S_VLOSS, S_BIN, S_DQMAX, S_COMIN,
-  S_VOL, S_CEX, S_CBO, S_CEV, S_CLAMP,
+  S_VOL, S_CEX, S_CBO, S_CEV, S_CLAMP, S_TR,
   S_XAPP, S_XDISC, S_XGAIN, S_XHELD, S_XSURF,  // coupling window (Exchange.cpp)
   S_COUNT
 };
-__constant__ int kSlotOp[S_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 2, 0, 0, 0, 0, 0,
+__constant__ int kSlotOp[S_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 2, 0, 0, 0, 0, 0, 0,
                                      0, 0, 0, 0, 0};
 
 struct Dev {
@@ -107,6 +107,10 @@ struct Dev {
   // (mode 3), and corrector faces that give both flanks the minus (lateral)
   // or lower (vertical) cell's Ks
   int pcLegacy;
+  // the root-uptake sink S [1/s] per cell (H), null when none is set: the
+  // predictor takes dtg S V from the rhs (Predictor.cpp:700-726) and the
+  // corrector from the theta update (Corrector.cpp:674-705)
+  double* sink;
   double *Ad, *Axp, *Ayp, *Azp, *rhs, *x;
   float *Ad32, *Axp32, *Ayp32, *Azp32;  // the stencil rounded to FP32 (mixed form), or null
   double* parts;
@@ -697,6 +701,10 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
     const double axp = (!east && k >= ktE) ? gxp : 0.0;
     const double ayp = (!north && k >= ktN) ? gyp : 0.0;
     const double azp = (k + 1 < g.nz) ? gzp : 0.0;
+    if (d.sink) {
+      const double sk = d.sink[c];
+      if (sk != 0.0) b -= dtg * sk * volume;
+    }
     d.Ad[c] = gct;
     d.Axp[c] = axp;
     d.Ayp[c] = ayp;
@@ -852,11 +860,14 @@ __global__ void k_cpl_reduce_audit(Dev d, const double* __restrict__ bottom) {
 }
 
 // ---- theta update (Corrector.cpp:629-672) ----------------------------------
+// With SINK the root-uptake sink leaves after the flux update, as a second
+// rounding step (Corrector.cpp:674-705), and its volume S V dtg is summed.
+template <bool SINK>
 __global__ void k_update_wc(Dev d, double dtg) {
   const Grid& g = d.g;
   const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
   const int64_t sy = static_cast<int64_t>(g.nx2) * g.nz;
-  KSum ss;
+  KSum ss, tr;
   for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
     const int k = static_cast<int>(r.t % g.nz);
     const int col = static_cast<int>(r.t / g.nz);
@@ -869,14 +880,29 @@ __global__ void k_update_wc(Dev d, double dtg) {
     const double dqx = dtg * (d.qx[c] - d.qx[c - g.nz]);
     const double dqy = dtg * (d.qy[c] - d.qy[c - sy]);
     const double dqz = dtg * (d.qzF[g.cF(j, i, k + 1)] - d.qzF[g.cF(j, i, k)]);
-    const double upd = (d.wcn[c] * volume + dqx + dqy + dqz) / coeff;
+    double upd = (d.wcn[c] * volume + dqx + dqy + dqz) / coeff;
+    ss.add(upd * volume * d.ss * dh / wcs_at(d, c));
+    if (SINK) {
+      const double sk = d.sink[c];
+      if (sk != 0.0) {
+        const double dtheta = dtg * sk * volume / coeff;
+        upd -= dtheta;
+        ss.add(-(dtheta * volume * d.ss * dh / wcs_at(d, c)));
+        tr.add(sk * volume * dtg);
+      }
+    }
     d.wc[c] = upd;
     d.sat[c] = (upd >= wcs_at(d, c) || d.h[c] > aev_at(d, c)) ? 1 : 0;
-    ss.add(upd * volume * d.ss * dh / wcs_at(d, c));
   }
-  double v[1] = {ss.value()};
-  const int s[1] = {S_SS};
-  put_parts<1>(d, s, v);
+  if (SINK) {
+    double v[2] = {ss.value(), tr.value()};
+    const int s[2] = {S_SS, S_TR};
+    put_parts<2>(d, s, v);
+  } else {
+    double v[1] = {ss.value()};
+    const int s[1] = {S_SS};
+    put_parts<1>(d, s, v);
+  }
 }
 
 // ---- post-allocation, phase 1 (Reallocate.cpp:181-391), head contact -------
@@ -1901,6 +1927,20 @@ struct GwSolver {
     if (surface) *surface = slotsHost[S_XSURF] * cell;
   }
   double* fieldPtr(int f, int64_t* n);
+  // the root-uptake sink: host_in = S [1/s] per cell (H) applies it from the
+  // next step on; null detaches it
+  void setSink(const double* host_in) {
+    if (!host_in) {
+      d.sink = nullptr;
+      return;
+    }
+    if (!sinkBuf) sinkBuf = alloc<double>(g.cellsH());
+    HIP_CHECK(hipMemcpyAsync(sinkBuf, host_in, static_cast<size_t>(g.cellsH()) * sizeof(double),
+                             hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    d.sink = sinkBuf;
+  }
+  double* sinkBuf = nullptr;
 };
 
 void GwSolver::buildGeometry(const frehg_gw_desc& ds, std::vector<double>& bath,
@@ -2354,7 +2394,8 @@ void GwSolver::step(double /*t*/, double dtg) {
     k_coupled_top<<<rb, kThreads, 0, stream>>>(d, dtg);
     HIP_CHECK(hipGetLastError());
   }
-  k_update_wc<<<rb, kThreads, 0, stream>>>(d, dtg);
+  if (d.sink) k_update_wc<true><<<rb, kThreads, 0, stream>>>(d, dtg);
+  else k_update_wc<false><<<rb, kThreads, 0, stream>>>(d, dtg);
   HIP_CHECK(hipGetLastError());
   // post-allocation, final clamp, boundary inflow and dt inputs: one column
   // kernel writing the second buffer pair, which then becomes the state
@@ -2404,6 +2445,7 @@ void GwSolver::step(double /*t*/, double dtg) {
   audit.cpl_bounce = s[S_CBO];
   audit.cpl_evap = s[S_CEV];
   audit.pc_clamp = s[S_CLAMP];
+  audit.tr_applied = d.sink ? s[S_TR] : 0.0;
   // adaptTimeStep (AdaptiveStep.cpp:100-117)
   double next = dtg;
   if (s[S_DQMAX] > dqShrink) next = dtg * 0.75;
@@ -2448,6 +2490,9 @@ double* GwSolver::fieldPtr(int f, int64_t* n) {
     case FREHG_F_SIN_Y: return d.siny;
     case FREHG_F_COS_Y: return d.cosy;
     case FREHG_F_CAPACITY: return d.ch;
+    case FREHG_F_SINK:
+      if (!d.sink) fail(FREHG_EINVAL, "no root-uptake sink is set");
+      return d.sink;
     default: fail(FREHG_EINVAL, "unknown field id");
   }
 }
@@ -2533,8 +2578,13 @@ int frehg_gw_get_field(frehg_gw_solver* s, int field, double* host_out) {
 int frehg_gw_set_field(frehg_gw_solver* s, int field, const double* host_in) {
   return guard([&] {
     GwSolver* g = frehg::S(s);
+    if (field == FREHG_F_SINK) {
+      g->setSink(host_in);
+      return;
+    }
     if (field != FREHG_F_HEAD && field != FREHG_F_WC)
-      frehg::fail(FREHG_EINVAL, "only head / water content are settable");
+      frehg::fail(FREHG_EINVAL, "only head, water content and the sink are settable");
+    if (!host_in) frehg::fail(FREHG_EINVAL, "null argument");
     int64_t n = 0;
     double* p = g->fieldPtr(field, &n);
     HIP_CHECK(hipMemcpyAsync(p, host_in, static_cast<size_t>(n) * sizeof(double),

@@ -64,6 +64,15 @@ class RichardsSolver:
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.field_shape(fid))
         check(self.lib, self.lib.frehg_gw_set_field(self._h, fid, _dptr(a)))
 
+    def set_sink(self, sink) -> None:
+        """The root-uptake sink S [1/s] per cell, halo layout (ny+2, nx+2, nz),
+        applied from the next step on (RichardsSolver::sink_,
+        RichardsSolver.hpp:656); None detaches it."""
+        if sink is None:
+            check(self.lib, self.lib.frehg_gw_set_field(self._h, abi.F_SINK, None))
+        else:
+            self.set(abi.F_SINK, sink)
+
     def set_bc_values(self, which: int, values) -> None:
         a = np.ascontiguousarray(values, dtype=np.float64).reshape(self.ny, self.nx)
         check(self.lib, self.lib.frehg_gw_set_bc_values(self._h, which, _dptr(a)))

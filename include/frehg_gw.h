@@ -130,6 +130,8 @@ typedef struct frehg_gw_audit {
   double cpl_evap;
   double cpl_vent_thin;
   double pc_clamp;                 /* mode-3 clamp volume of this step */
+  double tr_applied;               /* root-uptake sink volume S V dtg of this step
+                                      (GwStepAudit::trApplied, RichardsSolver.hpp:113) */
 } frehg_gw_audit;
 
 /* SolveStats (src/core/LinearSystem.hpp:49-53) plus refinement detail. */
@@ -165,6 +167,8 @@ typedef struct frehg_gw_solve_stats {
 #define FREHG_F_SIN_Y 17
 #define FREHG_F_COS_Y 18
 #define FREHG_F_CAPACITY 19        /* C(h^n) of the last assembly (H) */
+#define FREHG_F_SINK 20            /* root-uptake sink S [1/s] (H) rootUptakeSink(),
+                                      RichardsSolver.hpp:295,656 */
 
 /* Opaque handles. */
 typedef struct frehg_gw_solver frehg_gw_solver;
@@ -202,7 +206,11 @@ int frehg_gw_field_size(const frehg_gw_solver* s, int field, int64_t* n);
 int frehg_gw_get_field(frehg_gw_solver* s, int field, double* host_out);
 /* Overwrite h or theta from host memory and rebuild the derived state
  * (checkpoint restore + RichardsSolver::refreshDerivedState,
- * src/gw/RichardsSolver.cpp:880-890).  Only FREHG_F_HEAD / FREHG_F_WC. */
+ * src/gw/RichardsSolver.cpp:880-890): FREHG_F_HEAD / FREHG_F_WC.
+ * FREHG_F_SINK sets the root-uptake sink S [1/s] that the following steps
+ * apply (the vegetation module's sink_, src/gw/Evapotranspiration.cpp:
+ * 560-634, consumed at src/gw/Predictor.cpp:700-726 and
+ * src/gw/Corrector.cpp:674-705); host_in = NULL detaches it. */
 int frehg_gw_set_field(frehg_gw_solver* s, int field, const double* host_in);
 /* Refresh per-column boundary values (updateBoundaryValues,
  * src/gw/RichardsSolver.cpp:526-566); which = 0 top, 1 bottom, 2 x-, 3 x+,

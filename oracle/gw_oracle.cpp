@@ -17,7 +17,8 @@
 // PC = true is predictor_corrector: consistent (the default); the legacy pair
 // <false, false> (pcLegacy) differs in four rules, each marked "legacy".
 //   Corrector.cpp:539-627          coupled-top bookkeeping
-//   Corrector.cpp:629-672          flux-divergence theta update
+//   Corrector.cpp:629-705          flux-divergence theta update, root-uptake sink
+//   Predictor.cpp:700-726          the sink in the rhs
 //   Corrector.cpp:709-792          moisture ghosts, final clamp
 //   Reallocate.cpp:122-545         post-allocation (saturated_contact: head)
 //   AdaptiveStep.cpp:44-118        dtg controller
@@ -209,6 +210,8 @@ struct Oracle {
   int64_t nActive = 0;
   // fields
   F3 h, hn, wc, wcn, ch, ksx, ksy, ksz, vga, vgn, wcs, wcr, aev;
+  F3 sink;               // root-uptake sink S [1/s] (RichardsSolver.hpp:656)
+  bool sinkActive = false;
   F3 kx, ky, kzF, qx, qy, qzF, vloss, room, sendUp, sendDown, wcSnap, hSnap;
   // boundary markers
   F2<int> topCode, botCode, sxm, sxp, sym, syp, cplMode;
@@ -893,6 +896,15 @@ void Oracle::assemble(double dtg) {
         vals[base + 6] = gzp;
         rhs[row] = b;
       }
+  if (sinkActive) {  // Predictor.cpp:700-726, after the assembly
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i)
+        for (int k = 0; k < nz; ++k) {
+          const int64_t row = g(j, i, k);
+          if (row < 0 || sink(j, i, k) == 0.0) continue;
+          rhs[row] -= dtg * sink(j, i, k) * (az * dz3d(j, i, k));
+        }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1375,6 +1387,24 @@ void Oracle::updateWaterContent(double dtg) {
       }
     return s;
   });
+  if (!sinkActive) return;
+  // the root-uptake sink after the flux update (Corrector.cpp:674-705); the
+  // applied volume S V dtg is the vegetation module's trApplied
+  // (Evapotranspiration.cpp:619-633)
+  double ssDelta = 0.0, applied = 0.0;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i)
+      for (int k = 0; k < nz; ++k) {
+        if (g(j, i, k) < 0 || sink(j, i, k) == 0.0) continue;
+        const double volume = az * dz3d(j, i, k);
+        const double coeff = volume * (1.0 + ss * (h(j, i, k) - hn(j, i, k)) / wcs(j, i, k));
+        const double dTheta = dtg * sink(j, i, k) * volume / coeff;
+        wc(j, i, k) -= dTheta;
+        ssDelta -= dTheta * volume * ss * (h(j, i, k) - hn(j, i, k)) / wcs(j, i, k);
+        applied += sink(j, i, k) * volume * dtg;
+      }
+  audit.ss_storage += ssDelta;
+  audit.tr_applied = applied;
 }
 
 // --- Reallocate.cpp:122-545, saturated_contact: head ------------------------
@@ -1779,6 +1809,9 @@ const F3* field_of(Oracle& o, int f) {
     case FREHG_F_SIN_Y: return &o.siny;
     case FREHG_F_COS_Y: return &o.cosy;
     case FREHG_F_CAPACITY: return &o.ch;
+    case FREHG_F_SINK:
+      if (!o.sinkActive) fail(FREHG_EINVAL, "no root-uptake sink is set");
+      return &o.sink;
     default: fail(FREHG_EINVAL, "unknown field id");
   }
 }
@@ -1867,7 +1900,16 @@ int oracle_gw_get_field(void* p, int f, double* outp) {
 int oracle_gw_set_field(void* p, int f, const double* in) {
   return guard([&] {
     Oracle& o = *static_cast<Oracle*>(p);
-    if (f != FREHG_F_HEAD && f != FREHG_F_WC) fail(FREHG_EINVAL, "only head / water content are settable");
+    if (f == FREHG_F_SINK) {  // NULL detaches the sink
+      o.sinkActive = in != nullptr;
+      if (in) {
+        o.sink = o.h;  // halo-layout shape
+        std::memcpy(o.sink.v.data(), in, o.sink.v.size() * sizeof(double));
+      }
+      return;
+    }
+    if (f != FREHG_F_HEAD && f != FREHG_F_WC) fail(FREHG_EINVAL, "only head, water content and the sink are settable");
+    if (!in) fail(FREHG_EINVAL, "null argument");
     F3& fld = (f == FREHG_F_HEAD) ? o.h : o.wc;
     std::memcpy(fld.v.data(), in, fld.v.size() * sizeof(double));
     o.enforceHeadBc();

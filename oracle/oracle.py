@@ -132,6 +132,13 @@ class OracleRichards:
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.field_shape(fid))
         _check(lib().oracle_gw_set_field(self._h, fid, _dptr(a)))
 
+    def set_sink(self, sink) -> None:
+        """The root-uptake sink S [1/s], halo layout; None detaches it."""
+        if sink is None:
+            _check(lib().oracle_gw_set_field(self._h, abi.F_SINK, None))
+        else:
+            self.set(abi.F_SINK, sink)
+
     def set_bc_values(self, which: int, values) -> None:
         a = np.ascontiguousarray(values, dtype=np.float64).reshape(self.ny, self.nx)
         _check(lib().oracle_gw_set_bc_values(self._h, which, _dptr(a)))

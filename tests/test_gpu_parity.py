@@ -84,9 +84,12 @@ def _compare(o, g, tag, tol=TOL):
     return errs
 
 
-def _stepwise(case, steps, tol=TOL):
+def _stepwise(case, steps, tol=TOL, setup=None):
     """The gate: identical inputs at every step."""
     o, g = _pair(case)
+    if setup is not None:
+        setup(o)
+        setup(g)
     worst = {}
     dt_flips = 0
     for s in range(steps):
@@ -204,6 +207,17 @@ def test_stepwise_hetero_terrain_legacy_pair():
     c = cases.hetero_512x512x100(nx=32, ny=24, nz=100)
     c.predictor_corrector = "legacy"
     _stepwise(c, 4)
+
+
+def test_stepwise_hetero_terrain_with_sink():
+    # a root-uptake sink over the upper 20 layers, seeded, at most 1e-8 1/s
+    # (3e-6 of theta per 300 s step, well inside the supply)
+    c = cases.hetero_512x512x100(nx=32, ny=24, nz=100)
+    sink = np.zeros((c.ny + 2, c.nx + 2, c.nz))
+    sink[1:-1, 1:-1, :20] = np.random.default_rng(20261203).uniform(0.0, 1e-8, (c.ny, c.nx, 20))
+    o, g = _stepwise(c, 4, setup=lambda s: s.set_sink(sink))
+    assert g.audit()["tr_applied"] == pytest.approx(o.audit()["tr_applied"], rel=1e-12)
+    assert g.audit()["tr_applied"] > 0.0
 
 
 def test_stepwise_jacobi():

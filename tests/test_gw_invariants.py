@@ -401,3 +401,66 @@ def test_coupled_wet_top_face_conductivity(backend, pc):
             TEST_SOIL["theta_r"], TEST_SOIL["aev"])
     want = 0.5 * (ks + vg("conductivity", soil, h_top, ks)) if pc == "legacy" else ks
     assert sim.get(abi.F_KZF)[1, 1, 0] == pytest.approx(want, rel=1e-14)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_sink_in_saturated_confined_cell_drops_head_by_sink_over_ss(backend):
+    # test_gw.cpp:780-823: one sealed saturated cell, Ss 1e-4, a constant
+    # sink S = 1e-6 / 0.5 for one step: h - hn = -S dtg / Ss (1e-12) and
+    # theta stays theta_s (1e-14).  The sink is given directly (the
+    # vegetation module that derives it from T_pot is not part of the step).
+    c = gw_base_case(1, 1, 1, 0.5, 60.0, ks=0.0, specific_storage=1e-4, reallocation="disabled",
+                     ic_form="head", ic_const=2.0)
+    sim = make(backend, c)
+    sink = np.zeros(sim.field_shape(abi.F_SINK))
+    rate = 1e-6 / 0.5
+    sink[1, 1, 0] = rate
+    sim.set_sink(sink)
+    hn = sim.get(abi.F_HEAD)[1, 1, 0]
+    sim.step()
+    h = sim.get(abi.F_HEAD)[1, 1, 0]
+    expected = -rate * 60.0 / 1e-4
+    assert (h - hn) / expected == pytest.approx(1.0, abs=1e-12)
+    assert sim.get(abi.F_WC)[1, 1, 0] == pytest.approx(TEST_SOIL["theta_s"], abs=1e-14)
+    assert sim.audit()["tr_applied"] == pytest.approx(rate * 0.5 * 60.0, rel=1e-15)
+
+
+def _sink_column():
+    c = gw_base_case(2, 2, 10, 0.1, 30.0, specific_storage=1e-5)
+    c.set_bc("top", abi.BC_FLUX, -1e-6)
+    sink = np.zeros((4, 4, 10))
+    sink[1:3, 1:3, :5] = np.linspace(1e-7, 5e-7, 5)
+    return c, sink
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_sink_audit_closes(backend):
+    # the coupled closure's groundwater part with the transpiration channel:
+    # dV = boundary_in - ss_storage + realloc_adjust - vloss - tr_applied,
+    # and tr_applied = sum S V dtg (the sink leaves through no face)
+    c, sink = _sink_column()
+    sim = make(backend, c)
+    sim.set_sink(sink)
+    prev = sim.volume()
+    for n in range(30):
+        sim.step()
+        v, res = closure(sim, prev)
+        a = sim.audit()
+        assert a["tr_applied"] == pytest.approx(sink.sum() * 0.1 * 30.0, rel=1e-13)
+        assert abs(res + a["tr_applied"]) <= 1e-12, n
+        prev = v
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_detached_sink_is_no_sink(backend):
+    c, sink = _sink_column()
+    ref = make(backend, c)
+    sim = make(backend, c)
+    sim.set_sink(sink)
+    sim.set_sink(None)
+    for _ in range(5):
+        ref.step()
+        sim.step()
+    assert np.array_equal(ref.get(abi.F_HEAD), sim.get(abi.F_HEAD))
+    assert np.array_equal(ref.get(abi.F_WC), sim.get(abi.F_WC))
+    assert sim.audit()["tr_applied"] == 0.0
