@@ -81,6 +81,7 @@ class GwCase:
     use_full3d: bool = True
     face_conductivity: str = "arithmetic"
     predictor_corrector: str = "consistent"
+    density_coupling: Optional[dict] = None  # groundwater.density_coupling (betas, gradient)
     reallocation: str = "legacy"
     reallocation_surplus: str = "drop"
     dt_init: float = 1.0

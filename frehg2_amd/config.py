@@ -177,6 +177,16 @@ def load_config(src, base_dir=None):
     case.face_conductivity = str(gw.get("face_conductivity", "arithmetic"))
     case.reallocation = str(gw.get("reallocation", "legacy"))
     case.reallocation_surplus = str(gw.get("reallocation_surplus", "drop"))
+    dc = gw.get("density_coupling") or {}
+    if bool(dc.get("enabled", False)):
+        # src/core/Config.cpp:345-356; the transport module supplies the
+        # scalars, density_ratios() turns them into the solver's input
+        case.density_coupling = dict(
+            beta_saline=float(dc.get("beta_saline", 0.000744)),
+            beta_saline_viscosity=float(dc.get("beta_saline_viscosity", 0.0022)),
+            thermal_expansion=float(dc.get("thermal_expansion", 0.0)),
+            reference_temperature=float(dc.get("reference_temperature", 20.0)),
+            reallocation_gradient=str(dc.get("reallocation_gradient", "freshwater")))
     case.dt_init = float(ts.get("dt_init", 0.0))
     case.dt_min = float(ts.get("dt_min", 0.0))
     case.dt_max = float(ts.get("dt_max", 0.0))
@@ -268,6 +278,19 @@ def load_config(src, base_dir=None):
     for e in entries:
         case.set_bc(e.which, e.code, e.value(0.0), e.mask)
     return case, BoundarySchedule(entries, ny, nx)
+
+
+def density_ratios(coupling: dict, salinity=None, temperature=None):
+    """Cell ratios from the transport scalars (Baroclinic.cpp:78-86):
+    r_rho = 1 + beta_s s - beta_T (T - T0), r_visc = 1 / (1 + beta_sv s);
+    an absent scalar contributes 0.  Feed them to RichardsSolver.set_density
+    with reallocation_gradient=coupling["reallocation_gradient"]."""
+    s = 0.0 if salinity is None else np.asarray(salinity, dtype=np.float64)
+    dt = 0.0 if temperature is None else (np.asarray(temperature, dtype=np.float64)
+                                          - coupling["reference_temperature"])
+    r_rho = 1.0 + coupling["beta_saline"] * s - coupling["thermal_expansion"] * dt
+    r_visc = 1.0 / (1.0 + coupling["beta_saline_viscosity"] * s)
+    return r_rho, r_visc
 
 
 def run_groundwater_loop(solver, schedule: BoundarySchedule, t_end: float, t_start: float = 0.0,

@@ -28,8 +28,10 @@
 //   k_cpl_*             the coupling window around the step (include/
 //                       frehg_coupler.h): src/coupling/Exchange.cpp:21-128
 //                       and the surface depth rule src/swe/WetDry.cpp:24-43
-// Density / viscosity ratios are 1 (no transport module): their factors are
-// omitted, which is exact.
+//   k_density_faces     Baroclinic.cpp:46-187 (face means of the ratios)
+// Density / viscosity ratios: the kernels that carry them are instantiated
+// twice; without ratios (freshwater) each factor is the literal 1.0, which
+// folds away, so that path reads no ratio array.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -111,6 +113,13 @@ struct Dev {
   // predictor takes dtg S V from the rhs (Predictor.cpp:700-726) and the
   // corrector from the theta update (Corrector.cpp:674-705)
   double* sink;
+  // density / viscosity ratios (Baroclinic.cpp), null when none are set (all
+  // 1): cells rRho, rVisc (H); face means rXp, vXp, rYp, vYp (H, the slot of
+  // the face east / north of the cell) and rZp, vZp (F, plane k = the face
+  // above cell k).  densGrad: the reallocation's vertical head gradient
+  // subtracts rZp instead of 1 (reallocation_gradient: density)
+  double *rRho, *rVisc, *rXp, *rYp, *vXp, *vYp, *rZp, *vZp;
+  int densGrad;
   double *Ad, *Axp, *Ayp, *Azp, *rhs, *x;
   float *Ad32, *Axp32, *Ayp32, *Azp32;  // the stencil rounded to FP32 (mixed form), or null
   double* parts;
@@ -181,12 +190,18 @@ __device__ __forceinline__ void add_parts(const Dev& d, const int (&slot)[NV], d
 }
 
 // ---- ghosts (Predictor.cpp:66-87, :757-817; Corrector.cpp:709-756) --------
+// rFace: the boundary face's density ratio; the plus edges scale the depth
+// term by it, the minus edges do not (Predictor.cpp:56-62, 66-87)
 __device__ __forceinline__ double ghost_head(int code, double value, double interior, double zc,
-                                             bool coupled, double bed, double depth) {
+                                             bool coupled, double bed, double depth, double rFace,
+                                             bool plus) {
   if (code == FREHG_BC_HEAD) return value;
   if (code == FREHG_BC_HEAD_HYDROSTATIC) {
-    if (coupled && value > bed) return (bed - zc) + depth;
-    return value - zc;
+    if (coupled && value > bed) {
+      const double hydro = (bed - zc) * rFace;
+      return plus ? hydro + depth * rFace : hydro + depth;
+    }
+    return (value - zc) * rFace;
   }
   return interior;
 }
@@ -221,7 +236,10 @@ __global__ void k_ghosts(Dev d, int doH, int doWc) {
       if (doH) {
         const double zc = d.bot3d[cin] + 0.5 * d.dz3d[cin];
         const double dep = d.coupled ? d.dept[g.c2(ji, ii)] : 0.0;
-        d.h[cg] = ghost_head(code, value, d.h[cin], zc, d.coupled != 0, d.bath[g.c2(ji, ii)], dep);
+        const double* rf = xEdge ? d.rXp : d.rYp;  // face slot: the ghost's (minus), the cell's (plus)
+        const double rFace = rf ? rf[side == 0 ? cg : cin] : 1.0;
+        d.h[cg] = ghost_head(code, value, d.h[cin], zc, d.coupled != 0, d.bath[g.c2(ji, ii)], dep,
+                             rFace, side == 1);
       }
       if (doWc) d.wc[cg] = is_head(code) ? wcs_at(d, cin) : d.wc[cin];
     }
@@ -239,9 +257,13 @@ __global__ void k_classify_coupled(Dev d, double dtg) {
     const double dep = d.dept[c2];
     if (kt < 0 || dep <= 0.0) {
       // a pressurized dry top is a seepage face (mode 3), not under legacy
-      d.cplMode[c2] = (!d.pcLegacy && kt >= 0 && d.h[g.c3(j, i, kt)] > 0.5 * d.dz3d[g.c3(j, i, kt)])
-                          ? 3
-                          : 0;
+      bool pressurized = false;
+      if (!d.pcLegacy && kt >= 0) {
+        const int64_t c = g.c3(j, i, kt);
+        const double rz = d.rZp ? d.rZp[g.cF(j, i, kt)] : 1.0;
+        pressurized = d.h[c] > rz * 0.5 * d.dz3d[c];
+      }
+      d.cplMode[c2] = pressurized ? 3 : 0;
       continue;
     }
     const int64_t c = g.c3(j, i, kt);
@@ -282,7 +304,9 @@ __device__ __forceinline__ bool unsat(const Dev& d, int j, int i, int k) {
 
 // ---- lateral faces: K (Predictor.cpp:127-180, 236-268, 318-374) and, with
 // corr, the Darcy flux (Corrector.cpp:176-366).  dir 0 = x, 1 = y.
-template <int DIR>
+// With DENS the Darcy flux carries the face's viscosity ratio, and its
+// along-layer gravity term r v (consistent pair) (Corrector.cpp:194-247).
+template <int DIR, bool DENS>
 __device__ __forceinline__ void face_lat_at(const Dev& d, int corr, int j, int i,
                                             int k) {
   const Grid& g = d.g;
@@ -326,10 +350,12 @@ __device__ __forceinline__ void face_lat_at(const Dev& d, int corr, int j, int i
     if (!corr) return;
     // Darcy flux (Corrector.cpp:176-366), positive toward -axis
     double q;
+    const double vf = DENS ? (DIR == 0 ? d.vXp : d.vYp)[c] : 1.0;
+    const double rf = DENS ? (DIR == 0 ? d.rXp : d.rYp)[c] : 1.0;
     if (s == 0) {
       // the inner cell (j, 1) / (1, i) is cn and the ghost cell is c
       const double kface = Kn;
-      q = A * kface * cv * (hn - hc) / (0.5 * spacing);
+      q = A * kface * vf * cv * (hn - hc) / (0.5 * spacing);
       const int code = DIR == 0 ? d.sxm[g.c2(j, 1)] : d.sym[g.c2(1, i)];
       const double val = DIR == 0 ? d.vxm[g.c2(j, 1)] : d.vym[g.c2(1, i)];
       if (code == FREHG_BC_NOFLUX) q = 0.0;
@@ -346,7 +372,8 @@ __device__ __forceinline__ void face_lat_at(const Dev& d, int corr, int j, int i
           kface = 0.5 * (Kc + vg_conductivity(soil_at(d, cn), hn, ks[c]));
         if (!actA || !actB) kface = 0.0;
       }
-      q = A * (kface * cv * (hn - hc) / delta + static_cast<double>(sgn) * kface * sv);
+      const double rv = d.pcLegacy ? 1.0 : rf * vf;  // legacy: no ratios on the gravity term
+      q = A * (kface * vf * cv * (hn - hc) / delta + static_cast<double>(sgn) * kface * rv * sv);
       if (uns) q = 0.0;
       if (s == nlast) {
         const int code = DIR == 0 ? d.sxp[g.c2(j, i)] : d.syp[g.c2(j, i)];
@@ -361,6 +388,7 @@ __device__ __forceinline__ void face_lat_at(const Dev& d, int corr, int j, int i
 
 // ---- vertical faces: K (Predictor.cpp:181-232, 280-313, 375-385) and, with
 // corr, the Darcy flux (Corrector.cpp:373-527) with the coupled top modes.
+template <bool DENS>
 __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, int j, int i,
                                           int k, double& clamp) {
   const Grid& g = d.g;
@@ -405,6 +433,8 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
     d.kzF[g.cF(j, i, k)] = v;
     if (!corr) return;
     double q = 0.0;
+    const int64_t fz = g.cF(j, i, k);
+    const double rz = DENS ? d.rZp[fz] : 1.0, vz = DENS ? d.vZp[fz] : 1.0;
     if (actK && !actM) {
       const int64_t c = cb;
       if (d.coupled) {
@@ -418,7 +448,7 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
           q = -take * g.az / dtg;
           d.avail[c2] -= take;
         } else if (mode == 1) {
-          q = g.az * ks * ((hb - depth) / delta - 1.0);
+          q = g.az * ks * vz * ((hb - depth) / delta - rz);
           if (q < 0.0) {
             const double need = -q * dtg / g.az;
             if (need > d.avail[c2]) {
@@ -429,7 +459,7 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
             }
           }
         } else if (mode == 3) {
-          q = g.az * ks * ((hb - depth) / delta - 1.0);
+          q = g.az * ks * vz * ((hb - depth) / delta - rz);
           if (q <= 0.0) {
             clamp += -q * dtg;
             q = 0.0;
@@ -437,7 +467,7 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
         } else {
           const double kG = vg_conductivity(soil_at(d, c), depth, ks);
           const double kface = 0.5 * (kG + Kzb);
-          q = g.az * kface * ((hb - depth) / delta - 1.0);
+          q = g.az * kface * vz * ((hb - depth) / delta - rz);
           if (q <= 0.0) q = 0.0;
           if (d.topCode[c2] == FREHG_BC_FLUX) {
             const double tv = d.topValue[c2];
@@ -449,23 +479,23 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
         if (code == FREHG_BC_FLUX) {
           q = d.topValue[c2] * g.az;
         } else if (code == FREHG_BC_HEAD) {
-          q = g.az * d.ksz[c] * ((hb - d.topValue[c2]) / (0.5 * dzb) - 1.0);
+          q = g.az * d.ksz[c] * vz * ((hb - d.topValue[c2]) / (0.5 * dzb) - rz);
         }
       }
     } else if (k == g.nz && actM) {
       const double kface = Kza;
       const int code = d.botCode[c2];
       if (code == FREHG_BC_FLUX) q = d.botValue[c2] * g.az;
-      else if (code == FREHG_BC_GRAVITY) q = -kface * g.az;
+      else if (code == FREHG_BC_GRAVITY) q = -kface * vz * rz * g.az;
       else if (code == FREHG_BC_HEAD)
-        q = g.az * kface * ((d.botValue[c2] - ha) / (0.5 * dza) - 1.0);
+        q = g.az * kface * vz * ((d.botValue[c2] - ha) / (0.5 * dza) - rz);
     } else if (actM && actK) {
       double kface = base;
       if (d.pcLegacy && d.ksz[ca] != d.ksz[cb])  // legacy: the lower cell's Ks on both flanks
         kface = 0.5 * (vg_conductivity(soil_at(d, ca), ha, d.ksz[cb]) + Kzb);
-      q = g.az * kface * ((hb - ha) / (0.5 * (dza + dzb)) - 1.0);
+      q = g.az * kface * vz * ((hb - ha) / (0.5 * (dza + dzb)) - rz);
     }
-    d.qzF[g.cF(j, i, k)] = q;
+    d.qzF[fz] = q;
   }
 }
 
@@ -474,6 +504,7 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
 // the y face north of it (i >= 1, k < nz) and the z face above (j, i, k)
 // (j, i >= 1), so the operands the three share are fetched once (1.02 ms
 // against 0.36 + 0.48 + 0.27 ms for three kernels at 512 x 512 x 100).
+template <bool DENS>
 __global__ __launch_bounds__(kThreads) void k_faces(Dev d, int corr, double dtg) {
   const Grid& g = d.g;
   const int nzf = g.nz + 1, nxa = g.nx + 1;
@@ -484,10 +515,10 @@ __global__ __launch_bounds__(kThreads) void k_faces(Dev d, int corr, double dtg)
     const int64_t col = r.t / nzf;
     const int i = static_cast<int>(col % nxa), j = static_cast<int>(col / nxa);
     if (k < g.nz) {
-      if (j >= 1) face_lat_at<0>(d, corr, j, i, k);
-      if (i >= 1) face_lat_at<1>(d, corr, j, i, k);
+      if (j >= 1) face_lat_at<0, DENS>(d, corr, j, i, k);
+      if (i >= 1) face_lat_at<1, DENS>(d, corr, j, i, k);
     }
-    if (j >= 1 && i >= 1) face_z_at(d, corr, dtg, j, i, k, clamp);
+    if (j >= 1 && i >= 1) face_z_at<DENS>(d, corr, dtg, j, i, k, clamp);
   }
   if (corr) {
     double v[1] = {clamp};
@@ -496,7 +527,106 @@ __global__ __launch_bounds__(kThreads) void k_faces(Dev d, int corr, double dtg)
   }
 }
 
+// ---- density / viscosity face means (Baroclinic.cpp:46-187) ---------------
+__global__ void k_fill(double* p, int64_t n, double v) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    p[t] = v;
+}
+
+// Thread (j, i, k) of (ny+1) x (nx+1) x (nz+1) sets the x face east of (j, i),
+// the y face north of it and the z face above (j, i, k), as k_faces indexes
+// them.  Arithmetic means between active flanks; a minus-side edge face
+// averages the interior cell with its ghost (the ghost's own ratio under a
+// head condition), a plus-side edge face is 1 unless the edge holds a head
+// condition (then the ghost's ratio); the face above a column's top cell
+// averages the top cell with the ghost above it (top[] when given, else the
+// top cell's own ratio); the bottom face stays 1.  Slots no rule reaches
+// keep the 1 they were filled with.
+__global__ void k_density_faces(Dev d, const double* rrTop, const double* rvTop) {
+  const Grid& g = d.g;
+  const int nzf = g.nz + 1, nxa = g.nx + 1;
+  const int64_t n = static_cast<int64_t>(g.ny + 1) * nxa * nzf;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int k = static_cast<int>(t % nzf);
+    const int64_t col = t / nzf;
+    const int i = static_cast<int>(col % nxa), j = static_cast<int>(col / nxa);
+    if (k < g.nz && j >= 1) {  // x face east of (j, i), i in [0, nx]
+      const int64_t c = g.c3(j, i, k), ce = g.c3(j, i + 1, k);
+      double r = 1.0, v = 1.0;
+      if (i == 0) {
+        if (act(d, j, 1, k)) {
+          if (is_head(d.sxm[g.c2(j, 1)])) {
+            r = d.rRho[c];
+            v = d.rVisc[c];
+          } else {
+            r = 0.5 * (d.rRho[ce] + d.rRho[c]);
+            v = 0.5 * (d.rVisc[ce] + d.rVisc[c]);
+          }
+        }
+      } else if (i == g.nx) {
+        if (act(d, j, i, k) && is_head(d.sxp[g.c2(j, i)])) {
+          r = d.rRho[ce];
+          v = d.rVisc[ce];
+        }
+      } else if (act(d, j, i, k) && act(d, j, i + 1, k)) {
+        r = 0.5 * (d.rRho[c] + d.rRho[ce]);
+        v = 0.5 * (d.rVisc[c] + d.rVisc[ce]);
+      }
+      d.rXp[c] = r;
+      d.vXp[c] = v;
+    }
+    if (k < g.nz && i >= 1) {  // y face north of (j, i), j in [0, ny]
+      const int64_t c = g.c3(j, i, k), cn = g.c3(j + 1, i, k);
+      double r = 1.0, v = 1.0;
+      if (j == 0) {
+        if (act(d, 1, i, k)) {
+          if (is_head(d.sym[g.c2(1, i)])) {
+            r = d.rRho[c];
+            v = d.rVisc[c];
+          } else {
+            r = 0.5 * (d.rRho[cn] + d.rRho[c]);
+            v = 0.5 * (d.rVisc[cn] + d.rVisc[c]);
+          }
+        }
+      } else if (j == g.ny) {
+        if (act(d, j, i, k) && is_head(d.syp[g.c2(j, i)])) {
+          r = d.rRho[cn];
+          v = d.rVisc[cn];
+        }
+      } else if (act(d, j, i, k) && act(d, j + 1, i, k)) {
+        r = 0.5 * (d.rRho[c] + d.rRho[cn]);
+        v = 0.5 * (d.rVisc[c] + d.rVisc[cn]);
+      }
+      d.rYp[c] = r;
+      d.vYp[c] = v;
+    }
+    if (j >= 1 && i >= 1) {  // z face above (j, i, k), k in [0, nz]
+      const bool below = k < g.nz && act(d, j, i, k);
+      const bool above = k >= 1 && act(d, j, i, k - 1);
+      double r = 1.0, v = 1.0;
+      if (below && above) {
+        const int64_t c = g.c3(j, i, k);
+        r = 0.5 * (d.rRho[c - 1] + d.rRho[c]);
+        v = 0.5 * (d.rVisc[c - 1] + d.rVisc[c]);
+      } else if (below) {
+        const int64_t c = g.c3(j, i, k);
+        const int c2 = g.c2(j, i);
+        r = 0.5 * (d.rRho[c] + (rrTop ? rrTop[c2] : d.rRho[c]));
+        v = 0.5 * (d.rVisc[c] + (rvTop ? rvTop[c2] : d.rVisc[c]));
+      }
+      d.rZp[g.cF(j, i, k)] = r;
+      d.vZp[g.cF(j, i, k)] = v;
+    }
+  }
+}
+
 // ---- assembly (Predictor.cpp:400-698) into stencil storage ---------------
+// With DENS the legs carry r v of their face, the storage r of the cell and
+// the gravity terms r^2 v (Predictor.cpp:474-687); without, every ratio is
+// the literal 1.0 and folds away.
+template <bool DENS>
 __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
   const Grid& g = d.g;
   const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
@@ -540,6 +670,15 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
     const int cE = d.sxp[c2], cW = d.sxm[c2], cN = d.syp[c2], cS = d.sym[c2];
     const int bot = d.botCode[c2];
     const double wcnc = d.wcn[c], wcsc = wcs_at(d, c), hnc = d.hn[c];
+    double rc = 1.0, rxc = 1.0, rxW = 1.0, vxc = 1.0, vxW = 1.0, ryc = 1.0, ryS = 1.0, vyc = 1.0,
+           vyS = 1.0, rzT = 1.0, rzB = 1.0, vzT = 1.0, vzB = 1.0;
+    if (DENS) {
+      const int64_t f = g.cF(j, i, k);
+      rc = d.rRho[c];
+      rxc = d.rXp[c], rxW = d.rXp[cxm], vxc = d.vXp[c], vxW = d.vXp[cxm];
+      ryc = d.rYp[c], ryS = d.rYp[cym], vyc = d.vYp[c], vyS = d.vYp[cym];
+      rzT = d.rZp[f], rzB = d.rZp[f + 1], vzT = d.vZp[f], vzB = d.vZp[f + 1];
+    }
     double b3c = 0.0, b3E = 0.0, b3W = 0.0, b3N = 0.0, b3S = 0.0;
     double sinxc = 0.0, sinxW = 0.0, sinyc = 0.0, sinyS = 0.0;
     if (d.terrain) {
@@ -589,10 +728,10 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
     }
     if (!d.full3d && (unsat(d, j - 1, i, k) || unsatC)) kym = 0.0;
     if (!south && k < ktS) kym = 0.0;
-    double gxp = -kxp * dtg * cosxc / (dx * dx) * (axc * dx);
-    double gxm = -kxm * dtg * cosxW / (dx * dx) * (axW * dx);
-    double gyp = -kyp * dtg * cosyc / (dy * dy) * (ayc * dy);
-    double gym = -kym * dtg * cosyS / (dy * dy) * (ayS * dy);
+    double gxp = -kxp * dtg * rxc * vxc * cosxc / (dx * dx) * (axc * dx);
+    double gxm = -kxm * dtg * rxW * vxW * cosxW / (dx * dx) * (axW * dx);
+    double gyp = -kyp * dtg * ryc * vyc * cosyc / (dy * dy) * (ayc * dy);
+    double gym = -kym * dtg * ryS * vyS * cosyS / (dy * dy) * (ayS * dy);
     if (east && is_head(cE)) gxp *= 2.0;
     if (west && is_head(cW)) gxm *= 2.0;
     if (north && is_head(cN)) gyp *= 2.0;
@@ -620,11 +759,11 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
     if (k == g.nz - 1) kzBot = bot == FREHG_BC_NOFLUX ? 0.0 : Kzc;
     else kzBot = d.harmonic ? face_k_harmonic(Kzc, d.ksz[c], dzc, KzU, d.ksz[c + 1], dzU)
                             : 0.5 * (Kzc + KzU);
-    const double gzp = -kzBot * dtg / (dzc * dzf) * volume;
+    const double gzp = -kzBot * dtg * rzB * vzB / (dzc * dzf) * volume;
     dzf = isTop ? 0.5 * dzc : 0.5 * (dzc + dzD);
-    const double gzm = -kzTop * dtg / (dzc * dzf) * volume;
+    const double gzm = -kzTop * dtg * rzT * vzT / (dzc * dzf) * volume;
 
-    const double storage = (chv + d.ss * wcnc / wcsc);
+    const double storage = (chv + d.ss * wcnc / wcsc) * rc;
     double gct = storage * volume - (gxp + gxm + gyp + gym);
     if (east && cE == FREHG_BC_FLUX) gct += gxp;
     if (west && cW == FREHG_BC_FLUX) gct += gxm;
@@ -641,36 +780,39 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
     }
 
     double b = storage * hnc * volume;
-    b -= dtg * volume * kzBot / dzc;
-    b += dtg * volume * kzTop / dzc;
+    b -= dtg * volume * kzBot * rzB * rzB * vzB / dzc;
+    b += dtg * volume * kzTop * rzT * rzT * vzT / dzc;
     if (d.terrain) {
+      // the along-layer gravity term, K r^2 v sin (legacy: K sin)
+      const bool plain = d.pcLegacy;
       double sg = (b3E > b3c) ? 1.0 : -1.0;
-      b += sg * dtg * axc * kxp * sinxc;
+      b += sg * dtg * axc * kxp * (plain ? 1.0 : rxc * rxc * vxc) * sinxc;
       sg = (b3c > b3W) ? -1.0 : 1.0;
-      b += sg * dtg * axW * kxm * sinxW;
+      b += sg * dtg * axW * kxm * (plain ? 1.0 : rxW * rxW * vxW) * sinxW;
       sg = (b3N > b3c) ? 1.0 : -1.0;
-      b += sg * dtg * ayc * kyp * sinyc;
+      b += sg * dtg * ayc * kyp * (plain ? 1.0 : ryc * ryc * vyc) * sinyc;
       sg = (b3c > b3S) ? -1.0 : 1.0;
-      b += sg * dtg * ayS * kym * sinyS;
+      b += sg * dtg * ayS * kym * (plain ? 1.0 : ryS * ryS * vyS) * sinyS;
     }
     if (east) {
-      if (cE == FREHG_BC_FLUX) b += axc * d.vxp[c2] * dtg;
+      if (cE == FREHG_BC_FLUX) b += rxc * axc * d.vxp[c2] * dtg;
       else b -= gxp * d.hn[cxp];
     }
     if (west) {
-      if (cW == FREHG_BC_FLUX) b += axW * d.vxm[c2] * dtg;
+      if (cW == FREHG_BC_FLUX) b += rxW * axW * d.vxm[c2] * dtg;
       else b -= gxm * d.hn[cxm];
     }
     if (north) {
-      if (cN == FREHG_BC_FLUX) b += ayc * d.vyp[c2] * dtg;
+      if (cN == FREHG_BC_FLUX) b += ryc * ayc * d.vyp[c2] * dtg;
       else b -= gyp * d.hn[cyp];
     }
     if (south) {
-      if (cS == FREHG_BC_FLUX) b += ayS * d.vym[c2] * dtg;
+      if (cS == FREHG_BC_FLUX) b += ryS * ayS * d.vym[c2] * dtg;
       else b -= gym * d.hn[cym];
     }
     if (k == g.nz - 1) {
-      if (bot == FREHG_BC_FLUX) b += dtg * volume * (d.botValue[c2] + kzBot) / dzc;
+      if (bot == FREHG_BC_FLUX)
+        b += dtg * rzB * volume * (d.botValue[c2] + kzBot * rzB * vzB) / dzc;
       else if (bot == FREHG_BC_HEAD) b -= gzp * d.botValue[c2];
     }
     if (isTop) {
@@ -683,19 +825,19 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
           const double roomTop = fmax(0.0, (0.9999 * wcsc - wcnc) * dzc);
           const double take = fmin(d.avail[c2], roomTop);
           const double qSupply = -take / dtg;
-          b += -dtg * volume * (qSupply + kzTop) / dzc;
+          b += -dtg * rzT * volume * (qSupply + kzTop * rzT * vzT) / dzc;
         } else {
           double qtop = 0.0;
           if (d.topCode[c2] == FREHG_BC_FLUX) {
             const double tv = d.topValue[c2];
             if ((tv > 0.0 && wcnc < wcsc) || (tv < 0.0 && wcnc > wcr_at(d, c))) qtop = tv;
           }
-          b += -dtg * volume * (qtop + kzTop) / dzc;
+          b += -dtg * rzT * volume * (qtop + kzTop * rzT * vzT) / dzc;
         }
       } else if (d.topCode[c2] == FREHG_BC_HEAD) {
         b -= gzm * d.topValue[c2];
       } else if (d.topCode[c2] == FREHG_BC_FLUX) {
-        b += -dtg * volume * (d.topValue[c2] + kzTop) / dzc;
+        b += -dtg * rzT * volume * (d.topValue[c2] + kzTop * rzT * vzT) / dzc;
       }
     }
     const double axp = (!east && k >= ktE) ? gxp : 0.0;
@@ -703,7 +845,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
     const double azp = (k + 1 < g.nz) ? gzp : 0.0;
     if (d.sink) {
       const double sk = d.sink[c];
-      if (sk != 0.0) b -= dtg * sk * volume;
+      if (sk != 0.0) b -= dtg * rc * sk * volume;
     }
     d.Ad[c] = gct;
     d.Axp[c] = axp;
@@ -862,7 +1004,9 @@ __global__ void k_cpl_reduce_audit(Dev d, const double* __restrict__ bottom) {
 // ---- theta update (Corrector.cpp:629-672) ----------------------------------
 // With SINK the root-uptake sink leaves after the flux update, as a second
 // rounding step (Corrector.cpp:674-705), and its volume S V dtg is summed.
-template <bool SINK>
+// With DENS the fluxes are weighted by their face's density ratio and the
+// storage by the cell's (Corrector.cpp:653-669).
+template <bool SINK, bool DENS>
 __global__ void k_update_wc(Dev d, double dtg) {
   const Grid& g = d.g;
   const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
@@ -876,18 +1020,27 @@ __global__ void k_update_wc(Dev d, double dtg) {
     const int64_t c = g.c3(j, i, k);
     const double volume = g.az * d.dz3d[c];
     const double dh = d.h[c] - d.hn[c];
-    const double coeff = volume * (1.0 + d.ss * dh / wcs_at(d, c));
-    const double dqx = dtg * (d.qx[c] - d.qx[c - g.nz]);
-    const double dqy = dtg * (d.qy[c] - d.qy[c - sy]);
-    const double dqz = dtg * (d.qzF[g.cF(j, i, k + 1)] - d.qzF[g.cF(j, i, k)]);
-    double upd = (d.wcn[c] * volume + dqx + dqy + dqz) / coeff;
-    ss.add(upd * volume * d.ss * dh / wcs_at(d, c));
+    const int64_t f = g.cF(j, i, k);
+    const double rr = DENS ? d.rRho[c] : 1.0;
+    const double coeff = volume * (rr + rr * d.ss * dh / wcs_at(d, c));
+    double dqx, dqy, dqz;
+    if (DENS) {
+      dqx = dtg * (d.qx[c] * d.rXp[c] - d.qx[c - g.nz] * d.rXp[c - g.nz]);
+      dqy = dtg * (d.qy[c] * d.rYp[c] - d.qy[c - sy] * d.rYp[c - sy]);
+      dqz = dtg * (d.qzF[f + 1] * d.rZp[f + 1] - d.qzF[f] * d.rZp[f]);
+    } else {
+      dqx = dtg * (d.qx[c] - d.qx[c - g.nz]);
+      dqy = dtg * (d.qy[c] - d.qy[c - sy]);
+      dqz = dtg * (d.qzF[f + 1] - d.qzF[f]);
+    }
+    double upd = (d.wcn[c] * rr * volume + dqx + dqy + dqz) / coeff;
+    ss.add(upd * volume * rr * d.ss * dh / wcs_at(d, c));
     if (SINK) {
       const double sk = d.sink[c];
       if (sk != 0.0) {
-        const double dtheta = dtg * sk * volume / coeff;
+        const double dtheta = dtg * rr * sk * volume / coeff;
         upd -= dtheta;
-        ss.add(-(dtheta * volume * d.ss * dh / wcs_at(d, c)));
+        ss.add(-(dtheta * volume * rr * d.ss * dh / wcs_at(d, c)));
         tr.add(sk * volume * dtg);
       }
     }
@@ -933,13 +1086,13 @@ __device__ Split gradient_split(const Dev& d, int j, int i, int k) {
   d6[3] = (k >= ktS && kyS > 0.0) ? (h0 - hS) / g.dy : 0.0;
   if (k + 1 < g.nz && k + 1 >= kt0 && kzUp > 0.0) {
     const double dzp = 0.5 * (dz0 + dzU);
-    d6[4] = (hU - h0) / dzp - 1.0;
+    d6[4] = (hU - h0) / dzp - (d.densGrad ? d.rZp[f + 1] : 1.0);
   } else {
     d6[4] = 0.0;
   }
   if (k - 1 >= 0 && k - 1 >= kt0 && kzDn > 0.0) {
     const double dzm = 0.5 * (dz0 + dzD);
-    d6[5] = (h0 - hD) / dzm - 1.0;
+    d6[5] = (h0 - hD) / dzm - (d.densGrad ? d.rZp[f] : 1.0);
   } else {
     d6[5] = 0.0;
   }
@@ -1861,7 +2014,8 @@ struct GwSolver {
     HIP_CHECK(hipGetLastError());
     // the predictor's face K are formed inside the assembly
     if (!corr) return;
-    k_faces<<<rb, kThreads, 0, stream>>>(d, corr, dtg);
+    if (d.rRho) k_faces<true><<<rb, kThreads, 0, stream>>>(d, corr, dtg);
+    else k_faces<false><<<rb, kThreads, 0, stream>>>(d, corr, dtg);
     HIP_CHECK(hipGetLastError());
   }
   void copy(double* dst, const double* src, int64_t n) {
@@ -1941,6 +2095,51 @@ struct GwSolver {
     d.sink = sinkBuf;
   }
   double* sinkBuf = nullptr;
+  // density / viscosity ratios: rr, rv = cell ratios (H, ghosts included);
+  // rrTop, rvTop = the ghost above each column's top cell (2D halo layout) or
+  // null for the top cell's own; rr = null resets every ratio to 1
+  void setDensity(const double* rr, const double* rv, const double* rrTop, const double* rvTop,
+                  int gradient) {
+    if (!rr) {
+      d.rRho = d.rVisc = d.rXp = d.rYp = d.vXp = d.vYp = d.rZp = d.vZp = nullptr;
+      d.densGrad = 0;
+    } else {
+      const int64_t nH = g.cellsH(), nF = g.cellsF(), n2 = static_cast<int64_t>(g.ny2) * g.nx2;
+      if (!densBuf[0]) {
+        for (int q = 0; q < 6; ++q) densBuf[q] = alloc<double>(nH);
+        densBuf[6] = alloc<double>(nF);
+        densBuf[7] = alloc<double>(nF);
+        densTop[0] = alloc<double>(n2);
+        densTop[1] = alloc<double>(n2);
+      }
+      d.rRho = densBuf[0], d.rVisc = densBuf[1], d.rXp = densBuf[2], d.rYp = densBuf[3];
+      d.vXp = densBuf[4], d.vYp = densBuf[5], d.rZp = densBuf[6], d.vZp = densBuf[7];
+      const int fb = static_cast<int>(std::min<int64_t>((nF + kThreads - 1) / kThreads, 4096));
+      HIP_CHECK(hipMemcpyAsync(d.rRho, rr, nH * sizeof(double), hipMemcpyHostToDevice, stream));
+      if (rv)
+        HIP_CHECK(hipMemcpyAsync(d.rVisc, rv, nH * sizeof(double), hipMemcpyHostToDevice, stream));
+      else
+        k_fill<<<fb, kThreads, 0, stream>>>(d.rVisc, nH, 1.0);
+      for (int q = 2; q < 6; ++q) k_fill<<<fb, kThreads, 0, stream>>>(densBuf[q], nH, 1.0);
+      for (int q = 6; q < 8; ++q) k_fill<<<fb, kThreads, 0, stream>>>(densBuf[q], nF, 1.0);
+      HIP_CHECK(hipGetLastError());
+      const double* top[2] = {rrTop, rvTop};
+      const double* topDev[2] = {nullptr, nullptr};
+      for (int q = 0; q < 2; ++q)
+        if (top[q]) {
+          HIP_CHECK(hipMemcpyAsync(densTop[q], top[q], n2 * sizeof(double), hipMemcpyHostToDevice,
+                                   stream));
+          topDev[q] = densTop[q];
+        }
+      k_density_faces<<<fb, kThreads, 0, stream>>>(d, topDev[0], topDev[1]);
+      HIP_CHECK(hipGetLastError());
+      d.densGrad = gradient ? 1 : 0;
+    }
+    ghosts(true, false);  // hydrostatic ghosts carry the boundary-face ratio
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+  double* densBuf[8] = {};
+  double* densTop[2] = {};
 };
 
 void GwSolver::buildGeometry(const frehg_gw_desc& ds, std::vector<double>& bath,
@@ -2364,7 +2563,8 @@ void GwSolver::step(double /*t*/, double dtg) {
     HIP_CHECK(hipGetLastError());
   }
   faces(0, dtg);
-  k_assemble<<<rb, kThreads, 0, stream>>>(d, dtg);
+  if (d.rRho) k_assemble<true><<<rb, kThreads, 0, stream>>>(d, dtg);
+  else k_assemble<false><<<rb, kThreads, 0, stream>>>(d, dtg);
   HIP_CHECK(hipGetLastError());
   if (timing) HIP_CHECK(hipEventRecord(evs0, stream));
   Stencil<double> A{d.Ad, d.Axp, d.Ayp, d.Azp};
@@ -2394,8 +2594,10 @@ void GwSolver::step(double /*t*/, double dtg) {
     k_coupled_top<<<rb, kThreads, 0, stream>>>(d, dtg);
     HIP_CHECK(hipGetLastError());
   }
-  if (d.sink) k_update_wc<true><<<rb, kThreads, 0, stream>>>(d, dtg);
-  else k_update_wc<false><<<rb, kThreads, 0, stream>>>(d, dtg);
+  if (d.sink && d.rRho) k_update_wc<true, true><<<rb, kThreads, 0, stream>>>(d, dtg);
+  else if (d.sink) k_update_wc<true, false><<<rb, kThreads, 0, stream>>>(d, dtg);
+  else if (d.rRho) k_update_wc<false, true><<<rb, kThreads, 0, stream>>>(d, dtg);
+  else k_update_wc<false, false><<<rb, kThreads, 0, stream>>>(d, dtg);
   HIP_CHECK(hipGetLastError());
   // post-allocation, final clamp, boundary inflow and dt inputs: one column
   // kernel writing the second buffer pair, which then becomes the state
@@ -2593,6 +2795,13 @@ int frehg_gw_set_field(frehg_gw_solver* s, int field, const double* host_in) {
     g->copy(g->d.hn, g->d.h, g->g.cellsH());
     g->copy(g->d.wcn, g->d.wc, g->g.cellsH());
     HIP_CHECK(hipStreamSynchronize(g->stream));
+  });
+}
+int frehg_gw_set_density(frehg_gw_solver* s, const double* r_rho, const double* r_visc,
+                         const double* r_rho_top, const double* r_visc_top,
+                         int reallocation_gradient) {
+  return guard([&] {
+    frehg::S(s)->setDensity(r_rho, r_visc, r_rho_top, r_visc_top, reallocation_gradient);
   });
 }
 int frehg_gw_set_bc_values(frehg_gw_solver* s, int which, const double* values) {

@@ -64,6 +64,25 @@ class RichardsSolver:
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.field_shape(fid))
         check(self.lib, self.lib.frehg_gw_set_field(self._h, fid, _dptr(a)))
 
+    def set_density(self, r_rho, r_visc=None, r_rho_top=None, r_visc_top=None,
+                    reallocation_gradient="freshwater") -> None:
+        """Density / viscosity ratios (RichardsSolver::updateBaroclinicFaces,
+        Baroclinic.cpp:46-187): r_rho, r_visc per cell in halo layout
+        (ny+2, nx+2, nz), ghosts included; r_rho_top / r_visc_top the ghost
+        above each column's top cell, (ny+2, nx+2), None for the top cell's
+        own.  r_rho None resets every ratio to 1."""
+        keep, args = [], []
+        shapes = [self.field_shape(abi.F_HEAD)] * 2 + [(self.ny + 2, self.nx + 2)] * 2
+        for a, shape in zip((r_rho, r_visc, r_rho_top, r_visc_top), shapes):
+            if a is None:
+                args.append(None)
+            else:
+                arr = np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.float64)
+                keep.append(arr)
+                args.append(_dptr(arr))
+        check(self.lib, self.lib.frehg_gw_set_density(
+            self._h, *args, int(reallocation_gradient == "density")))
+
     def set_sink(self, sink) -> None:
         """The root-uptake sink S [1/s] per cell, halo layout (ny+2, nx+2, nz),
         applied from the next step on (RichardsSolver::sink_,

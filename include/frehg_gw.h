@@ -212,6 +212,22 @@ int frehg_gw_get_field(frehg_gw_solver* s, int field, double* host_out);
  * 560-634, consumed at src/gw/Predictor.cpp:700-726 and
  * src/gw/Corrector.cpp:674-705); host_in = NULL detaches it. */
 int frehg_gw_set_field(frehg_gw_solver* s, int field, const double* host_in);
+/* Density / viscosity ratios from the transport module's scalars
+ * (RichardsSolver::updateBaroclinicFaces, src/gw/Baroclinic.cpp:46-187, with
+ * groundwater.density_coupling, src/core/Config.cpp:345-356): r_rho = 1 +
+ * beta_s s - beta_T (T - T0) and r_visc = 1 / (1 + beta_sv s) per cell, halo
+ * layout including ghosts (the caller evaluates them from its scalars).
+ * r_rho_top / r_visc_top: the ratios of the ghost above each column's top cell
+ * (2D (ny+2)(nx+2); the surface scalar's in coupled runs), NULL for the top
+ * cell's own.  r_visc NULL = 1.  The face means follow Baroclinic.cpp's edge
+ * rules; the ratios then enter the assembly, the Darcy fluxes, the theta
+ * update, the hydrostatic ghosts and the pressurized-top test as in
+ * Predictor.cpp / Corrector.cpp.  reallocation_gradient = 1 is
+ * density_coupling.reallocation_gradient: density (Reallocate.cpp:221-233).
+ * r_rho NULL resets every ratio to 1. */
+int frehg_gw_set_density(frehg_gw_solver* s, const double* r_rho, const double* r_visc,
+                         const double* r_rho_top, const double* r_visc_top,
+                         int reallocation_gradient);
 /* Refresh per-column boundary values (updateBoundaryValues,
  * src/gw/RichardsSolver.cpp:526-566); which = 0 top, 1 bottom, 2 x-, 3 x+,
  * 4 y-, 5 y+; values is ny*nx. */

@@ -27,8 +27,9 @@
 //   Grid.cpp:160-245               compressed (j,i,k) global ids
 //   LinearSystem.cpp:147-227,332-484  CG semantics (PETSc KSPCG replaced by
 //                                  an explicit PCG; see the solver notes)
-// Density/viscosity ratios are 1 in every formula (no transport module), so
-// their factors are omitted: multiplication by 1.0 is exact.
+// Density/viscosity ratios (r_rho, r_visc and their face means, Baroclinic.cpp)
+// enter every formula where the reference has them; they are 1 until a caller
+// sets cell ratios (oracle_gw_set_density), and multiplying by 1.0 is exact.
 //
 // Parity pins (tests/test_oracle_*.py): the reference's hand values
 // (tests/unit/test_vangenuchten.cpp:43-126), its gw invariants
@@ -212,6 +213,12 @@ struct Oracle {
   F3 h, hn, wc, wcn, ch, ksx, ksy, ksz, vga, vgn, wcs, wcr, aev;
   F3 sink;               // root-uptake sink S [1/s] (RichardsSolver.hpp:656)
   bool sinkActive = false;
+  // density / viscosity ratios: cells (H, ghosts included), x+ / y+ face means
+  // (H) and z face means (F, plane k = face above cell k); all 1 by default
+  F3 rRho, rVisc, rXp, rYp, vXp, vYp, rZp, vZp;
+  bool densityGradient = false;  // reallocation_gradient: density
+  void setDensity(const double* rr, const double* rv, const double* rrTop, const double* rvTop,
+                  bool gradient);
   F3 kx, ky, kzF, qx, qy, qzF, vloss, room, sendUp, sendDown, wcSnap, hSnap;
   // boundary markers
   F2<int> topCode, botCode, sxm, sxp, sym, syp, cplMode;
@@ -552,6 +559,9 @@ void Oracle::create(const frehg_gw_desc& d) {
                 &qx, &qy, &vloss, &room, &sendUp, &sendDown, &wcSnap, &hSnap})
     f->init(ny2, nx2, nz, 0.0);
   kzF.init(ny2, nx2, nz + 1, 0.0);
+  for (F3* f : {&rRho, &rVisc, &rXp, &rYp, &vXp, &vYp}) f->init(ny2, nx2, nz, 1.0);
+  rZp.init(ny2, nx2, nz + 1, 1.0);
+  vZp.init(ny2, nx2, nz + 1, 1.0);
   qzF.init(ny2, nx2, nz + 1, 0.0);
   // soil staging: interior fill + mirrored edge ghosts (RichardsSolver.cpp:207-329, :24-57)
   const double* src[8] = {d.ksx, d.ksy, d.ksz, d.vg_alpha, d.vg_n, d.theta_s, d.theta_r, d.aev};
@@ -598,16 +608,106 @@ void Oracle::create(const frehg_gw_desc& d) {
   applyIC(d);
 }
 
+// --- Baroclinic.cpp:46-187: face means of the density / viscosity ratios ----
+// rr, rv: cell ratios over the halo box (H); rrTop, rvTop: the ratios of the
+// ghost above each column's top cell (2D, (ny+2)(nx+2); the surface scalar's
+// in coupled runs), null for the top cell's own.  rr = null resets every
+// ratio to 1.
+void Oracle::setDensity(const double* rr, const double* rv, const double* rrTop,
+                        const double* rvTop, bool gradient) {
+  for (F3* f : {&rRho, &rVisc, &rXp, &rYp, &vXp, &vYp}) std::fill(f->v.begin(), f->v.end(), 1.0);
+  std::fill(rZp.v.begin(), rZp.v.end(), 1.0);
+  std::fill(vZp.v.begin(), vZp.v.end(), 1.0);
+  densityGradient = gradient && rr != nullptr;
+  if (!rr) return;
+  std::copy(rr, rr + rRho.v.size(), rRho.v.begin());
+  if (rv) std::copy(rv, rv + rVisc.v.size(), rVisc.v.begin());
+  const auto headCode = [](int c) { return isHead(c); };
+  // x faces: slot i is the face east of cell i, slot 0 the west boundary face
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 0; i <= nx; ++i)
+      for (int k = 0; k < nz; ++k) {
+        double r = 1.0, v = 1.0;
+        if (i == 0) {
+          if (g(j, 1, k) >= 0) {
+            if (headCode(sxm(j, 1))) {
+              r = rRho(j, 0, k);
+              v = rVisc(j, 0, k);
+            } else {
+              r = 0.5 * (rRho(j, 1, k) + rRho(j, 0, k));
+              v = 0.5 * (rVisc(j, 1, k) + rVisc(j, 0, k));
+            }
+          }
+        } else if (i == nx) {
+          if (g(j, i, k) >= 0 && headCode(sxp(j, i))) {
+            r = rRho(j, i + 1, k);
+            v = rVisc(j, i + 1, k);
+          }
+        } else if (g(j, i, k) >= 0 && g(j, i + 1, k) >= 0) {
+          r = 0.5 * (rRho(j, i, k) + rRho(j, i + 1, k));
+          v = 0.5 * (rVisc(j, i, k) + rVisc(j, i + 1, k));
+        }
+        rXp(j, i, k) = r;
+        vXp(j, i, k) = v;
+      }
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i)
+      for (int k = 0; k < nz; ++k) {
+        double r = 1.0, v = 1.0;
+        if (j == 0) {
+          if (g(1, i, k) >= 0) {
+            if (headCode(sym(1, i))) {
+              r = rRho(0, i, k);
+              v = rVisc(0, i, k);
+            } else {
+              r = 0.5 * (rRho(1, i, k) + rRho(0, i, k));
+              v = 0.5 * (rVisc(1, i, k) + rVisc(0, i, k));
+            }
+          }
+        } else if (j == ny) {
+          if (g(j, i, k) >= 0 && headCode(syp(j, i))) {
+            r = rRho(j + 1, i, k);
+            v = rVisc(j + 1, i, k);
+          }
+        } else if (g(j, i, k) >= 0 && g(j + 1, i, k) >= 0) {
+          r = 0.5 * (rRho(j, i, k) + rRho(j + 1, i, k));
+          v = 0.5 * (rVisc(j, i, k) + rVisc(j + 1, i, k));
+        }
+        rYp(j, i, k) = r;
+        vYp(j, i, k) = v;
+      }
+  // z faces: plane k above cell k; the bottom plane nz stays 1
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i)
+      for (int k = 0; k <= nz; ++k) {
+        const bool below = k < nz && g(j, i, k) >= 0;
+        const bool above = k >= 1 && g(j, i, k - 1) >= 0;
+        double r = 1.0, v = 1.0;
+        if (below && above) {
+          r = 0.5 * (rRho(j, i, k - 1) + rRho(j, i, k));
+          v = 0.5 * (rVisc(j, i, k - 1) + rVisc(j, i, k));
+        } else if (below) {
+          const size_t c2 = static_cast<size_t>(j) * (nx + 2) + i;
+          r = 0.5 * (rRho(j, i, k) + (rrTop ? rrTop[c2] : rRho(j, i, k)));
+          v = 0.5 * (rVisc(j, i, k) + (rvTop ? rvTop[c2] : rVisc(j, i, k)));
+        }
+        rZp(j, i, k) = r;
+        vZp(j, i, k) = v;
+      }
+}
+
 // --- ghost heads (Predictor.cpp:66-87, :757-817) --------------------------
+// rFace: the boundary face's density ratio; the plus edges scale the depth
+// term by it, the minus edges do not (Predictor.cpp:56-62)
 static double ghost_head(int code, double value, double interior, double zc, bool coupled,
-                         double bed, double depth, bool plus) {
+                         double bed, double depth, double rFace, bool plus) {
   if (code == FREHG_BC_HEAD) return value;
   if (code == FREHG_BC_HEAD_HYDROSTATIC) {
     if (coupled && value > bed) {
-      const double hydro = (bed - zc);
-      return plus ? hydro + depth : hydro + depth;
+      const double hydro = (bed - zc) * rFace;
+      return plus ? hydro + depth * rFace : hydro + depth;
     }
-    return (value - zc);
+    return (value - zc) * rFace;
   }
   return interior;
 }
@@ -617,18 +717,22 @@ void Oracle::enforceHeadBc() {
   for (int j = 1; j <= ny; ++j)
     for (int k = 0; k < nz; ++k) {
       double zc = bot3d(j, 1, k) + 0.5 * dz3d(j, 1, k);
-      h(j, 0, k) = ghost_head(sxm(j, 1), vxm(j, 1), h(j, 1, k), zc, coupled, bath(j, 1), dep(j, 1), false);
+      h(j, 0, k) = ghost_head(sxm(j, 1), vxm(j, 1), h(j, 1, k), zc, coupled, bath(j, 1), dep(j, 1),
+                              rXp(j, 0, k), false);
       zc = bot3d(j, nx, k) + 0.5 * dz3d(j, nx, k);
       h(j, nx + 1, k) =
-          ghost_head(sxp(j, nx), vxp(j, nx), h(j, nx, k), zc, coupled, bath(j, nx), dep(j, nx), true);
+          ghost_head(sxp(j, nx), vxp(j, nx), h(j, nx, k), zc, coupled, bath(j, nx), dep(j, nx),
+                     rXp(j, nx, k), true);
     }
   for (int i = 1; i <= nx; ++i)
     for (int k = 0; k < nz; ++k) {
       double zc = bot3d(1, i, k) + 0.5 * dz3d(1, i, k);
-      h(0, i, k) = ghost_head(sym(1, i), vym(1, i), h(1, i, k), zc, coupled, bath(1, i), dep(1, i), false);
+      h(0, i, k) = ghost_head(sym(1, i), vym(1, i), h(1, i, k), zc, coupled, bath(1, i), dep(1, i),
+                              rYp(0, i, k), false);
       zc = bot3d(ny, i, k) + 0.5 * dz3d(ny, i, k);
       h(ny + 1, i, k) =
-          ghost_head(syp(ny, i), vyp(ny, i), h(ny, i, k), zc, coupled, bath(ny, i), dep(ny, i), true);
+          ghost_head(syp(ny, i), vyp(ny, i), h(ny, i, k), zc, coupled, bath(ny, i), dep(ny, i),
+                     rYp(ny, i, k), true);
     }
 }
 
@@ -659,7 +763,7 @@ void Oracle::classifyCoupledTop(double dtg) {
       const size_t c2 = static_cast<size_t>(j) * (nx + 2) + i;
       if (kt < 0 || dept[c2] <= 0.0) {
         // legacy: no pressurized seepage face (mode 3), Corrector.cpp:96-106
-        if (!pcLegacy && kt >= 0 && h(j, i, kt) > 0.5 * dz3d(j, i, kt)) {
+        if (!pcLegacy && kt >= 0 && h(j, i, kt) > rZp(j, i, kt) * 0.5 * dz3d(j, i, kt)) {
           cplMode(j, i) = 3;
           continue;
         }
@@ -796,20 +900,26 @@ void Oracle::assemble(double dtg) {
         const size_t c2 = static_cast<size_t>(j) * (nx + 2) + i;
         const bool isTop = (k == 0) || (g(j, i, k - 1) < 0);
         const double volume = az * dz3d(j, i, k);
-        double gxp = -kx(j, i, k) * dtg * cosx(j, i, k) / (dx * dx) * (ax(j, i, k) * dx);
-        double gxm = -kx(j, i - 1, k) * dtg * cosx(j, i - 1, k) / (dx * dx) * (ax(j, i - 1, k) * dx);
-        double gyp = -ky(j, i, k) * dtg * cosy(j, i, k) / (dy * dy) * (ay(j, i, k) * dy);
-        double gym = -ky(j - 1, i, k) * dtg * cosy(j - 1, i, k) / (dy * dy) * (ay(j - 1, i, k) * dy);
+        double gxp = -kx(j, i, k) * dtg * rXp(j, i, k) * vXp(j, i, k) * cosx(j, i, k) / (dx * dx) *
+                     (ax(j, i, k) * dx);
+        double gxm = -kx(j, i - 1, k) * dtg * rXp(j, i - 1, k) * vXp(j, i - 1, k) *
+                     cosx(j, i - 1, k) / (dx * dx) * (ax(j, i - 1, k) * dx);
+        double gyp = -ky(j, i, k) * dtg * rYp(j, i, k) * vYp(j, i, k) * cosy(j, i, k) / (dy * dy) *
+                     (ay(j, i, k) * dy);
+        double gym = -ky(j - 1, i, k) * dtg * rYp(j - 1, i, k) * vYp(j - 1, i, k) *
+                     cosy(j - 1, i, k) / (dy * dy) * (ay(j - 1, i, k) * dy);
         if (east(i) && isHead(sxp(j, i))) gxp *= 2.0;
         if (west(i) && isHead(sxm(j, i))) gxm *= 2.0;
         if (north(j) && isHead(syp(j, i))) gyp *= 2.0;
         if (south(j) && isHead(sym(j, i))) gym *= 2.0;
         double dzf = (k == nz - 1) ? 0.5 * dz3d(j, i, k) : 0.5 * (dz3d(j, i, k) + dz3d(j, i, k + 1));
-        const double gzp = -kzF(j, i, k + 1) * dtg / (dz3d(j, i, k) * dzf) * volume;
+        const double gzp = -kzF(j, i, k + 1) * dtg * rZp(j, i, k + 1) * vZp(j, i, k + 1) /
+                           (dz3d(j, i, k) * dzf) * volume;
         dzf = isTop ? 0.5 * dz3d(j, i, k) : 0.5 * (dz3d(j, i, k) + dz3d(j, i, k - 1));
-        const double gzm = -kzF(j, i, k) * dtg / (dz3d(j, i, k) * dzf) * volume;
+        const double gzm = -kzF(j, i, k) * dtg * rZp(j, i, k) * vZp(j, i, k) /
+                           (dz3d(j, i, k) * dzf) * volume;
 
-        const double storage = (ch(j, i, k) + ss * wcn(j, i, k) / wcs(j, i, k));
+        const double storage = (ch(j, i, k) + ss * wcn(j, i, k) / wcs(j, i, k)) * rRho(j, i, k);
         double gct = storage * volume - (gxp + gxm + gyp + gym);
         if (east(i) && sxp(j, i) == FREHG_BC_FLUX) gct += gxp;
         if (west(i) && sxm(j, i) == FREHG_BC_FLUX) gct += gxm;
@@ -825,37 +935,43 @@ void Oracle::assemble(double dtg) {
         }
 
         double b = storage * hn(j, i, k) * volume;
-        b -= dtg * volume * kzF(j, i, k + 1) / dz3d(j, i, k);
-        b += dtg * volume * kzF(j, i, k) / dz3d(j, i, k);
+        b -= dtg * volume * kzF(j, i, k + 1) * rZp(j, i, k + 1) * rZp(j, i, k + 1) *
+             vZp(j, i, k + 1) / dz3d(j, i, k);
+        b += dtg * volume * kzF(j, i, k) * rZp(j, i, k) * rZp(j, i, k) * vZp(j, i, k) / dz3d(j, i, k);
         if (followTerrain) {
+          // the along-layer gravity term: K v r^2 sin (consistent), K sin (legacy)
+          const auto rv2 = [&](const F3& r, const F3& v, int jj, int ii) {
+            return pcLegacy ? 1.0 : r(jj, ii, k) * r(jj, ii, k) * v(jj, ii, k);
+          };
           double sgn = (bot3d(j, i + 1, k) > bot3d(j, i, k)) ? 1.0 : -1.0;
-          b += sgn * dtg * ax(j, i, k) * kx(j, i, k) * sinx(j, i, k);
+          b += sgn * dtg * ax(j, i, k) * kx(j, i, k) * rv2(rXp, vXp, j, i) * sinx(j, i, k);
           sgn = (bot3d(j, i, k) > bot3d(j, i - 1, k)) ? -1.0 : 1.0;
-          b += sgn * dtg * ax(j, i - 1, k) * kx(j, i - 1, k) * sinx(j, i - 1, k);
+          b += sgn * dtg * ax(j, i - 1, k) * kx(j, i - 1, k) * rv2(rXp, vXp, j, i - 1) * sinx(j, i - 1, k);
           sgn = (bot3d(j + 1, i, k) > bot3d(j, i, k)) ? 1.0 : -1.0;
-          b += sgn * dtg * ay(j, i, k) * ky(j, i, k) * siny(j, i, k);
+          b += sgn * dtg * ay(j, i, k) * ky(j, i, k) * rv2(rYp, vYp, j, i) * siny(j, i, k);
           sgn = (bot3d(j, i, k) > bot3d(j - 1, i, k)) ? -1.0 : 1.0;
-          b += sgn * dtg * ay(j - 1, i, k) * ky(j - 1, i, k) * siny(j - 1, i, k);
+          b += sgn * dtg * ay(j - 1, i, k) * ky(j - 1, i, k) * rv2(rYp, vYp, j - 1, i) * siny(j - 1, i, k);
         }
         if (east(i)) {
-          if (sxp(j, i) == FREHG_BC_FLUX) b += ax(j, i, k) * vxp(j, i) * dtg;
+          if (sxp(j, i) == FREHG_BC_FLUX) b += rXp(j, i, k) * ax(j, i, k) * vxp(j, i) * dtg;
           else b -= gxp * hn(j, i + 1, k);
         }
         if (west(i)) {
-          if (sxm(j, i) == FREHG_BC_FLUX) b += ax(j, i - 1, k) * vxm(j, i) * dtg;
+          if (sxm(j, i) == FREHG_BC_FLUX) b += rXp(j, i - 1, k) * ax(j, i - 1, k) * vxm(j, i) * dtg;
           else b -= gxm * hn(j, i - 1, k);
         }
         if (north(j)) {
-          if (syp(j, i) == FREHG_BC_FLUX) b += ay(j, i, k) * vyp(j, i) * dtg;
+          if (syp(j, i) == FREHG_BC_FLUX) b += rYp(j, i, k) * ay(j, i, k) * vyp(j, i) * dtg;
           else b -= gyp * hn(j + 1, i, k);
         }
         if (south(j)) {
-          if (sym(j, i) == FREHG_BC_FLUX) b += ay(j - 1, i, k) * vym(j, i) * dtg;
+          if (sym(j, i) == FREHG_BC_FLUX) b += rYp(j - 1, i, k) * ay(j - 1, i, k) * vym(j, i) * dtg;
           else b -= gym * hn(j - 1, i, k);
         }
         if (k == nz - 1) {
           if (botCode(j, i) == FREHG_BC_FLUX)
-            b += dtg * volume * (botValue(j, i) + kzF(j, i, nz)) / dz3d(j, i, k);
+            b += dtg * rZp(j, i, nz) * volume *
+                 (botValue(j, i) + kzF(j, i, nz) * rZp(j, i, nz) * vZp(j, i, nz)) / dz3d(j, i, k);
           else if (botCode(j, i) == FREHG_BC_HEAD)
             b -= gzp * botValue(j, i);
         }
@@ -870,7 +986,8 @@ void Oracle::assemble(double dtg) {
               const double roomTop = std::fmax(0.0, (0.9999 * wcs(j, i, k) - wcn(j, i, k)) * dz3d(j, i, k));
               const double take = std::fmin(avail[c2], roomTop);
               const double qSupply = -take / dtg;
-              b += -dtg * volume * (qSupply + kzF(j, i, k)) / dz3d(j, i, k);
+              b += -dtg * rZp(j, i, k) * volume * (qSupply + kzF(j, i, k) * rZp(j, i, k) * vZp(j, i, k)) /
+                   dz3d(j, i, k);
             } else {
               double qtop = 0.0;
               if (topCode(j, i) == FREHG_BC_FLUX) {
@@ -878,12 +995,14 @@ void Oracle::assemble(double dtg) {
                 if ((v > 0.0 && wcn(j, i, k) < wcs(j, i, k)) || (v < 0.0 && wcn(j, i, k) > wcr(j, i, k)))
                   qtop = v;
               }
-              b += -dtg * volume * (qtop + kzF(j, i, k)) / dz3d(j, i, k);
+              b += -dtg * rZp(j, i, k) * volume * (qtop + kzF(j, i, k) * rZp(j, i, k) * vZp(j, i, k)) /
+                   dz3d(j, i, k);
             }
           } else if (topCode(j, i) == FREHG_BC_HEAD) {
             b -= gzm * topValue(j, i);
           } else if (topCode(j, i) == FREHG_BC_FLUX) {
-            b += -dtg * volume * (topValue(j, i) + kzF(j, i, k)) / dz3d(j, i, k);
+            b += -dtg * rZp(j, i, k) * volume *
+                 (topValue(j, i) + kzF(j, i, k) * rZp(j, i, k) * vZp(j, i, k)) / dz3d(j, i, k);
           }
         }
         const size_t base = 7 * static_cast<size_t>(row);
@@ -902,7 +1021,7 @@ void Oracle::assemble(double dtg) {
         for (int k = 0; k < nz; ++k) {
           const int64_t row = g(j, i, k);
           if (row < 0 || sink(j, i, k) == 0.0) continue;
-          rhs[row] -= dtg * sink(j, i, k) * (az * dz3d(j, i, k));
+          rhs[row] -= dtg * rRho(j, i, k) * sink(j, i, k) * (az * dz3d(j, i, k));
         }
   }
 }
@@ -1173,7 +1292,7 @@ void Oracle::fluxes(double dtg) {
         double q = 0.0;
         if (westFace) {
           const double kf = kh(j, 1, k, ksx);
-          q = A * kf * cosx(j, 0, k) * (h(j, 1, k) - h(j, 0, k)) / (0.5 * dx);
+          q = A * kf * vXp(j, 0, k) * cosx(j, 0, k) * (h(j, 1, k) - h(j, 0, k)) / (0.5 * dx);
         } else {
           const double kc = kh(j, i, k, ksx);
           double kf, delta = dx;
@@ -1185,8 +1304,10 @@ void Oracle::fluxes(double dtg) {
             kf = faceKx(j, i, k, j, i + 1, ksx, 1.0, 1.0);
             if (g(j, i, k) < 0 || g(j, i + 1, k) < 0) kf = 0.0;
           }
-          q = A * (kf * cosx(j, i, k) * (h(j, i + 1, k) - h(j, i, k)) / delta +
-                   static_cast<double>(sgn) * kf * sinx(j, i, k));
+          // the along-layer term carries r v (consistent) or neither (legacy)
+          const double rvx = pcLegacy ? 1.0 : rXp(j, i, k) * vXp(j, i, k);
+          q = A * (kf * vXp(j, i, k) * cosx(j, i, k) * (h(j, i + 1, k) - h(j, i, k)) / delta +
+                   static_cast<double>(sgn) * kf * rvx * sinx(j, i, k));
           if (!full3d && ((g(j, i, k) >= 0 && wc(j, i, k) < wcs(j, i, k)) ||
                           (g(j, i + 1, k) >= 0 && wc(j, i + 1, k) < wcs(j, i + 1, k))))
             q = 0.0;
@@ -1212,7 +1333,7 @@ void Oracle::fluxes(double dtg) {
         double q = 0.0;
         if (southFace) {
           const double kf = kh(1, i, k, ksy);
-          q = A * kf * cosy(0, i, k) * (h(1, i, k) - h(0, i, k)) / (0.5 * dy);
+          q = A * kf * vYp(0, i, k) * cosy(0, i, k) * (h(1, i, k) - h(0, i, k)) / (0.5 * dy);
         } else {
           const double kc = kh(j, i, k, ksy);
           double kf, delta = dy;
@@ -1224,8 +1345,9 @@ void Oracle::fluxes(double dtg) {
             kf = faceKx(j, i, k, j + 1, i, ksy, 1.0, 1.0);
             if (g(j, i, k) < 0 || g(j + 1, i, k) < 0) kf = 0.0;
           }
-          q = A * (kf * cosy(j, i, k) * (h(j + 1, i, k) - h(j, i, k)) / delta +
-                   static_cast<double>(sgn) * kf * siny(j, i, k));
+          const double rvy = pcLegacy ? 1.0 : rYp(j, i, k) * vYp(j, i, k);
+          q = A * (kf * vYp(j, i, k) * cosy(j, i, k) * (h(j + 1, i, k) - h(j, i, k)) / delta +
+                   static_cast<double>(sgn) * kf * rvy * siny(j, i, k));
           if (!full3d && ((g(j, i, k) >= 0 && wc(j, i, k) < wcs(j, i, k)) ||
                           (g(j + 1, i, k) >= 0 && wc(j + 1, i, k) < wcs(j + 1, i, k))))
             q = 0.0;
@@ -1262,7 +1384,7 @@ void Oracle::fluxes(double dtg) {
               avail[c2] -= take;
             } else if (mode == 1) {
               const double delta = 0.5 * dz3d(j, i, k);
-              q = az * ks * ((h(j, i, k) - depth) / delta - 1.0);
+              q = az * ks * vZp(j, i, k) * ((h(j, i, k) - depth) / delta - rZp(j, i, k));
               if (q < 0.0) {
                 const double need = -q * dtg / az;
                 if (need > avail[c2]) {
@@ -1274,7 +1396,7 @@ void Oracle::fluxes(double dtg) {
               }
             } else if (mode == 3) {
               const double delta = 0.5 * dz3d(j, i, k);
-              q = az * ks * ((h(j, i, k) - depth) / delta - 1.0);
+              q = az * ks * vZp(j, i, k) * ((h(j, i, k) - depth) / delta - rZp(j, i, k));
               if (q <= 0.0) {
                 clamp += -q * dtg;
                 q = 0.0;
@@ -1284,7 +1406,7 @@ void Oracle::fluxes(double dtg) {
               const double kC = k_of_h(s, h(j, i, k), ks);
               const double kf = 0.5 * (kG + kC);
               const double delta = 0.5 * dz3d(j, i, k);
-              q = az * kf * ((h(j, i, k) - depth) / delta - 1.0);
+              q = az * kf * vZp(j, i, k) * ((h(j, i, k) - depth) / delta - rZp(j, i, k));
               if (q <= 0.0) q = 0.0;
               if (topCode(j, i) == FREHG_BC_FLUX) {
                 const double v = topValue(j, i);
@@ -1297,7 +1419,7 @@ void Oracle::fluxes(double dtg) {
               q = topValue(j, i) * az;
             } else if (code == FREHG_BC_HEAD) {
               const double delta = 0.5 * dz3d(j, i, k);
-              q = az * ksz(j, i, k) * ((h(j, i, k) - topValue(j, i)) / delta - 1.0);
+              q = az * ksz(j, i, k) * vZp(j, i, k) * ((h(j, i, k) - topValue(j, i)) / delta - rZp(j, i, k));
             } else {
               q = 0.0;
             }
@@ -1306,10 +1428,10 @@ void Oracle::fluxes(double dtg) {
           const int code = botCode(j, i);
           const double kf = kh(j, i, k - 1, ksz);
           if (code == FREHG_BC_FLUX) q = botValue(j, i) * az;
-          else if (code == FREHG_BC_GRAVITY) q = -kf * az;
+          else if (code == FREHG_BC_GRAVITY) q = -kf * vZp(j, i, k) * rZp(j, i, k) * az;
           else if (code == FREHG_BC_HEAD) {
             const double delta = 0.5 * dz3d(j, i, k - 1);
-            q = az * kf * ((botValue(j, i) - h(j, i, k - 1)) / delta - 1.0);
+            q = az * kf * vZp(j, i, k) * ((botValue(j, i) - h(j, i, k - 1)) / delta - rZp(j, i, k));
           } else q = 0.0;
         } else if (above && below) {
           double kf;
@@ -1321,7 +1443,7 @@ void Oracle::fluxes(double dtg) {
           else
             kf = 0.5 * (kh(j, i, k - 1, ksz) + kh(j, i, k, ksz));
           const double delta = 0.5 * (dz3d(j, i, k - 1) + dz3d(j, i, k));
-          q = az * kf * ((h(j, i, k) - h(j, i, k - 1)) / delta - 1.0);
+          q = az * kf * vZp(j, i, k) * ((h(j, i, k) - h(j, i, k - 1)) / delta - rZp(j, i, k));
         }
         qzF(j, i, k) = q;
       }
@@ -1377,13 +1499,14 @@ void Oracle::updateWaterContent(double dtg) {
       for (int k = 0; k < nz; ++k) {
         if (g(j, i, k) < 0) continue;
         const double volume = az * dz3d(j, i, k);
-        const double coeff = volume * (1.0 + ss * (h(j, i, k) - hn(j, i, k)) / wcs(j, i, k));
-        const double dqx = dtg * (qx(j, i, k) - qx(j, i - 1, k));
-        const double dqy = dtg * (qy(j, i, k) - qy(j - 1, i, k));
-        const double dqz = dtg * (qzF(j, i, k + 1) - qzF(j, i, k));
-        const double upd = (wcn(j, i, k) * volume + dqx + dqy + dqz) / coeff;
+        const double r = rRho(j, i, k);
+        const double coeff = volume * (r + r * ss * (h(j, i, k) - hn(j, i, k)) / wcs(j, i, k));
+        const double dqx = dtg * (qx(j, i, k) * rXp(j, i, k) - qx(j, i - 1, k) * rXp(j, i - 1, k));
+        const double dqy = dtg * (qy(j, i, k) * rYp(j, i, k) - qy(j - 1, i, k) * rYp(j - 1, i, k));
+        const double dqz = dtg * (qzF(j, i, k + 1) * rZp(j, i, k + 1) - qzF(j, i, k) * rZp(j, i, k));
+        const double upd = (wcn(j, i, k) * r * volume + dqx + dqy + dqz) / coeff;
         wc(j, i, k) = upd;
-        s += upd * volume * ss * (h(j, i, k) - hn(j, i, k)) / wcs(j, i, k);
+        s += upd * volume * r * ss * (h(j, i, k) - hn(j, i, k)) / wcs(j, i, k);
       }
     return s;
   });
@@ -1397,10 +1520,11 @@ void Oracle::updateWaterContent(double dtg) {
       for (int k = 0; k < nz; ++k) {
         if (g(j, i, k) < 0 || sink(j, i, k) == 0.0) continue;
         const double volume = az * dz3d(j, i, k);
-        const double coeff = volume * (1.0 + ss * (h(j, i, k) - hn(j, i, k)) / wcs(j, i, k));
-        const double dTheta = dtg * sink(j, i, k) * volume / coeff;
+        const double r = rRho(j, i, k);
+        const double coeff = volume * (r + r * ss * (h(j, i, k) - hn(j, i, k)) / wcs(j, i, k));
+        const double dTheta = dtg * r * sink(j, i, k) * volume / coeff;
         wc(j, i, k) -= dTheta;
-        ssDelta -= dTheta * volume * ss * (h(j, i, k) - hn(j, i, k)) / wcs(j, i, k);
+        ssDelta -= dTheta * volume * r * ss * (h(j, i, k) - hn(j, i, k)) / wcs(j, i, k);
         applied += sink(j, i, k) * volume * dtg;
       }
   audit.ss_storage += ssDelta;
@@ -1426,13 +1550,13 @@ void Oracle::reallocate() {
     d6[3] = (act(jc - 1, ic, kc) && ky(jc - 1, ic, kc) > 0.0) ? (hR(jc, ic, kc) - hR(jc - 1, ic, kc)) / dy : 0.0;
     if (act(jc, ic, kc + 1) && kzF(jc, ic, kc + 1) > 0.0) {
       const double dzp = 0.5 * (dz3d(jc, ic, kc) + dz3d(jc, ic, kc + 1));
-      d6[4] = (hR(jc, ic, kc + 1) - hR(jc, ic, kc)) / dzp - 1.0;
+      d6[4] = (hR(jc, ic, kc + 1) - hR(jc, ic, kc)) / dzp - (densityGradient ? rZp(jc, ic, kc + 1) : 1.0);
     } else {
       d6[4] = 0.0;
     }
     if (act(jc, ic, kc - 1) && kzF(jc, ic, kc) > 0.0) {
       const double dzm = 0.5 * (dz3d(jc, ic, kc) + dz3d(jc, ic, kc - 1));
-      d6[5] = (hR(jc, ic, kc) - hR(jc, ic, kc - 1)) / dzm - 1.0;
+      d6[5] = (hR(jc, ic, kc) - hR(jc, ic, kc - 1)) / dzm - (densityGradient ? rZp(jc, ic, kc) : 1.0);
     } else {
       d6[5] = 0.0;
     }
@@ -1916,6 +2040,14 @@ int oracle_gw_set_field(void* p, int f, const double* in) {
     o.enforceMoistureBc();
     o.hn = o.h;
     o.wcn = o.wc;
+  });
+}
+int oracle_gw_set_density(void* p, const double* r_rho, const double* r_visc,
+                          const double* r_rho_top, const double* r_visc_top, int gradient) {
+  return guard([&] {
+    Oracle& o = *static_cast<Oracle*>(p);
+    o.setDensity(r_rho, r_visc, r_rho_top, r_visc_top, gradient != 0);
+    o.enforceHeadBc();  // hydrostatic ghosts carry the boundary-face ratio
   });
 }
 int oracle_gw_set_bc_values(void* p, int which, const double* values) {

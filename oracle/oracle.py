@@ -42,6 +42,7 @@ _SIG = {
     "oracle_gw_get_field": ([_vp, C.c_int, _dp], C.c_int),
     "oracle_gw_set_field": ([_vp, C.c_int, _dp], C.c_int),
     "oracle_gw_set_bc_values": ([_vp, C.c_int, _dp], C.c_int),
+    "oracle_gw_set_density": ([_vp, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
     "oracle_gw_attach_coupling": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int],
                                   C.c_int),
     "oracle_gw_ktop": ([_vp, C.POINTER(C.c_int)], C.c_int),
@@ -131,6 +132,13 @@ class OracleRichards:
     def set(self, fid, arr) -> None:
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.field_shape(fid))
         _check(lib().oracle_gw_set_field(self._h, fid, _dptr(a)))
+
+    def set_density(self, r_rho, r_visc=None, r_rho_top=None, r_visc_top=None,
+                    reallocation_gradient="freshwater") -> None:
+        """Density / viscosity ratios per cell (halo layout); None resets to 1."""
+        args, keep = _density_args(self, r_rho, r_visc, r_rho_top, r_visc_top)
+        _check(lib().oracle_gw_set_density(self._h, *args,
+                                           int(reallocation_gradient == "density")))
 
     def set_sink(self, sink) -> None:
         """The root-uptake sink S [1/s], halo layout; None detaches it."""
@@ -226,6 +234,20 @@ class OracleRichards:
 def vg(fn: str, soil, *args) -> float:
     """Call one oracle closure: soil = (alpha, n, theta_s, theta_r, aev)."""
     return getattr(lib(), f"oracle_vg_{fn}")(*soil, *args)
+
+
+def _density_args(sim, r_rho, r_visc, r_rho_top, r_visc_top):
+    """Contiguous float64 arrays (or null pointers) for the set_density calls."""
+    keep, args = [], []
+    shapes = [sim.field_shape(abi.F_HEAD)] * 2 + [(sim.ny + 2, sim.nx + 2)] * 2
+    for a, shape in zip((r_rho, r_visc, r_rho_top, r_visc_top), shapes):
+        if a is None:
+            args.append(None)
+        else:
+            arr = np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.float64)
+            keep.append(arr)
+            args.append(_dptr(arr))
+    return args, keep
 
 
 class OracleLinearSystem:

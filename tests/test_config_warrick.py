@@ -12,7 +12,8 @@ import numpy as np
 import pytest
 
 from frehg2_amd import abi
-from frehg2_amd.config import TimeSeries, load_config, polygon_contains, run_groundwater_loop
+from frehg2_amd.config import (TimeSeries, density_ratios, load_config, polygon_contains,
+                               run_groundwater_loop)
 
 DECK = {
     "simulation": {"id": "re-1d-infiltration"},
@@ -67,6 +68,14 @@ def test_scheme_keys():
     assert case.predictor_corrector == "legacy" and case.to_desc()[0].predictor_corrector == 0
     # saturated_contact: legacy reads cells its own sweep is rewriting
     # (Reallocate.cpp:177): its result depends on the loop order
+    deck["groundwater"]["density_coupling"] = {"enabled": True, "reallocation_gradient": "density"}
+    case, _ = load_config(deck)
+    dc = case.density_coupling
+    assert dc["beta_saline"] == 0.000744 and dc["reallocation_gradient"] == "density"
+    # sea water at 35 PSU (Baroclinic.cpp:78-86 with the default betas)
+    r_rho, r_visc = density_ratios(dc, salinity=np.array([35.0]), temperature=np.array([20.0]))
+    assert r_rho[0] == pytest.approx(1.02604, rel=1e-15)
+    assert r_visc[0] == pytest.approx(1.0 / 1.077, rel=1e-15)
     deck["groundwater"]["saturated_contact"] = "legacy"
     with pytest.raises(ValueError):
         load_config(deck)

@@ -317,8 +317,11 @@ def test_mixed_precision_stepwise_stiff():
         assert eh <= 1e-8 and ew <= 1e-8, (s, eh, ew)
 
 
-def _mixed_stepwise(c, steps):
+def _mixed_stepwise(c, steps, setup=None):
     o, g = _pair(c)
+    if setup is not None:
+        setup(o)
+        setup(g)
     fp64_its = 0
     for s in range(steps):
         _schedule(c, o, g, s)
@@ -430,3 +433,42 @@ def test_ls_seam_errors():
     with pytest.raises(abi.FrehgError) as e:
         ls.solve(DeviceArray(b), DeviceArray.zeros(n))
     assert e.value.code == abi.FREHG_EDIVERGED
+
+
+def _ratios(c, seed):
+    """Seeded density ratios in [1, 1.03] and viscosity ratios in [0.9, 1],
+    cells (ghosts included) and the ghosts above the column tops."""
+    rng = np.random.default_rng(seed)
+    shape, top = (c.ny + 2, c.nx + 2, c.nz), (c.ny + 2, c.nx + 2)
+    return (1.0 + 0.03 * rng.random(shape), 0.9 + 0.1 * rng.random(shape),
+            1.0 + 0.03 * rng.random(top), 0.9 + 0.1 * rng.random(top))
+
+
+@pytest.mark.parametrize("gradient", ["freshwater", "density"])
+def test_stepwise_side_boundaries_with_density(gradient):
+    # head / hydrostatic / flux sides: every edge rule of the face means and
+    # the ratio-scaled hydrostatic ghosts
+    c = _side_case()
+    rr, rv, rt, vt = _ratios(c, 11)
+    _stepwise(c, 8, setup=lambda s: s.set_density(rr, rv, rt, vt, reallocation_gradient=gradient))
+
+
+def test_stepwise_coupled_with_density():
+    c = cases.coupled_128x128x64(nx=16, ny=12, nz=20, precision="fp64")
+    rr, rv, rt, vt = _ratios(c, 12)
+    _stepwise(c, 10, setup=lambda s: s.set_density(rr, rv, rt, vt))
+
+
+@pytest.mark.parametrize("pc", ["consistent", "legacy"])
+def test_stepwise_hetero_terrain_with_density(pc):
+    c = cases.hetero_512x512x100(nx=32, ny=24, nz=100)
+    c.predictor_corrector = pc
+    rr, rv, _, _ = _ratios(c, 13)
+    _stepwise(c, 3, setup=lambda s: s.set_density(rr, rv))
+
+
+def test_mixed_precision_with_density():
+    c = _side_case()
+    c.precision = "mixed"
+    rr, rv, rt, vt = _ratios(c, 14)
+    _mixed_stepwise(c, 6, setup=lambda s: s.set_density(rr, rv, rt, vt))

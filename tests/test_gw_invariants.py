@@ -464,3 +464,79 @@ def test_detached_sink_is_no_sink(backend):
     assert np.array_equal(ref.get(abi.F_HEAD), sim.get(abi.F_HEAD))
     assert np.array_equal(ref.get(abi.F_WC), sim.get(abi.F_WC))
     assert sim.audit()["tr_applied"] == 0.0
+
+
+def _density_column(ratios, gradient_ratios):
+    """A closed saturated 1x1x10 column whose head rises downward by the face
+    mean of `gradient_ratios` per spacing (hydrostatic for those ratios)."""
+    nz, dz = 10, 0.1
+    h = np.empty(nz)
+    h[0] = 0.3
+    for k in range(1, nz):
+        h[k] = h[k - 1] + dz * 0.5 * (gradient_ratios[k - 1] + gradient_ratios[k])
+    c = gw_base_case(1, 1, nz, dz, 60.0, specific_storage=1e-5, ic_form="head",
+                     ic_values=h.reshape(1, 1, nz))
+    r = np.broadcast_to(ratios, (3, 3, nz)).copy()
+    return c, r
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_density_hydrostatic_column_stays_at_rest(backend):
+    # Baroclinic.cpp face means + Corrector.cpp:502-525: a z face carries no
+    # flux when dh/dz equals its density ratio, the mean of its two cells; a
+    # column built on that rule stays at rest under any viscosity ratio,
+    # and one built on the freshwater gradient does not
+    rng = np.random.default_rng(7)
+    ratios = 1.0 + 0.05 * rng.random(10)
+    visc = 0.9 + 0.1 * rng.random((3, 3, 10))
+    c, r = _density_column(ratios, ratios)
+    sim = make(backend, c)
+    sim.set_density(r, visc)
+    h0 = sim.get(abi.F_HEAD)[1, 1, :].copy()
+    for _ in range(10):
+        sim.step()
+    assert np.max(np.abs(sim.get(abi.F_HEAD)[1, 1, :] - h0)) <= 1e-10
+    assert np.max(np.abs(sim.get(abi.F_QZF)[1, 1, 1:10])) <= 1e-18
+    c, r = _density_column(ratios, np.ones(10))
+    sim = make(backend, c)
+    sim.set_density(r, visc)
+    h0 = sim.get(abi.F_HEAD)[1, 1, :].copy()
+    sim.step()
+    assert np.max(np.abs(sim.get(abi.F_HEAD)[1, 1, :] - h0)) > 1e-6
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_uniform_viscosity_ratio_scales_conductivity(backend):
+    # K enters every Darcy term as K v: a uniform r_visc = c is Ks scaled by c
+    # wherever no face keeps the default 1 (flux top, no-flux bottom and sides)
+    cv = 0.8
+
+    def case(ks):
+        c = gw_base_case(2, 2, 10, 0.1, 30.0, ks=ks, specific_storage=1e-5)
+        c.set_bc("top", abi.BC_FLUX, -1e-6)
+        return c
+    a = make(backend, case(1e-5))
+    a.set_density(np.ones((4, 4, 10)), np.full((4, 4, 10), cv))
+    b = make(backend, case(1e-5 * cv))
+    for _ in range(20):
+        a.step()
+        b.step()
+    ha, hb = a.get(abi.F_HEAD)[1:-1, 1:-1], b.get(abi.F_HEAD)[1:-1, 1:-1]
+    assert np.max(np.abs(ha - hb)) <= 1e-10 * np.max(np.abs(hb))
+    plain = make(backend, case(1e-5))
+    for _ in range(20):
+        plain.step()
+    assert np.max(np.abs(plain.get(abi.F_HEAD)[1:-1, 1:-1] - ha)) > 1e-6  # the ratio mattered
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_unit_ratios_are_no_ratios(backend):
+    c, sink = _sink_column()
+    ref = make(backend, c)
+    sim = make(backend, c)
+    sim.set_density(np.ones((4, 4, 10)), np.ones((4, 4, 10)), np.ones((4, 4)), np.ones((4, 4)))
+    for _ in range(5):
+        ref.step()
+        sim.step()
+    assert np.array_equal(ref.get(abi.F_HEAD), sim.get(abi.F_HEAD))
+    assert np.array_equal(ref.get(abi.F_WC), sim.get(abi.F_WC))
