@@ -106,6 +106,7 @@ SYMBOLS = {
     "frehg_gw_set_field": ([_vp, C.c_int, _dp], C.c_int),
     "frehg_gw_set_bc_values": ([_vp, C.c_int, _dp], C.c_int),
     "frehg_gw_set_density": ([_vp, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
+    "frehg_gw_set_et": ([_vp, _ip, _dp], C.c_int),
     "frehg_gw_attach_coupling": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int],
                                  C.c_int),
     "frehg_gw_set_timing": ([_vp, C.c_int], C.c_int),

@@ -120,6 +120,11 @@ struct Dev {
   // subtracts rZp instead of 1 (reallocation_gradient: density)
   double *rRho, *rVisc, *rXp, *rYp, *vXp, *vYp, *rZp, *vZp;
   int densGrad;
+  // the evapotranspiration soil channel (RichardsSolver.hpp:620-636), null
+  // when off: columns with etCol != 0 take the applied flux etQ [m/s] at the
+  // top face, guard-free (2D halo layout)
+  int* etCol;
+  double* etQ;
   double *Ad, *Axp, *Ayp, *Azp, *rhs, *x;
   float *Ad32, *Axp32, *Ayp32, *Azp32;  // the stencil rounded to FP32 (mixed form), or null
   double* parts;
@@ -469,14 +474,18 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
           const double kface = 0.5 * (kG + Kzb);
           q = g.az * kface * vz * ((hb - depth) / delta - rz);
           if (q <= 0.0) q = 0.0;
-          if (d.topCode[c2] == FREHG_BC_FLUX) {
+          if (d.etCol && d.etCol[c2]) {
+            q += d.etQ[c2] * g.az;
+          } else if (d.topCode[c2] == FREHG_BC_FLUX) {
             const double tv = d.topValue[c2];
             if ((tv > 0.0 && d.wc[c] > wcr_at(d, c)) || tv < 0.0) q += tv * g.az;
           }
         }
       } else {
         const int code = d.topCode[c2];
-        if (code == FREHG_BC_FLUX) {
+        if (d.etCol && d.etCol[c2]) {
+          q = d.etQ[c2] * g.az;
+        } else if (code == FREHG_BC_FLUX) {
           q = d.topValue[c2] * g.az;
         } else if (code == FREHG_BC_HEAD) {
           q = g.az * d.ksz[c] * vz * ((hb - d.topValue[c2]) / (0.5 * dzb) - rz);
@@ -828,12 +837,16 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
           b += -dtg * rzT * volume * (qSupply + kzTop * rzT * vzT) / dzc;
         } else {
           double qtop = 0.0;
-          if (d.topCode[c2] == FREHG_BC_FLUX) {
+          if (d.etCol && d.etCol[c2]) {
+            qtop = d.etQ[c2];
+          } else if (d.topCode[c2] == FREHG_BC_FLUX) {
             const double tv = d.topValue[c2];
             if ((tv > 0.0 && wcnc < wcsc) || (tv < 0.0 && wcnc > wcr_at(d, c))) qtop = tv;
           }
           b += -dtg * rzT * volume * (qtop + kzTop * rzT * vzT) / dzc;
         }
+      } else if (d.etCol && d.etCol[c2]) {
+        b += -dtg * rzT * volume * (d.etQ[c2] + kzTop * rzT * vzT) / dzc;
       } else if (d.topCode[c2] == FREHG_BC_HEAD) {
         b -= gzm * d.topValue[c2];
       } else if (d.topCode[c2] == FREHG_BC_FLUX) {
@@ -901,8 +914,11 @@ __global__ void k_coupled_top(Dev d, double dtg) {
       }
     }
     d.seep[c2] += q * dtg / g.az;
-    if (d.topCode[c2] == FREHG_BC_FLUX && d.topValue[c2] > 0.0 && d.wc[c] > wcr_at(d, c) &&
-        d.dept[c2] <= d.minDepth) {
+    if (d.etCol && d.etCol[c2]) {  // the channel's exact volume (Corrector.cpp:609-615)
+      d.seep[c2] -= d.etQ[c2] * dtg;
+      ev.add(d.etQ[c2] * dtg * g.az);
+    } else if (d.topCode[c2] == FREHG_BC_FLUX && d.topValue[c2] > 0.0 && d.wc[c] > wcr_at(d, c) &&
+               d.dept[c2] <= d.minDepth) {
       d.seep[c2] -= d.topValue[c2] * dtg;
       ev.add(d.topValue[c2] * dtg * g.az);
     }
@@ -2139,6 +2155,27 @@ struct GwSolver {
     HIP_CHECK(hipStreamSynchronize(stream));
   }
   double* densBuf[8] = {};
+  // the evapotranspiration soil channel: columns (2D int mask) and the
+  // applied flux qvap [m/s] (2D), both halo layout; null detaches it
+  void setEt(const int* columns, const double* qvap) {
+    if (!columns || !qvap) {
+      d.etCol = nullptr;
+      d.etQ = nullptr;
+      return;
+    }
+    const int64_t n2 = static_cast<int64_t>(g.ny2) * g.nx2;
+    if (!etColBuf) {
+      etColBuf = alloc<int>(n2);
+      etQBuf = alloc<double>(n2);
+    }
+    HIP_CHECK(hipMemcpyAsync(etColBuf, columns, n2 * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(etQBuf, qvap, n2 * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    d.etCol = etColBuf;
+    d.etQ = etQBuf;
+  }
+  int* etColBuf = nullptr;
+  double* etQBuf = nullptr;
   double* densTop[2] = {};
 };
 
@@ -2796,6 +2833,9 @@ int frehg_gw_set_field(frehg_gw_solver* s, int field, const double* host_in) {
     g->copy(g->d.wcn, g->d.wc, g->g.cellsH());
     HIP_CHECK(hipStreamSynchronize(g->stream));
   });
+}
+int frehg_gw_set_et(frehg_gw_solver* s, const int* columns, const double* qvap) {
+  return guard([&] { frehg::S(s)->setEt(columns, qvap); });
 }
 int frehg_gw_set_density(frehg_gw_solver* s, const double* r_rho, const double* r_visc,
                          const double* r_rho_top, const double* r_visc_top,

@@ -83,6 +83,17 @@ class RichardsSolver:
         check(self.lib, self.lib.frehg_gw_set_density(
             self._h, *args, int(reallocation_gradient == "density")))
 
+    def set_et(self, columns, qvap) -> None:
+        """The evapotranspiration soil channel (RichardsSolver::etColumn_ /
+        qVap_): a (ny+2, nx+2) mask of channel columns and their applied top
+        flux [m/s], positive upward; None detaches it."""
+        if columns is None:
+            check(self.lib, self.lib.frehg_gw_set_et(self._h, None, None))
+            return
+        m = np.ascontiguousarray(np.broadcast_to(columns, (self.ny + 2, self.nx + 2)), dtype=np.int32)
+        q = np.ascontiguousarray(np.broadcast_to(qvap, (self.ny + 2, self.nx + 2)), dtype=np.float64)
+        check(self.lib, self.lib.frehg_gw_set_et(self._h, m.ctypes.data_as(abi._ip), _dptr(q)))
+
     def set_sink(self, sink) -> None:
         """The root-uptake sink S [1/s] per cell, halo layout (ny+2, nx+2, nz),
         applied from the next step on (RichardsSolver::sink_,

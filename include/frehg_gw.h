@@ -212,6 +212,15 @@ int frehg_gw_get_field(frehg_gw_solver* s, int field, double* host_out);
  * 560-634, consumed at src/gw/Predictor.cpp:700-726 and
  * src/gw/Corrector.cpp:674-705); host_in = NULL detaches it. */
 int frehg_gw_set_field(frehg_gw_solver* s, int field, const double* host_in);
+/* The evapotranspiration soil channel (RichardsSolver::qVap_ / etColumn_,
+ * src/gw/RichardsSolver.hpp:620-636, computed by the ET module each
+ * substep): columns = 2D (ny+2)(nx+2) int mask of channel columns, qvap =
+ * 2D applied top flux [m/s].  A marked column's top face takes qvap
+ * guard-free in the assembly and the Darcy flux (Predictor.cpp:660-679,
+ * Corrector.cpp:458-472), and in coupled runs the channel's volume leaves
+ * the seepage accumulator as evaporation (Corrector.cpp:609-615).  NULL
+ * detaches the channel. */
+int frehg_gw_set_et(frehg_gw_solver* s, const int* columns, const double* qvap);
 /* Density / viscosity ratios from the transport module's scalars
  * (RichardsSolver::updateBaroclinicFaces, src/gw/Baroclinic.cpp:46-187, with
  * groundwater.density_coupling, src/core/Config.cpp:345-356): r_rho = 1 +

@@ -217,6 +217,11 @@ struct Oracle {
   // (H) and z face means (F, plane k = face above cell k); all 1 by default
   F3 rRho, rVisc, rXp, rYp, vXp, vYp, rZp, vZp;
   bool densityGradient = false;  // reallocation_gradient: density
+  // the evapotranspiration soil channel (RichardsSolver.hpp:620-636): columns
+  // marked in etCol take the applied flux etQ [m/s] at their top face
+  std::vector<int> etCol;
+  std::vector<double> etQ;
+  bool etActive = false;
   void setDensity(const double* rr, const double* rv, const double* rrTop, const double* rvTop,
                   bool gradient);
   F3 kx, ky, kzF, qx, qy, qzF, vloss, room, sendUp, sendDown, wcSnap, hSnap;
@@ -990,7 +995,9 @@ void Oracle::assemble(double dtg) {
                    dz3d(j, i, k);
             } else {
               double qtop = 0.0;
-              if (topCode(j, i) == FREHG_BC_FLUX) {
+              if (etActive && etCol[c2]) {
+                qtop = etQ[c2];  // the channel's capped flux, no guard (Predictor.cpp:660-664)
+              } else if (topCode(j, i) == FREHG_BC_FLUX) {
                 const double v = topValue(j, i);
                 if ((v > 0.0 && wcn(j, i, k) < wcs(j, i, k)) || (v < 0.0 && wcn(j, i, k) > wcr(j, i, k)))
                   qtop = v;
@@ -998,6 +1005,9 @@ void Oracle::assemble(double dtg) {
               b += -dtg * rZp(j, i, k) * volume * (qtop + kzF(j, i, k) * rZp(j, i, k) * vZp(j, i, k)) /
                    dz3d(j, i, k);
             }
+          } else if (etActive && etCol[c2]) {  // Predictor.cpp:675-679
+            b += -dtg * rZp(j, i, k) * volume * (etQ[c2] + kzF(j, i, k) * rZp(j, i, k) * vZp(j, i, k)) /
+                 dz3d(j, i, k);
           } else if (topCode(j, i) == FREHG_BC_HEAD) {
             b -= gzm * topValue(j, i);
           } else if (topCode(j, i) == FREHG_BC_FLUX) {
@@ -1408,14 +1418,18 @@ void Oracle::fluxes(double dtg) {
               const double delta = 0.5 * dz3d(j, i, k);
               q = az * kf * vZp(j, i, k) * ((h(j, i, k) - depth) / delta - rZp(j, i, k));
               if (q <= 0.0) q = 0.0;
-              if (topCode(j, i) == FREHG_BC_FLUX) {
+              if (etActive && etCol[c2]) {
+                q += etQ[c2] * az;  // Corrector.cpp:458-462
+              } else if (topCode(j, i) == FREHG_BC_FLUX) {
                 const double v = topValue(j, i);
                 if ((v > 0.0 && wc(j, i, k) > wcr(j, i, k)) || v < 0.0) q += v * az;
               }
             }
           } else {
             const int code = topCode(j, i);
-            if (code == FREHG_BC_FLUX) {
+            if (etActive && etCol[c2]) {
+              q = etQ[c2] * az;  // Corrector.cpp:471-472
+            } else if (code == FREHG_BC_FLUX) {
               q = topValue(j, i) * az;
             } else if (code == FREHG_BC_HEAD) {
               const double delta = 0.5 * dz3d(j, i, k);
@@ -1480,8 +1494,11 @@ void Oracle::coupledBookkeeping(double dtg) {
         }
       }
       seep[c2] += q * dtg / az;
-      if (topCode(j, i) == FREHG_BC_FLUX && topValue(j, i) > 0.0 && wc(j, i, kt) > wcr(j, i, kt) &&
-          dept[c2] <= minDepth) {
+      if (etActive && etCol[c2]) {  // the channel's exact volume (Corrector.cpp:609-615)
+        seep[c2] -= etQ[c2] * dtg;
+        ev += etQ[c2] * dtg * az;
+      } else if (topCode(j, i) == FREHG_BC_FLUX && topValue(j, i) > 0.0 && wc(j, i, kt) > wcr(j, i, kt) &&
+                 dept[c2] <= minDepth) {
         seep[c2] -= topValue(j, i) * dtg;
         ev += topValue(j, i) * dtg * az;
       }
@@ -2040,6 +2057,16 @@ int oracle_gw_set_field(void* p, int f, const double* in) {
     o.enforceMoistureBc();
     o.hn = o.h;
     o.wcn = o.wc;
+  });
+}
+int oracle_gw_set_et(void* p, const int* columns, const double* qvap) {
+  return guard([&] {
+    Oracle& o = *static_cast<Oracle*>(p);
+    o.etActive = columns != nullptr && qvap != nullptr;
+    if (!o.etActive) return;
+    const size_t n2 = static_cast<size_t>(o.ny + 2) * (o.nx + 2);
+    o.etCol.assign(columns, columns + n2);
+    o.etQ.assign(qvap, qvap + n2);
   });
 }
 int oracle_gw_set_density(void* p, const double* r_rho, const double* r_visc,

@@ -43,6 +43,7 @@ _SIG = {
     "oracle_gw_set_field": ([_vp, C.c_int, _dp], C.c_int),
     "oracle_gw_set_bc_values": ([_vp, C.c_int, _dp], C.c_int),
     "oracle_gw_set_density": ([_vp, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
+    "oracle_gw_set_et": ([_vp, C.POINTER(C.c_int), _dp], C.c_int),
     "oracle_gw_attach_coupling": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int],
                                   C.c_int),
     "oracle_gw_ktop": ([_vp, C.POINTER(C.c_int)], C.c_int),
@@ -139,6 +140,15 @@ class OracleRichards:
         args, keep = _density_args(self, r_rho, r_visc, r_rho_top, r_visc_top)
         _check(lib().oracle_gw_set_density(self._h, *args,
                                            int(reallocation_gradient == "density")))
+
+    def set_et(self, columns, qvap) -> None:
+        """The ET soil channel: 2D halo-layout column mask and flux; None detaches."""
+        if columns is None:
+            _check(lib().oracle_gw_set_et(self._h, None, None))
+            return
+        m = np.ascontiguousarray(np.broadcast_to(columns, (self.ny + 2, self.nx + 2)), dtype=np.int32)
+        q = np.ascontiguousarray(np.broadcast_to(qvap, (self.ny + 2, self.nx + 2)), dtype=np.float64)
+        _check(lib().oracle_gw_set_et(self._h, m.ctypes.data_as(C.POINTER(C.c_int)), _dptr(q)))
 
     def set_sink(self, sink) -> None:
         """The root-uptake sink S [1/s], halo layout; None detaches it."""

@@ -472,3 +472,28 @@ def test_mixed_precision_with_density():
     c.precision = "mixed"
     rr, rv, rt, vt = _ratios(c, 14)
     _mixed_stepwise(c, 6, setup=lambda s: s.set_density(rr, rv, rt, vt))
+
+
+def _et(c, seed):
+    """Seeded channel columns (about a third) with fluxes of either sign."""
+    rng = np.random.default_rng(seed)
+    cols = np.zeros((c.ny + 2, c.nx + 2), dtype=np.int32)
+    cols[1:-1, 1:-1] = rng.random((c.ny, c.nx)) < 0.33
+    return cols, rng.uniform(-1e-7, 3e-7, (c.ny + 2, c.nx + 2))
+
+
+def test_stepwise_coupled_with_et_channel():
+    c = cases.coupled_128x128x64(nx=16, ny=12, nz=20, precision="fp64")
+    cols, q = _et(c, 21)
+    _stepwise(c, 10, setup=lambda s: s.set_et(cols, q))
+
+
+def test_stepwise_side_boundaries_with_et_channel_and_density():
+    c = _side_case()
+    cols, q = _et(c, 22)
+    rr, rv, rt, vt = _ratios(c, 23)
+
+    def setup(s):
+        s.set_et(cols, q)
+        s.set_density(rr, rv, rt, vt)
+    _stepwise(c, 8, setup=setup)

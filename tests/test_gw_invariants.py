@@ -540,3 +540,45 @@ def test_unit_ratios_are_no_ratios(backend):
         sim.step()
     assert np.array_equal(ref.get(abi.F_HEAD), sim.get(abi.F_HEAD))
     assert np.array_equal(ref.get(abi.F_WC), sim.get(abi.F_WC))
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_et_channel_replaces_the_top_condition(backend):
+    # Corrector.cpp:471-472 and Predictor.cpp:675-679: an uncoupled channel
+    # column's top face carries exactly qVap A, guard-free, whatever its
+    # configured top flux; the others keep theirs.  The audit closes with it
+    # in boundary_in.
+    c = gw_base_case(2, 1, 10, 0.1, 30.0, specific_storage=1e-5)
+    c.set_bc("top", abi.BC_FLUX, -1e-6)
+    sim = make(backend, c)
+    cols = np.zeros((3, 4), dtype=np.int32)
+    cols[1, 1] = 1
+    qvap = np.zeros((3, 4))
+    qvap[1, 1] = 3e-7
+    sim.set_et(cols, qvap)
+    prev = sim.volume()
+    for n in range(10):
+        sim.step()
+        q = sim.get(abi.F_QZF)
+        assert q[1, 1, 0] == 3e-7 * 1.0 and q[1, 2, 0] == -1e-6 * 1.0
+        v, res = closure(sim, prev)
+        assert abs(res) <= 1e-12
+        assert sim.audit()["boundary_in"] == pytest.approx((1e-6 - 3e-7) * 30.0, rel=1e-12)
+        prev = v
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_et_channel_books_evaporation_in_coupled_runs(backend):
+    # Corrector.cpp:609-615: the channel's volume leaves the seepage
+    # accumulator and is booked as evaporation, any sign, no depth test
+    c = gw_base_case(1, 1, 8, 0.1, 30.0, ic_const=0.3)
+    z = np.zeros((3, 3))
+    c.coupling = dict(eta=z.copy(), dept=z.copy(), seep_accum=z.copy(), avail=z.copy(),
+                      gain=z.copy(), bounce=z.copy(), min_depth=1e-4, vent_all=0)
+    sim = make(backend, c)
+    sim.set_et(np.ones((3, 3), dtype=np.int32), np.full((3, 3), 2e-7))
+    sim.step()
+    qtop = sim.get(abi.F_QZF)[1, 1, 0]
+    assert sim.audit()["cpl_evap"] == pytest.approx(2e-7 * 30.0, rel=1e-14)
+    assert sim.coupling_state("seep_accum")[1, 1] == pytest.approx(qtop * 30.0 - 2e-7 * 30.0,
+                                                                   rel=1e-12)
