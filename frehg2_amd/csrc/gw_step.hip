@@ -172,8 +172,8 @@ __device__ __forceinline__ Range xcd_range(int64_t n) {
 // neighbours: i-tiles of kTileW columns, each walked j-major.  An XCD's consecutive
 // threads then meet a cell's j-1 neighbour kTileW * nz cells back (still in
 // that XCD's L2) instead of a whole (nx+2) * nz row back (assembly 1.31 ->
-// 1.26 ms at 512 x 512 x 100; the reallocation's classification, bound by
-// vector instructions, gains nothing).  A bijection of [0, nx * ny) onto the
+// 1.25 ms at 512 x 512 x 100; the reallocation's classification, bound by
+// vector instructions, and the face pass gain nothing).  A bijection of [0, nx * ny) onto the
 // interior columns.
 constexpr int kTileW = 64;
 __device__ __forceinline__ void tiled_column(const Grid& g, int col, int& j, int& i) {
