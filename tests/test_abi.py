@@ -111,3 +111,30 @@ def test_hbm_probe_measures_both_streams():
     r = hbm.measure(4 * 1024 * 1024, 5)
     # 8 x 32 MB fits the on-package cache, so only sanity is checked here
     assert 100.0 < r["copy_gbs"] < 1e5 and 100.0 < r["stream_6r2w_gbs"] < 1e5, r
+
+
+@pytest.mark.gpu
+def test_step_inputs_reject_what_they_cannot_use():
+    # the sink is readable only once set; head and theta need data; the
+    # predictor-corrector pair is 0 or 1; a detached sink and a reset
+    # density are accepted at any time
+    from frehg2_amd import cases
+    from frehg2_amd.gw import RichardsSolver
+    g = RichardsSolver(cases.loam_32x32x50(steps=1, nx=4, ny=3))
+    with pytest.raises(abi.FrehgError) as e:
+        g.get(abi.F_SINK)
+    assert e.value.code == abi.FREHG_EINVAL
+    with pytest.raises(abi.FrehgError) as e:
+        abi.check(g.lib, g.lib.frehg_gw_set_field(g._h, abi.F_HEAD, None))
+    assert e.value.code == abi.FREHG_EINVAL
+    g.set_sink(None)
+    g.set_density(None)
+    g.set_et(None, None)
+    g.step(60.0)
+    assert g.audit()["tr_applied"] == 0.0
+    g.close()
+    c = cases.loam_32x32x50(steps=1, nx=4, ny=3)
+    desc, keep = c.to_desc()
+    desc.predictor_corrector = 2
+    h = abi.C.c_void_p()
+    assert g.lib.frehg_gw_create(abi.C.byref(desc), abi.C.byref(h)) == abi.FREHG_EINVAL
