@@ -145,6 +145,15 @@ PROBE_SYMBOLS = {
 }
 
 
+# Test exports (include/frehg_gw_testing.h; no reference counterpart): the
+# device closures and the post-allocation sweep, for the test suite.
+VG_THETA, VG_HEAD, VG_CAPACITY, VG_CONDUCTIVITY, VG_DKDTHETA, VG_CONDUCTIVITY_C = range(6)
+TESTING_SYMBOLS = {
+    "frehg_test_vg": ([C.c_int, C.c_int64, _dp, _dp, _dp, _dp], C.c_int),
+    "frehg_gw_test_post_allocate": ([_vp, C.c_double], C.c_int),
+}
+
+
 class FrehgError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"[frehg error {code}] {message}")
@@ -164,7 +173,8 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
         raise FrehgError(FREHG_EDEVICE,
                          f"HIP library {p} is not built; run __graft_entry__.build() first")
     lib = C.CDLL(str(p))
-    for name, (args, res) in {**SYMBOLS, **COUPLER_SYMBOLS, **PROBE_SYMBOLS}.items():
+    for name, (args, res) in {**SYMBOLS, **COUPLER_SYMBOLS, **PROBE_SYMBOLS,
+                             **TESTING_SYMBOLS}.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res

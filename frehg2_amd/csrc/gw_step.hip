@@ -43,6 +43,7 @@
 
 #include "../../include/frehg_coupler.h"
 #include "../../include/frehg_gw.h"
+#include "../../include/frehg_gw_testing.h"
 #include "gw_device.h"
 #include "gw_pcg.h"
 
@@ -1958,6 +1959,47 @@ __global__ void k_initial(Dev d, int form, const double* vals, double vconst, do
   }
 }
 
+// ---- test exports (include/frehg_gw_testing.h) -----------------------------
+// One closure of the device path per point (soil as 5 doubles a point).
+__global__ void k_test_vg(int closure, int64_t n, const double* __restrict__ soil,
+                          const double* __restrict__ x, const double* __restrict__ ks,
+                          double* __restrict__ out) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const VgSoil s{soil[5 * t], soil[5 * t + 1], soil[5 * t + 2], soil[5 * t + 3],
+                   soil[5 * t + 4]};
+    double v = 0.0, cap = 0.0;
+    switch (closure) {
+      case 0: v = vg_theta(s, x[t]); break;
+      case 1: v = vg_head(s, x[t]); break;
+      case 2: (void)vg_mualem_capacity(s, x[t], cap); v = cap; break;
+      case 3: v = vg_conductivity(s, x[t], ks[t]); break;
+      case 4: v = vg_dkdtheta(s, x[t], ks[t]); break;
+      default: v = vg_k_from(vg_mualem_capacity(s, x[t], cap), ks[t]); break;
+    }
+    out[t] = v;
+  }
+}
+
+// Saturation flags of every halo-layout cell from the installed theta and h
+// (what k_update_wc and k_sat_halo leave for the classification); inactive
+// interior cells are never saturated neighbours.
+__global__ void k_sat_all(Dev d) {
+  const Grid& g = d.g;
+  const int64_t n = g.cellsH();
+  for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int k = static_cast<int>(c % g.nz);
+    const int64_t q = c / g.nz;
+    const int i = static_cast<int>(q % g.nx2), j = static_cast<int>(q / g.nx2);
+    const bool halo = i == 0 || j == 0 || i == g.nx + 1 || j == g.ny + 1;
+    d.sat[c] = (halo || act(d, j, i, k)) &&
+                       (d.wc[c] >= wcs_at(d, c) || d.h[c] > aev_at(d, c))
+                   ? 1
+                   : 0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
@@ -2066,6 +2108,41 @@ struct GwSolver {
     HIP_CHECK(hipStreamSynchronize(stream));
   }
   void step(double t, double dtg);
+  // Post-allocation and final clamp (Reallocate.cpp:122-545,
+  // Corrector.cpp:758-792): the classification writes the second buffer pair,
+  // which then becomes the state; the saturation flags must be current.
+  void postAllocation(double dtg) {
+    k_realloc_classify<<<rb, kThreads, 0, stream>>>(d);
+    HIP_CHECK(hipGetLastError());
+    dispatch_col(g.nz, [&](auto RC) {
+      constexpr int R = decltype(RC)::value;
+      k_realloc_walk<R><<<rbCol, kThreads, 0, stream>>>(d, dtg);
+      k_realloc_walk_serial<R><<<rbCol, kThreads, 0, stream>>>(d, dtg);
+    });
+    HIP_CHECK(hipGetLastError());
+    std::swap(d.h, d.hOut);
+    std::swap(d.wc, d.wcOut);
+    ghosts(false, true);
+  }
+  // The sweep alone on the installed state (frehg_gw_test_post_allocate).
+  void testPostAllocate(double dtg) {
+    if (!(dtg > 0.0)) fail(FREHG_EINVAL, "dtg must be positive");
+    dtgCurrent = dtg;
+    k_reset_parts<<<64, kThreads, 0, stream>>>(d.parts, rb);
+    k_sat_all<<<rb, kThreads, 0, stream>>>(d);
+    HIP_CHECK(hipGetLastError());
+    postAllocation(dtg);
+    reduceSlots();
+    const double* s = slotsHost;
+    audit = frehg_gw_audit{};
+    if (!d.reallocOff) {
+      audit.realloc_adjust = s[S_ADJ] + s[S_LOST];
+      audit.realloc_dropped = s[S_DISC] - s[S_LOST];
+      audit.cpl_vent = s[S_VENT];
+      audit.cpl_vent_thin = s[S_THIN];
+    }
+    audit.vloss = s[S_VLOSS];
+  }
   double volume() {
     k_reset_parts<<<64, kThreads, 0, stream>>>(d.parts, rb);
     k_volume<<<rb, kThreads, 0, stream>>>(d);
@@ -2665,17 +2742,7 @@ void GwSolver::step(double /*t*/, double dtg) {
                  kThreads, 0, stream>>>(d);
     HIP_CHECK(hipGetLastError());
     const int adapt = dtMin < dtMax ? 1 : 0;
-    k_realloc_classify<<<rb, kThreads, 0, stream>>>(d);
-    HIP_CHECK(hipGetLastError());
-    dispatch_col(g.nz, [&](auto RC) {
-      constexpr int R = decltype(RC)::value;
-      k_realloc_walk<R><<<rbCol, kThreads, 0, stream>>>(d, dtg);
-      k_realloc_walk_serial<R><<<rbCol, kThreads, 0, stream>>>(d, dtg);
-    });
-    HIP_CHECK(hipGetLastError());
-    std::swap(d.h, d.hOut);
-    std::swap(d.wc, d.wcOut);
-    ghosts(false, true);
+    postAllocation(dtg);
     k_boundary_in<<<std::min(rb, rbCol * 4), kThreads, 0, stream>>>(d, dtg);
     HIP_CHECK(hipGetLastError());
     if (adapt) {
@@ -2981,6 +3048,47 @@ int frehg_memcpy_dtoh(void* dst, const void* src, int64_t bytes) {
 }
 int frehg_gw_synchronize(frehg_gw_solver* s) {
   return guard([&] { HIP_CHECK(hipStreamSynchronize(frehg::S(s)->stream)); });
+}
+
+// ---- test exports (include/frehg_gw_testing.h) ------------------------------
+int frehg_test_vg(int closure, int64_t n, const double* soil, const double* x, const double* ks,
+                  double* out) {
+  return guard([&] {
+    if (closure < FREHG_VG_THETA || closure > FREHG_VG_CONDUCTIVITY_C || n < 0 || !soil || !x ||
+        !out)
+      fail(FREHG_EINVAL, "frehg_test_vg: invalid arguments");
+    const bool needKs = closure == FREHG_VG_CONDUCTIVITY || closure == FREHG_VG_DKDTHETA ||
+                        closure == FREHG_VG_CONDUCTIVITY_C;
+    if (needKs && !ks) fail(FREHG_EINVAL, "frehg_test_vg: this closure needs ks");
+    if (n == 0) return;
+    double *dS = nullptr, *dX = nullptr, *dK = nullptr, *dO = nullptr;
+    auto release = [&] {
+      for (double* p : {dS, dX, dK, dO})
+        if (p) (void)hipFree(p);
+    };
+    try {
+      const size_t b = static_cast<size_t>(n) * sizeof(double);
+      HIP_CHECK(hipMalloc(&dS, 5 * b));
+      HIP_CHECK(hipMalloc(&dX, b));
+      HIP_CHECK(hipMalloc(&dK, b));
+      HIP_CHECK(hipMalloc(&dO, b));
+      HIP_CHECK(hipMemcpy(dS, soil, 5 * b, hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemcpy(dX, x, b, hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemset(dK, 0, b));
+      if (ks) HIP_CHECK(hipMemcpy(dK, ks, b, hipMemcpyHostToDevice));
+      const int blocks = static_cast<int>(std::min<int64_t>((n + 255) / 256, 1024));
+      frehg::k_test_vg<<<blocks, 256>>>(closure, n, dS, dX, dK, dO);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpy(out, dO, b, hipMemcpyDeviceToHost));
+    } catch (...) {
+      release();
+      throw;
+    }
+    release();
+  });
+}
+int frehg_gw_test_post_allocate(frehg_gw_solver* s, double dtg) {
+  return guard([&] { frehg::S(s)->testPostAllocate(dtg); });
 }
 
 }  // extern "C"

@@ -164,6 +164,11 @@ class RichardsSolver:
         check(self.lib, self.lib.frehg_gw_active_cells(self._h, C.byref(v)))
         return v.value
 
+    def test_post_allocate(self, dtg: float) -> None:
+        """Test export (include/frehg_gw_testing.h): the post-allocation sweep
+        and final clamp alone on the current state."""
+        check(self.lib, self.lib.frehg_gw_test_post_allocate(self._h, float(dtg)))
+
     def set_timing(self, enable) -> None:
         """0/False off; n > 0 (True = 1): step and solve events every step and
         one SpMV launch in n bracketed by events."""

@@ -2151,6 +2151,23 @@ int oracle_gw_assemble_only(void* p, double dtg) {
   });
 }
 
+// The post-allocation sweep alone on the installed state, as the
+// reference's tests run RichardsSolver::reallocateWaterContent
+// (tests/unit/test_gw.cpp:1037), followed by the final clamp
+// (Corrector.cpp:758-792) and the moisture ghosts, the same sequence the
+// device path's test export runs.
+int oracle_gw_test_post_allocate(void* p, double dtg) {
+  return guard([&] {
+    Oracle& o = *static_cast<Oracle*>(p);
+    if (!(dtg > 0.0)) fail(FREHG_EINVAL, "dtg must be positive");
+    o.audit = frehg_gw_audit{};
+    o.dtgCurrent = dtg;
+    o.reallocate();
+    o.finalize();
+    o.enforceMoistureBc();
+  });
+}
+
 // ---- generic COO linear system ---------------------------------------------
 int oracle_ls_create(int64_t n, double rtol, double atol, int maxIt, int pc, void** out) {
   return guard([&] {

@@ -59,6 +59,7 @@ _SIG = {
     "oracle_ls_set_values": ([_vp, _dp], C.c_int),
     "oracle_ls_solve": ([_vp, _dp, _dp, C.POINTER(abi.SolveStats)], C.c_int),
     "oracle_ls_is_symmetric": ([_vp, C.c_double, C.POINTER(C.c_int)], C.c_int),
+    "oracle_gw_test_post_allocate": ([_vp, C.c_double], C.c_int),
 }
 
 _LIB = None
@@ -226,6 +227,10 @@ class OracleRichards:
         out = np.empty(3)
         _check(lib().oracle_gw_geometry_scalars(self._h, _dptr(out)))
         return {"offset": out[0], "box_top": out[1], "pc_clamp_total": out[2]}
+
+    def test_post_allocate(self, dtg: float) -> None:
+        """The post-allocation sweep and final clamp alone on the current state."""
+        _check(lib().oracle_gw_test_post_allocate(self._h, dtg))
 
     def assemble_only(self, dtg: float):
         _check(lib().oracle_gw_assemble_only(self._h, dtg))

@@ -15,6 +15,7 @@ HEADER = ROOT / "include" / "frehg_gw.h"
 
 PROBE_HEADER = ROOT / "include" / "frehg_probe.h"
 COUPLER_HEADER = ROOT / "include" / "frehg_coupler.h"
+TESTING_HEADER = ROOT / "include" / "frehg_gw_testing.h"
 
 
 def declared_functions(header=HEADER):
@@ -34,7 +35,7 @@ def test_header_declares_the_seams():
 def test_library_exports_every_declared_symbol():
     lib = abi.load_library()
     for name in (declared_functions() + declared_functions(PROBE_HEADER) +
-                 declared_functions(COUPLER_HEADER)):
+                 declared_functions(COUPLER_HEADER) + declared_functions(TESTING_HEADER)):
         assert hasattr(lib, name), name
     assert lib.frehg_abi_version() == abi.ABI_VERSION
 
@@ -45,6 +46,22 @@ def test_probe_header_matches_its_ctypes_table():
 
 def test_coupler_header_matches_its_ctypes_table():
     assert set(declared_functions(COUPLER_HEADER)) == set(abi.COUPLER_SYMBOLS)
+
+
+def test_testing_header_matches_its_ctypes_table():
+    assert set(declared_functions(TESTING_HEADER)) == set(abi.TESTING_SYMBOLS)
+
+
+def test_every_header_is_checked():
+    checked = {HEADER, PROBE_HEADER, COUPLER_HEADER, TESTING_HEADER}
+    assert set((ROOT / "include").glob("*.h")) <= checked
+
+
+def test_testing_calls_reject_bad_arguments():
+    """Argument checks run before any device work."""
+    lib = abi.load_library()
+    assert lib.frehg_test_vg(99, 1, None, None, None, None) == abi.FREHG_EINVAL
+    assert lib.frehg_gw_test_post_allocate(None, 1.0) != 0
 
 
 def test_coupler_header_compiles_as_c(tmp_path):
