@@ -293,6 +293,30 @@ def density_ratios(coupling: dict, salinity=None, temperature=None):
     return r_rho, r_visc
 
 
+def coupled_density_ratios(coupling: dict, salinity=None, temperature=None,
+                           surface_salinity=None, surface_temperature=None) -> dict:
+    """All four ratio arrays of a coupled run, as keyword arguments of
+    RichardsSolver.set_density: the cell ratios from the subsurface scalars
+    (halo layout) and the ratios of the ghost above each column's top cell
+    from the surface scalars ((ny+2, nx+2) arrays), which the reference takes
+    for the top face in coupled runs (Baroclinic.cpp:172-180).  Without the
+    top ratios the top face would use the top cell's own ratio, the
+    reference's uncoupled rule."""
+    r_rho, r_visc = density_ratios(coupling, salinity, temperature)
+    top_rho, top_visc = density_ratios(coupling, surface_salinity, surface_temperature)
+    shape2 = None
+    for a in (surface_salinity, surface_temperature):
+        if a is not None:
+            shape2 = np.shape(a)
+    if shape2 is None:
+        raise ValueError("coupled_density_ratios: a surface scalar (salinity or temperature) "
+                         "is required for the top-face ratios")
+    return dict(r_rho=r_rho, r_visc=r_visc,
+                r_rho_top=np.broadcast_to(top_rho, shape2).astype(np.float64),
+                r_visc_top=np.broadcast_to(top_visc, shape2).astype(np.float64),
+                reallocation_gradient=coupling.get("reallocation_gradient", "freshwater"))
+
+
 def run_groundwater_loop(solver, schedule: BoundarySchedule, t_end: float, t_start: float = 0.0,
                          on_step=None):
     """The gw-only driver loop (src/driver/Simulation.cpp:1672-1770): t += dtg;
@@ -311,7 +335,8 @@ def run_groundwater_loop(solver, schedule: BoundarySchedule, t_end: float, t_sta
         v = solver.volume()
         a.update(t=t, dtg=dtg, volume=v,
                  residual=(v - v_prev) - (a["boundary_in"] - a["ss_storage"] +
-                                          a["realloc_adjust"] - a["vloss"]))
+                                          a["realloc_adjust"] - a["vloss"] -
+                                          a.get("tr_applied", 0.0)))
         records.append(a)
         if on_step is not None:
             on_step(solver, a)

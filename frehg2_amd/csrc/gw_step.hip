@@ -2255,7 +2255,10 @@ struct GwSolver {
   // the evapotranspiration soil channel: columns (2D int mask) and the
   // applied flux qvap [m/s] (2D), both halo layout; null detaches it
   void setEt(const int* columns, const double* qvap) {
-    if (!columns || !qvap) {
+    if (!columns != !qvap)
+      fail(FREHG_EINVAL, "frehg_gw_set_et: columns and qvap are given together (both NULL "
+                         "detaches the channel)");
+    if (!columns) {
       d.etCol = nullptr;
       d.etQ = nullptr;
       return;

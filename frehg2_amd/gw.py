@@ -86,7 +86,10 @@ class RichardsSolver:
     def set_et(self, columns, qvap) -> None:
         """The evapotranspiration soil channel (RichardsSolver::etColumn_ /
         qVap_): a (ny+2, nx+2) mask of channel columns and their applied top
-        flux [m/s], positive upward; None detaches it."""
+        flux [m/s], positive upward; both None detach it."""
+        if (columns is None) != (qvap is None):
+            raise FrehgError(abi.FREHG_EINVAL, "set_et: columns and qvap are given together "
+                                               "(both None detaches the channel)")
         if columns is None:
             check(self.lib, self.lib.frehg_gw_set_et(self._h, None, None))
             return

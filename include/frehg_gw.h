@@ -210,7 +210,12 @@ int frehg_gw_get_field(frehg_gw_solver* s, int field, double* host_out);
  * FREHG_F_SINK sets the root-uptake sink S [1/s] that the following steps
  * apply (the vegetation module's sink_, src/gw/Evapotranspiration.cpp:
  * 560-634, consumed at src/gw/Predictor.cpp:700-726 and
- * src/gw/Corrector.cpp:674-705); host_in = NULL detaches it. */
+ * src/gw/Corrector.cpp:674-705); host_in = NULL detaches it.  The step
+ * applies S as given: the reference's vegetation module recomputes it before
+ * every step with the supply cap min(want, (theta^n - theta_r) / dtg)
+ * (Evapotranspiration.cpp:606-611), so a caller refreshes S, capped at that
+ * step's dtg, before each step (a sink computed for a smaller dtg can draw
+ * theta below theta_r, which the final clamp then books as vloss). */
 int frehg_gw_set_field(frehg_gw_solver* s, int field, const double* host_in);
 /* The evapotranspiration soil channel (RichardsSolver::qVap_ / etColumn_,
  * src/gw/RichardsSolver.hpp:620-636, computed by the ET module each

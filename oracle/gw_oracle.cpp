@@ -2062,6 +2062,7 @@ int oracle_gw_set_field(void* p, int f, const double* in) {
 int oracle_gw_set_et(void* p, const int* columns, const double* qvap) {
   return guard([&] {
     Oracle& o = *static_cast<Oracle*>(p);
+    if (!columns != !qvap) fail(FREHG_EINVAL, "set_et: columns and qvap are given together");
     o.etActive = columns != nullptr && qvap != nullptr;
     if (!o.etActive) return;
     const size_t n2 = static_cast<size_t>(o.ny + 2) * (o.nx + 2);

@@ -143,7 +143,9 @@ class OracleRichards:
                                            int(reallocation_gradient == "density")))
 
     def set_et(self, columns, qvap) -> None:
-        """The ET soil channel: 2D halo-layout column mask and flux; None detaches."""
+        """The ET soil channel: 2D halo-layout column mask and flux; both None detach."""
+        if (columns is None) != (qvap is None):
+            raise abi.FrehgError(abi.FREHG_EINVAL, "set_et: columns and qvap are given together")
         if columns is None:
             _check(lib().oracle_gw_set_et(self._h, None, None))
             return

@@ -5,6 +5,7 @@ import re
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from frehg2_amd import abi
@@ -147,6 +148,15 @@ def test_step_inputs_reject_what_they_cannot_use():
     g.set_sink(None)
     g.set_density(None)
     g.set_et(None, None)
+    # a channel mask without its flux (or the reverse) is an error, not a detach
+    n2 = (g.ny + 2) * (g.nx + 2)
+    cols = np.ones(n2, dtype=np.int32)
+    q = np.full(n2, 1e-7)
+    assert g.lib.frehg_gw_set_et(g._h, cols.ctypes.data_as(abi._ip), None) == abi.FREHG_EINVAL
+    assert g.lib.frehg_gw_set_et(g._h, None, q.ctypes.data_as(abi._dp)) == abi.FREHG_EINVAL
+    with pytest.raises(abi.FrehgError) as e:
+        g.set_et(cols.reshape(g.ny + 2, g.nx + 2), None)
+    assert e.value.code == abi.FREHG_EINVAL
     g.step(60.0)
     assert g.audit()["tr_applied"] == 0.0
     g.close()

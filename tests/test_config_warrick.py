@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from frehg2_amd import abi
-from frehg2_amd.config import (TimeSeries, density_ratios, load_config, polygon_contains,
+from frehg2_amd.config import (TimeSeries, coupled_density_ratios, density_ratios, load_config, polygon_contains,
                                run_groundwater_loop)
 
 DECK = {
@@ -79,6 +79,37 @@ def test_scheme_keys():
     deck["groundwater"]["saturated_contact"] = "legacy"
     with pytest.raises(ValueError):
         load_config(deck)
+
+
+def test_coupled_density_ratios_take_the_top_face_from_the_surface():
+    """Coupled runs take the top face's ghost ratio from the surface scalars
+    (Baroclinic.cpp:172-180): the helper hands them to set_density, and they
+    change the top face's flux against the uncoupled own-cell rule."""
+    from frehg2_amd import cases
+    from oracle.oracle import OracleRichards
+    dc = {"beta_saline": 0.000744, "thermal_expansion": 0.00021, "reference_temperature": 20.0,
+          "beta_saline_viscosity": 0.0022, "reallocation_gradient": "freshwater"}
+    c = cases.coupled_128x128x64(member=0, nx=6, ny=5, nz=8)
+    shape3 = (c.ny + 2, c.nx + 2, c.nz)
+    shape2 = (c.ny + 2, c.nx + 2)
+    s3 = np.full(shape3, 5.0)
+    s2 = np.full(shape2, 30.0)
+    kw = coupled_density_ratios(dc, salinity=s3, surface_salinity=s2)
+    assert kw["r_rho_top"].shape == shape2
+    assert kw["r_rho_top"][0, 0] == pytest.approx(1.0 + 0.000744 * 30.0, rel=1e-15)
+    assert kw["r_visc_top"][0, 0] == pytest.approx(1.0 / (1.0 + 0.0022 * 30.0), rel=1e-15)
+    with pytest.raises(ValueError):
+        coupled_density_ratios(dc, salinity=s3)
+    runs = []
+    for top in (True, False):
+        o = OracleRichards(c)
+        if top:
+            o.set_density(**kw)
+        else:
+            o.set_density(kw["r_rho"], kw["r_visc"], reallocation_gradient="freshwater")
+        o.step(c.dt)
+        runs.append(o.get(abi.F_QZF)[1:-1, 1:-1, 0])  # the top faces
+    assert np.max(np.abs(runs[0] - runs[1])) > 0.0
 
 
 def test_polygon_and_series_semantics(tmp_path):

@@ -452,6 +452,21 @@ def test_sink_audit_closes(backend):
 
 
 @pytest.mark.parametrize("backend", BACKENDS)
+def test_groundwater_loop_residual_closes_with_a_sink(backend):
+    # run_groundwater_loop books the transpiration channel in its per-step
+    # mass residual (Simulation.cpp:1082-1126 with trApplied)
+    from frehg2_amd.config import run_groundwater_loop
+    c, sink = _sink_column()
+    sim = make(backend, c)
+    sim.set_sink(sink)
+    recs = run_groundwater_loop(sim, None, t_end=30.0 * 12)
+    assert len(recs) == 12
+    for r in recs:
+        assert r["tr_applied"] > 0.0
+        assert abs(r["residual"]) <= 1e-12, r
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
 def test_detached_sink_is_no_sink(backend):
     c, sink = _sink_column()
     ref = make(backend, c)

@@ -255,6 +255,50 @@ def test_head_classification_restores_the_whole_fringe(backend, axis, seed_at_mi
     assert wc == [THETA_S] * len(wc), (axis, seed_at_minus, wc)
 
 
+def _snapshot_pair(backend, axis, a_first):
+    """Two live cells among dry ones: A (theta < theta_s, h above air entry:
+    saturated by its head, with no saturated neighbour, so the sweep moves its
+    head onto the retention curve) and B (unsaturated, below air entry)."""
+    n = 4
+    c = gw_base_case(n if axis == 0 else 1, n if axis == 1 else 1, n if axis == 2 else 1,
+                     0.05, 0.5)
+    sim = make(backend, c)
+    sim.step()
+    shape = sim.field_shape(abi.F_HEAD)
+    wc = np.full(shape, 0.1)
+    h = np.full(shape, -10.0)
+
+    def at(m):
+        return (m + 1 if axis == 1 else 1, m + 1 if axis == 0 else 1, m if axis == 2 else 0)
+
+    pa, pb = (at(1), at(2)) if a_first else (at(2), at(1))
+    wc[pa], h[pa] = 0.30, 0.05
+    wc[pb], h[pb] = 0.25, -1.0
+    sim.set(abi.F_HEAD, h)
+    sim.set(abi.F_WC, wc)
+    sim.test_post_allocate(0.5)
+    w, hh = sim.get(abi.F_WC), sim.get(abi.F_HEAD)
+    return (w[pa], hh[pa]), (w[pb], hh[pb])
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_head_rule_reads_the_pre_sweep_heads(backend, axis):
+    """The head rule classifies against the state before the sweep
+    (Reallocate.cpp:303-337 reads the snapshot): B sees A saturated by its
+    pre-sweep head whichever of the two the sweep visits first, and takes
+    theta(h_B); A, with no saturated neighbour, moves onto h(theta_A).  A sweep
+    that read heads in place would leave B alone when A comes first."""
+    from oracle import oracle as orc
+    soil = (TEST_SOIL["vg_alpha"], TEST_SOIL["vg_n"], TEST_SOIL["theta_s"],
+            TEST_SOIL["theta_r"], TEST_SOIL["aev"])
+    for a_first in (True, False):
+        (wa, ha), (wb, hb) = _snapshot_pair(backend, axis, a_first)
+        assert hb == -1.0 and wa == 0.30
+        assert wb == pytest.approx(orc.vg("theta", soil, -1.0), rel=1e-13), (axis, a_first)
+        assert ha == pytest.approx(orc.vg("head", soil, 0.30), rel=1e-12), (axis, a_first)
+
+
 def _oversaturated_state(sim):
     """A state the corrector could leave: an over-saturated block above a
     fringe just below theta_s, heads from the last step."""
