@@ -157,37 +157,56 @@ def run_form(args, precision, rtol):
 
 
 def accuracy(args, solvers):
-    """One step of each form from the FP64 form's final state, against that
-    step converged to rtol 1e-13 with FP64 residuals."""
+    """One step of each form from the FP64 form's final state, against the
+    FP64 oracle's step from the same state (rtol 1e-13, the host's threads):
+    head and theta error normwise, and each side's mass-audit residual
+    dV - (boundary_in - ss_storage + realloc_adjust - vloss) relative to V."""
     from frehg2_amd import abi
-    from frehg2_amd.gw import RichardsSolver
+    from oracle import oracle as orc
+    threads = min(os.cpu_count() or 1, 16)
+    orc.set_threads(threads)
     g64, case = solvers["fp64"]
     state = (g64.get(abi.F_HEAD), g64.get(abi.F_WC))
-    tcase = make_case(args, "mixed", 1e-13)
-    tcase.inner_rtol = 1e-3
-    truth = RichardsSolver(tcase)
-    out = {}
-    heads = {}
-    for name, g in [("truth", truth)] + [(f, solvers[f][0]) for f in solvers]:
+
+    def audit_residual(sim, v0):
+        a = sim.audit()
+        v = sim.volume()
+        return ((v - v0) - (a["boundary_in"] - a["ss_storage"] + a["realloc_adjust"]
+                            - a["vloss"])) / v
+
+    t0 = time.perf_counter()
+    ocase = make_case(args, "fp64", 1e-13)
+    ocase.atol = 1e-300
+    o = orc.OracleRichards(ocase)
+    o.set(abi.F_HEAD, state[0])
+    o.set(abi.F_WC, state[1])
+    v0 = o.volume()
+    o.step(case.dt)
+    hT, wT = o.get(abi.F_HEAD)[1:-1, 1:-1], o.get(abi.F_WC)[1:-1, 1:-1]
+    sT = o.last_solve()
+    out = {"reference": (f"the FP64 oracle's step from the FP64 form's final state, rtol 1e-13 "
+                         f"(residual {sT['residual_norm']:.3e}, ||b|| {sT['rhs_norm']:.3e}), "
+                         f"{threads} host threads, {time.perf_counter() - t0:.1f} s"),
+           "oracle_audit_residual": audit_residual(o, v0)}
+    o.close()
+    del o
+    for f in solvers:
+        g = solvers[f][0]
         g.set(abi.F_HEAD, state[0])
         g.set(abi.F_WC, state[1])
+        v0 = g.volume()
         g.step(case.dt)
-        heads[name] = (g.get(abi.F_HEAD)[1:-1, 1:-1], g.get(abi.F_WC)[1:-1, 1:-1],
-                       g.last_solve())
-    truth.close()
-    hT, wT, sT = heads["truth"]
-    out["reference"] = ("one step from the FP64 form's final state, converged to rtol 1e-13 "
-                        f"(residual {sT['residual_norm']:.3e}, ||b|| {sT['rhs_norm']:.3e})")
-    for f in solvers:
-        h, w, st = heads[f]
+        h, w = g.get(abi.F_HEAD)[1:-1, 1:-1], g.get(abi.F_WC)[1:-1, 1:-1]
         out[f"{f}_head_rel_err"] = float(np.max(np.abs(h - hT)) / np.max(np.abs(hT)))
         out[f"{f}_theta_rel_err"] = float(np.max(np.abs(w - wT)) / np.max(np.abs(wT)))
-        out[f"{f}_residual"] = st["residual_norm"]
+        out[f"{f}_residual"] = g.last_solve()["residual_norm"]
+        out[f"{f}_audit_residual"] = audit_residual(g, v0)
     out["target"] = ERR_TARGET
     out["met"] = all(out[f"{f}_head_rel_err"] <= ERR_TARGET for f in solvers)
     for f in solvers:
-        log(f"accuracy {f}: head {out[f'{f}_head_rel_err']:.3e}, "
-            f"theta {out[f'{f}_theta_rel_err']:.3e}")
+        log(f"accuracy {f} vs the FP64 oracle: head {out[f'{f}_head_rel_err']:.3e}, "
+            f"theta {out[f'{f}_theta_rel_err']:.3e}, audit residual "
+            f"{out[f'{f}_audit_residual']:.1e} (oracle {out['oracle_audit_residual']:.1e})")
     return out
 
 
@@ -588,12 +607,19 @@ def main():
     if args.columns:
         columns_bench(args)
         return
-    # the achievable streaming rate on this device, measured live: a copy and
-    # a 6-read/2-write FP64 stream over eight fields of this workload's size
+    # the achievable streaming rate on this device, measured live: the best
+    # copy and 6-read/2-write FP64 stream over access width, cache policy and
+    # occupancy, on eight fields of this workload's size
     from frehg2_amd import hbm
-    stream = hbm.measure((args.nx + 2) * (args.ny + 2) * args.nz)
-    log(f"hbm probe: copy {stream['copy_gbs']:.0f} GB/s, 6r2w stream "
-        f"{stream['stream_6r2w_gbs']:.0f} GB/s")
+    sw = hbm.sweep((args.nx + 2) * (args.ny + 2) * args.nz)
+    stream = {"copy_gbs": sw["best"]["copy"]["gbs"], "stream_6r2w_gbs": sw["best"]["6r2w"]["gbs"],
+              "stream_5r1w_gbs": sw["best"]["5r1w"]["gbs"],
+              "best_configs": {k: {kk: v[kk] for kk in ("bytes_per_lane", "nontemporal",
+                                                        "blocks_per_cu")}
+                               for k, v in sw["best"].items()}}
+    log(f"hbm probe (best of {len(sw['table'])} configurations): copy "
+        f"{stream['copy_gbs']:.0f} GB/s, 6r2w stream {stream['stream_6r2w_gbs']:.0f} GB/s, "
+        f"5r1w {stream['stream_5r1w_gbs']:.0f} GB/s")
     forms = [f for f in args.forms.split(",") if f]
     results = {}
     solvers = {}
@@ -627,7 +653,9 @@ def main():
                        "avg_launch_ms": r64["spmv_avg_ms"],
                        "algorithmic_bytes_per_launch": r64["spmv_bytes"],
                        "measured_stream_gbs": {"copy": stream["copy_gbs"],
-                                               "stream_6r2w": stream["stream_6r2w_gbs"]},
+                                               "stream_6r2w": stream["stream_6r2w_gbs"],
+                                               "stream_5r1w": stream["stream_5r1w_gbs"],
+                                               "best_configs": stream["best_configs"]},
                        "frac_of_measured_stream": bw / stream["stream_6r2w_gbs"]}
     out["fp64"] = r64
     if "mixed" in results:

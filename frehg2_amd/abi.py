@@ -138,9 +138,17 @@ COUPLER_SYMBOLS = {
 }
 
 
+class HbmProbeResult(C.Structure):
+    _fields_ = [("reads", C.c_int), ("writes", C.c_int), ("bytes_per_lane", C.c_int),
+                ("nontemporal", C.c_int), ("blocks_per_cu", C.c_int), ("mean_ms", C.c_double),
+                ("gbs", C.c_double)]
+
+
 # The achievable-bandwidth probe (include/frehg_probe.h; no reference counterpart).
 PROBE_SYMBOLS = {
     "frehg_hbm_probe": ([C.c_int64, C.c_int, _dp, _dp], C.c_int),
+    "frehg_hbm_probe_sweep": ([C.c_int64, C.c_int, C.POINTER(HbmProbeResult), C.c_int, _ip],
+                              C.c_int),
     "frehg_probe_last_error": ([], C.c_char_p),
 }
 
