@@ -126,6 +126,8 @@ SYMBOLS = {
     "frehg_ls_set_values": ([_vp, _vp], C.c_int),
     "frehg_ls_solve": ([_vp, _vp, _vp, C.POINTER(SolveStats)], C.c_int),
     "frehg_ls_is_symmetric": ([_vp, C.c_double, _ip], C.c_int),
+    "frehg_ls_set_precision": ([_vp, C.c_int, C.c_double], C.c_int),
+    "frehg_ls_structured": ([_vp, _ip, _dp], C.c_int),
 }
 
 

@@ -21,6 +21,7 @@ linear interpolation, clamped at both ends.
 from __future__ import annotations
 
 import math
+import warnings
 from dataclasses import dataclass
 from pathlib import Path
 from typing import Optional
@@ -144,6 +145,10 @@ def _read_field(base: Path, spec, shape, name):
     return arr.reshape(shape)
 
 
+# The reference's PETSc/hypre preconditioner names and what runs here.
+PC_REMAP = {"bjacobi-icc": "zline", "amg": "zline", "gamg": "zline"}
+
+
 def load_config(src, base_dir=None):
     """Build (GwCase, BoundarySchedule) from a YAML path or an already-parsed dict."""
     if isinstance(src, (str, Path)):
@@ -235,8 +240,21 @@ def load_config(src, base_dir=None):
     case.rtol = float(sg.get("rtol", 1e-8))
     case.atol = float(sg.get("atol", 1e-14))
     case.max_iterations = int(sg.get("max_iterations", 1000))
-    pc = str(sg.get("preconditioner", "zline"))
-    case.preconditioner = {"bjacobi-icc": "zline", "amg": "zline", "gamg": "zline"}.get(pc, pc)
+    pc = str(sg.get("preconditioner", "bjacobi-icc"))
+    # The reference's preconditioners (LinearSystem.cpp:158-221) run on
+    # PETSc/hypre; this path's PCG offers point Jacobi and the z-line block
+    # Jacobi.  bjacobi-icc (the reference default), amg and gamg map to
+    # z-line: the remap is recorded on the case and announced.
+    if pc in PC_REMAP:
+        case.preconditioner = PC_REMAP[pc]
+        case.preconditioner_requested = pc
+        if pc != "bjacobi-icc":
+            warnings.warn(f"solver.groundwater.preconditioner: {pc} runs as the z-line block "
+                          f"Jacobi PCG on this path", stacklevel=2)
+    elif pc in ("zline", "jacobi"):
+        case.preconditioner = pc
+    else:
+        raise ValueError(f"solver.groundwater.preconditioner: unknown value {pc!r}")
     case.precision = str(sg.get("precision", "fp64"))
     # boundary conditions
     entries = []

@@ -236,3 +236,15 @@ class LinearSystem:
         s = C.c_int()
         check(self.lib, self.lib.frehg_ls_is_symmetric(self._h, tol, C.byref(s)))
         return bool(s.value)
+
+    def set_precision(self, precision, inner_rtol=0.0) -> None:
+        """FP64 (default) or the FP32-inner mixed form (structured path only)."""
+        prec = {"fp64": abi.PREC_FP64, "mixed": abi.PREC_MIXED}.get(precision, precision)
+        check(self.lib, self.lib.frehg_ls_set_precision(self._h, int(prec), float(inner_rtol)))
+
+    def path(self) -> dict:
+        """Which path runs: 'structured' (the 7-point stencil PCG), 'pattern'
+        (the 7-point pattern, values not set or not symmetric) or 'generic'."""
+        s, a = C.c_int(), C.c_double()
+        check(self.lib, self.lib.frehg_ls_structured(self._h, C.byref(s), C.byref(a)))
+        return {"path": ("generic", "pattern", "structured")[s.value], "asymmetry": a.value}

@@ -296,12 +296,19 @@ int frehg_memcpy_dtoh(void* dst_host, const void* src_device, int64_t bytes);
 /* ---- solve seam (replaces frehg::LinearSystem / PETSc KSPCG) ---------- */
 
 /* LinearSystem(comm, prefix, nLocalRows, nGlobalRows, settings)
- * (src/core/LinearSystem.hpp:109-110; defaults :72-97). */
+ * (src/core/LinearSystem.hpp:109-110; defaults :72-97).  preconditioner is
+ * FREHG_PC_JACOBI or FREHG_PC_ZLINE (the z-line block Jacobi, the seam's
+ * counterpart of the reference's default bjacobi + icc(0),
+ * LinearSystem.cpp:206-221; it needs the 7-point pattern below). */
 int frehg_ls_create(int64_t n_rows, double rtol, double atol, int max_iterations,
                     int preconditioner, frehg_linear_system** out);
 int frehg_ls_destroy(frehg_linear_system* ls);
 /* LinearSystem::setPattern (src/core/LinearSystem.cpp:273-296): host COO
- * index arrays; entries with row or column -1 are ignored, duplicates sum. */
+ * index arrays; entries with row or column -1 are ignored, duplicates sum.
+ * The 7-point pattern RichardsSolver::buildCooPattern emits
+ * (src/gw/RichardsSolver.cpp:680-727: per row [diag, xm, xp, ym, yp, zm, zp],
+ * rows in gid order) is recognised and solved on a structured embedding by
+ * the step's stencil PCG; any other pattern runs a generic CSR PCG. */
 int frehg_ls_set_pattern(frehg_linear_system* ls, int64_t n_coo, const int64_t* rows,
                          const int64_t* cols);
 /* LinearSystem::setValues (src/core/LinearSystem.cpp:298-309): device array
@@ -315,6 +322,19 @@ int frehg_ls_solve(frehg_linear_system* ls, const double* rhs_device, double* so
                    frehg_gw_solve_stats* stats);
 /* LinearSystem::isSymmetric (src/core/LinearSystem.hpp:137). */
 int frehg_ls_is_symmetric(frehg_linear_system* ls, double tol, int* symmetric);
+/* Arithmetic of the seam's solve (no counterpart in the reference's
+ * SolverSettings, LinearSystem.hpp:72-97; the YAML extension
+ * solver.groundwater.precision): FREHG_PREC_FP64 (default) or
+ * FREHG_PREC_MIXED (FP32 inner PCG in FP64 refinement, structured path
+ * only); inner_rtol as in frehg_gw_desc (0 -> 1e-4). */
+int frehg_ls_set_precision(frehg_linear_system* ls, int precision, double inner_rtol);
+/* Which path the seam runs (no counterpart in the reference): *structured =
+ * 2 when the pattern is the 7-point pattern and the current values are
+ * symmetric (the stencil PCG runs), 1 when the pattern is but values are not
+ * set or not symmetric, 0 for the generic CSR path.  *asymmetry (may be
+ * NULL) is the largest |minus leg - mirror plus leg| / |diagonal| of the last
+ * values. */
+int frehg_ls_structured(frehg_linear_system* ls, int* structured, double* asymmetry);
 
 #ifdef __cplusplus
 }

@@ -112,6 +112,24 @@ def test_coupled_density_ratios_take_the_top_face_from_the_surface():
     assert np.max(np.abs(runs[0] - runs[1])) > 0.0
 
 
+def test_reference_preconditioners_map_to_zline_visibly():
+    import copy
+    import warnings
+    deck = copy.deepcopy(DECK)
+    deck.setdefault("solver", {}).setdefault("groundwater", {})["preconditioner"] = "amg"
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        case, _ = load_config(deck)
+    assert case.preconditioner == "zline" and case.preconditioner_requested == "amg"
+    assert any("amg" in str(x.message) for x in w)
+    deck["solver"]["groundwater"]["preconditioner"] = "jacobi"
+    case, _ = load_config(deck)
+    assert case.preconditioner == "jacobi"
+    deck["solver"]["groundwater"]["preconditioner"] = "ilu-magic"
+    with pytest.raises(ValueError):
+        load_config(deck)
+
+
 def test_polygon_and_series_semantics(tmp_path):
     sq = [(0, 0), (2, 0), (2, 2), (0, 2)]
     assert polygon_contains(sq, 1, 1) and polygon_contains(sq, 2, 1)  # edge counts as inside
