@@ -128,7 +128,12 @@ SYMBOLS = {
     "frehg_ls_is_symmetric": ([_vp, C.c_double, _ip], C.c_int),
     "frehg_ls_set_precision": ([_vp, C.c_int, C.c_double], C.c_int),
     "frehg_ls_structured": ([_vp, _ip, _dp], C.c_int),
+    "frehg_ls_solve_stencil_noalloc": ([C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                        C.c_double, C.c_double, C.c_int, C.c_int,
+                                        C.POINTER(SolveStats)], C.c_int),
 }
+
+NOALLOC_MAX_CELLS = 1048576  # FREHG_NOALLOC_MAX_CELLS
 
 
 # The coupling window (include/frehg_coupler.h; src/coupling/Exchange.cpp).

@@ -42,6 +42,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -753,6 +754,42 @@ struct Ls {
   }
 };
 
+// ---------------------------------------------------------------------------
+// The allocation-free variant of the structured solve
+// (frehg_ls_solve_stencil_noalloc): every work vector in one file-scope
+// device array.  Each GPU holds its own copy of it from the library's load
+// until the process ends.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocCells = FREHG_NOALLOC_MAX_CELLS;
+constexpr int64_t kNoAllocDoubles = 7 * (kNoAllocCells + 32) + 4 * 4096 + 256;
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+double g_scratch_red[kMaxDevices][4];
+alignas(64) unsigned char g_scratch_state[kMaxDevices][256];
+
+// The device address of the current device's scratch array, looked up once
+// per device with that device current (the array's name in host code is a
+// host-side stand-in, not a device address).
+double* scratch_of_current_device(int* devOut) {
+  int dev = 0;
+  LS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) ls_fail(FREHG_EDEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      ls_fail(FREHG_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *devOut = dev;
+  return g_scratch_ptr[dev];
+}
+
 template <class Fn>
 int ls_guard(Fn fn) {
   try {
@@ -901,6 +938,58 @@ int frehg_ls_structured(frehg_linear_system* h, int* structured, double* asymmet
     if (!ls || !structured) frehg::ls_fail(FREHG_EINVAL, "invalid arguments");
     *structured = ls->structuredPattern ? (ls->values ? (ls->structuredValues ? 2 : 1) : 1) : 0;
     if (asymmetry) *asymmetry = ls->asymmetry;
+  });
+}
+
+int frehg_ls_solve_stencil_noalloc(int nx, int ny, int nz, const double* Ad, const double* Axp,
+                                   const double* Ayp, const double* Azp, const double* rhs,
+                                   double* sol, double rtol, double atol, int max_iterations,
+                                   int preconditioner, frehg_gw_solve_stats* stats) {
+  return frehg::ls_guard([&] {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !Ad || !Axp || !Ayp || !Azp || !rhs || !sol)
+      frehg::ls_fail(FREHG_EINVAL, "invalid arguments");
+    if (preconditioner != FREHG_PC_JACOBI && preconditioner != FREHG_PC_ZLINE)
+      frehg::ls_fail(FREHG_EUNSUPPORTED, "point Jacobi or z-line");
+    if (nz > 512) frehg::ls_fail(FREHG_EUNSUPPORTED, "nz > 512 is not supported by the column kernels");
+    const int64_t nH = static_cast<int64_t>(nx + 2) * (ny + 2) * nz;
+    if (nH > frehg::kNoAllocCells)
+      frehg::ls_fail(FREHG_EINVAL, "(nx+2)(ny+2)nz = " + std::to_string(nH) +
+                                       " exceeds the allocation-free scratch maximum of " +
+                                       std::to_string(frehg::kNoAllocCells) + " cells");
+    const frehg::Grid g{nx, ny, nz, nx + 2, ny + 2, 1.0, 1.0, 1.0};
+    const int64_t need = frehg::Pcg::workspaceDoubles(g);
+    if (need > frehg::kNoAllocDoubles)
+      frehg::ls_fail(FREHG_EINVAL, "the grid needs more scratch than the array holds");
+    int dev = 0;
+    double* base = frehg::scratch_of_current_device(&dev);
+    frehg::PcgWorkspace ws;
+    ws.device = base;
+    ws.doubles = frehg::kNoAllocDoubles;
+    ws.hostRed = frehg::g_scratch_red[dev];
+    ws.hostState = frehg::g_scratch_state[dev];
+    const hipStream_t s = nullptr;  // the null stream: the call returns when the solve is done
+    frehg::Pcg pcg(g, s, ws);
+    pcg.clearVectors();
+    frehg::k_ls_fill<<<std::max(1, static_cast<int>(std::min<int64_t>(2048, (nH + 255) / 256))),
+                       frehg::kT, 0, s>>>(nH, sol, 0.0);
+    LS_HIP(hipGetLastError());
+    frehg::PcgOptions o;
+    o.rtol = rtol;
+    o.atol = atol;
+    o.max_iterations = max_iterations > 0 ? max_iterations : 500;
+    o.zline = preconditioner == FREHG_PC_ZLINE;
+    frehg::PcgStats ps;
+    const frehg::Stencil<double> A{const_cast<double*>(Ad), const_cast<double*>(Axp),
+                                   const_cast<double*>(Ayp), const_cast<double*>(Azp)};
+    pcg.solve(A, rhs, sol, o, &ps);
+    LS_HIP(hipStreamSynchronize(s));
+    frehg_gw_solve_stats st{};
+    st.iterations = ps.iterations;
+    st.converged = ps.converged;
+    st.residual_norm = ps.residual_norm;
+    st.rhs_norm = ps.rhs_norm;
+    if (stats) *stats = st;
+    if (!st.converged) frehg::ls_fail(FREHG_EDIVERGED, "linear solve did not converge");
   });
 }
 

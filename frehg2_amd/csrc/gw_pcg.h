@@ -49,11 +49,32 @@ struct PcgOptions {
   bool stencil32_ready = false;  // mixed: the caller wrote the FP32 stencil (stencil32())
 };
 
+// Caller-owned memory for FP64 solves (Pcg's external mode, which allocates
+// nothing): device doubles for the work vectors, partial sums and state, and
+// host buffers for the reduced values and the polled state.
+struct PcgWorkspace {
+  double* device = nullptr;    // >= Pcg::workspaceDoubles(g) doubles, 256-byte aligned
+  int64_t doubles = 0;
+  double* hostRed = nullptr;   // 4 doubles
+  void* hostState = nullptr;   // Pcg::hostStateBytes() bytes
+};
+
 // Device-resident PCG workspace for one grid (allocated once).
 class Pcg {
  public:
   Pcg(const Grid& g, hipStream_t stream);
+  // External mode: the work vectors live in ws (nothing is allocated or
+  // freed); FP64 solves only (the mixed form's FP32 arrays are allocated).
+  Pcg(const Grid& g, hipStream_t stream, const PcgWorkspace& ws);
   ~Pcg();
+  // Device doubles the external mode needs for grid g at the default launch
+  // shapes on the current device, and the host state size.
+  static int64_t workspaceDoubles(const Grid& g);
+  static size_t hostStateBytes();
+  // Zero r, z and both p buffers (a fill kernel): the stencil kernels read z
+  // and p at the ghost cells around the grid, which a reused workspace may
+  // hold from an earlier, larger grid.
+  void clearVectors();
   Pcg(const Pcg&) = delete;
   Pcg& operator=(const Pcg&) = delete;
 
@@ -94,7 +115,13 @@ class Pcg {
                       const PcgOptions& opt, PcgStats* st, int* itDone);
   void reduceHost(int nparts, int nvals, double* out);
   void applyShapes();
+  void initCommon();
 
+  struct Shapes {
+    int blocks = 0, pstride = 0;
+    March march64, march32;
+  };
+  static Shapes shapesFor(const Grid& g, int cus, int wpcCol, int wpc64, int wpc32);
   static March makeMarch(const Grid& g, int cus, int wavesPerCu);
 
   Grid g_;
@@ -117,8 +144,8 @@ class Pcg {
   double* red_ = nullptr;     // reduced values (device)
   double* redHost_ = nullptr; // pinned mirror
   void* state_ = nullptr;     // PcgState (device)
-  void* stateHost_ = nullptr; // pinned mirror
-  hipEvent_t evA_[2] = {nullptr, nullptr};
+  void* stateHost_ = nullptr; // pinned mirror (external mode: the caller's host buffer)
+  bool external_ = false;     // memory owned by the caller (PcgWorkspace)
 };
 
 }  // namespace frehg

@@ -987,16 +987,82 @@ T* dalloc(int64_t n) {
 
 }  // namespace
 
-Pcg::Pcg(const Grid& g, hipStream_t stream) : g_(g), stream_(stream) {
-  if (g.nz > 512) throw std::runtime_error("nz > 512 is not supported by the column kernels");
-  int cus = 256;
-  hipDeviceProp_t prop;
-  int dev = 0;
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-    cus = prop.multiProcessorCount;
-  cus_ = cus;
-  pairs_ = g.nz % 2 == 0;
+__global__ void k_fill_vectors(int64_t n, double* __restrict__ a, double* __restrict__ b,
+                               double* __restrict__ c, double* __restrict__ d) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    a[i] = 0.0;
+    b[i] = 0.0;
+    c[i] = 0.0;
+    d[i] = 0.0;
+  }
+}
+
+int device_cus() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      cus > 0)
+    return cus;
+  return 256;
+}
+
+void Pcg::initCommon() {
+  if (g_.nz > 512) throw std::runtime_error("nz > 512 is not supported by the column kernels");
+  cus_ = device_cus();
+  pairs_ = g_.nz % 2 == 0;
   applyShapes();
+}
+
+// A vector slot of the external workspace: whole 256-byte lines.
+constexpr int64_t padded(int64_t n) { return (n + 31) / 32 * 32; }
+
+int64_t Pcg::workspaceDoubles(const Grid& g) {
+  const int cus = device_cus();
+  const Pcg::Shapes sh = shapesFor(g, cus, 16, 16, 32);
+  return 7 * padded(g.cellsH()) + padded(4 * static_cast<int64_t>(sh.pstride)) + padded(4) +
+         padded((sizeof(PcgState) + sizeof(double) - 1) / sizeof(double));
+}
+
+size_t Pcg::hostStateBytes() { return sizeof(PcgState); }
+
+Pcg::Pcg(const Grid& g, hipStream_t stream, const PcgWorkspace& ws)
+    : g_(g), stream_(stream), external_(true) {
+  initCommon();
+  const int64_t nH = g.cellsH();
+  int64_t off = 0;
+  auto take = [&](int64_t n) {
+    double* p = ws.device + off;
+    off += padded(n);
+    return p;
+  };
+  r_ = take(nH);
+  z_ = take(nH);
+  p0_ = take(nH);
+  p1_ = take(nH);
+  q_ = take(nH);
+  invd_ = take(nH);
+  w_ = take(nH);
+  parts_ = take(4 * static_cast<int64_t>(pstride_));
+  partsCap_ = pstride_;
+  red_ = take(4);
+  state_ = take((sizeof(PcgState) + sizeof(double) - 1) / sizeof(double));
+  if (!ws.device || !ws.hostRed || !ws.hostState || off > ws.doubles)
+    throw std::runtime_error("PCG workspace too small for this grid");
+  redHost_ = ws.hostRed;
+  stateHost_ = ws.hostState;
+}
+
+void Pcg::clearVectors() {
+  const int64_t nH = g_.cellsH();
+  const int blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(
+      int64_t(cus_) * 8, (nH + kBlock - 1) / kBlock)));
+  k_fill_vectors<<<blocks, kBlock, 0, stream_>>>(nH, r_, z_, p0_, p1_);
+  HIP_OK(hipGetLastError());
+}
+
+Pcg::Pcg(const Grid& g, hipStream_t stream) : g_(g), stream_(stream) {
+  initCommon();
   const int64_t nH = g.cellsH();
   r_ = dalloc<double>(nH);
   z_ = dalloc<double>(nH);
@@ -1011,11 +1077,10 @@ Pcg::Pcg(const Grid& g, hipStream_t stream) : g_(g), stream_(stream) {
   state_ = dalloc<PcgState>(1);
   HIP_OK(hipHostMalloc(&redHost_, 4 * sizeof(double), hipHostMallocDefault));
   HIP_OK(hipHostMalloc(&stateHost_, sizeof(PcgState), hipHostMallocDefault));
-  HIP_OK(hipEventCreate(&evA_[0]));
-  HIP_OK(hipEventCreate(&evA_[1]));
 }
 
 Pcg::~Pcg() {
+  if (external_) return;
   for (void* p : {(void*)r_, (void*)z_, (void*)p0_, (void*)p1_, (void*)q_, (void*)invd_, (void*)w_,
                   (void*)Ad32_, (void*)Axp32_, (void*)Ayp32_, (void*)Azp32_, (void*)r32_,
                   (void*)z32_, (void*)p032_, (void*)p132_, (void*)q32_, (void*)d32_,
@@ -1023,8 +1088,6 @@ Pcg::~Pcg() {
     if (p) (void)hipFree(p);
   if (redHost_) (void)hipHostFree(redHost_);
   if (stateHost_) (void)hipHostFree(stateHost_);
-  for (hipEvent_t e : evA_)
-    if (e) (void)hipEventDestroy(e);
 }
 
 double Pcg::spmvBytesPerLaunch(bool mixed) const {
@@ -1033,14 +1096,25 @@ double Pcg::spmvBytesPerLaunch(bool mixed) const {
   return per * static_cast<double>(g_.columns()) * g_.nz;
 }
 
+Pcg::Shapes Pcg::shapesFor(const Grid& g, int cus, int wpcCol, int wpc64, int wpc32) {
+  Shapes sh;
+  const int64_t ncol = g.columns();
+  const int64_t wavesWanted = std::max<int64_t>(8, std::min<int64_t>(ncol, int64_t(cus) * wpcCol));
+  sh.blocks = static_cast<int>(std::max<int64_t>(8, ((wavesWanted + kWpb - 1) / kWpb + 7) / 8 * 8));
+  sh.march64 = makeMarch(g, cus, wpc64);
+  sh.march32 = makeMarch(g, cus, wpc32);
+  sh.pstride = std::max({sh.blocks, sh.march64.blocks, sh.march32.blocks});
+  return sh;
+}
+
 void Pcg::applyShapes() {
-  const int64_t ncol = g_.columns();
-  const int64_t wavesWanted =
-      std::max<int64_t>(8, std::min<int64_t>(ncol, int64_t(cus_) * wpcCol_));
-  blocks_ = static_cast<int>(std::max<int64_t>(8, ((wavesWanted + kWpb - 1) / kWpb + 7) / 8 * 8));
-  march64_ = makeMarch(g_, cus_, wpc64_);
-  march32_ = makeMarch(g_, cus_, wpc32_);
-  pstride_ = std::max({blocks_, march64_.blocks, march32_.blocks});
+  const Shapes sh = shapesFor(g_, cus_, wpcCol_, wpc64_, wpc32_);
+  blocks_ = sh.blocks;
+  march64_ = sh.march64;
+  march32_ = sh.march32;
+  pstride_ = sh.pstride;
+  if (parts_ && pstride_ > partsCap_ && external_)
+    throw std::runtime_error("launch shapes beyond the external workspace's partial sums");
   if (parts_ && pstride_ > partsCap_) {
     HIP_OK(hipStreamSynchronize(stream_));
     HIP_OK(hipFree(parts_));
@@ -1279,6 +1353,7 @@ int Pcg::solveFp64(const Stencil<double>& A, const double* b, double* x, const P
 }
 
 Stencil<float> Pcg::stencil32() {
+  if (external_) throw std::runtime_error("the mixed form allocates its FP32 workspace");
   const int64_t nH = g_.cellsH();
   if (!Ad32_) {
     Ad32_ = dalloc<float>(nH);

@@ -336,6 +336,32 @@ int frehg_ls_set_precision(frehg_linear_system* ls, int precision, double inner_
  * values. */
 int frehg_ls_structured(frehg_linear_system* ls, int* structured, double* asymmetry);
 
+/* Largest grid of frehg_ls_solve_stencil_noalloc, in halo cells
+ * (nx+2)(ny+2)nz: 2^20, about 100 x 100 x 100 cells.  Its scratch array is
+ * 7 vectors of this many doubles plus partial sums, ~59 MB per GPU. */
+#define FREHG_NOALLOC_MAX_CELLS 1048576
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the allocation-free variant of frehg_ls_solve's
+ * structured path.  It runs the same FP64 stencil PCG kernels (point Jacobi
+ * or z-line, zero initial guess, the same convergence test and
+ * FREHG_EDIVERGED on failure) on a system given directly in the symmetric
+ * 7-point stencil storage: device arrays in halo layout (ny+2)(nx+2)nz, Ad
+ * the diagonal, Axp / Ayp / Azp the coupling of a cell with its i+1 / j+1 /
+ * k+1 neighbour (zero toward missing neighbours and in the ghost cells),
+ * inactive cells as identity rows; rhs and sol likewise (sol is zeroed
+ * first).  Every work vector lives in a static device array of the library
+ * (FREHG_NOALLOC_MAX_CELLS halo cells; larger grids return FREHG_EINVAL
+ * before any device work); no hipMalloc, hipFree or other allocating call is
+ * made.  The call is synchronous on the null stream.  All calls on one GPU
+ * share that array and it keeps the last call's contents: calls that overlap
+ * on one GPU race. */
+int frehg_ls_solve_stencil_noalloc(int nx, int ny, int nz, const double* Ad, const double* Axp,
+                                   const double* Ayp, const double* Azp, const double* rhs,
+                                   double* sol, double rtol, double atol, int max_iterations,
+                                   int preconditioner, frehg_gw_solve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
