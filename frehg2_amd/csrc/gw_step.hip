@@ -2911,13 +2911,17 @@ int frehg_gw_set_field(frehg_gw_solver* s, int field, const double* host_in) {
       g->setSink(host_in);
       return;
     }
-    if (field != FREHG_F_HEAD && field != FREHG_F_WC)
-      frehg::fail(FREHG_EINVAL, "only head, water content and the sink are settable");
+    if (field != FREHG_F_HEAD && field != FREHG_F_WC && field != FREHG_F_VLOSS)
+      frehg::fail(FREHG_EINVAL, "only head, water content, vloss and the sink are settable");
     if (!host_in) frehg::fail(FREHG_EINVAL, "null argument");
     int64_t n = 0;
     double* p = g->fieldPtr(field, &n);
     HIP_CHECK(hipMemcpyAsync(p, host_in, static_cast<size_t>(n) * sizeof(double),
                              hipMemcpyHostToDevice, g->stream));
+    if (field == FREHG_F_VLOSS) {  // a cumulative diagnostic: nothing derives from it
+      HIP_CHECK(hipStreamSynchronize(g->stream));
+      return;
+    }
     g->ghosts(true, true);
     g->copy(g->d.hn, g->d.h, g->g.cellsH());
     g->copy(g->d.wcn, g->d.wc, g->g.cellsH());

@@ -207,6 +207,8 @@ int frehg_gw_get_field(frehg_gw_solver* s, int field, double* host_out);
 /* Overwrite h or theta from host memory and rebuild the derived state
  * (checkpoint restore + RichardsSolver::refreshDerivedState,
  * src/gw/RichardsSolver.cpp:880-890): FREHG_F_HEAD / FREHG_F_WC.
+ * FREHG_F_VLOSS restores the cumulative clamp volume, the checkpoint's
+ * vloss_cum (src/driver/Simulation.cpp:720-723), with nothing rebuilt.
  * FREHG_F_SINK sets the root-uptake sink S [1/s] that the following steps
  * apply (the vegetation module's sink_, src/gw/Evapotranspiration.cpp:
  * 560-634, consumed at src/gw/Predictor.cpp:700-726 and

@@ -2049,7 +2049,13 @@ int oracle_gw_set_field(void* p, int f, const double* in) {
       }
       return;
     }
-    if (f != FREHG_F_HEAD && f != FREHG_F_WC) fail(FREHG_EINVAL, "only head, water content and the sink are settable");
+    if (f == FREHG_F_VLOSS) {  // the checkpoint's cumulative vloss_cum
+      if (!in) fail(FREHG_EINVAL, "null argument");
+      std::memcpy(o.vloss.v.data(), in, o.vloss.v.size() * sizeof(double));
+      return;
+    }
+    if (f != FREHG_F_HEAD && f != FREHG_F_WC)
+      fail(FREHG_EINVAL, "only head, water content, vloss and the sink are settable");
     if (!in) fail(FREHG_EINVAL, "null argument");
     F3& fld = (f == FREHG_F_HEAD) ? o.h : o.wc;
     std::memcpy(fld.v.data(), in, fld.v.size() * sizeof(double));

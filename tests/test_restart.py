@@ -43,6 +43,7 @@ def test_restart_continues_bit_identically(backend, name):
     b = make(backend, case)
     b.set(abi.F_HEAD, a.get(abi.F_HEAD))
     b.set(abi.F_WC, a.get(abi.F_WC))
+    b.set(abi.F_VLOSS, a.get(abi.F_VLOSS))  # the checkpoint's vloss_cum
     if case.coupling is not None:
         for n in CPL:
             b.set_coupling_state(n, a.coupling_state(n))
@@ -50,7 +51,7 @@ def test_restart_continues_bit_identically(backend, name):
         dt = a.next_dt() if case.adaptive else case.dt
         a.step(dt)
         b.step(dt)
-        for f in (abi.F_HEAD, abi.F_WC, abi.F_QX, abi.F_QY, abi.F_QZF):
+        for f in (abi.F_HEAD, abi.F_WC, abi.F_QX, abi.F_QY, abi.F_QZF, abi.F_VLOSS):
             assert np.array_equal(a.get(f), b.get(f)), (name, s, f)
         aa, ab = a.audit(), b.audit()
         assert aa == ab, (name, s, aa, ab)
@@ -59,3 +60,26 @@ def test_restart_continues_bit_identically(backend, name):
                 assert np.array_equal(a.coupling_state(n), b.coupling_state(n)), (name, s, n)
         if case.adaptive:
             assert a.next_dt() == b.next_dt()
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_vloss_cum_restores_and_keeps_accumulating(backend):
+    """A column driven past theta_s and clamped books vloss each step; the
+    restored cumulative field continues from the checkpoint's value
+    (src/driver/Simulation.cpp:720-723)."""
+    from helpers import gw_base_case
+    c = gw_base_case(1, 1, 10, 0.05, 0.5, ic_const=0.45, reallocation="disabled")
+    c.set_bc("top", abi.BC_FLUX, -4.0e-4)
+    a = make(backend, c)
+    for _ in range(4):
+        a.step()
+    cum = a.get(abi.F_VLOSS)
+    assert np.abs(cum).sum() > 0.0
+    b = make(backend, c)
+    b.set(abi.F_HEAD, a.get(abi.F_HEAD))
+    b.set(abi.F_WC, a.get(abi.F_WC))
+    b.set(abi.F_VLOSS, cum)
+    a.step()
+    b.step()
+    assert np.array_equal(a.get(abi.F_VLOSS), b.get(abi.F_VLOSS))
+    assert a.audit()["vloss"] == b.audit()["vloss"]
