@@ -33,8 +33,10 @@ F_VLOSS, F_DZ3D, F_BOT3D = 10, 11, 12
 F_AREA_X, F_AREA_Y, F_SIN_X, F_COS_X, F_SIN_Y, F_COS_Y = 13, 14, 15, 16, 17, 18
 F_CAPACITY = 19
 F_SINK = 20
+F_WIN_X, F_WIN_Y, F_WIN_Z, F_WIN_SINK, F_QZ_PATH_CORR, F_WIN_VENT, F_WIN_VAP = range(21, 28)
 
-FACE_FIELDS = (F_KZF, F_QZF)
+FACE_FIELDS = (F_KZF, F_QZF, F_WIN_Z, F_QZ_PATH_CORR)
+PLANE_FIELDS = (F_WIN_VENT, F_WIN_VAP)
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -105,6 +107,8 @@ SYMBOLS = {
     "frehg_gw_get_field": ([_vp, C.c_int, _dp], C.c_int),
     "frehg_gw_set_field": ([_vp, C.c_int, _dp], C.c_int),
     "frehg_gw_set_bc_values": ([_vp, C.c_int, _dp], C.c_int),
+    "frehg_gw_enable_transport_window": ([_vp], C.c_int),
+    "frehg_gw_reset_transport_window": ([_vp], C.c_int),
     "frehg_gw_set_density": ([_vp, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
     "frehg_gw_set_et": ([_vp, _ip, _dp], C.c_int),
     "frehg_gw_attach_coupling": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int],

@@ -130,6 +130,11 @@ struct Dev {
   float *Ad32, *Axp32, *Ayp32, *Azp32;  // the stencil rounded to FP32 (mixed form), or null
   double* parts;
   int rb;
+  // the transport window (RichardsSolver.cpp:796-878), when tw: face volumes
+  // since the last reset (winX, winY, winSink: H; winZ: F; winVent, winVap:
+  // 2D) and the walks' path corrections of the step (pathCorr: F)
+  int tw;
+  double *winX, *winY, *winZ, *winSink, *pathCorr, *winVent, *winVap;
 };
 
 __device__ __forceinline__ bool act(const Dev& d, int j, int i, int k) {
@@ -1199,6 +1204,13 @@ __device__ __forceinline__ void col_set(double (&v)[N], int k, double x, int lan
   for (int mm = 0; mm < N; ++mm)
     if (m == mm && lane == (k & 63)) v[mm] = x;
 }
+template <int N>
+__device__ __forceinline__ void col_add(double (&v)[N], int k, double x, int lane) {
+  const int m = k >> 6;
+#pragma unroll
+  for (int mm = 0; mm < N; ++mm)
+    if (m == mm && lane == (k & 63)) v[mm] += x;
+}
 // Highest set row below k, or -1.
 template <int R>
 __device__ __forceinline__ int mask_prev(const unsigned long long (&mk)[R], int k) {
@@ -1612,6 +1624,7 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk(Dev d, double dtg) {
       if constexpr (R > kParallelWalkRows) {
         continue;  // k_realloc_walk_serial
       } else {
+        if (d.tw) continue;  // path recording: k_realloc_walk_serial
         const int64_t f0 = g.cF(j, i, 0);
         double room[R], sUp[R], sDn[R], qz[RF];
         unsigned long long sendMask[R], roomMask[R];
@@ -1646,7 +1659,7 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk(Dev d, double dtg) {
 // reference's loop visits the full cells in between and leaves them
 // untouched).  Every arithmetic operation and its order are the reference's.
 // Same grid as k_realloc_walk: its audit partials are added to that kernel's.
-template <int R>
+template <int R, bool TW>
 __global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double dtg) {
   constexpr int RF = R + 1;
   const Grid& g = d.g;
@@ -1716,8 +1729,34 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double 
           cQ = col_get<RF>(qz, f);
         }
       };
+      // Path recording (the transport window, Reallocate.cpp:425-445): a
+      // transfer from cell `from` to cell `to` (the cell above the column
+      // top for a vent) takes its legacy mirror back off `legacyFace` (pt)
+      // and puts its rate on every face it crosses, as a difference array
+      // (df) summed along the column at the end.
+      double pt[TW ? RF : 1], df[TW ? RF : 1];
+      if constexpr (TW) {
+#pragma unroll
+        for (int m = 0; m < RF; ++m) {
+          pt[m] = 0.0;
+          df[m] = 0.0;
+        }
+      }
+      auto record = [&](int from, int to, double vol, int legacyFace, double legacySign) {
+        if constexpr (TW) {
+          const double rate = vol / dtg;
+          col_add<RF>(pt, legacyFace, -legacySign * rate, lane);
+          if (to < from) {  // faces to+1 .. from, upward
+            col_add<RF>(df, to + 1, rate, lane);
+            col_add<RF>(df, from + 1, -rate, lane);
+          } else {          // faces from+1 .. to, downward
+            col_add<RF>(df, from + 1, -rate, lane);
+            col_add<RF>(df, to + 1, rate, lane);
+          }
+        }
+      };
       // One receiving cell of a walk: room, theta and the mirror face flux.
-      auto take_at = [&](int ll, double& dV, int faceRow, bool faceAdd) {
+      auto take_at = [&](int from, int ll, double& dV, int faceRow, bool faceAdd) {
         use_cell(ll);
         const double take = (cRoom > dV) ? dV : cRoom;
         cW = cW + take / (g.az * cDz);
@@ -1726,6 +1765,7 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double 
         if (!(left > 0.0)) mask_clear<R>(roomMask, ll);
         use_face(faceRow);
         cQ = faceAdd ? cQ + take / dtg : cQ - take / dtg;
+        record(from, ll, take, faceRow, faceAdd ? 1.0 : -1.0);
         dV -= take;
       };
 #pragma unroll 1
@@ -1748,7 +1788,7 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double 
                 break;
               }
               ll = nxt;
-              take_at(ll, dV, ll + 1, true);
+              take_at(k, ll, dV, ll + 1, true);
             }
             if (dV > 0.0) {
               if (d.coupled) {
@@ -1764,6 +1804,8 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double 
                   dept += dd;
                   use_face(ll);
                   cQ = cQ + dV / dtg;
+                  record(k, ll - 1, dV, ll, 1.0);
+                  if (TW && lead) d.winVent[c2] += dV;
                   if (lead) vent.add(dV);
                 } else if (lead) {
                   lost.add(-dV);
@@ -1772,13 +1814,14 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double 
               } else if (topIsHead) {
                 use_face(ll);
                 cQ = cQ + dV / dtg;
+                record(k, ll - 1, dV, ll, 1.0);
                 dV = 0.0;
               } else {
                 while (dV > 0.0) {
                   const int nxt = mask_next<R>(roomMask, ll);
                   if (nxt < 0) break;
                   ll = nxt;
-                  take_at(ll, dV, ll, false);
+                  take_at(k, ll, dV, ll, false);
                 }
                 if (lead) lost.add(-dV);
                 dV = 0.0;
@@ -1792,7 +1835,7 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double 
               const int nxt = mask_next<R>(roomMask, ll);
               if (nxt < 0) break;
               ll = nxt;
-              take_at(ll, dV, ll, false);
+              take_at(k, ll, dV, ll, false);
             }
             if (lead) lost.add(-dV);
           }
@@ -1804,6 +1847,15 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double 
       for (int m = 0; m < RF; ++m) {
         const int k = lane + kWave * m;
         if (k <= nz) d.qzF[f0 + k] = qz[m];
+      }
+      if constexpr (TW) {
+        double carry = 0.0;
+#pragma unroll
+        for (int m = 0; m < RF; ++m) {
+          const int k = lane + kWave * m;
+          const double run = scan_affine_up(df[m], 1.0, carry, lane);
+          if (k <= nz) d.pathCorr[f0 + k] = pt[m] + run;
+        }
       }
       clamp_column<R>(d, lane, kt, b0, wcv, dzv, vl);
     }
@@ -1956,6 +2008,25 @@ __global__ void k_initial(Dev d, int form, const double* vals, double vconst, do
     }
     d.h[c] = hv;
     d.wc[c] = w;
+  }
+}
+
+// ---- the transport window (RichardsSolver.cpp:826-878) ----------------------
+// Every face slot, halos included: x, y, the root-uptake volumes (H), the z
+// faces with the walks' path corrections (F) and the ET vapour (2D).
+__global__ void k_transport_window(Dev d, double dtg) {
+  const Grid& g = d.g;
+  const int64_t nH = g.cellsH(), nF = g.cellsF();
+  const int64_t n2 = static_cast<int64_t>(g.ny2) * g.nx2;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < nF;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    if (t < nH) {
+      d.winX[t] += d.qx[t] * dtg;
+      d.winY[t] += d.qy[t] * dtg;
+      if (d.sink) d.winSink[t] += d.sink[t] * g.az * d.dz3d[t] * dtg;
+    }
+    d.winZ[t] += (d.qzF[t] + d.pathCorr[t]) * dtg;
+    if (t < n2 && d.etCol) d.winVap[t] += d.etQ[t] * g.az * dtg;
   }
 }
 
@@ -2114,15 +2185,47 @@ struct GwSolver {
   void postAllocation(double dtg) {
     k_realloc_classify<<<rb, kThreads, 0, stream>>>(d);
     HIP_CHECK(hipGetLastError());
+    if (d.tw) fill(d.pathCorr, g.cellsF(), 0.0);
     dispatch_col(g.nz, [&](auto RC) {
       constexpr int R = decltype(RC)::value;
       k_realloc_walk<R><<<rbCol, kThreads, 0, stream>>>(d, dtg);
-      k_realloc_walk_serial<R><<<rbCol, kThreads, 0, stream>>>(d, dtg);
+      if (d.tw) k_realloc_walk_serial<R, true><<<rbCol, kThreads, 0, stream>>>(d, dtg);
+      else k_realloc_walk_serial<R, false><<<rbCol, kThreads, 0, stream>>>(d, dtg);
     });
     HIP_CHECK(hipGetLastError());
     std::swap(d.h, d.hOut);
     std::swap(d.wc, d.wcOut);
     ghosts(false, true);
+  }
+  void fill(double* p, int64_t n, double v) {
+    k_fill<<<static_cast<int>(std::min<int64_t>(2048, (n + kThreads - 1) / kThreads)), kThreads, 0,
+             stream>>>(p, n, v);
+    HIP_CHECK(hipGetLastError());
+  }
+  // RichardsSolver::enableTransportWindow / resetTransportWindow
+  // (RichardsSolver.cpp:796-824).
+  void enableTransportWindow() {
+    if (d.tw) return;
+    const int64_t nH = g.cellsH(), nF = g.cellsF();
+    const int64_t n2 = static_cast<int64_t>(g.ny2) * g.nx2;
+    d.winX = alloc<double>(nH);
+    d.winY = alloc<double>(nH);
+    d.winSink = alloc<double>(nH);
+    d.winZ = alloc<double>(nF);
+    d.pathCorr = alloc<double>(nF);
+    d.winVent = alloc<double>(n2);
+    d.winVap = alloc<double>(n2);
+    d.tw = 1;
+  }
+  void resetTransportWindow() {
+    if (!d.tw) return;
+    const int64_t nH = g.cellsH(), nF = g.cellsF();
+    const int64_t n2 = static_cast<int64_t>(g.ny2) * g.nx2;
+    for (double* p : {d.winX, d.winY, d.winSink}) fill(p, nH, 0.0);
+    fill(d.winZ, nF, 0.0);
+    fill(d.winVent, n2, 0.0);
+    fill(d.winVap, n2, 0.0);
+    HIP_CHECK(hipStreamSynchronize(stream));
   }
   // The sweep alone on the installed state (frehg_gw_test_post_allocate).
   void testPostAllocate(double dtg) {
@@ -2748,6 +2851,10 @@ void GwSolver::step(double /*t*/, double dtg) {
     postAllocation(dtg);
     k_boundary_in<<<std::min(rb, rbCol * 4), kThreads, 0, stream>>>(d, dtg);
     HIP_CHECK(hipGetLastError());
+    if (d.tw) {
+      k_transport_window<<<rb, kThreads, 0, stream>>>(d, dtg);
+      HIP_CHECK(hipGetLastError());
+    }
     if (adapt) {
       k_adapt_inputs<<<rb, kThreads, 0, stream>>>(d, dtg);
       HIP_CHECK(hipGetLastError());
@@ -2822,6 +2929,24 @@ double* GwSolver::fieldPtr(int f, int64_t* n) {
     case FREHG_F_SINK:
       if (!d.sink) fail(FREHG_EINVAL, "no root-uptake sink is set");
       return d.sink;
+    case FREHG_F_WIN_X:
+    case FREHG_F_WIN_Y:
+    case FREHG_F_WIN_Z:
+    case FREHG_F_WIN_SINK:
+    case FREHG_F_QZ_PATH_CORR:
+    case FREHG_F_WIN_VENT:
+    case FREHG_F_WIN_VAP:
+      if (!d.tw) fail(FREHG_EINVAL, "the transport window is not enabled");
+      if (f == FREHG_F_WIN_Z || f == FREHG_F_QZ_PATH_CORR) *n = g.cellsF();
+      if (f == FREHG_F_WIN_VENT || f == FREHG_F_WIN_VAP)
+        *n = static_cast<int64_t>(g.ny2) * g.nx2;
+      return f == FREHG_F_WIN_X      ? d.winX
+             : f == FREHG_F_WIN_Y    ? d.winY
+             : f == FREHG_F_WIN_Z    ? d.winZ
+             : f == FREHG_F_WIN_SINK ? d.winSink
+             : f == FREHG_F_QZ_PATH_CORR ? d.pathCorr
+             : f == FREHG_F_WIN_VENT ? d.winVent
+                                     : d.winVap;
     default: fail(FREHG_EINVAL, "unknown field id");
   }
 }
@@ -2937,6 +3062,12 @@ int frehg_gw_set_density(frehg_gw_solver* s, const double* r_rho, const double* 
   return guard([&] {
     frehg::S(s)->setDensity(r_rho, r_visc, r_rho_top, r_visc_top, reallocation_gradient);
   });
+}
+int frehg_gw_enable_transport_window(frehg_gw_solver* s) {
+  return guard([&] { frehg::S(s)->enableTransportWindow(); });
+}
+int frehg_gw_reset_transport_window(frehg_gw_solver* s) {
+  return guard([&] { frehg::S(s)->resetTransportWindow(); });
 }
 int frehg_gw_set_bc_values(frehg_gw_solver* s, int which, const double* values) {
   return guard([&] {

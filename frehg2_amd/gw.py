@@ -51,7 +51,17 @@ class RichardsSolver:
         except Exception:
             pass
 
+    def enable_transport_window(self) -> None:
+        """RichardsSolver::enableTransportWindow: window volumes from now on."""
+        check(self.lib, self.lib.frehg_gw_enable_transport_window(self._h))
+
+    def reset_transport_window(self) -> None:
+        """RichardsSolver::resetTransportWindow: zero the window volumes."""
+        check(self.lib, self.lib.frehg_gw_reset_transport_window(self._h))
+
     def field_shape(self, fid):
+        if fid in abi.PLANE_FIELDS:
+            return (self.ny + 2, self.nx + 2)
         nk = self.nz + 1 if fid in abi.FACE_FIELDS else self.nz
         return (self.ny + 2, self.nx + 2, nk)
 

@@ -169,6 +169,16 @@ typedef struct frehg_gw_solve_stats {
 #define FREHG_F_CAPACITY 19        /* C(h^n) of the last assembly (H) */
 #define FREHG_F_SINK 20            /* root-uptake sink S [1/s] (H) rootUptakeSink(),
                                       RichardsSolver.hpp:295,656 */
+/* The transport window (frehg_gw_enable_transport_window; RichardsSolver.hpp:
+ * 226-249): face volumes [m^3] summed over the steps since the last reset. */
+#define FREHG_F_WIN_X 21           /* through the x+ faces (H) windowVolumeX() */
+#define FREHG_F_WIN_Y 22           /* through the y+ faces (H) windowVolumeY() */
+#define FREHG_F_WIN_Z 23           /* through the z faces, incl. the post-allocation
+                                      transfers on every face crossed (F) windowVolumeZ() */
+#define FREHG_F_WIN_SINK 24        /* root-uptake volume (H) windowSinkVolume() */
+#define FREHG_F_QZ_PATH_CORR 25    /* the last step's path corrections [m^3/s] (F) qzPathCorr_ */
+#define FREHG_F_WIN_VENT 26        /* vented volume, 2D (ny+2)(nx+2) windowVentVolume() */
+#define FREHG_F_WIN_VAP 27         /* ET vapour volume, 2D windowVapourVolume() */
 
 /* Opaque handles. */
 typedef struct frehg_gw_solver frehg_gw_solver;
@@ -244,6 +254,17 @@ int frehg_gw_set_et(frehg_gw_solver* s, const int* columns, const double* qvap);
 int frehg_gw_set_density(frehg_gw_solver* s, const double* r_rho, const double* r_visc,
                          const double* r_rho_top, const double* r_visc_top,
                          int reallocation_gradient);
+/* RichardsSolver::enableTransportWindow (src/gw/RichardsSolver.cpp:796-813,
+ * transport.subsurface_update: consistent): from now on every step adds
+ * q dtg of each face into the window fields (FREHG_F_WIN_*) until
+ * frehg_gw_reset_transport_window, and the post-allocation walks record
+ * each transfer on every face it crosses (FREHG_F_QZ_PATH_CORR, added to
+ * the z window; the legacy mirror in the Darcy flux qzF is unchanged).
+ * Idempotent. */
+int frehg_gw_enable_transport_window(frehg_gw_solver* s);
+/* RichardsSolver::resetTransportWindow (src/gw/RichardsSolver.cpp:815-824):
+ * zero the window volumes (the transport calls it after consuming them). */
+int frehg_gw_reset_transport_window(frehg_gw_solver* s);
 /* Refresh per-column boundary values (updateBoundaryValues,
  * src/gw/RichardsSolver.cpp:526-566); which = 0 top, 1 bottom, 2 x-, 3 x+,
  * 4 y-, 5 y+; values is ny*nx. */

@@ -225,6 +225,14 @@ struct Oracle {
   void setDensity(const double* rr, const double* rv, const double* rrTop, const double* rvTop,
                   bool gradient);
   F3 kx, ky, kzF, qx, qy, qzF, vloss, room, sendUp, sendDown, wcSnap, hSnap;
+  // the transport window (RichardsSolver.cpp:796-878): face volumes summed
+  // over the steps since the last reset, and the walks' path corrections
+  bool transportWindow = false;
+  F3 winX, winY, winZ, winSink, pathCorr;
+  F2<double> winVent, winVap;
+  void enableTransportWindow();
+  void resetTransportWindow();
+  void accumulateTransportWindow(double dtg);
   // boundary markers
   F2<int> topCode, botCode, sxm, sxp, sym, syp, cplMode;
   F2<double> topValue, botValue, vxm, vxp, vym, vyp;
@@ -1714,6 +1722,20 @@ void Oracle::reallocate() {
       for (int k = 0; k < nz; ++k) {
         if (g(j, i, k) < 0) continue;
         const bool topIsHead = topCode(j, i) == FREHG_BC_HEAD;
+        // Path recording (transport window only, Reallocate.cpp:425-445):
+        // the legacy mirror of a transfer from cell `from` to cell `to`
+        // (the cell above the column top for a vent), recorded at face
+        // `legacyFace`, is replaced by the flux on every face crossed.
+        const auto record = [&](int from, int to, double vol, int legacyFace, double legacySign) {
+          if (!transportWindow) return;
+          const double rate = vol / dtg;
+          pathCorr(j, i, legacyFace) -= legacySign * rate;
+          if (to < from) {
+            for (int f = to + 1; f <= from; ++f) pathCorr(j, i, f) += rate;
+          } else {
+            for (int f = from + 1; f <= to; ++f) pathCorr(j, i, f) -= rate;
+          }
+        };
         double dV = sendUp(j, i, k);
         if (dV > 0.0) {
           int ll = k;
@@ -1724,6 +1746,7 @@ void Oracle::reallocate() {
               wc(j, i, ll) += take / (az * dz3d(j, i, ll));
               room(j, i, ll) -= take;
               qzF(j, i, ll + 1) += take / dtg;
+              record(k, ll, take, ll + 1, 1.0);
               dV -= take;
             }
           }
@@ -1737,6 +1760,8 @@ void Oracle::reallocate() {
                 avail[c2] += d;
                 gain[c2] += d;
                 qzF(j, i, ll) += dV / dtg;
+                record(k, ll - 1, dV, ll, 1.0);
+                if (transportWindow) winVent(j, i) += dV;
                 vent += dV;
               } else {
                 lost -= dV;
@@ -1744,6 +1769,7 @@ void Oracle::reallocate() {
               dV = 0.0;
             } else if (topIsHead) {
               qzF(j, i, ll) += dV / dtg;
+              record(k, ll - 1, dV, ll, 1.0);
               dV = 0.0;
             } else {
               while (ll + 1 < nz && dV > 0.0) {
@@ -1753,6 +1779,7 @@ void Oracle::reallocate() {
                   wc(j, i, ll) += take / (az * dz3d(j, i, ll));
                   room(j, i, ll) -= take;
                   qzF(j, i, ll) -= take / dtg;
+                  record(k, ll, take, ll, -1.0);
                   dV -= take;
                 }
               }
@@ -1771,6 +1798,7 @@ void Oracle::reallocate() {
               wc(j, i, ll) += take / (az * dz3d(j, i, ll));
               room(j, i, ll) -= take;
               qzF(j, i, ll) -= take / dtg;
+              record(k, ll, take, ll, -1.0);
               dV -= take;
             }
           }
@@ -1792,6 +1820,47 @@ void Oracle::reallocate() {
   audit.realloc_dropped = discarded - dropped;
   audit.cpl_vent = vented;
   audit.cpl_vent_thin = thin;
+}
+
+// --- RichardsSolver.cpp:796-878: the transport window --------------------------
+void Oracle::enableTransportWindow() {
+  if (transportWindow) return;
+  transportWindow = true;
+  winX = qx;
+  winY = qy;
+  winSink = qx;
+  winZ = qzF;
+  pathCorr = qzF;
+  winVent = F2<double>();
+  winVent.init(ny + 2, nx + 2, 0.0);
+  winVap = winVent;
+  resetTransportWindow();
+  std::fill(pathCorr.v.begin(), pathCorr.v.end(), 0.0);
+}
+
+void Oracle::resetTransportWindow() {
+  if (!transportWindow) return;
+  for (F3* f : {&winX, &winY, &winZ, &winSink}) std::fill(f->v.begin(), f->v.end(), 0.0);
+  std::fill(winVent.v.begin(), winVent.v.end(), 0.0);
+  std::fill(winVap.v.begin(), winVap.v.end(), 0.0);
+}
+
+// Every face slot, halos included (RichardsSolver.cpp:826-878).
+void Oracle::accumulateTransportWindow(double dtg) {
+  for (size_t t = 0; t < winX.v.size(); ++t) {
+    winX.v[t] += qx.v[t] * dtg;
+    winY.v[t] += qy.v[t] * dtg;
+  }
+  for (size_t t = 0; t < winZ.v.size(); ++t) winZ.v[t] += (qzF.v[t] + pathCorr.v[t]) * dtg;
+  if (etActive)
+    for (int j = 0; j < ny + 2; ++j)
+      for (int i = 0; i < nx + 2; ++i)
+        winVap(j, i) += etQ[static_cast<size_t>(j) * (nx + 2) + i] * az * dtg;
+  if (sinkActive)
+    for (int j = 0; j < ny + 2; ++j)
+      for (int i = 0; i < nx + 2; ++i)
+        for (int k = 0; k < nz; ++k)
+          winSink(j, i, k) += sink(j, i, k) * az * dz3d(j, i, k) * dtg;
 }
 
 // --- Corrector.cpp:758-792 ---------------------------------------------------
@@ -1894,10 +1963,12 @@ void Oracle::step(double /*t*/, double dtg) {
   fluxes(dtg);
   if (coupled) coupledBookkeeping(dtg);
   updateWaterContent(dtg);
+  if (transportWindow) std::fill(pathCorr.v.begin(), pathCorr.v.end(), 0.0);
   reallocate();
   finalize();
   enforceMoistureBc();
   boundaryFlux(dtg);
+  if (transportWindow) accumulateTransportWindow(dtg);
   adaptTimeStep(dtg);
   tStep += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -1953,8 +2024,32 @@ const F3* field_of(Oracle& o, int f) {
     case FREHG_F_SINK:
       if (!o.sinkActive) fail(FREHG_EINVAL, "no root-uptake sink is set");
       return &o.sink;
+    case FREHG_F_WIN_X:
+    case FREHG_F_WIN_Y:
+    case FREHG_F_WIN_Z:
+    case FREHG_F_WIN_SINK:
+    case FREHG_F_QZ_PATH_CORR:
+      if (!o.transportWindow) fail(FREHG_EINVAL, "the transport window is not enabled");
+      return f == FREHG_F_WIN_X   ? &o.winX
+             : f == FREHG_F_WIN_Y ? &o.winY
+             : f == FREHG_F_WIN_Z ? &o.winZ
+             : f == FREHG_F_WIN_SINK ? &o.winSink
+                                     : &o.pathCorr;
     default: fail(FREHG_EINVAL, "unknown field id");
   }
+}
+
+// Data and element count of any field, the 2D window planes included.
+const double* field_data(Oracle& o, int f, size_t* n) {
+  if (f == FREHG_F_WIN_VENT || f == FREHG_F_WIN_VAP) {
+    if (!o.transportWindow) fail(FREHG_EINVAL, "the transport window is not enabled");
+    const F2<double>& p = f == FREHG_F_WIN_VENT ? o.winVent : o.winVap;
+    *n = p.v.size();
+    return p.v.data();
+  }
+  const F3* fld = field_of(o, f);
+  *n = fld->v.size();
+  return fld->v.data();
 }
 
 }  // namespace
@@ -2030,13 +2125,24 @@ int oracle_gw_active_cells(void* p, int64_t* n) {
   return FREHG_OK;
 }
 int oracle_gw_field_size(void* p, int f, int64_t* n) {
-  return guard([&] { *n = static_cast<int64_t>(field_of(*static_cast<Oracle*>(p), f)->v.size()); });
+  return guard([&] {
+    size_t m = 0;
+    (void)field_data(*static_cast<Oracle*>(p), f, &m);
+    *n = static_cast<int64_t>(m);
+  });
 }
 int oracle_gw_get_field(void* p, int f, double* outp) {
   return guard([&] {
-    const F3* fld = field_of(*static_cast<Oracle*>(p), f);
-    std::memcpy(outp, fld->v.data(), fld->v.size() * sizeof(double));
+    size_t m = 0;
+    const double* d = field_data(*static_cast<Oracle*>(p), f, &m);
+    std::memcpy(outp, d, m * sizeof(double));
   });
+}
+int oracle_gw_enable_transport_window(void* p) {
+  return guard([&] { static_cast<Oracle*>(p)->enableTransportWindow(); });
+}
+int oracle_gw_reset_transport_window(void* p) {
+  return guard([&] { static_cast<Oracle*>(p)->resetTransportWindow(); });
 }
 int oracle_gw_set_field(void* p, int f, const double* in) {
   return guard([&] {
@@ -2169,6 +2275,7 @@ int oracle_gw_test_post_allocate(void* p, double dtg) {
     if (!(dtg > 0.0)) fail(FREHG_EINVAL, "dtg must be positive");
     o.audit = frehg_gw_audit{};
     o.dtgCurrent = dtg;
+    if (o.transportWindow) std::fill(o.pathCorr.v.begin(), o.pathCorr.v.end(), 0.0);
     o.reallocate();
     o.finalize();
     o.enforceMoistureBc();

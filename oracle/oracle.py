@@ -60,6 +60,8 @@ _SIG = {
     "oracle_ls_solve": ([_vp, _dp, _dp, C.POINTER(abi.SolveStats)], C.c_int),
     "oracle_ls_is_symmetric": ([_vp, C.c_double, C.POINTER(C.c_int)], C.c_int),
     "oracle_gw_test_post_allocate": ([_vp, C.c_double], C.c_int),
+    "oracle_gw_enable_transport_window": ([_vp], C.c_int),
+    "oracle_gw_reset_transport_window": ([_vp], C.c_int),
 }
 
 _LIB = None
@@ -122,7 +124,15 @@ class OracleRichards:
         except Exception:
             pass
 
+    def enable_transport_window(self) -> None:
+        _check(lib().oracle_gw_enable_transport_window(self._h))
+
+    def reset_transport_window(self) -> None:
+        _check(lib().oracle_gw_reset_transport_window(self._h))
+
     def field_shape(self, fid):
+        if fid in abi.PLANE_FIELDS:
+            return (self.ny + 2, self.nx + 2)
         nk = self.nz + 1 if fid in abi.FACE_FIELDS else self.nz
         return (self.ny + 2, self.nx + 2, nk)
 
