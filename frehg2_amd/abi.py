@@ -25,6 +25,7 @@ BC_NOFLUX, BC_HEAD, BC_FLUX, BC_GRAVITY, BC_HEAD_HYDROSTATIC = 0, 1, 2, 3, 4
 IC_MOISTURE, IC_HEAD, IC_WATER_TABLE = 0, 1, 2
 PC_JACOBI, PC_ZLINE, PC_ICC0 = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+STEP_HOST, STEP_DEVICE, STEP_GRAPH = 0, 1, 2
 
 F_HEAD, F_WC, F_HEAD_N, F_WC_N = 0, 1, 2, 3
 F_KX, F_KY, F_KZF = 4, 5, 6
@@ -90,6 +91,12 @@ class GwTiming(C.Structure):
                 ("steps", C.c_int64), ("pcg_iterations", C.c_int64)]
 
 
+class GraphStats(C.Structure):
+    _fields_ = [("mode", C.c_int), ("budget_rounds", C.c_int), ("budget_iterations", C.c_int),
+                ("captures", C.c_int64), ("replays", C.c_int64), ("device_steps", C.c_int64),
+                ("host_finishes", C.c_int64)]
+
+
 # Exported symbols of the product library, with argtypes (the C-ABI surface).
 _vp = C.c_void_p
 SYMBOLS = {
@@ -118,6 +125,8 @@ SYMBOLS = {
     "frehg_gw_synchronize": ([_vp], C.c_int),
     "frehg_gw_set_pcg_shapes": ([_vp, C.c_int, C.c_int, C.c_int], C.c_int),
     "frehg_gw_get_pcg_shapes": ([_vp, _ip, _ip, _ip], C.c_int),
+    "frehg_gw_set_step_mode": ([_vp, C.c_int, C.c_int, C.c_int], C.c_int),
+    "frehg_gw_get_graph_stats": ([_vp, C.POINTER(GraphStats)], C.c_int),
     "frehg_device_malloc": ([C.c_int64, C.POINTER(_vp)], C.c_int),
     "frehg_device_free": ([_vp], C.c_int),
     "frehg_memcpy_htod": ([_vp, _vp, C.c_int64], C.c_int),

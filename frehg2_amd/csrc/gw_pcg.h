@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "gw_device.h"
 
 namespace frehg {
@@ -49,6 +51,21 @@ struct PcgOptions {
   bool stencil32_ready = false;  // mixed: the caller wrote the FP32 stencil (stencil32())
 };
 
+// Launches of a device-controlled solve (Pcg::enqueueDevice): `rounds` FP64
+// rounds (restarted recursions, each confirmed on the true residual) or mixed
+// refinement cycles, of `iterations` direction/update launch pairs each.
+struct PcgBudget {
+  int rounds = 1;
+  int iterations = 32;
+};
+
+// What a device-controlled solve left for the budget's next sizing.
+struct DeviceSummary {
+  int max_inner = 0;  // most iterations one round / cycle took
+  int stopped = 0;    // the solve ended inside the budget (converged or not)
+  int fallback = 0;   // mixed: a cycle failed to halve the residual
+};
+
 // Caller-owned memory for FP64 solves (Pcg's external mode, which allocates
 // nothing): device doubles for the work vectors, partial sums and state, and
 // host buffers for the reduced values and the polled state.
@@ -83,6 +100,22 @@ class Pcg {
   int solve(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
             PcgStats* stats);
 
+  // Device-controlled solve: the same launches and decisions as solve(), as
+  // a fixed sequence with no host synchronization (capturable in a graph).
+  // It ends by copying the control state to its host mirror; after the
+  // stream has reached that point, deviceResult() reads it.  A solve that
+  // did not converge inside the budget leaves `gate` (gateFlag(), device
+  // memory) at 1 and resumeDevice() finishes it the host-polled way from
+  // where it stopped.  prepareDevice() allocates what the form needs
+  // (before any capture).
+  void prepareDevice(const PcgOptions& opt);
+  void enqueueDevice(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                     const PcgBudget& budget);
+  const int* gateFlag() const;
+  int deviceResult(PcgStats* stats, DeviceSummary* summary) const;  // returns the gate
+  int resumeDevice(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                   PcgStats* stats);
+
   // The FP32 stencil of the mixed form (allocated on the first call).  A
   // caller that writes it as the rounded FP64 stencil sets
   // PcgOptions::stencil32_ready, and the solve skips its own cast.
@@ -108,6 +141,39 @@ class Pcg {
                 PcgStats* st);
   int solveMixed(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
                  PcgStats* st);
+  // mixed-loop state carried between cycles (and handed over by resumeDevice)
+  struct MixedCarry {
+    double* xa = nullptr;  // current iterate
+    double* xb = nullptr;  // the other x buffer
+    double rr = 0.0, tol2 = 0.0, s = 1.0;
+    int total = 0, n0 = 0;
+    bool restart = true;
+    float* pOld = nullptr;
+    float* pNew = nullptr;
+  };
+  int fp64Loop(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+               PcgStats* st, double rr, double tol2, int total, int stalls);
+  void mixedSetup(const Stencil<double>& A, const PcgOptions& opt);
+  int mixedLoop(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                PcgStats* st, MixedCarry c);
+  int fp64Finish(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                 PcgStats* st, const double* best, int total);
+  std::pair<double, double> residualHost(const Stencil<double>& A, const double* b,
+                                         const double* x, bool zline);
+  std::pair<double, double> refineHost(const Stencil<double>& A, const double* b, double* xi,
+                                       double* xo, double sc, bool hasD);
+  template <class T>
+  void launchDir(const Stencil<T>& A, const T* z, const T* pOld, T* pNew, T* q, int np, int n,
+                 bool restart, const March& mj);
+  template <class T>
+  void launchUpdate(T* x, const T* pPrev, const T* p, T* r, const T* q, T* z, const T* invd,
+                    const T* w, bool zline, bool two, int n, const March& mj);
+  void launchFactor(const Stencil<double>& A, bool zline, bool fp32);
+  void launchResidual(const Stencil<double>& A, const double* b, const double* x, bool zline,
+                      const void* ctl);
+  void launchRefine(const Stencil<double>& A, const double* b, double* xa, double* xb, double sc,
+                    bool hasD, const void* ctl);
+  void launchInnerInit(bool zline, double s, const void* ctl);
   template <class T>
   int innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T* z, T* q,
                       const T* invd, const T* w, bool zline, double tol2, int maxIt,

@@ -50,6 +50,25 @@ struct PcgState {
   double alphaH[2];  // alpha of iteration n in slot n & 1 (deferred x updates)
   int it;
   int done;
+  // read by the direction kernels: the recursion stops once rr <= tol2, or
+  // at iteration `limit` (the iteration budget of a run of launches)
+  double tol2;
+  int limit;
+  // device-side control (Pcg::enqueueDevice): the solve's scalars and
+  // decisions live here, so no host round trip sits inside the solve
+  int skip;         // the current round / cycle is over before it starts (no-ops)
+  int stop;         // the solve is over: converged, diverged or out of iterations
+  int converged;
+  int fallback;     // mixed: a cycle failed to halve the FP64 residual
+  int keepNew;      // ... and its output iterate was still the better one
+  int xInB;         // mixed: the current iterate is in the second x buffer
+  int total;        // iterations so far
+  int refinements;
+  int stalls;       // FP64 rounds that made no iteration
+  int maxInner;     // most iterations of one round / cycle
+  int gate;         // 1: the solve did not converge inside the budget
+  double tolF2;     // final tolerance, squared
+  double rrTrue, bbTrue, rrPrev, s;
 };
 
 constexpr int kBlock = 256;
@@ -294,7 +313,9 @@ __global__ __launch_bounds__(kBlock) void k_residual64(Grid g, March mj, Stencil
                                                        double* __restrict__ z,
                                                        const double* __restrict__ invd,
                                                        const double* __restrict__ w,
-                                                       double* __restrict__ parts, int pstride) {
+                                                       double* __restrict__ parts, int pstride,
+                                                       const PcgState* __restrict__ ctl) {
+  if (ctl && ctl->skip) return;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t sx = g.nz;
   KSum rz, rr, bb;
@@ -356,16 +377,23 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Grid g, March mj, Stencil<T>
                                                     const T* __restrict__ pOld,
                                                     T* __restrict__ pNew, T* __restrict__ q,
                                                     double* __restrict__ parts, int pstride,
-                                                    int nprev, PcgState* __restrict__ st, int n,
-                                                    int restart, double tol2) {
+                                                    int nprev, PcgState* __restrict__ st, int n) {
+  // Launches queued behind the one that ended the recursion (convergence or
+  // the budget) are no-ops and leave the count alone.
+  if (st->done) return;
+  if (n >= st->limit) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->it = n;
+    }
+    return;
+  }
   double red[2];
   const int rs[2] = {0, 1};
   reduce_parts<2>(parts, pstride, nprev, rs, red);
   const double rzv = red[0], rrv = red[1];
-  if (rrv <= tol2) {
-    // The first launch that sees convergence records it; the no-op launches
-    // queued behind it in the same polling chunk leave the count alone.
-    if (blockIdx.x == 0 && threadIdx.x == 0 && !st->done) {
+  if (rrv <= st->tol2) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->done = 1;
       st->it = n;
       st->rr = rrv;
@@ -506,14 +534,27 @@ __global__ __launch_bounds__(kBlock) void k_flush_x(Grid g, T* __restrict__ x,
 
 // Mixed form, outer sweep: xo = xi + sc d (all 7 stencil points), r = b - A xo
 // in FP64; partials rr (slot 1), bb (slot 3).  Without HASD, xo is xi.  j-march.
+// Under device control (ctl set) the sweep is skipped once the solve is over,
+// and xi / xo swap when the current iterate is in the second buffer; sc is
+// then 1 / s of the control state.
 template <int R, bool HASD>
 __global__ __launch_bounds__(kBlock) void k_refine64(Grid g, March mj, Stencil<double> A,
                                                      const double* __restrict__ b,
-                                                     const double* __restrict__ xi,
-                                                     double* __restrict__ xo,
+                                                     double* xa, double* xb,
                                                      const float* __restrict__ d, double sc,
                                                      double* __restrict__ r,
-                                                     double* __restrict__ parts, int pstride) {
+                                                     double* __restrict__ parts, int pstride,
+                                                     const PcgState* __restrict__ ctl) {
+  const double* __restrict__ xi = xa;
+  double* __restrict__ xo = xb;
+  if (ctl) {
+    if (ctl->skip) return;
+    if (ctl->xInB) {
+      xi = xb;
+      xo = xa;
+    }
+    sc = 1.0 / ctl->s;
+  }
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
   KSum rr, bb;
@@ -573,7 +614,12 @@ __global__ __launch_bounds__(kBlock) void k_inner_init(Grid g, const double* __r
                                                        float* __restrict__ d32,
                                                        const float* __restrict__ invd,
                                                        const float* __restrict__ w,
-                                                       double* __restrict__ parts, int pstride) {
+                                                       double* __restrict__ parts, int pstride,
+                                                       const PcgState* __restrict__ ctl) {
+  if (ctl) {
+    if (ctl->skip) return;
+    s = ctl->s;
+  }
   const int lane = threadIdx.x & (kWave - 1);
   KSum rz, rr;
   int64_t col, end, stride;
@@ -706,14 +752,22 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p(Grid g, March mj, Stencil<
                                                       float* __restrict__ pNew,
                                                       float* __restrict__ q,
                                                       double* __restrict__ parts, int pstride,
-                                                      int nprev, PcgState* __restrict__ st, int n,
-                                                      double tol2) {
+                                                      int nprev, PcgState* __restrict__ st,
+                                                      int n) {
+  if (st->done) return;
+  if (n >= st->limit) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->it = n;
+    }
+    return;
+  }
   double red[2];
   const int rs[2] = {0, 1};
   reduce_parts<2>(parts, pstride, nprev, rs, red);
   const double rzv = red[0], rrv = red[1];
-  if (rrv <= tol2) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && !st->done) {
+  if (rrv <= st->tol2) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->done = 1;
       st->it = n;
       st->rr = rrv;
@@ -881,7 +935,12 @@ __global__ __launch_bounds__(kBlock) void k_inner_init_p(Grid g, const double* _
                                                          const float* __restrict__ invd,
                                                          const float* __restrict__ w,
                                                          double* __restrict__ parts,
-                                                         int pstride) {
+                                                         int pstride,
+                                                         const PcgState* __restrict__ ctl) {
+  if (ctl) {
+    if (ctl->skip) return;
+    s = ctl->s;
+  }
   const int lane = threadIdx.x & (kWave - 1);
   const int nz = g.nz;
   KSum rz, rr;
@@ -945,9 +1004,192 @@ __global__ __launch_bounds__(kBlock) void k_reduce_parts(const double* __restric
   }
 }
 
-__global__ void k_state_reset(PcgState* st) {
+__global__ void k_state_reset(PcgState* st, double tol2, int limit) {
   st->done = 0;
   st->it = 0;
+  st->tol2 = tol2;
+  st->limit = limit;
+}
+
+// ---------------------------------------------------------------------------
+// Device-side control.  The host-polled solve decides between launches on
+// the host; these one-block kernels take the same decisions, in the same
+// order and with the same arithmetic, on the device, so a solve is a fixed
+// sequence of launches (capturable in a graph): a round (FP64) or cycle
+// (mixed) of B direction/update pairs, whose launches past the end of the
+// recursion return at once, then the control kernel that reads the result.
+// Iterations inside a round are numbered from 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int round_its(const PcgState* st, int budget) {
+  return st->done ? st->it : budget;
+}
+
+// FP64 start, after the first residual: tolerance, first decisions.
+__global__ __launch_bounds__(kBlock) void k_dc_begin64(const double* __restrict__ parts,
+                                                       int pstride, int nparts, PcgState* st,
+                                                       double rtol, double atol, int maxIt) {
+  double v[2];
+  const int slots[2] = {1, 3};
+  reduce_parts<2>(parts, pstride, nparts, slots, v);
+  if (threadIdx.x != 0) return;
+  const double rr = v[0], bb = v[1];
+  const double tol = fmax(rtol * sqrt(bb), atol);
+  st->tolF2 = tol * tol;
+  st->tol2 = st->tolF2;
+  st->rrTrue = rr;
+  st->bbTrue = bb;
+  st->total = 0;
+  st->refinements = 0;
+  st->stalls = 0;
+  st->maxInner = 0;
+  st->fallback = 0;
+  st->keepNew = 0;
+  st->xInB = 0;
+  st->converged = rr <= st->tolF2 ? 1 : 0;
+  st->stop = (st->converged || !isfinite(rr) || maxIt <= 0) ? 1 : 0;
+  st->gate = 1;
+}
+
+// Before an FP64 round: its launches are no-ops if the solve is over.
+__global__ void k_dc_round64(PcgState* st, int maxIt) {
+  st->skip = st->stop;
+  st->done = st->stop;
+  st->it = 0;
+  st->limit = maxIt - st->total;
+}
+
+// After a round's launches: the x term an odd count left deferred (the p of
+// launch its - 1: even launches write pEven, odd ones pOdd).  With COPY the
+// last p is also left in pOdd, which the next cycle's first launch reads as
+// p_old (mixed cycles continue the recursion).
+template <class T, bool COPY>
+__global__ __launch_bounds__(kBlock) void k_dc_flush(Grid g, T* __restrict__ x, T* pEven,
+                                                     T* pOdd, const PcgState* __restrict__ st,
+                                                     int budget) {
+  if (st->skip) return;
+  const int its = round_its(st, budget);
+  if (!(its & 1)) return;
+  const T alpha = static_cast<T>(st->alphaH[(its - 1) & 1]);
+  const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t c = column_base(g, t / g.nz) + t % g.nz;
+    const T p = pEven[c];
+    x[c] = x[c] + alpha * p;
+    if (COPY) pOdd[c] = p;
+  }
+}
+
+// After an FP64 round's true residual: the host loop's decisions.
+__global__ __launch_bounds__(kBlock) void k_dc_end64(const double* __restrict__ parts,
+                                                     int pstride, int nparts, PcgState* st,
+                                                     int budget, int maxIt) {
+  double v[2];
+  const int slots[2] = {1, 3};
+  reduce_parts<2>(parts, pstride, nparts, slots, v);
+  if (threadIdx.x != 0 || st->skip) return;
+  const int its = round_its(st, budget);
+  st->total += its;
+  st->maxInner = max(st->maxInner, its);
+  const double rr = v[0];
+  st->rrTrue = rr;
+  if (rr <= st->tolF2) {
+    st->converged = 1;
+    st->stop = 1;
+  } else if (!isfinite(rr) || st->total >= maxIt) {
+    st->stop = 1;
+  } else if (its == 0 && ++st->stalls > 2) {
+    st->stop = 1;
+  }
+}
+
+// Mixed start, after the first FP64 residual: tolerance, scale, decisions.
+__global__ __launch_bounds__(kBlock) void k_dc_begin_mixed(const double* __restrict__ parts,
+                                                           int pstride, int nparts, PcgState* st,
+                                                           double rtol, double atol, int maxIt,
+                                                           int maxRef, double ncell) {
+  double v[2];
+  const int slots[2] = {1, 3};
+  reduce_parts<2>(parts, pstride, nparts, slots, v);
+  if (threadIdx.x != 0) return;
+  const double rr = v[0], bb = v[1];
+  const double tol = fmax(rtol * sqrt(bb), atol);
+  st->tolF2 = tol * tol;
+  st->rrTrue = rr;
+  st->bbTrue = bb;
+  st->s = rr > 0.0 ? 1.0 / sqrt(rr / ncell) : 1.0;
+  st->total = 0;
+  st->refinements = 0;
+  st->stalls = 0;
+  st->maxInner = 0;
+  st->fallback = 0;
+  st->keepNew = 0;
+  st->xInB = 0;
+  st->converged = rr <= st->tolF2 ? 1 : 0;
+  st->stop = (st->converged || !isfinite(rr) || maxRef <= 0 || maxIt <= 0) ? 1 : 0;
+  st->gate = 1;
+}
+
+// Before a mixed cycle: the FP32 recursion's tolerance and iteration cap.
+__global__ void k_dc_cycle(PcgState* st, int maxIt, double innerRtol) {
+  st->skip = st->stop;
+  st->done = st->stop;
+  st->it = 0;
+  if (st->stop) return;
+  const double s = st->s;
+  const double rrS = st->rrTrue * s * s;
+  st->tol2 = fmax(innerRtol * innerRtol * rrS, 0.25 * st->tolF2 * s * s);
+  st->limit = min(maxIt - st->total, 400);
+  st->rrPrev = st->rrTrue;
+}
+
+// After a cycle's launches: count, and leave the last rz where the next
+// cycle's first launch (iteration 0 of its numbering) reads the previous one.
+__global__ void k_dc_mid(PcgState* st, int budget) {
+  if (st->skip) return;
+  const int its = round_its(st, budget);
+  st->total += its;
+  st->maxInner = max(st->maxInner, its);
+  if (its > 0) st->rz[1] = st->rz[(its - 1) & 1];
+}
+
+// After a cycle's FP64 residual: the host loop's decisions.
+__global__ __launch_bounds__(kBlock) void k_dc_end_mixed(const double* __restrict__ parts,
+                                                         int pstride, int nparts, PcgState* st,
+                                                         int maxIt, int maxRef) {
+  double v[2];
+  const int slots[2] = {1, 3};
+  reduce_parts<2>(parts, pstride, nparts, slots, v);
+  if (threadIdx.x != 0 || st->skip) return;
+  const double rr = v[0];
+  st->refinements += 1;
+  if (!isfinite(rr) || rr > 0.25 * st->rrPrev) {
+    st->fallback = 1;
+    st->keepNew = (isfinite(rr) && rr < st->rrPrev) ? 1 : 0;
+    st->stop = 1;
+    return;
+  }
+  st->xInB ^= 1;
+  st->rrTrue = rr;
+  if (rr <= st->tolF2) {
+    st->converged = 1;
+    st->stop = 1;
+  } else if (st->refinements >= maxRef || st->total >= maxIt) {
+    st->stop = 1;
+  }
+}
+
+__global__ void k_dc_finish(PcgState* st) { st->gate = st->converged ? 0 : 1; }
+
+// Converged mixed solve whose iterate is in the second buffer: x = xb.
+__global__ __launch_bounds__(kBlock) void k_dc_take_x(Grid g, double* __restrict__ x,
+                                                      const double* __restrict__ xb,
+                                                      const PcgState* __restrict__ st) {
+  if (!(st->converged && st->xInB)) return;
+  const int64_t n = g.cellsH();
+  for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    x[c] = xb[c];
 }
 
 template <class F>
@@ -1156,6 +1398,153 @@ int Pcg::solve(const Stencil<double>& A, const double* b, double* x, const PcgOp
   return opt.mixed ? solveMixed(A, b, x, opt, st) : solveFp64(A, b, x, opt, st);
 }
 
+// ---------------------------------------------------------------------------
+// Launch helpers shared by the host-polled and the device-controlled solves.
+// ---------------------------------------------------------------------------
+template <class T>
+void Pcg::launchDir(const Stencil<T>& A, const T* z, const T* pOld, T* pNew, T* q, int np, int n,
+                    bool restart, const March& mj) {
+  PcgState* dst = static_cast<PcgState*>(state_);
+  if constexpr (std::is_same<T, float>::value) {
+    if (pairs_) {
+      dispatch_R2(g_.nz, [&](auto RC) {
+        constexpr int R2 = decltype(RC)::value;
+        if (restart)
+          k_pcg_dir_p<R2, true><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, z, pOld, pNew, q,
+                                                                   parts_, pstride_, np, dst, n);
+        else
+          k_pcg_dir_p<R2, false><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, z, pOld, pNew, q,
+                                                                    parts_, pstride_, np, dst, n);
+      });
+      HIP_OK(hipGetLastError());
+      return;
+    }
+  }
+  dispatch_R(g_.nz, [&](auto RC) {
+    constexpr int R = decltype(RC)::value;
+    if (restart)
+      k_pcg_dir<T, R, true><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, z, pOld, pNew, q,
+                                                               parts_, pstride_, np, dst, n);
+    else
+      k_pcg_dir<T, R, false><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, z, pOld, pNew, q,
+                                                                parts_, pstride_, np, dst, n);
+  });
+  HIP_OK(hipGetLastError());
+}
+
+template <class T>
+void Pcg::launchUpdate(T* x, const T* pPrev, const T* p, T* r, const T* q, T* z, const T* invd,
+                       const T* w, bool zline, bool two, int n, const March& mj) {
+  PcgState* dst = static_cast<PcgState*>(state_);
+  if constexpr (std::is_same<T, float>::value) {
+    if (pairs_) {
+      dispatch_R2(g_.nz, [&](auto RC) {
+        constexpr int R2 = decltype(RC)::value;
+        if (zline && two)
+          k_pcg_update_p<R2, true, 2><<<blocks_, kBlock, 0, stream_>>>(
+              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+        else if (zline)
+          k_pcg_update_p<R2, true, 0><<<blocks_, kBlock, 0, stream_>>>(
+              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+        else if (two)
+          k_pcg_update_p<R2, false, 2><<<blocks_, kBlock, 0, stream_>>>(
+              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+        else
+          k_pcg_update_p<R2, false, 0><<<blocks_, kBlock, 0, stream_>>>(
+              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+      });
+      HIP_OK(hipGetLastError());
+      return;
+    }
+  }
+  dispatch_R(g_.nz, [&](auto RC) {
+    constexpr int R = decltype(RC)::value;
+    if (zline && two)
+      k_pcg_update<T, R, true, 2><<<blocks_, kBlock, 0, stream_>>>(
+          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+    else if (zline)
+      k_pcg_update<T, R, true, 0><<<blocks_, kBlock, 0, stream_>>>(
+          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+    else if (two)
+      k_pcg_update<T, R, false, 2><<<blocks_, kBlock, 0, stream_>>>(
+          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+    else
+      k_pcg_update<T, R, false, 0><<<blocks_, kBlock, 0, stream_>>>(
+          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+  });
+  HIP_OK(hipGetLastError());
+}
+
+void Pcg::launchFactor(const Stencil<double>& A, bool zline, bool fp32) {
+  const int nFactorBlocks = (g_.columns() + kWave - 1) / kWave;
+  if (fp32 && zline)
+    k_factor<float, true><<<nFactorBlocks, kWave, 0, stream_>>>(g_, Ad32_, Azp32_, invd32_, w32_);
+  else if (fp32)
+    k_factor<float, false><<<nFactorBlocks, kWave, 0, stream_>>>(g_, Ad32_, Azp32_, invd32_, w32_);
+  else if (zline)
+    k_factor<double, true><<<nFactorBlocks, kWave, 0, stream_>>>(g_, A.Ad, A.Azp, invd_, w_);
+  else
+    k_factor<double, false><<<nFactorBlocks, kWave, 0, stream_>>>(g_, A.Ad, A.Azp, invd_, w_);
+  HIP_OK(hipGetLastError());
+}
+
+void Pcg::launchResidual(const Stencil<double>& A, const double* b, const double* x, bool zline,
+                         const void* ctl) {
+  const PcgState* c = static_cast<const PcgState*>(ctl);
+  dispatch_R(g_.nz, [&](auto RC) {
+    constexpr int R = decltype(RC)::value;
+    const March& mj = march64_;
+    if (zline)
+      k_residual64<R, true><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, b, x, r_, z_, invd_, w_,
+                                                               parts_, pstride_, c);
+    else
+      k_residual64<R, false><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, b, x, r_, z_, invd_,
+                                                                w_, parts_, pstride_, c);
+  });
+  HIP_OK(hipGetLastError());
+}
+
+void Pcg::launchRefine(const Stencil<double>& A, const double* b, double* xa, double* xb,
+                       double sc, bool hasD, const void* ctl) {
+  const PcgState* c = static_cast<const PcgState*>(ctl);
+  dispatch_R(g_.nz, [&](auto RC) {
+    constexpr int R = decltype(RC)::value;
+    if (hasD)
+      k_refine64<R, true><<<march64_.blocks, kBlock, 0, stream_>>>(
+          g_, march64_, A, b, xa, xb, d32_, sc, r_, parts_, pstride_, c);
+    else
+      k_refine64<R, false><<<march64_.blocks, kBlock, 0, stream_>>>(
+          g_, march64_, A, b, xa, xb, d32_, sc, r_, parts_, pstride_, c);
+  });
+  HIP_OK(hipGetLastError());
+}
+
+void Pcg::launchInnerInit(bool zline, double s, const void* ctl) {
+  const PcgState* c = static_cast<const PcgState*>(ctl);
+  if (pairs_) {
+    dispatch_R2(g_.nz, [&](auto RC) {
+      constexpr int R2 = decltype(RC)::value;
+      if (zline)
+        k_inner_init_p<R2, true><<<blocks_, kBlock, 0, stream_>>>(
+            g_, r_, s, r32_, z32_, d32_, invd32_, w32_, parts_, pstride_, c);
+      else
+        k_inner_init_p<R2, false><<<blocks_, kBlock, 0, stream_>>>(
+            g_, r_, s, r32_, z32_, d32_, invd32_, w32_, parts_, pstride_, c);
+    });
+  } else {
+    dispatch_R(g_.nz, [&](auto RC) {
+      constexpr int R = decltype(RC)::value;
+      if (zline)
+        k_inner_init<R, true><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
+                                                               invd32_, w32_, parts_, pstride_, c);
+      else
+        k_inner_init<R, false><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
+                                                                invd32_, w32_, parts_, pstride_, c);
+    });
+  }
+  HIP_OK(hipGetLastError());
+}
+
 template <class T>
 int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T* z, T* q,
                          const T* invd, const T* w, bool zline, double tol2, int maxIt,
@@ -1165,10 +1554,10 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
   // previous rz in the state slot the numbering expects.
   PcgState* dst = static_cast<PcgState*>(state_);
   PcgState* hst = static_cast<PcgState*>(stateHost_);
-  k_state_reset<<<1, 1, 0, stream_>>>(dst);
-  HIP_OK(hipGetLastError());
   int n = n0;
   maxIt += n0;
+  k_state_reset<<<1, 1, 0, stream_>>>(dst, tol2, maxIt);
+  HIP_OK(hipGetLastError());
   T* const bufOld = pOld;  // dir launch n writes p into (n - n0 even ? bufNew : bufOld)
   T* const bufNew = pNew;
   hipEvent_t evs[2 * kPoll];
@@ -1187,73 +1576,10 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
       sampled[c] = opt.timing && (n - n0) % every == every / 2;
       if (sampled[c]) HIP_OK(hipEventRecord(evs[2 * c], stream_));
       const int np = n == n0 ? nprev0 : blocks_;
-      const int rst = (n == n0 && restart) ? 1 : 0;
-      if constexpr (std::is_same<T, float>::value) {
-        if (pairs_) {
-          dispatch_R2(g_.nz, [&](auto RC) {
-            constexpr int R2 = decltype(RC)::value;
-            if (rst)
-              k_pcg_dir_p<R2, true><<<mj.blocks, kBlock, 0, stream_>>>(
-                  g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, tol2);
-            else
-              k_pcg_dir_p<R2, false><<<mj.blocks, kBlock, 0, stream_>>>(
-                  g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, tol2);
-          });
-        }
-      }
-      if (!(std::is_same<T, float>::value && pairs_)) {
-        dispatch_R(g_.nz, [&](auto RC) {
-          constexpr int R = decltype(RC)::value;
-          if (rst) {
-            k_pcg_dir<T, R, true><<<mj.blocks, kBlock, 0, stream_>>>(
-                g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, rst, tol2);
-          } else {
-            k_pcg_dir<T, R, false><<<mj.blocks, kBlock, 0, stream_>>>(
-                g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n, rst, tol2);
-          }
-        });
-      }
-      HIP_OK(hipGetLastError());
+      launchDir<T>(A, z, pOld, pNew, q, np, n, n == n0 && restart, mj);
       if (sampled[c]) HIP_OK(hipEventRecord(evs[2 * c + 1], stream_));
-      if constexpr (std::is_same<T, float>::value) {
-        if (pairs_) {
-          dispatch_R2(g_.nz, [&](auto RC) {
-            constexpr int R2 = decltype(RC)::value;
-            const bool two = ((n - n0) & 1) != 0;  // x takes this and the previous term
-            if (zline && two)
-              k_pcg_update_p<R2, true, 2><<<blocks_, kBlock, 0, stream_>>>(
-                  g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
-            else if (zline)
-              k_pcg_update_p<R2, true, 0><<<blocks_, kBlock, 0, stream_>>>(
-                  g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
-            else if (two)
-              k_pcg_update_p<R2, false, 2><<<blocks_, kBlock, 0, stream_>>>(
-                  g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
-            else
-              k_pcg_update_p<R2, false, 0><<<blocks_, kBlock, 0, stream_>>>(
-                  g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
-          });
-        }
-      }
-      if (!(std::is_same<T, float>::value && pairs_)) {
-        const bool two = ((n - n0) & 1) != 0;  // x takes this and the previous term
-        dispatch_R(g_.nz, [&](auto RC) {
-          constexpr int R = decltype(RC)::value;
-          if (zline && two)
-            k_pcg_update<T, R, true, 2><<<blocks_, kBlock, 0, stream_>>>(
-                g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
-          else if (zline)
-            k_pcg_update<T, R, true, 0><<<blocks_, kBlock, 0, stream_>>>(
-                g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
-          else if (two)
-            k_pcg_update<T, R, false, 2><<<blocks_, kBlock, 0, stream_>>>(
-                g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
-          else
-            k_pcg_update<T, R, false, 0><<<blocks_, kBlock, 0, stream_>>>(
-                g_, x, pOld, pNew, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
-        });
-      }
-      HIP_OK(hipGetLastError());
+      const bool two = ((n - n0) & 1) != 0;  // x takes this and the previous term
+      launchUpdate<T>(x, pOld, pNew, r, q, z, invd, w, zline, two, n, mj);
       std::swap(pOld, pNew);
     }
     HIP_OK(hipMemcpyAsync(hst, dst, sizeof(PcgState), hipMemcpyDeviceToHost, stream_));
@@ -1295,35 +1621,27 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
   return done;
 }
 
+std::pair<double, double> Pcg::residualHost(const Stencil<double>& A, const double* b,
+                                            const double* x, bool zline) {
+  launchResidual(A, b, x, zline, nullptr);
+  double red[4];
+  reduceHost(march64_.blocks, 4, red);
+  return std::make_pair(red[1], red[3]);
+}
+
 int Pcg::solveFp64(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
                    PcgStats* st) {
-  const int nFactorBlocks = (g_.columns() + kWave - 1) / kWave;
-  if (opt.zline)
-    k_factor<double, true><<<nFactorBlocks, kWave, 0, stream_>>>(g_, A.Ad, A.Azp, invd_, w_);
-  else
-    k_factor<double, false><<<nFactorBlocks, kWave, 0, stream_>>>(g_, A.Ad, A.Azp, invd_, w_);
-  HIP_OK(hipGetLastError());
-  auto residual = [&]() {
-    dispatch_R(g_.nz, [&](auto RC) {
-      constexpr int R = decltype(RC)::value;
-      const March& mj = march64_;
-      if (opt.zline)
-        k_residual64<R, true><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, b, x, r_, z_, invd_,
-                                                                 w_, parts_, pstride_);
-      else
-        k_residual64<R, false><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, b, x, r_, z_, invd_,
-                                                                  w_, parts_, pstride_);
-    });
-    HIP_OK(hipGetLastError());
-    double red[4];
-    reduceHost(march64_.blocks, 4, red);
-    return std::make_pair(red[1], red[3]);
-  };
-  auto [rr, bb] = residual();
+  launchFactor(A, opt.zline, false);
+  const auto [rr, bb] = residualHost(A, b, x, opt.zline);
   const double tol = std::max(opt.rtol * std::sqrt(bb), opt.atol);
-  const double tol2 = tol * tol;
   st->rhs_norm = std::sqrt(bb);
-  int total = 0, stalls = 0;
+  return fp64Loop(A, b, x, opt, st, rr, tol * tol, 0, 0);
+}
+
+// The outer loop of the FP64 solve from a true residual rr: rounds of
+// recursion, each confirmed (or restarted) on the true residual.
+int Pcg::fp64Loop(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                  PcgStats* st, double rr, double tol2, int total, int stalls) {
   double* pOld = p0_;
   double* pNew = p1_;
   while (true) {
@@ -1337,12 +1655,11 @@ int Pcg::solveFp64(const Stencil<double>& A, const double* b, double* x, const P
       break;
     }
     int its = 0;
-    Stencil<double> Ac = A;
-    innerIterations<double>(Ac, x, pOld, pNew, r_, z_, q_, invd_, w_, opt.zline, tol2,
+    innerIterations<double>(A, x, pOld, pNew, r_, z_, q_, invd_, w_, opt.zline, tol2,
                             opt.max_iterations - total, march64_, march64_.blocks, 0, true, opt,
                             st, &its);
     total += its;
-    std::tie(rr, bb) = residual();  // true residual (restart point if not converged)
+    rr = residualHost(A, b, x, opt.zline).first;  // true residual (restart point if not converged)
     if (its == 0 && ++stalls > 2 && !(rr <= tol2)) {
       st->residual_norm = std::sqrt(rr);
       break;
@@ -1372,41 +1689,34 @@ Stencil<float> Pcg::stencil32() {
   return Stencil<float>{Ad32_, Axp32_, Ayp32_, Azp32_};
 }
 
-int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
-                    PcgStats* st) {
-  const int64_t nH = g_.cellsH();
+std::pair<double, double> Pcg::refineHost(const Stencil<double>& A, const double* b, double* xi,
+                                          double* xo, double sc, bool hasD) {
+  launchRefine(A, b, xi, xo, sc, hasD, nullptr);
+  double red[4];
+  reduceHost(march64_.blocks, 4, red);
+  return std::make_pair(red[1], red[3]);
+}
+
+void Pcg::mixedSetup(const Stencil<double>& A, const PcgOptions& opt) {
   const Stencil<float> A32 = stencil32();
   if (!opt.stencil32_ready) {
     k_cast_stencil<<<blocks_ * 4, kBlock, 0, stream_>>>(g_, A, A32);
     HIP_OK(hipGetLastError());
   }
-  const int nFactorBlocks = (g_.columns() + kWave - 1) / kWave;
-  if (opt.zline)
-    k_factor<float, true><<<nFactorBlocks, kWave, 0, stream_>>>(g_, Ad32_, Azp32_, invd32_, w32_);
-  else
-    k_factor<float, false><<<nFactorBlocks, kWave, 0, stream_>>>(g_, Ad32_, Azp32_, invd32_, w32_);
-  HIP_OK(hipGetLastError());
+  launchFactor(A, opt.zline, true);
+}
+
+int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                    PcgStats* st) {
+  mixedSetup(A, opt);
   // x ping-pongs between the caller's array and the FP64 q_ buffer (idle in this form).
-  double* xa = x;
-  double* xb = q_;
-  auto refine = [&](const double* xi, double* xo, double sc, int hasD) {
-    dispatch_R(g_.nz, [&](auto RC) {
-      constexpr int R = decltype(RC)::value;
-      if (hasD)
-        k_refine64<R, true><<<march64_.blocks, kBlock, 0, stream_>>>(
-            g_, march64_, A, b, xi, xo, d32_, sc, r_, parts_, pstride_);
-      else
-        k_refine64<R, false><<<march64_.blocks, kBlock, 0, stream_>>>(
-            g_, march64_, A, b, xi, xo, d32_, sc, r_, parts_, pstride_);
-    });
-    HIP_OK(hipGetLastError());
-    double red[4];
-    reduceHost(march64_.blocks, 4, red);
-    return std::make_pair(red[1], red[3]);
-  };
-  auto [rr, bb] = refine(xa, xb, 0.0, 0);
+  MixedCarry c;
+  c.xa = x;
+  c.xb = q_;
+  const auto [rr, bb] = refineHost(A, b, c.xa, c.xb, 0.0, false);
   const double tol = std::max(opt.rtol * std::sqrt(bb), opt.atol);
-  const double tol2 = tol * tol;
+  c.rr = rr;
+  c.tol2 = tol * tol;
   st->rhs_norm = std::sqrt(bb);
   const double ncell = static_cast<double>(g_.columns()) * g_.nz;
   // Reliable updates: the FP32 CG runs on s r (one fixed scale, so the FP32
@@ -1415,84 +1725,209 @@ int Pcg::solveMixed(const Stencil<double>& A, const double* b, double* x, const 
   // to x in FP64, r = b - A x is recomputed in FP64 and handed back to the
   // FP32 recursion, which carries on with its search direction instead of
   // restarting, so the Krylov space built so far is kept.
-  const double s = rr > 0.0 ? 1.0 / std::sqrt(rr / ncell) : 1.0;
-  float* pOld = p032_;
-  float* pNew = p132_;
-  int total = 0, n0 = 0;
-  bool restart = true;
+  c.s = rr > 0.0 ? 1.0 / std::sqrt(rr / ncell) : 1.0;
+  c.pOld = p032_;
+  c.pNew = p132_;
+  return mixedLoop(A, b, x, opt, st, c);
+}
+
+int Pcg::mixedLoop(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                   PcgStats* st, MixedCarry c) {
+  const int64_t nH = g_.cellsH();
+  const Stencil<float> A32{Ad32_, Axp32_, Ayp32_, Azp32_};
   while (true) {
-    if (rr <= tol2) {
+    if (c.rr <= c.tol2) {
       st->converged = 1;
       break;
     }
-    if (!std::isfinite(rr) || st->refinements >= opt.max_refinements ||
-        total >= opt.max_iterations)
+    if (!std::isfinite(c.rr) || st->refinements >= opt.max_refinements ||
+        c.total >= opt.max_iterations)
       break;
-    if (pairs_) {
-      dispatch_R2(g_.nz, [&](auto RC) {
-        constexpr int R2 = decltype(RC)::value;
-        if (opt.zline)
-          k_inner_init_p<R2, true><<<blocks_, kBlock, 0, stream_>>>(
-              g_, r_, s, r32_, z32_, d32_, invd32_, w32_, parts_, pstride_);
-        else
-          k_inner_init_p<R2, false><<<blocks_, kBlock, 0, stream_>>>(
-              g_, r_, s, r32_, z32_, d32_, invd32_, w32_, parts_, pstride_);
-      });
-    } else {
-      dispatch_R(g_.nz, [&](auto RC) {
-        constexpr int R = decltype(RC)::value;
-        if (opt.zline)
-          k_inner_init<R, true><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
-                                                                 invd32_, w32_, parts_, pstride_);
-        else
-          k_inner_init<R, false><<<blocks_, kBlock, 0, stream_>>>(g_, r_, s, r32_, z32_, d32_,
-                                                                  invd32_, w32_, parts_, pstride_);
-      });
-    }
-    HIP_OK(hipGetLastError());
-    const double rrS = rr * s * s;
+    launchInnerInit(opt.zline, c.s, nullptr);
+    const double rrS = c.rr * c.s * c.s;
     const double tolIn2 =
-        std::max(opt.inner_rtol * opt.inner_rtol * rrS, 0.25 * tol2 * s * s);
+        std::max(opt.inner_rtol * opt.inner_rtol * rrS, 0.25 * c.tol2 * c.s * c.s);
     int its = 0;
-    innerIterations<float>(A32, d32_, pOld, pNew, r32_, z32_, q32_, invd32_, w32_, opt.zline,
-                           tolIn2, std::min(opt.max_iterations - total, 400), march32_, blocks_,
-                           n0, restart, opt, st, &its);
-    total += its;
-    n0 += its;
-    restart = false;
-    const double rrPrev = rr;
-    std::tie(rr, bb) = refine(xa, xb, 1.0 / s, 1);
+    innerIterations<float>(A32, d32_, c.pOld, c.pNew, r32_, z32_, q32_, invd32_, w32_, opt.zline,
+                           tolIn2, std::min(opt.max_iterations - c.total, 400), march32_,
+                           blocks_, c.n0, c.restart, opt, st, &its);
+    c.total += its;
+    c.n0 += its;
+    c.restart = false;
+    const double rrPrev = c.rr;
+    c.rr = refineHost(A, b, c.xa, c.xb, 1.0 / c.s, true).first;
     st->refinements += 1;
-    if (!std::isfinite(rr) || rr > 0.25 * rrPrev) {
+    if (!std::isfinite(c.rr) || c.rr > 0.25 * rrPrev) {
       // The FP32 cycle did not halve the FP64 residual (an operator too
       // ill-conditioned for FP32, or a breakdown): finish in FP64 from the
       // better of the two iterates.
-      const bool keepNew = std::isfinite(rr) && rr < rrPrev;
-      double* best = keepNew ? xb : xa;
-      if (best != x)
-        HIP_OK(hipMemcpyAsync(x, best, static_cast<size_t>(nH) * sizeof(double),
-                              hipMemcpyDeviceToDevice, stream_));
-      PcgOptions o64 = opt;
-      o64.mixed = false;
-      o64.max_iterations = std::max(1, opt.max_iterations - total);
-      PcgStats s64;
-      const int rc = solveFp64(A, b, x, o64, &s64);
-      st->fp64_iterations = s64.iterations;
-      st->iterations = total + s64.iterations;
-      st->residual_norm = s64.residual_norm;
-      st->converged = s64.converged;
-      st->spmv_ms += s64.spmv_ms;
-      st->spmv_launches += s64.spmv_launches;
-      return rc;
+      const bool keepNew = std::isfinite(c.rr) && c.rr < rrPrev;
+      return fp64Finish(A, b, x, opt, st, keepNew ? c.xb : c.xa, c.total);
     }
-    std::swap(xa, xb);
+    std::swap(c.xa, c.xb);
   }
-  if (xa != x)
-    HIP_OK(hipMemcpyAsync(x, xa, static_cast<size_t>(nH) * sizeof(double),
+  if (c.xa != x)
+    HIP_OK(hipMemcpyAsync(x, c.xa, static_cast<size_t>(nH) * sizeof(double),
                           hipMemcpyDeviceToDevice, stream_));
-  st->iterations = total;
-  st->residual_norm = std::sqrt(rr);
+  st->iterations = c.total;
+  st->residual_norm = std::sqrt(c.rr);
   return st->converged ? FREHG_OK : FREHG_EDIVERGED;
+}
+
+int Pcg::fp64Finish(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                    PcgStats* st, const double* best, int total) {
+  const int64_t nH = g_.cellsH();
+  if (best != x)
+    HIP_OK(hipMemcpyAsync(x, best, static_cast<size_t>(nH) * sizeof(double),
+                          hipMemcpyDeviceToDevice, stream_));
+  PcgOptions o64 = opt;
+  o64.mixed = false;
+  o64.max_iterations = std::max(1, opt.max_iterations - total);
+  PcgStats s64;
+  const int rc = solveFp64(A, b, x, o64, &s64);
+  st->fp64_iterations = s64.iterations;
+  st->iterations = total + s64.iterations;
+  st->residual_norm = s64.residual_norm;
+  st->converged = s64.converged;
+  st->spmv_ms += s64.spmv_ms;
+  st->spmv_launches += s64.spmv_launches;
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Device-controlled solve: the host-polled solve's launches and decisions as
+// a fixed sequence (rounds / cycles of `iterations` launch pairs each), no
+// host synchronization, ending with a copy of the control state to the
+// host mirror.  Capturable in a graph.
+// ---------------------------------------------------------------------------
+void Pcg::prepareDevice(const PcgOptions& opt) {
+  if (opt.mixed) (void)stencil32();  // the FP32 arrays exist before any capture
+}
+
+void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
+                        const PcgOptions& opt, const PcgBudget& bud) {
+  if (bud.rounds < 1 || bud.iterations < 1 || bud.iterations > 4096)
+    throw std::runtime_error("device PCG budget out of range");
+  PcgState* dst = static_cast<PcgState*>(state_);
+  const int B = bud.iterations;
+  if (!opt.mixed) {
+    launchFactor(A, opt.zline, false);
+    launchResidual(A, b, x, opt.zline, nullptr);
+    k_dc_begin64<<<1, kBlock, 0, stream_>>>(parts_, pstride_, march64_.blocks, dst, opt.rtol,
+                                            opt.atol, opt.max_iterations);
+    HIP_OK(hipGetLastError());
+    for (int round = 0; round < bud.rounds; ++round) {
+      k_dc_round64<<<1, 1, 0, stream_>>>(dst, opt.max_iterations);
+      HIP_OK(hipGetLastError());
+      double* pOld = p0_;
+      double* pNew = p1_;
+      for (int n = 0; n < B; ++n) {
+        launchDir<double>(A, z_, pOld, pNew, q_, n == 0 ? march64_.blocks : blocks_, n, n == 0,
+                          march64_);
+        launchUpdate<double>(x, pOld, pNew, r_, q_, z_, invd_, w_, opt.zline, (n & 1) != 0, n,
+                             march64_);
+        std::swap(pOld, pNew);
+      }
+      k_dc_flush<double, false><<<blocks_ * 4, kBlock, 0, stream_>>>(g_, x, p1_, p0_, dst, B);
+      HIP_OK(hipGetLastError());
+      launchResidual(A, b, x, opt.zline, dst);
+      k_dc_end64<<<1, kBlock, 0, stream_>>>(parts_, pstride_, march64_.blocks, dst, B,
+                                            opt.max_iterations);
+      HIP_OK(hipGetLastError());
+    }
+  } else {
+    mixedSetup(A, opt);
+    const Stencil<float> A32{Ad32_, Axp32_, Ayp32_, Azp32_};
+    launchRefine(A, b, x, q_, 0.0, false, nullptr);
+    const double ncell = static_cast<double>(g_.columns()) * g_.nz;
+    k_dc_begin_mixed<<<1, kBlock, 0, stream_>>>(parts_, pstride_, march64_.blocks, dst, opt.rtol,
+                                                opt.atol, opt.max_iterations,
+                                                opt.max_refinements, ncell);
+    HIP_OK(hipGetLastError());
+    for (int cycle = 0; cycle < bud.rounds; ++cycle) {
+      k_dc_cycle<<<1, 1, 0, stream_>>>(dst, opt.max_iterations, opt.inner_rtol);
+      HIP_OK(hipGetLastError());
+      launchInnerInit(opt.zline, 0.0, dst);
+      float* pOld = p032_;
+      float* pNew = p132_;
+      for (int n = 0; n < B; ++n) {
+        launchDir<float>(A32, z32_, pOld, pNew, q32_, blocks_, n, cycle == 0 && n == 0, march32_);
+        launchUpdate<float>(d32_, pOld, pNew, r32_, q32_, z32_, invd32_, w32_, opt.zline,
+                            (n & 1) != 0, n, march32_);
+        std::swap(pOld, pNew);
+      }
+      k_dc_flush<float, true><<<blocks_ * 4, kBlock, 0, stream_>>>(g_, d32_, p132_, p032_, dst,
+                                                                   B);
+      k_dc_mid<<<1, 1, 0, stream_>>>(dst, B);
+      HIP_OK(hipGetLastError());
+      launchRefine(A, b, x, q_, 0.0, true, dst);
+      k_dc_end_mixed<<<1, kBlock, 0, stream_>>>(parts_, pstride_, march64_.blocks, dst,
+                                                opt.max_iterations, opt.max_refinements);
+      HIP_OK(hipGetLastError());
+    }
+    k_dc_take_x<<<blocks_ * 4, kBlock, 0, stream_>>>(g_, x, q_, dst);
+    HIP_OK(hipGetLastError());
+  }
+  k_dc_finish<<<1, 1, 0, stream_>>>(dst);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(stateHost_, dst, sizeof(PcgState), hipMemcpyDeviceToHost, stream_));
+}
+
+const int* Pcg::gateFlag() const { return &static_cast<const PcgState*>(state_)->gate; }
+
+int Pcg::deviceResult(PcgStats* st, DeviceSummary* sum) const {
+  const PcgState* h = static_cast<const PcgState*>(stateHost_);
+  *st = PcgStats{};
+  st->iterations = h->total;
+  st->refinements = h->refinements;
+  st->converged = h->converged;
+  st->residual_norm = std::sqrt(h->rrTrue);
+  st->rhs_norm = std::sqrt(h->bbTrue);
+  if (sum) {
+    sum->max_inner = h->maxInner;
+    sum->stopped = h->stop;
+    sum->fallback = h->fallback;
+  }
+  return h->gate;
+}
+
+int Pcg::resumeDevice(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                      PcgStats* st) {
+  const PcgState h = *static_cast<const PcgState*>(stateHost_);
+  *st = PcgStats{};
+  st->rhs_norm = std::sqrt(h.bbTrue);
+  st->refinements = h.refinements;
+  if (!opt.mixed) {
+    if (h.stop && !h.converged) {
+      st->iterations = h.total;
+      st->residual_norm = std::sqrt(h.rrTrue);
+      return FREHG_EDIVERGED;
+    }
+    return fp64Loop(A, b, x, opt, st, h.rrTrue, h.tolF2, h.total, h.stalls);
+  }
+  MixedCarry c;
+  c.xa = h.xInB ? q_ : x;
+  c.xb = h.xInB ? x : q_;
+  if (h.fallback) return fp64Finish(A, b, x, opt, st, h.keepNew ? c.xb : c.xa, h.total);
+  if (h.stop) {
+    if (c.xa != x)
+      HIP_OK(hipMemcpyAsync(x, c.xa, static_cast<size_t>(g_.cellsH()) * sizeof(double),
+                            hipMemcpyDeviceToDevice, stream_));
+    st->iterations = h.total;
+    st->residual_norm = std::sqrt(h.rrTrue);
+    return st->converged ? FREHG_OK : FREHG_EDIVERGED;
+  }
+  // out of cycles: the host loop carries on from the device's state (the
+  // last p is in p032_, the last rz in the slot an even numbering reads)
+  c.rr = h.rrTrue;
+  c.tol2 = h.tolF2;
+  c.s = h.s;
+  c.total = h.total;
+  c.n0 = 0;
+  c.restart = h.refinements == 0;
+  c.pOld = p032_;
+  c.pNew = p132_;
+  return mixedLoop(A, b, x, opt, st, c);
 }
 
 template int Pcg::innerIterations<double>(const Stencil<double>&, double*, double*&, double*&,

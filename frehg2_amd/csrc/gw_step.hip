@@ -135,7 +135,16 @@ struct Dev {
   // 2D) and the walks' path corrections of the step (pathCorr: F)
   int tw;
   double *winX, *winY, *winZ, *winSink, *pathCorr, *winVent, *winVap;
+  // the step's dt [s], in device memory (a replayed graph reads the value of
+  // the step it runs), and the gate of the kernels after the solve: when set
+  // and nonzero they return at once (the solve did not converge inside a
+  // device-controlled budget and the host finishes the step)
+  const double* dtgp;
+  const int* gate;
 };
+
+// The kernels after the solve return at once when their gate is closed.
+__device__ __forceinline__ bool gated(const Dev& d) { return d.gate && *d.gate; }
 
 __device__ __forceinline__ bool act(const Dev& d, int j, int i, int k) {
   return k >= d.kt[d.g.c2(j, i)];
@@ -236,6 +245,7 @@ __device__ __forceinline__ double ghost_head(int code, double value, double inte
 }
 
 __global__ void k_ghosts(Dev d, int doH, int doWc) {
+  if (gated(d)) return;
   const Grid& g = d.g;
   const int64_t nxe = static_cast<int64_t>(g.ny) * g.nz;
   const int64_t n = nxe + static_cast<int64_t>(g.nx) * g.nz;
@@ -276,7 +286,8 @@ __global__ void k_ghosts(Dev d, int doH, int doWc) {
 }
 
 // ---- coupled-top classification (Corrector.cpp:55-118) -------------------
-__global__ void k_classify_coupled(Dev d, double dtg) {
+__global__ void k_classify_coupled(Dev d) {
+  const double dtg = *d.dtgp;
   const Grid& g = d.g;
   const int n = g.columns();
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
@@ -307,6 +318,7 @@ __global__ void k_classify_coupled(Dev d, double dtg) {
 // assembly.
 template <bool ISO, bool CAP>
 __global__ void k_cell_k(Dev d) {
+  if (gated(d)) return;
   const int64_t n = d.g.cellsH();
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -538,7 +550,9 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
 // (j, i >= 1), so the operands the three share are fetched once (1.02 ms
 // against 0.36 + 0.48 + 0.27 ms for three kernels at 512 x 512 x 100).
 template <bool DENS>
-__global__ __launch_bounds__(kThreads) void k_faces(Dev d, int corr, double dtg) {
+__global__ __launch_bounds__(kThreads) void k_faces(Dev d, int corr) {
+  if (gated(d)) return;
+  const double dtg = *d.dtgp;
   const Grid& g = d.g;
   const int nzf = g.nz + 1, nxa = g.nx + 1;
   const int64_t n = static_cast<int64_t>(g.ny + 1) * nxa * nzf;
@@ -561,7 +575,8 @@ __global__ __launch_bounds__(kThreads) void k_faces(Dev d, int corr, double dtg)
 }
 
 // ---- density / viscosity face means (Baroclinic.cpp:46-187) ---------------
-__global__ void k_fill(double* p, int64_t n, double v) {
+__global__ void k_fill(double* p, int64_t n, double v, const int* gate) {
+  if (gate && *gate) return;
   for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
        t += static_cast<int64_t>(gridDim.x) * blockDim.x)
     p[t] = v;
@@ -660,7 +675,8 @@ __global__ void k_density_faces(Dev d, const double* rrTop, const double* rvTop)
 // the gravity terms r^2 v (Predictor.cpp:474-687); without, every ratio is
 // the literal 1.0 and folds away.
 template <bool DENS>
-__global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
+__global__ __launch_bounds__(kThreads) void k_assemble(Dev d) {
+  const double dtg = *d.dtgp;
   const Grid& g = d.g;
   const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
   const double dx = g.dx, dy = g.dy, az = g.az;
@@ -901,6 +917,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(Dev d, double dtg) {
 }
 
 __global__ void k_scatter(Dev d) {
+  if (gated(d)) return;
   const Grid& g = d.g;
   const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
   for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
@@ -915,7 +932,9 @@ __global__ void k_scatter(Dev d) {
 }
 
 // ---- coupled top bookkeeping (Corrector.cpp:539-627) ----------------------
-__global__ void k_coupled_top(Dev d, double dtg) {
+__global__ void k_coupled_top(Dev d) {
+  if (gated(d)) return;
+  const double dtg = *d.dtgp;
   const Grid& g = d.g;
   KSum ex, bo, ev;
   for (Range r = xcd_range(g.columns()); r.t < r.end; r.t += r.stride) {
@@ -1048,7 +1067,9 @@ __global__ void k_cpl_reduce_audit(Dev d, const double* __restrict__ bottom) {
 // With DENS the fluxes are weighted by their face's density ratio and the
 // storage by the cell's (Corrector.cpp:653-669).
 template <bool SINK, bool DENS>
-__global__ void k_update_wc(Dev d, double dtg) {
+__global__ void k_update_wc(Dev d) {
+  if (gated(d)) return;
+  const double dtg = *d.dtgp;
   const Grid& g = d.g;
   const int64_t n = static_cast<int64_t>(g.columns()) * g.nz;
   const int64_t sy = static_cast<int64_t>(g.nx2) * g.nz;
@@ -1428,6 +1449,7 @@ __device__ __forceinline__ bool walk_parallel(const Grid& g, int lane, int kt, d
 // "sat": wc >= theta_s or h > aev), for the halo columns; interior cells get
 // theirs from k_update_wc.
 __global__ void k_sat_halo(Dev d) {
+  if (gated(d)) return;
   const Grid& g = d.g;
   const int nh = 2 * (g.nx2 + g.ny);
   const int64_t n = static_cast<int64_t>(nh) * g.nz;
@@ -1449,6 +1471,7 @@ __global__ void k_sat_halo(Dev d) {
 // the cell's outcome into the second buffer pair (wcOut, hOut), its send
 // volumes, and flags the columns that have senders.
 __global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
+  if (gated(d)) return;
   const Grid& g = d.g;
   const int nz = g.nz;
   const int64_t n = static_cast<int64_t>(g.columns()) * nz;
@@ -1594,7 +1617,9 @@ __device__ __forceinline__ void walk_inputs(const Dev& d, int lane, int kt, int6
 }
 
 template <int R>
-__global__ __launch_bounds__(kThreads) void k_realloc_walk(Dev d, double dtg) {
+__global__ __launch_bounds__(kThreads) void k_realloc_walk(Dev d) {
+  if (gated(d)) return;
+  const double dtg = *d.dtgp;
   constexpr int RF = R + 1;  // face rows 0..nz
   constexpr int RS = R <= kParallelWalkRows ? R : 1;
   __shared__ double shWalk[kThreads / kWave][7 * kWave * RS];
@@ -1660,7 +1685,9 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk(Dev d, double dtg) {
 // untouched).  Every arithmetic operation and its order are the reference's.
 // Same grid as k_realloc_walk: its audit partials are added to that kernel's.
 template <int R, bool TW>
-__global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double dtg) {
+__global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d) {
+  if (gated(d)) return;
+  const double dtg = *d.dtgp;
   constexpr int RF = R + 1;
   const Grid& g = d.g;
   const int lane = threadIdx.x & (kWave - 1);
@@ -1868,7 +1895,9 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk_serial(Dev d, double 
 // Boundary inflow of the step (RichardsSolver.cpp:961-1006): the net volume
 // through the domain's faces, one thread per column for the top and bottom
 // faces and one per face for the sides.
-__global__ __launch_bounds__(kThreads) void k_boundary_in(Dev d, double dtg) {
+__global__ __launch_bounds__(kThreads) void k_boundary_in(Dev d) {
+  if (gated(d)) return;
+  const double dtg = *d.dtgp;
   const Grid& g = d.g;
   const int nz = g.nz;
   const int64_t sy = static_cast<int64_t>(g.nx2) * nz;
@@ -1913,7 +1942,9 @@ __global__ __launch_bounds__(kThreads) void k_boundary_in(Dev d, double dtg) {
 // Adaptive-step inputs (AdaptiveStep.cpp:59-98): the largest flux imbalance
 // and the Courant limit of the unsaturated cells, top cells excluded.  Only
 // launched when dt can change (dt_min < dt_max).
-__global__ __launch_bounds__(kThreads) void k_adapt_inputs(Dev d, double dtg) {
+__global__ __launch_bounds__(kThreads) void k_adapt_inputs(Dev d) {
+  if (gated(d)) return;
+  const double dtg = *d.dtgp;
   const Grid& g = d.g;
   const int nz = g.nz;
   const int64_t n = static_cast<int64_t>(g.columns()) * nz;
@@ -1960,7 +1991,9 @@ __global__ void k_volume(Dev d) {
 }
 
 // Fixed-order reduction of every slot: one block per slot.
-__global__ void k_reduce_slots(const double* __restrict__ parts, int rb, double* __restrict__ out) {
+__global__ void k_reduce_slots(const double* __restrict__ parts, int rb, double* __restrict__ out,
+                               const int* gate) {
+  if (gate && *gate) return;
   const int slot = blockIdx.x;
   const int op = kSlotOp[slot];
   double v[1] = {op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY)};
@@ -2014,7 +2047,9 @@ __global__ void k_initial(Dev d, int form, const double* vals, double vconst, do
 // ---- the transport window (RichardsSolver.cpp:826-878) ----------------------
 // Every face slot, halos included: x, y, the root-uptake volumes (H), the z
 // faces with the walks' path corrections (F) and the ET vapour (2D).
-__global__ void k_transport_window(Dev d, double dtg) {
+__global__ void k_transport_window(Dev d) {
+  if (gated(d)) return;
+  const double dtg = *d.dtgp;
   const Grid& g = d.g;
   const int64_t nH = g.cellsH(), nF = g.cellsF();
   const int64_t n2 = static_cast<int64_t>(g.ny2) * g.nx2;
@@ -2126,6 +2161,21 @@ struct GwSolver {
   int timingEvery = 1;  // SpMV launches bracketed: one in timingEvery
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evs0 = nullptr, evs1 = nullptr;
   frehg_gw_timing tm{};
+  // the step's dt: device word read by the kernels, pinned host word it is
+  // copied from
+  double* dtgDev = nullptr;
+  double* dtgPinned = nullptr;
+  // step mode (FREHG_STEP_*): host-polled, device-controlled, or a
+  // device-controlled step captured in a graph (one per buffer parity) and
+  // replayed; the budget of the device-controlled solve
+  int stepMode = FREHG_STEP_HOST;
+  PcgBudget budget;
+  bool budgetSet = false;    // given by the caller (else sized from the last solve)
+  bool budgetReady = false;  // a host step has sized it, or the caller set it
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  Dev capturedDev[2]{};      // the device view each graph was captured with
+  double* hParity0 = nullptr;
+  frehg_gw_graph_stats gstats{};
 
   template <class T>
   T* alloc(int64_t n) {
@@ -2136,6 +2186,8 @@ struct GwSolver {
 
   ~GwSolver() {
     if (stream) (void)hipStreamSynchronize(stream);
+    invalidateGraphs();
+    if (dtgPinned) (void)hipHostFree(dtgPinned);
     pcg.reset();
     for (void* p : allocations) (void)hipFree(p);
     if (slotsHost) (void)hipHostFree(slotsHost);
@@ -2155,7 +2207,7 @@ struct GwSolver {
     k_ghosts<<<std::max(blocks, 1), kThreads, 0, stream>>>(d, doH ? 1 : 0, doWc ? 1 : 0);
     HIP_CHECK(hipGetLastError());
   }
-  void faces(int corr, double dtg) {
+  void faces(int corr) {
     if (iso && !corr) k_cell_k<true, true><<<rb, kThreads, 0, stream>>>(d);
     else if (iso) k_cell_k<true, false><<<rb, kThreads, 0, stream>>>(d);
     else if (!corr) k_cell_k<false, true><<<rb, kThreads, 0, stream>>>(d);
@@ -2163,43 +2215,77 @@ struct GwSolver {
     HIP_CHECK(hipGetLastError());
     // the predictor's face K are formed inside the assembly
     if (!corr) return;
-    if (d.rRho) k_faces<true><<<rb, kThreads, 0, stream>>>(d, corr, dtg);
-    else k_faces<false><<<rb, kThreads, 0, stream>>>(d, corr, dtg);
+    if (d.rRho) k_faces<true><<<rb, kThreads, 0, stream>>>(d, corr);
+    else k_faces<false><<<rb, kThreads, 0, stream>>>(d, corr);
     HIP_CHECK(hipGetLastError());
   }
   void copy(double* dst, const double* src, int64_t n) {
     HIP_CHECK(hipMemcpyAsync(dst, src, static_cast<size_t>(n) * sizeof(double),
                              hipMemcpyDeviceToDevice, stream));
   }
-  void reduceSlots() {
-    k_reduce_slots<<<S_COUNT, kThreads, 0, stream>>>(d.parts, rb, slots);
+  // The step's sums to the host mirror; reduceSlots() waits for them.
+  void enqueueSlots() {
+    k_reduce_slots<<<S_COUNT, kThreads, 0, stream>>>(d.parts, rb, slots, d.gate);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(slotsHost, slots, S_COUNT * sizeof(double), hipMemcpyDeviceToHost,
                              stream));
+  }
+  void reduceSlots() {
+    enqueueSlots();
     HIP_CHECK(hipStreamSynchronize(stream));
   }
+  // The step's dt into device memory (every kernel reads it there), from a
+  // pinned host word: a replayed graph copies the value its step was given.
+  void enqueueDtg() {
+    HIP_CHECK(hipMemcpyAsync(dtgDev, dtgPinned, sizeof(double), hipMemcpyHostToDevice, stream));
+  }
+  void setDtg(double dtg) {
+    *dtgPinned = dtg;
+    enqueueDtg();
+  }
   void step(double t, double dtg);
+  // Step phases: before the solve (refresh, predictor assembly) and after it
+  // (scatter, corrector, post-allocation, boundary inflow, dt inputs).
+  void enqueuePre();
+  void enqueuePost();
+  void hostStep();
+  void deviceStep();
+  void finishStep(const PcgStats& ps);
+  void captureStep(int parity);
+  PcgOptions stepOptions() const;
+  void flipParity() {
+    std::swap(d.h, d.hOut);
+    std::swap(d.wc, d.wcOut);
+  }
+  int parity() const { return d.h == hParity0 ? 0 : 1; }
+  void invalidateGraphs() {
+    for (int p = 0; p < 2; ++p)
+      if (gexec[p]) {
+        (void)hipGraphExecDestroy(gexec[p]);
+        gexec[p] = nullptr;
+      }
+  }
+  void setStepMode(int mode, int rounds, int iterations);
   // Post-allocation and final clamp (Reallocate.cpp:122-545,
   // Corrector.cpp:758-792): the classification writes the second buffer pair,
   // which then becomes the state; the saturation flags must be current.
-  void postAllocation(double dtg) {
+  void postAllocation() {
     k_realloc_classify<<<rb, kThreads, 0, stream>>>(d);
     HIP_CHECK(hipGetLastError());
     if (d.tw) fill(d.pathCorr, g.cellsF(), 0.0);
     dispatch_col(g.nz, [&](auto RC) {
       constexpr int R = decltype(RC)::value;
-      k_realloc_walk<R><<<rbCol, kThreads, 0, stream>>>(d, dtg);
-      if (d.tw) k_realloc_walk_serial<R, true><<<rbCol, kThreads, 0, stream>>>(d, dtg);
-      else k_realloc_walk_serial<R, false><<<rbCol, kThreads, 0, stream>>>(d, dtg);
+      k_realloc_walk<R><<<rbCol, kThreads, 0, stream>>>(d);
+      if (d.tw) k_realloc_walk_serial<R, true><<<rbCol, kThreads, 0, stream>>>(d);
+      else k_realloc_walk_serial<R, false><<<rbCol, kThreads, 0, stream>>>(d);
     });
     HIP_CHECK(hipGetLastError());
-    std::swap(d.h, d.hOut);
-    std::swap(d.wc, d.wcOut);
+    flipParity();
     ghosts(false, true);
   }
   void fill(double* p, int64_t n, double v) {
     k_fill<<<static_cast<int>(std::min<int64_t>(2048, (n + kThreads - 1) / kThreads)), kThreads, 0,
-             stream>>>(p, n, v);
+             stream>>>(p, n, v, d.gate);
     HIP_CHECK(hipGetLastError());
   }
   // RichardsSolver::enableTransportWindow / resetTransportWindow
@@ -2231,10 +2317,11 @@ struct GwSolver {
   void testPostAllocate(double dtg) {
     if (!(dtg > 0.0)) fail(FREHG_EINVAL, "dtg must be positive");
     dtgCurrent = dtg;
+    setDtg(dtg);
     k_reset_parts<<<64, kThreads, 0, stream>>>(d.parts, rb);
     k_sat_all<<<rb, kThreads, 0, stream>>>(d);
     HIP_CHECK(hipGetLastError());
-    postAllocation(dtg);
+    postAllocation();
     reduceSlots();
     const double* s = slotsHost;
     audit = frehg_gw_audit{};
@@ -2335,9 +2422,11 @@ struct GwSolver {
       if (rv)
         HIP_CHECK(hipMemcpyAsync(d.rVisc, rv, nH * sizeof(double), hipMemcpyHostToDevice, stream));
       else
-        k_fill<<<fb, kThreads, 0, stream>>>(d.rVisc, nH, 1.0);
-      for (int q = 2; q < 6; ++q) k_fill<<<fb, kThreads, 0, stream>>>(densBuf[q], nH, 1.0);
-      for (int q = 6; q < 8; ++q) k_fill<<<fb, kThreads, 0, stream>>>(densBuf[q], nF, 1.0);
+        k_fill<<<fb, kThreads, 0, stream>>>(d.rVisc, nH, 1.0, nullptr);
+      for (int q = 2; q < 6; ++q)
+        k_fill<<<fb, kThreads, 0, stream>>>(densBuf[q], nH, 1.0, nullptr);
+      for (int q = 6; q < 8; ++q)
+        k_fill<<<fb, kThreads, 0, stream>>>(densBuf[q], nF, 1.0, nullptr);
       HIP_CHECK(hipGetLastError());
       const double* top[2] = {rrTop, rvTop};
       const double* topDev[2] = {nullptr, nullptr};
@@ -2693,6 +2782,13 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   slots = alloc<double>(S_COUNT);
   HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&slotsHost), S_COUNT * sizeof(double),
                           hipHostMallocDefault));
+  dtgDev = alloc<double>(1);
+  HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&dtgPinned), sizeof(double),
+                          hipHostMallocDefault));
+  *dtgPinned = 0.0;
+  d.dtgp = dtgDev;
+  d.gate = nullptr;
+  hParity0 = d.h;
 
   // boundary markers (RichardsSolver.cpp:429-566)
   {
@@ -2784,12 +2880,16 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   tm.spmv_bytes = pcg->spmvBytesPerLaunch(popt.mixed);
 }
 
-void GwSolver::step(double /*t*/, double dtg) {
-  if (!(dtg > 0.0)) fail(FREHG_EINVAL, "dtg must be positive");
+PcgOptions GwSolver::stepOptions() const {
+  PcgOptions o = popt;
+  o.timing = timing;
+  o.timing_every = timingEvery;
+  o.stencil32_ready = d.Ad32 != nullptr;
+  return o;
+}
+
+void GwSolver::enqueuePre() {
   const int64_t nH = g.cellsH();
-  if (timing) HIP_CHECK(hipEventRecord(ev0, stream));
-  dtgCurrent = dtg;
-  audit = frehg_gw_audit{};
   k_reset_parts<<<64, kThreads, 0, stream>>>(d.parts, rb);
   HIP_CHECK(hipGetLastError());
   // step-start refresh (RichardsSolver.cpp:743-748)
@@ -2799,21 +2899,69 @@ void GwSolver::step(double /*t*/, double dtg) {
   // predictor
   if (d.coupled) {
     k_classify_coupled<<<std::max(1, std::min(4096, (g.columns() + kThreads - 1) / kThreads)),
-                         kThreads, 0, stream>>>(d, dtg);
+                         kThreads, 0, stream>>>(d);
     HIP_CHECK(hipGetLastError());
   }
-  faces(0, dtg);
-  if (d.rRho) k_assemble<true><<<rb, kThreads, 0, stream>>>(d, dtg);
-  else k_assemble<false><<<rb, kThreads, 0, stream>>>(d, dtg);
+  faces(0);
+  if (d.rRho) k_assemble<true><<<rb, kThreads, 0, stream>>>(d);
+  else k_assemble<false><<<rb, kThreads, 0, stream>>>(d);
   HIP_CHECK(hipGetLastError());
+}
+
+void GwSolver::enqueuePost() {
+  k_scatter<<<rb, kThreads, 0, stream>>>(d);
+  HIP_CHECK(hipGetLastError());
+  ghosts(true, false);
+  // corrector
+  faces(1);
+  if (d.coupled) {
+    k_coupled_top<<<rb, kThreads, 0, stream>>>(d);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (d.sink && d.rRho) k_update_wc<true, true><<<rb, kThreads, 0, stream>>>(d);
+  else if (d.sink) k_update_wc<true, false><<<rb, kThreads, 0, stream>>>(d);
+  else if (d.rRho) k_update_wc<false, true><<<rb, kThreads, 0, stream>>>(d);
+  else k_update_wc<false, false><<<rb, kThreads, 0, stream>>>(d);
+  HIP_CHECK(hipGetLastError());
+  // post-allocation, final clamp, boundary inflow and dt inputs: one column
+  // kernel writing the second buffer pair, which then becomes the state
+  const int nh = 2 * (g.nx2 + g.ny);
+  const int64_t nhc = static_cast<int64_t>(nh) * g.nz;
+  k_sat_halo<<<static_cast<int>(std::min<int64_t>((nhc + kThreads - 1) / kThreads, 1024)),
+               kThreads, 0, stream>>>(d);
+  HIP_CHECK(hipGetLastError());
+  postAllocation();
+  k_boundary_in<<<std::min(rb, rbCol * 4), kThreads, 0, stream>>>(d);
+  HIP_CHECK(hipGetLastError());
+  if (d.tw) {
+    k_transport_window<<<rb, kThreads, 0, stream>>>(d);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (dtMin < dtMax) {
+    k_adapt_inputs<<<rb, kThreads, 0, stream>>>(d);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+void GwSolver::step(double /*t*/, double dtg) {
+  if (!(dtg > 0.0)) fail(FREHG_EINVAL, "dtg must be positive");
+  dtgCurrent = dtg;
+  audit = frehg_gw_audit{};
+  *dtgPinned = dtg;
+  if (stepMode == FREHG_STEP_HOST || timing || !budgetReady) hostStep();
+  else deviceStep();
+}
+
+// Host-polled step: the solve's decisions are taken on the host between
+// launches.
+void GwSolver::hostStep() {
+  if (timing) HIP_CHECK(hipEventRecord(ev0, stream));
+  enqueueDtg();
+  enqueuePre();
   if (timing) HIP_CHECK(hipEventRecord(evs0, stream));
   Stencil<double> A{d.Ad, d.Axp, d.Ayp, d.Azp};
   PcgStats ps;
-  PcgOptions o = popt;
-  o.timing = timing;
-  o.timing_every = timingEvery;
-  o.stencil32_ready = d.Ad32 != nullptr;
-  const int rc = pcg->solve(A, d.rhs, d.x, o, &ps);
+  const int rc = pcg->solve(A, d.rhs, d.x, stepOptions(), &ps);
   if (timing) HIP_CHECK(hipEventRecord(evs1, stream));
   last.iterations = ps.iterations;
   last.refinements = ps.refinements;
@@ -2825,43 +2973,149 @@ void GwSolver::step(double /*t*/, double dtg) {
     fail(FREHG_EDIVERGED, "gw linear solve did not converge in " + std::to_string(ps.iterations) +
                               " iterations (residual " + std::to_string(ps.residual_norm) +
                               ", ||b|| " + std::to_string(ps.rhs_norm) + ")");
-  k_scatter<<<rb, kThreads, 0, stream>>>(d);
-  HIP_CHECK(hipGetLastError());
-  ghosts(true, false);
-  // corrector
-  faces(1, dtg);
-  if (d.coupled) {
-    k_coupled_top<<<rb, kThreads, 0, stream>>>(d, dtg);
-    HIP_CHECK(hipGetLastError());
-  }
-  if (d.sink && d.rRho) k_update_wc<true, true><<<rb, kThreads, 0, stream>>>(d, dtg);
-  else if (d.sink) k_update_wc<true, false><<<rb, kThreads, 0, stream>>>(d, dtg);
-  else if (d.rRho) k_update_wc<false, true><<<rb, kThreads, 0, stream>>>(d, dtg);
-  else k_update_wc<false, false><<<rb, kThreads, 0, stream>>>(d, dtg);
-  HIP_CHECK(hipGetLastError());
-  // post-allocation, final clamp, boundary inflow and dt inputs: one column
-  // kernel writing the second buffer pair, which then becomes the state
-  {
-    const int nh = 2 * (g.nx2 + g.ny);
-    const int64_t nhc = static_cast<int64_t>(nh) * g.nz;
-    k_sat_halo<<<static_cast<int>(std::min<int64_t>((nhc + kThreads - 1) / kThreads, 1024)),
-                 kThreads, 0, stream>>>(d);
-    HIP_CHECK(hipGetLastError());
-    const int adapt = dtMin < dtMax ? 1 : 0;
-    postAllocation(dtg);
-    k_boundary_in<<<std::min(rb, rbCol * 4), kThreads, 0, stream>>>(d, dtg);
-    HIP_CHECK(hipGetLastError());
-    if (d.tw) {
-      k_transport_window<<<rb, kThreads, 0, stream>>>(d, dtg);
-      HIP_CHECK(hipGetLastError());
+  if (stepMode != FREHG_STEP_HOST && !budgetSet && !budgetReady) {
+    // the first step of a device-controlled mode sizes the budget
+    budget.iterations = std::min(4096, std::max(8, ps.iterations + ps.iterations / 2 + 4));
+    budget.rounds = popt.mixed ? std::max(2, ps.refinements + 1) : 1;
+    if (popt.mixed) {
+      const int perCycle = (ps.iterations + std::max(1, ps.refinements) - 1) /
+                           std::max(1, ps.refinements);
+      budget.iterations = std::min(400, std::max(8, perCycle + perCycle / 2 + 4));
     }
-    if (adapt) {
-      k_adapt_inputs<<<rb, kThreads, 0, stream>>>(d, dtg);
-      HIP_CHECK(hipGetLastError());
-    }
+    budgetReady = true;
   }
+  enqueuePost();
   if (timing) HIP_CHECK(hipEventRecord(ev1, stream));
   reduceSlots();
+  finishStep(ps);
+}
+
+// The whole device-controlled step, launches only: no host synchronization,
+// so the sequence can be captured.  The kernels after the solve carry the
+// solve's gate.
+void GwSolver::captureStep(int p) {
+  hipGraph_t graph = nullptr;
+  HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  const Dev before = d;
+  try {
+    enqueueDtg();
+    enqueuePre();
+    pcg->enqueueDevice(Stencil<double>{d.Ad, d.Axp, d.Ayp, d.Azp}, d.rhs, d.x, stepOptions(),
+                       budget);
+    d.gate = pcg->gateFlag();
+    enqueuePost();
+    enqueueSlots();
+    d.gate = nullptr;
+  } catch (...) {
+    d = before;
+    hipGraph_t junk = nullptr;
+    (void)hipStreamEndCapture(stream, &junk);
+    if (junk) (void)hipGraphDestroy(junk);
+    throw;
+  }
+  HIP_CHECK(hipStreamEndCapture(stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIP_CHECK(e);
+  if (p >= 0) {
+    gexec[p] = exec;
+    capturedDev[p] = before;
+    gstats.captures += 1;
+  } else {
+    (void)hipGraphExecDestroy(exec);
+  }
+}
+
+void GwSolver::deviceStep() {
+  const PcgOptions o = stepOptions();
+  pcg->prepareDevice(o);
+  const Stencil<double> A{d.Ad, d.Axp, d.Ayp, d.Azp};
+  if (stepMode == FREHG_STEP_GRAPH) {
+    const int p = parity();
+    if (gexec[p] && std::memcmp(&capturedDev[p], &d, sizeof(Dev)) != 0) invalidateGraphs();
+    if (!gexec[p]) {
+      captureStep(p);  // runs the host side of the step: the parity flips
+    } else {
+      flipParity();
+    }
+    HIP_CHECK(hipGraphLaunch(gexec[p], stream));
+    gstats.replays += 1;
+  } else {
+    enqueueDtg();
+    enqueuePre();
+    pcg->enqueueDevice(A, d.rhs, d.x, o, budget);
+    d.gate = pcg->gateFlag();
+    enqueuePost();
+    enqueueSlots();
+    d.gate = nullptr;
+  }
+  HIP_CHECK(hipStreamSynchronize(stream));
+  gstats.device_steps += 1;
+  PcgStats ps;
+  DeviceSummary sum;
+  if (pcg->deviceResult(&ps, &sum)) {
+    // The solve did not converge inside the budget and the kernels after it
+    // returned at once: finish the solve host-polled from where it stopped,
+    // then run the rest of the step.
+    gstats.host_finishes += 1;
+    flipParity();
+    const int rc = pcg->resumeDevice(A, d.rhs, d.x, o, &ps);
+    last.iterations = ps.iterations;
+    last.refinements = ps.refinements;
+    last.converged = ps.converged;
+    last.fp64_iterations = ps.fp64_iterations;
+    last.residual_norm = ps.residual_norm;
+    last.rhs_norm = ps.rhs_norm;
+    if (rc != FREHG_OK)
+      fail(FREHG_EDIVERGED, "gw linear solve did not converge in " +
+                                std::to_string(ps.iterations) + " iterations (residual " +
+                                std::to_string(ps.residual_norm) + ", ||b|| " +
+                                std::to_string(ps.rhs_norm) + ")");
+    if (!budgetSet) {
+      // a larger budget for the next steps (graphs are captured again)
+      const int need = popt.mixed ? std::max(sum.max_inner, budget.iterations)
+                                  : ps.iterations;
+      budget.iterations =
+          std::min(popt.mixed ? 400 : 4096, std::max(budget.iterations + budget.iterations / 2,
+                                                      need + need / 4 + 4));
+      if (popt.mixed) budget.rounds = std::max(budget.rounds, ps.refinements + 1);
+      invalidateGraphs();
+    }
+    enqueuePost();
+    reduceSlots();
+  }
+  last.iterations = ps.iterations;
+  last.refinements = ps.refinements;
+  last.converged = ps.converged;
+  last.fp64_iterations = ps.fp64_iterations;
+  last.residual_norm = ps.residual_norm;
+  last.rhs_norm = ps.rhs_norm;
+  finishStep(ps);
+}
+
+void GwSolver::setStepMode(int mode, int rounds, int iterations) {
+  if (mode < FREHG_STEP_HOST || mode > FREHG_STEP_GRAPH) fail(FREHG_EINVAL, "unknown step mode");
+  if (rounds < 0 || iterations < 0 || rounds > 64 || iterations > 4096)
+    fail(FREHG_EINVAL, "step budget out of range (rounds <= 64, iterations <= 4096)");
+  if ((rounds > 0) != (iterations > 0))
+    fail(FREHG_EINVAL, "give both budget values, or neither (sized from the first step)");
+  if (popt.mixed && iterations > 400) fail(FREHG_EINVAL, "mixed cycles are capped at 400 iterations");
+  invalidateGraphs();
+  stepMode = mode;
+  budgetSet = rounds > 0;
+  budgetReady = budgetSet;
+  if (budgetSet) {
+    budget.rounds = rounds;
+    budget.iterations = iterations;
+  }
+  gstats.budget_rounds = budget.rounds;
+  gstats.budget_iterations = budget.iterations;
+}
+
+// Audit and the next dt from the step's sums (slotsHost).
+void GwSolver::finishStep(const PcgStats& ps) {
+  const double dtg = dtgCurrent;
   const double* s = slotsHost;
   audit.ss_storage = s[S_SS];
   if (d.reallocOff) {
@@ -2890,6 +3144,8 @@ void GwSolver::step(double /*t*/, double dtg) {
   if (next > dtMax) next = dtMax;
   if (next < dtMin) next = dtMin;
   dtgNext = next;
+  gstats.budget_rounds = budget.rounds;
+  gstats.budget_iterations = budget.iterations;
   if (timing) {
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
@@ -3137,7 +3393,19 @@ int frehg_gw_set_pcg_shapes(frehg_gw_solver* s, int march64_waves_per_cu,
         frehg::fail(FREHG_EINVAL, "frehg_gw_set_pcg_shapes: waves per CU must be in [1, 256] "
                                   "(0 keeps a value)");
     GwSolver* g = frehg::S(s);
+    g->invalidateGraphs();
     g->pcg->setShapes(march64_waves_per_cu, march32_waves_per_cu, column_waves_per_cu);
+  });
+}
+int frehg_gw_set_step_mode(frehg_gw_solver* s, int mode, int rounds, int iterations) {
+  return guard([&] { frehg::S(s)->setStepMode(mode, rounds, iterations); });
+}
+int frehg_gw_get_graph_stats(const frehg_gw_solver* s, frehg_gw_graph_stats* out) {
+  return guard([&] {
+    if (!out) frehg::fail(FREHG_EINVAL, "frehg_gw_get_graph_stats: output is required");
+    const GwSolver* g = frehg::S(s);
+    *out = g->gstats;
+    out->mode = g->stepMode;
   });
 }
 int frehg_gw_get_pcg_shapes(const frehg_gw_solver* s, int* march64_waves_per_cu,

@@ -195,6 +195,20 @@ class RichardsSolver:
     def synchronize(self) -> None:
         check(self.lib, self.lib.frehg_gw_synchronize(self._h))
 
+    def set_step_mode(self, mode, rounds: int = 0, iterations: int = 0) -> None:
+        """'host' (host-polled PCG), 'device' (device-controlled, one host
+        synchronization a step) or 'graph' (the device-controlled step
+        captured in a hipGraph and replayed).  rounds / iterations: the
+        device-controlled solve's budget (0, 0: sized from the first step)."""
+        modes = {"host": abi.STEP_HOST, "device": abi.STEP_DEVICE, "graph": abi.STEP_GRAPH}
+        m = modes[mode] if isinstance(mode, str) else int(mode)
+        check(self.lib, self.lib.frehg_gw_set_step_mode(self._h, m, int(rounds), int(iterations)))
+
+    def graph_stats(self) -> dict:
+        t = abi.GraphStats()
+        check(self.lib, self.lib.frehg_gw_get_graph_stats(self._h, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in t._fields_}
+
     def set_pcg_shapes(self, march64=0, march32=0, column=0) -> None:
         """PCG launch shapes in waves per CU (0 keeps a value); see
         frehg2_amd.autotune."""

@@ -295,6 +295,45 @@ int frehg_gw_get_timing(frehg_gw_solver* s, frehg_gw_timing* out);
 /* Block until all work on the solver's stream is done. */
 int frehg_gw_synchronize(frehg_gw_solver* s);
 
+/* Step modes.  The reference's coupled substep (src/coupling/Coupler.cpp:
+ * 106-119) calls RichardsSolver::step, whose KSP solve
+ * (src/core/LinearSystem.cpp:320) iterates under host control.  HOST keeps
+ * that shape: the PCG's convergence is polled from the host every 8
+ * iterations.  DEVICE takes the same decisions on the device (one-block
+ * control kernels between rounds of launches), so a step is a fixed launch
+ * sequence with one host synchronization at its end.  GRAPH captures that
+ * sequence once per buffer parity in a hipGraph and replays it (the step's
+ * dt travels through a pinned host word, so any dt replays the same graph).
+ * A device-controlled solve runs `rounds` rounds (FP64: restarted
+ * recursions, each confirmed on the true residual; mixed: refinement
+ * cycles) of `iterations` direction/update launch pairs; launches past the
+ * end of the recursion return at once.  A solve that does not converge
+ * inside that budget closes a gate: the step's later kernels return at
+ * once, the host finishes the solve host-polled from where it stopped and
+ * then runs the rest of the step (the budget then grows, unless the caller
+ * fixed it).  DEVICE and GRAPH give bitwise the same results for the same
+ * budget; with a budget the solves fit in they give HOST's (FP64) or agree
+ * with it to rounding (mixed: the FP32 scale is computed on the device).
+ * No counterpart in the reference. */
+#define FREHG_STEP_HOST 0
+#define FREHG_STEP_DEVICE 1
+#define FREHG_STEP_GRAPH 2
+typedef struct frehg_gw_graph_stats {
+  int mode;
+  int budget_rounds;
+  int budget_iterations;
+  int64_t captures;        /* graphs captured (one per parity and budget) */
+  int64_t replays;         /* graph launches */
+  int64_t device_steps;    /* steps in DEVICE or GRAPH mode */
+  int64_t host_finishes;   /* of those, steps whose solve the host finished */
+} frehg_gw_graph_stats;
+/* rounds = iterations = 0: the first step runs host-polled and sizes the
+ * budget (and it grows when a solve outruns it); else the budget is fixed
+ * (rounds <= 64, iterations <= 4096; mixed cycles <= 400).  Timing mode
+ * (frehg_gw_set_timing) runs host-polled steps. */
+int frehg_gw_set_step_mode(frehg_gw_solver* s, int mode, int rounds, int iterations);
+int frehg_gw_get_graph_stats(const frehg_gw_solver* s, frehg_gw_graph_stats* out);
+
 /* Launch shapes of the PCG (no counterpart in the reference, where PETSc
  * makes these choices): waves per CU of the FP64 and FP32 stencil kernels
  * and of the column kernels, each in [1, 256]; 0 keeps a value.  Defaults
