@@ -126,6 +126,9 @@ SYMBOLS = {
     "frehg_gw_set_pcg_shapes": ([_vp, C.c_int, C.c_int, C.c_int], C.c_int),
     "frehg_gw_get_pcg_shapes": ([_vp, _ip, _ip, _ip], C.c_int),
     "frehg_gw_set_step_mode": ([_vp, C.c_int, C.c_int, C.c_int], C.c_int),
+    "frehg_gw_get_column_tiles": ([_vp, _ip, _ip, _ip, C.c_int, _ip], C.c_int),
+    "frehg_gw_set_column_tiles": ([_vp, _ip, C.c_int], C.c_int),
+    "frehg_gw_enable_column_solve": ([_vp, C.c_int], C.c_int),
     "frehg_gw_get_graph_stats": ([_vp, C.POINTER(GraphStats)], C.c_int),
     "frehg_device_malloc": ([C.c_int64, C.POINTER(_vp)], C.c_int),
     "frehg_device_free": ([_vp], C.c_int),

@@ -34,7 +34,7 @@ COUPLING_NAMES = ("eta", "dept", "seep_accum", "avail", "gain")
 
 
 def _snapshot(g):
-    snap = {"h": g.get(abi.F_HEAD), "wc": g.get(abi.F_WC)}
+    snap = {"h": g.get(abi.F_HEAD), "wc": g.get(abi.F_WC), "vloss": g.get(abi.F_VLOSS)}
     if getattr(g, "_cpl", None) is not None:
         snap["cpl"] = {n: g.coupling_state(n) for n in COUPLING_NAMES}
     return snap
@@ -43,6 +43,7 @@ def _snapshot(g):
 def _restore(g, snap):
     g.set(abi.F_HEAD, snap["h"])
     g.set(abi.F_WC, snap["wc"])
+    g.set(abi.F_VLOSS, snap["vloss"])
     for n, v in snap.get("cpl", {}).items():
         g.set_coupling_state(n, v)
 
@@ -96,6 +97,53 @@ def tune(g, dt=None, repeats=2, candidates=None):
     g.t = t0
     g.synchronize()
     return {"shapes": best, "solve_ms": best_ms, "trials_ms": trials}
+
+
+COLUMN_TILES = (1, 2, 4, 8)
+
+
+def tune_columns(g, dt=None, repeats=3, candidates=COLUMN_TILES):
+    """Pick the tile (columns per wave) of each depth bucket of the batched
+    column solve (include/frehg_gw.h frehg_gw_set_column_tiles) of a
+    laterally decoupled solver ``g``: coordinate descent over the buckets,
+    deepest first, each trial one step of ``dt`` from the same saved state,
+    timed by the step's solve events (median of ``repeats``).  The tile
+    changes which wave solves which column, not the arithmetic: the
+    solution is bitwise the same for every tile.  Leaves the fastest tiles
+    set and the state restored (the dt controller's next value is the trial
+    step's); returns them with the trial times (ms)."""
+    buckets = g.column_tiles()
+    if not buckets:
+        raise ValueError("the solver has no column solve (lateral conductivity is nonzero)")
+    dt = g.next_dt() if dt is None else dt
+    t0 = g.t
+    snap = _snapshot(g)
+    best = [b["cpw"] for b in buckets]
+    trials = {}
+
+    def measure(tiles):
+        k = ",".join(map(str, tiles))
+        if k not in trials:
+            g.set_column_tiles(tiles)
+            trials[k] = _time_step(g, snap, dt, repeats)
+            g.t = t0
+        return trials[k]
+
+    best_ms = measure(best)
+    for b in range(len(best)):
+        for v in candidates:
+            if v > buckets[b]["columns"]:
+                continue
+            tiles = best[:b] + [v] + best[b + 1:]
+            ms = measure(tiles)
+            if ms < best_ms:
+                best, best_ms = tiles, ms
+    g.set_column_tiles(best)
+    _restore(g, snap)
+    g.t = t0
+    g.synchronize()
+    return {"tiles": [dict(b, cpw=c) for b, c in zip(buckets, best)], "solve_ms": best_ms,
+            "trials_ms": trials}
 
 
 def cache_key(g, device: str = "") -> str:

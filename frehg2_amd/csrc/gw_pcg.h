@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include <utility>
+#include <vector>
 
 #include "gw_device.h"
 
@@ -49,6 +50,17 @@ struct PcgOptions {
   bool timing = false;        // bracket SpMV launches with HIP events
   int timing_every = 1;       // ... one launch in timing_every
   bool stencil32_ready = false;  // mixed: the caller wrote the FP32 stencil (stencil32())
+};
+
+// Laterally decoupled systems (no x or y legs): the columns with active rows
+// in buckets of equal R = ceil(active rows / 64), deepest first, and the
+// launch tile of each bucket (columns per wave).
+struct ColumnTable {
+  int buckets = 0;
+  int R[8] = {};
+  int cpw[8] = {};
+  int colStart[9] = {};   // bucket b's columns: order[colStart[b] .. colStart[b + 1])
+  int waveStart[9] = {};  // ... and its waves
 };
 
 // Launches of a device-controlled solve (Pcg::enqueueDevice): `rounds` FP64
@@ -116,6 +128,18 @@ class Pcg {
   int resumeDevice(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
                    PcgStats* stats);
 
+  // Laterally decoupled operators: solve(), with the z-line preconditioner,
+  // becomes one direct solve of every column (k_column_solve), checked on
+  // its true residual (the PCG finishes a solve that misses the tolerance).
+  // kt: first active row of each column (device, 2D halo layout), ktHost
+  // the same on the host.  Tiles: columns per wave of each depth bucket.
+  void setColumnSolve(const int* kt, const std::vector<int>& ktHost);
+  bool columnSolve() const { return direct_; }
+  bool columnSolveAvailable() const { return !bucketR_.empty() || kt_ != nullptr; }
+  void enableColumnSolve(bool on) { direct_ = on && kt_ != nullptr; }
+  void setColumnTiles(const int* cpw, int n);
+  int columnTiles(int* R, int* counts, int* cpw, int cap) const;
+
   // The FP32 stencil of the mixed form (allocated on the first call).  A
   // caller that writes it as the rounded FP64 stencil sets
   // PcgOptions::stencil32_ready, and the solve skips its own cast.
@@ -151,6 +175,10 @@ class Pcg {
     float* pOld = nullptr;
     float* pNew = nullptr;
   };
+  int solveDirect(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                  PcgStats* st);
+  void launchColumnSolve(const Stencil<double>& A, const double* b, double* x);
+  void buildColumnTable();
   int fp64Loop(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
                PcgStats* st, double rr, double tol2, int total, int stalls);
   void mixedSetup(const Stencil<double>& A, const PcgOptions& opt);
@@ -212,6 +240,14 @@ class Pcg {
   void* state_ = nullptr;     // PcgState (device)
   void* stateHost_ = nullptr; // pinned mirror (external mode: the caller's host buffer)
   bool external_ = false;     // memory owned by the caller (PcgWorkspace)
+  // direct column solve
+  bool direct_ = false;
+  const int* kt_ = nullptr;
+  int* order_ = nullptr;          // columns, bucket by bucket (device)
+  std::vector<int> bucketCols_;   // column indices of each bucket, concatenated (host)
+  std::vector<int> bucketCount_, bucketR_, bucketCpw_;
+  ColumnTable tab_;
+  int colBlocks_ = 0;
 };
 
 }  // namespace frehg

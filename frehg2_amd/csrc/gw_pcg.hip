@@ -182,6 +182,25 @@ __device__ __forceinline__ void shift_rows(const T (&v)[R], int lane, T (&below)
   }
 }
 
+// The two recurrences of the column solve with LDL^T factors held in
+// registers (iv = 1 / delta, wv = Azp / delta; zero past row nz - 1).
+template <class T, int R>
+__device__ __forceinline__ void line_solve(int lane, int nz, const T (&iv)[R], const T (&wv)[R],
+                                           const T (&r)[R], T (&z)[R]) {
+  T wPrev[R], dummy[R];
+  shift_rows<T, R>(wv, lane, dummy, wPrev);
+  T y[R];
+  T carry = T(0);
+#pragma unroll
+  for (int m = 0; m < R; ++m) y[m] = scan_affine_up(r[m], -wPrev[m], carry, lane);
+  carry = T(0);
+#pragma unroll
+  for (int m = R - 1; m >= 0; --m) {
+    const int k = lane + kWave * m;
+    z[m] = scan_affine_down(k < nz ? y[m] * iv[m] : T(0), k < nz ? -wv[m] : T(0), carry, lane);
+  }
+}
+
 // z = M^-1 r for one column: point Jacobi (LINE = false) or the exact
 // tridiagonal solve of the column block (LINE = true) with the LDL^T factors
 // invd_k = 1 / delta_k and w_k = Azp_k / delta_k:
@@ -204,24 +223,14 @@ __device__ __forceinline__ void apply_precond(int lane, int nz, int64_t base, co
     for (int m = 0; m < R; ++m) z[m] = r[m] * iv[m];
     return;
   } else {
-    T wv[R], wPrev[R], dummy[R];
+    T wv[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const int k = lane + kWave * m;
       const T t = w[base + row_in(k, nz)];
       wv[m] = k < nz ? t : T(0);
     }
-    shift_rows<T, R>(wv, lane, dummy, wPrev);
-    T y[R];
-    T carry = T(0);
-#pragma unroll
-    for (int m = 0; m < R; ++m) y[m] = scan_affine_up(r[m], -wPrev[m], carry, lane);
-    carry = T(0);
-#pragma unroll
-    for (int m = R - 1; m >= 0; --m) {
-      const int k = lane + kWave * m;
-      z[m] = scan_affine_down(k < nz ? y[m] * iv[m] : T(0), k < nz ? -wv[m] : T(0), carry, lane);
-    }
+    line_solve<T, R>(lane, nz, iv, wv, r, z);
   }
 }
 
@@ -302,6 +311,163 @@ __global__ __launch_bounds__(64) void k_factor(Grid g, const T* __restrict__ Ad,
     }
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// Direct solve of laterally decoupled systems (no x or y legs: a batch of
+// independent tridiagonal columns).  One wave owns one column's active rows
+// k0 .. nz - 1 (row k0 + lane + 64 m), so every load is coalesced and the
+// column never leaves registers.
+//
+// Factor: the pivots delta_k = a_k - c_{k-1}^2 / delta_{k-1} (c_k = Azp_k,
+// the leg to row k + 1) are the ratios p_k / p_{k-1} of the continuants
+// p_k = a_k p_{k-1} - c_{k-1}^2 p_{k-2}, i.e. v_k = M_k v_{k-1} with
+// M_k = [[a_k, -c_{k-1}^2], [1, 0]] and v_{-1} = (1, 0).  The prefix products
+// M_k ... M_0 are an inclusive scan over the lanes of each row segment
+// (Hillis-Steele, six levels, 2 x 2 products), each product rescaled by a
+// power of two (exact) so it stays in range; the segment's last vector
+// carries into the next segment.  Then the LDL^T solve (line_solve), one
+// FP64 refinement x += M^-1 (b - A x) (the scan's pivots can lose digits to
+// cancellation on near-singular columns) and the residual partials.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void rescale2(double& a, double& b, double& c, double& d) {
+  const double mx = fmax(fmax(fabs(a), fabs(b)), fmax(fabs(c), fabs(d)));
+  if (!(mx > 0.0) || !isfinite(mx)) return;
+  int e = 0;
+  (void)frexp(mx, &e);
+  a = ldexp(a, -e);
+  b = ldexp(b, -e);
+  c = ldexp(c, -e);
+  d = ldexp(d, -e);
+}
+
+template <int R>
+__device__ __forceinline__ void continuant_factor(int lane, int L, const double (&a)[R],
+                                                  const double (&c)[R], double (&iv)[R],
+                                                  double (&wv)[R]) {
+  double cUp[R], dummy[R];
+  shift_rows<double, R>(c, lane, dummy, cUp);  // cUp[m]: the leg of row k - 1
+  double v0 = 1.0, v1 = 0.0;                   // v_{k0 - 1}
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    // rows past the column take the identity map (their pivots repeat the
+    // last one and are never used)
+    const bool in = lane + kWave * m < L;
+    double m00 = in ? a[m] : 1.0, m01 = in ? -cUp[m] * cUp[m] : 0.0;
+    double m10 = in ? 1.0 : 0.0, m11 = in ? 0.0 : 1.0;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const double p00 = __shfl_up(m00, off, kWave), p01 = __shfl_up(m01, off, kWave);
+      const double p10 = __shfl_up(m10, off, kWave), p11 = __shfl_up(m11, off, kWave);
+      if (lane >= off) {
+        double n00 = m00 * p00 + m01 * p10, n01 = m00 * p01 + m01 * p11;
+        double n10 = m10 * p00 + m11 * p10, n11 = m10 * p01 + m11 * p11;
+        rescale2(n00, n01, n10, n11);
+        m00 = n00;
+        m01 = n01;
+        m10 = n10;
+        m11 = n11;
+      }
+    }
+    const double u0 = m00 * v0 + m01 * v1, u1 = m10 * v0 + m11 * v1;
+    const double delta = u0 / u1;
+    iv[m] = 1.0 / delta;
+    wv[m] = c[m] * iv[m];
+    v0 = __shfl(u0, kWave - 1, kWave);
+    v1 = __shfl(u1, kWave - 1, kWave);
+    double z0 = 0.0, z1 = 0.0;
+    rescale2(v0, v1, z0, z1);
+  }
+}
+
+// b - A x on a column in registers (tridiagonal: a, and c to the next row).
+template <int R>
+__device__ __forceinline__ void column_residual(int lane, int L, const double (&a)[R],
+                                                const double (&c)[R], const double (&bv)[R],
+                                                const double (&x)[R], double (&r)[R]) {
+  double xDn[R], xUp[R], cDn[R], cUp[R];
+  shift_rows<double, R>(x, lane, xDn, xUp);
+  shift_rows<double, R>(c, lane, cDn, cUp);
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int kk = lane + kWave * m;
+    r[m] = kk < L ? bv[m] - (a[m] * x[m] + c[m] * xDn[m] + cUp[m] * xUp[m]) : 0.0;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void column_direct(const Grid& g, const Stencil<double>& A,
+                                              const double* __restrict__ b,
+                                              double* __restrict__ x, int64_t base, int k0,
+                                              int lane, KSum& rr, KSum& bb) {
+  const int L = g.nz - k0;
+  double a[R], c[R], bv[R];
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int kk = lane + kWave * m;
+    const int64_t cc = base + k0 + (kk < L ? kk : L - 1);
+    const double ta = A.Ad[cc], tc = A.Azp[cc], tb = b[cc];
+    a[m] = kk < L ? ta : 1.0;  // rows past the column: identity, decoupled
+    c[m] = kk < L - 1 ? tc : 0.0;
+    bv[m] = kk < L ? tb : 0.0;
+  }
+  double iv[R], wv[R], xv[R], r[R], d[R];
+  continuant_factor<R>(lane, L, a, c, iv, wv);
+  line_solve<double, R>(lane, L, iv, wv, bv, xv);
+  column_residual<R>(lane, L, a, c, bv, xv, r);
+  line_solve<double, R>(lane, L, iv, wv, r, d);
+#pragma unroll
+  for (int m = 0; m < R; ++m) xv[m] += d[m];
+  column_residual<R>(lane, L, a, c, bv, xv, r);
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int kk = lane + kWave * m;
+    if (kk < L) {
+      x[base + k0 + kk] = xv[m];
+      rr.add(r[m] * r[m]);
+      bb.add(bv[m] * bv[m]);
+    }
+  }
+}
+
+// One launch over every column with active rows, deepest bucket first:
+// wave w of bucket b solves columns order[colStart + (w - waveStart) * cpw
+// + q], q < cpw.  Partials rr (slot 1), bb (slot 3).
+__global__ __launch_bounds__(kBlock) void k_column_solve(Grid g, Stencil<double> A,
+                                                         const double* __restrict__ b,
+                                                         double* __restrict__ x,
+                                                         const int* __restrict__ kt,
+                                                         const int* __restrict__ order,
+                                                         ColumnTable tab,
+                                                         double* __restrict__ parts, int pstride) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wg = blockIdx.x * kWpb + threadIdx.x / kWave;
+  KSum rr, bb;
+  int bk = 0;
+  while (bk < tab.buckets && wg >= tab.waveStart[bk + 1]) ++bk;
+  if (bk < tab.buckets) {
+    const int first = tab.colStart[bk] + (wg - tab.waveStart[bk]) * tab.cpw[bk];
+    const int last = min(first + tab.cpw[bk], tab.colStart[bk + 1]);
+    for (int q = first; q < last; ++q) {
+      const int col = order[q];
+      const int j = col / g.nx + 1, i = col % g.nx + 1;
+      const int64_t base = g.c3(j, i, 0);
+      const int k0 = kt[g.c2(j, i)];
+      switch (tab.R[bk]) {
+        case 1: column_direct<1>(g, A, b, x, base, k0, lane, rr, bb); break;
+        case 2: column_direct<2>(g, A, b, x, base, k0, lane, rr, bb); break;
+        case 3: column_direct<3>(g, A, b, x, base, k0, lane, rr, bb); break;
+        case 4: column_direct<4>(g, A, b, x, base, k0, lane, rr, bb); break;
+        case 5: column_direct<5>(g, A, b, x, base, k0, lane, rr, bb); break;
+        case 6: column_direct<6>(g, A, b, x, base, k0, lane, rr, bb); break;
+        case 7: column_direct<7>(g, A, b, x, base, k0, lane, rr, bb); break;
+        default: column_direct<8>(g, A, b, x, base, k0, lane, rr, bb); break;
+      }
+    }
+  }
+  double v[2] = {rr.value(), bb.value()};
+  const int slots[2] = {1, 3};
+  write_parts<2>(parts, pstride, slots, v);
 }
 
 // r = b - A x, z = M^-1 r; partials rz (slot 0), rr (1), bb (3).  j-march.
@@ -1181,6 +1347,13 @@ __global__ __launch_bounds__(kBlock) void k_dc_end_mixed(const double* __restric
 
 __global__ void k_dc_finish(PcgState* st) { st->gate = st->converged ? 0 : 1; }
 
+// After the direct column solve: one solve with one refinement.
+__global__ void k_dc_direct(PcgState* st) {
+  st->total = 1;
+  st->refinements = 1;
+  st->maxInner = 1;
+}
+
 // Converged mixed solve whose iterate is in the second buffer: x = xb.
 __global__ __launch_bounds__(kBlock) void k_dc_take_x(Grid g, double* __restrict__ x,
                                                       const double* __restrict__ xb,
@@ -1326,7 +1499,7 @@ Pcg::~Pcg() {
   for (void* p : {(void*)r_, (void*)z_, (void*)p0_, (void*)p1_, (void*)q_, (void*)invd_, (void*)w_,
                   (void*)Ad32_, (void*)Axp32_, (void*)Ayp32_, (void*)Azp32_, (void*)r32_,
                   (void*)z32_, (void*)p032_, (void*)p132_, (void*)q32_, (void*)d32_,
-                  (void*)invd32_, (void*)w32_, (void*)parts_, (void*)red_, state_})
+                  (void*)invd32_, (void*)w32_, (void*)parts_, (void*)red_, state_, (void*)order_})
     if (p) (void)hipFree(p);
   if (redHost_) (void)hipHostFree(redHost_);
   if (stateHost_) (void)hipHostFree(stateHost_);
@@ -1354,7 +1527,7 @@ void Pcg::applyShapes() {
   blocks_ = sh.blocks;
   march64_ = sh.march64;
   march32_ = sh.march32;
-  pstride_ = sh.pstride;
+  pstride_ = std::max(sh.pstride, colBlocks_);
   if (parts_ && pstride_ > partsCap_ && external_)
     throw std::runtime_error("launch shapes beyond the external workspace's partial sums");
   if (parts_ && pstride_ > partsCap_) {
@@ -1395,7 +1568,105 @@ void Pcg::reduceHost(int nparts, int nvals, double* out) {
 int Pcg::solve(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
                PcgStats* st) {
   *st = PcgStats{};
+  if (direct_ && opt.zline) return solveDirect(A, b, x, opt, st);
   return opt.mixed ? solveMixed(A, b, x, opt, st) : solveFp64(A, b, x, opt, st);
+}
+
+// ---------------------------------------------------------------------------
+// Direct column solve (laterally decoupled operators).
+// ---------------------------------------------------------------------------
+void Pcg::setColumnSolve(const int* kt, const std::vector<int>& ktHost) {
+  if (external_) throw std::runtime_error("the column solve needs its own workspace");
+  kt_ = kt;
+  std::vector<std::vector<int>> byR(9);
+  for (int j = 1; j <= g_.ny; ++j)
+    for (int i = 1; i <= g_.nx; ++i) {
+      const int L = g_.nz - ktHost[static_cast<size_t>(g_.c2(j, i))];
+      if (L <= 0) continue;
+      byR[(L + kWave - 1) / kWave].push_back((j - 1) * g_.nx + (i - 1));
+    }
+  bucketCols_.clear();
+  bucketCount_.clear();
+  bucketR_.clear();
+  for (int R = 8; R >= 1; --R) {  // deepest first: the longest waves start first
+    if (byR[R].empty()) continue;
+    bucketR_.push_back(R);
+    bucketCount_.push_back(static_cast<int>(byR[R].size()));
+    bucketCols_.insert(bucketCols_.end(), byR[R].begin(), byR[R].end());
+  }
+  bucketCpw_.assign(bucketR_.size(), 1);
+  if (order_) HIP_OK(hipFree(order_));
+  order_ = dalloc<int>(std::max<int64_t>(1, static_cast<int64_t>(bucketCols_.size())));
+  if (!bucketCols_.empty())
+    HIP_OK(hipMemcpy(order_, bucketCols_.data(), bucketCols_.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  direct_ = true;
+  buildColumnTable();
+}
+
+void Pcg::buildColumnTable() {
+  tab_ = ColumnTable{};
+  tab_.buckets = static_cast<int>(bucketR_.size());
+  int col = 0, wave = 0;
+  for (int b = 0; b < tab_.buckets; ++b) {
+    tab_.R[b] = bucketR_[b];
+    tab_.cpw[b] = bucketCpw_[b];
+    tab_.colStart[b] = col;
+    tab_.waveStart[b] = wave;
+    col += bucketCount_[b];
+    wave += (bucketCount_[b] + bucketCpw_[b] - 1) / bucketCpw_[b];
+  }
+  tab_.colStart[tab_.buckets] = col;
+  tab_.waveStart[tab_.buckets] = wave;
+  colBlocks_ = std::max(1, (wave + kWpb - 1) / kWpb);
+  applyShapes();
+}
+
+void Pcg::setColumnTiles(const int* cpw, int n) {
+  if (!kt_) throw std::runtime_error("no column solve on this operator");
+  if (n != static_cast<int>(bucketR_.size()))
+    throw std::runtime_error("one tile per depth bucket is expected");
+  for (int b = 0; b < n; ++b)
+    if (cpw[b] < 1 || cpw[b] > 64) throw std::runtime_error("columns per wave must be in [1, 64]");
+  for (int b = 0; b < n; ++b) bucketCpw_[b] = cpw[b];
+  buildColumnTable();
+}
+
+int Pcg::columnTiles(int* R, int* counts, int* cpw, int cap) const {
+  const int n = static_cast<int>(bucketR_.size());
+  for (int b = 0; b < n && b < cap; ++b) {
+    if (R) R[b] = bucketR_[b];
+    if (counts) counts[b] = bucketCount_[b];
+    if (cpw) cpw[b] = bucketCpw_[b];
+  }
+  return n;
+}
+
+void Pcg::launchColumnSolve(const Stencil<double>& A, const double* b, double* x) {
+  k_column_solve<<<colBlocks_, kBlock, 0, stream_>>>(g_, A, b, x, kt_, order_, tab_, parts_,
+                                                    pstride_);
+  HIP_OK(hipGetLastError());
+}
+
+int Pcg::solveDirect(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
+                     PcgStats* st) {
+  launchColumnSolve(A, b, x);
+  double red[4];
+  reduceHost(colBlocks_, 4, red);
+  const double rr = red[1], bb = red[3];
+  const double tol = std::max(opt.rtol * std::sqrt(bb), opt.atol);
+  st->rhs_norm = std::sqrt(bb);
+  st->refinements = 1;
+  if (rr <= tol * tol) {
+    st->iterations = 1;
+    st->converged = 1;
+    st->residual_norm = std::sqrt(rr);
+    return FREHG_OK;
+  }
+  // the PCG finishes from the direct solution
+  launchFactor(A, opt.zline, false);
+  const double rr2 = residualHost(A, b, x, opt.zline).first;
+  return fp64Loop(A, b, x, opt, st, rr2, tol * tol, 1, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1801,7 +2072,7 @@ int Pcg::fp64Finish(const Stencil<double>& A, const double* b, double* x, const 
 // host mirror.  Capturable in a graph.
 // ---------------------------------------------------------------------------
 void Pcg::prepareDevice(const PcgOptions& opt) {
-  if (opt.mixed) (void)stencil32();  // the FP32 arrays exist before any capture
+  if (opt.mixed && !(direct_ && opt.zline)) (void)stencil32();  // allocated before any capture
 }
 
 void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
@@ -1810,7 +2081,13 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
     throw std::runtime_error("device PCG budget out of range");
   PcgState* dst = static_cast<PcgState*>(state_);
   const int B = bud.iterations;
-  if (!opt.mixed) {
+  if (direct_ && opt.zline) {
+    launchColumnSolve(A, b, x);
+    k_dc_begin64<<<1, kBlock, 0, stream_>>>(parts_, pstride_, colBlocks_, dst, opt.rtol, opt.atol,
+                                            opt.max_iterations);
+    k_dc_direct<<<1, 1, 0, stream_>>>(dst);
+    HIP_OK(hipGetLastError());
+  } else if (!opt.mixed) {
     launchFactor(A, opt.zline, false);
     launchResidual(A, b, x, opt.zline, nullptr);
     k_dc_begin64<<<1, kBlock, 0, stream_>>>(parts_, pstride_, march64_.blocks, dst, opt.rtol,
@@ -1897,6 +2174,16 @@ int Pcg::resumeDevice(const Stencil<double>& A, const double* b, double* x, cons
   *st = PcgStats{};
   st->rhs_norm = std::sqrt(h.bbTrue);
   st->refinements = h.refinements;
+  if (direct_ && opt.zline) {
+    if (h.stop && !h.converged) {
+      st->iterations = h.total;
+      st->residual_norm = std::sqrt(h.rrTrue);
+      return FREHG_EDIVERGED;
+    }
+    launchFactor(A, opt.zline, false);
+    const double rr = residualHost(A, b, x, opt.zline).first;
+    return fp64Loop(A, b, x, opt, st, rr, h.tolF2, h.total, 0);
+  }
   if (!opt.mixed) {
     if (h.stop && !h.converged) {
       st->iterations = h.total;

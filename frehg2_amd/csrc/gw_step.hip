@@ -2866,7 +2866,16 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   HIP_CHECK(hipStreamSynchronize(stream));
   if (icDev) (void)hipFree(icDev);
   pcg = std::make_unique<Pcg>(g, stream);
-  if (popt.mixed) {  // the assembly writes the FP32 stencil beside the FP64 one
+  // No lateral conductivity anywhere: every x and y leg of the operator is
+  // zero and the columns are independent tridiagonal systems, which the
+  // z-line form solves directly (one launch over all columns).
+  {
+    const size_t nIn = static_cast<size_t>(nx) * ny * nz;
+    bool lateral = false;
+    for (size_t q = 0; q < nIn && !lateral; ++q) lateral = ds.ksx[q] != 0.0 || ds.ksy[q] != 0.0;
+    if (!lateral && popt.zline) pcg->setColumnSolve(d.kt, ktop);
+  }
+  if (popt.mixed && !pcg->columnSolve()) {  // the assembly writes the FP32 stencil beside the FP64 one
     const Stencil<float> a32 = pcg->stencil32();
     d.Ad32 = a32.Ad;
     d.Axp32 = a32.Axp;
@@ -3395,6 +3404,37 @@ int frehg_gw_set_pcg_shapes(frehg_gw_solver* s, int march64_waves_per_cu,
     GwSolver* g = frehg::S(s);
     g->invalidateGraphs();
     g->pcg->setShapes(march64_waves_per_cu, march32_waves_per_cu, column_waves_per_cu);
+  });
+}
+int frehg_gw_get_column_tiles(const frehg_gw_solver* s, int* rows_per_lane, int* columns,
+                              int* columns_per_wave, int capacity, int* buckets) {
+  return guard([&] {
+    if (!buckets) frehg::fail(FREHG_EINVAL, "frehg_gw_get_column_tiles: buckets is required");
+    *buckets = frehg::S(s)->pcg->columnTiles(rows_per_lane, columns, columns_per_wave,
+                                             capacity < 0 ? 0 : capacity);
+  });
+}
+int frehg_gw_enable_column_solve(frehg_gw_solver* s, int enable) {
+  return guard([&] {
+    GwSolver* g = frehg::S(s);
+    if (enable && !g->pcg->columnSolveAvailable())
+      frehg::fail(FREHG_EUNSUPPORTED, "the operator has lateral legs: no column solve");
+    g->pcg->enableColumnSolve(enable != 0);
+    g->invalidateGraphs();
+  });
+}
+int frehg_gw_set_column_tiles(frehg_gw_solver* s, const int* columns_per_wave, int buckets) {
+  return guard([&] {
+    GwSolver* g = frehg::S(s);
+    if (!columns_per_wave) frehg::fail(FREHG_EINVAL, "frehg_gw_set_column_tiles: tiles required");
+    if (!g->pcg->columnSolveAvailable())
+      frehg::fail(FREHG_EUNSUPPORTED, "the operator has lateral legs: no column solve");
+    try {
+      g->pcg->setColumnTiles(columns_per_wave, buckets);
+    } catch (const std::runtime_error& e) {
+      frehg::fail(FREHG_EINVAL, e.what());
+    }
+    g->invalidateGraphs();
   });
 }
 int frehg_gw_set_step_mode(frehg_gw_solver* s, int mode, int rounds, int iterations) {

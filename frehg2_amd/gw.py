@@ -204,6 +204,22 @@ class RichardsSolver:
         m = modes[mode] if isinstance(mode, str) else int(mode)
         check(self.lib, self.lib.frehg_gw_set_step_mode(self._h, m, int(rounds), int(iterations)))
 
+    def column_tiles(self) -> list:
+        """Depth buckets of the batched column solve: [{R, columns, cpw}]
+        (empty when the operator has lateral legs)."""
+        cap = 8
+        R, n, cpw = (C.c_int * cap)(), (C.c_int * cap)(), (C.c_int * cap)()
+        nb = C.c_int()
+        check(self.lib, self.lib.frehg_gw_get_column_tiles(self._h, R, n, cpw, cap, C.byref(nb)))
+        return [{"R": R[b], "columns": n[b], "cpw": cpw[b]} for b in range(nb.value)]
+
+    def enable_column_solve(self, enable=True) -> None:
+        check(self.lib, self.lib.frehg_gw_enable_column_solve(self._h, int(bool(enable))))
+
+    def set_column_tiles(self, cpw) -> None:
+        arr = (C.c_int * len(cpw))(*[int(v) for v in cpw])
+        check(self.lib, self.lib.frehg_gw_set_column_tiles(self._h, arr, len(cpw)))
+
     def graph_stats(self) -> dict:
         t = abi.GraphStats()
         check(self.lib, self.lib.frehg_gw_get_graph_stats(self._h, C.byref(t)))

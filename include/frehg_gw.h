@@ -334,6 +334,25 @@ typedef struct frehg_gw_graph_stats {
 int frehg_gw_set_step_mode(frehg_gw_solver* s, int mode, int rounds, int iterations);
 int frehg_gw_get_graph_stats(const frehg_gw_solver* s, frehg_gw_graph_stats* out);
 
+/* Batched column solve.  When no cell has lateral conductivity (ksx = ksy
+ * = 0 everywhere) the operator is a batch of independent tridiagonal
+ * columns, and with the z-line preconditioner the step solves every column
+ * directly in one launch (one wave a column, active rows only, in buckets of
+ * equal depth R = ceil(active rows / 64), deepest first), with one FP64
+ * refinement; the solve is checked on its true residual against the PCG
+ * tolerance and the PCG finishes any solve that misses it.  The reference
+ * runs the same systems through KSPCG (src/core/LinearSystem.cpp:320).
+ * Tiles: columns per wave of each bucket, in [1, 64]; the tuner
+ * frehg2_amd.autotune.tune_columns picks them by timing steps.
+ * get: the bucket count (0: no column solve) and, up to capacity, each
+ * bucket's R, column count and tile. */
+int frehg_gw_get_column_tiles(const frehg_gw_solver* s, int* rows_per_lane, int* columns,
+                              int* columns_per_wave, int capacity, int* buckets);
+int frehg_gw_set_column_tiles(frehg_gw_solver* s, const int* columns_per_wave, int buckets);
+/* 0: solve laterally decoupled systems with the z-line PCG instead (1, the
+ * default, restores the column solve; EUNSUPPORTED where there is none). */
+int frehg_gw_enable_column_solve(frehg_gw_solver* s, int enable);
+
 /* Launch shapes of the PCG (no counterpart in the reference, where PETSc
  * makes these choices): waves per CU of the FP64 and FP32 stencil kernels
  * and of the column kernels, each in [1, 256]; 0 keeps a value.  Defaults
