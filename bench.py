@@ -77,6 +77,10 @@ def parse():
     ap.add_argument("--member-shape", default="128,128,64")
     ap.add_argument("--columns", action="store_true",
                     help="run the uneven-columns workload instead (4096 columns of 50-400 cells)")
+    ap.add_argument("--pc-sweep", action="store_true",
+                    help="mixed form's inner preconditioners (zline, jacobi, chebyshev 3/5/7) on "
+                         "the stiff 32x32x50 and the 512x512x100 workloads: time, iterations "
+                         "and error against the FP64 oracle")
     ap.add_argument("--ens-rtol", type=float, default=0.0,
                     help="ensemble: PCG rtol of every leg (0: the workload's 1e-10)")
     return ap.parse_args()
@@ -586,6 +590,74 @@ def columns_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def pc_sweep(args):
+    """The mixed form's inner preconditioners on the stiff 32x32x50 workload
+    (adaptive dt, 20 steps) and on the 512x512x100 one (dt 300 s, 5 steps):
+    ms per step, PCG iterations per solve, and head / theta error of one
+    more step against the FP64 oracle's step from the same state."""
+    import dataclasses
+    from frehg2_amd import abi, cases
+    from frehg2_amd.gw import RichardsSolver
+    from oracle import oracle as orc
+    from oracle.oracle import OracleRichards
+    orc.set_threads(min(os.cpu_count() or 1, 16))
+    variants = [("zline", 0), ("jacobi", 0), ("chebyshev", 3), ("chebyshev", 5), ("chebyshev", 7)]
+
+    def nerr(a, b):
+        return float(np.max(np.abs(a - b))) / float(np.max(np.abs(b)))
+
+    legs = {}
+    for label, make, steps, dt_fixed, rtol in (
+            ("richards_stiff_32x32x50", lambda: cases.stiff_32x32x50(precision="mixed"), 20,
+             None, 1e-10),
+            ("richards_hetero_512x512x100", lambda: cases.hetero_512x512x100(precision="mixed"),
+             5, 300.0, args.rtol_mixed)):
+        base = make()
+        for pc, deg in variants:
+            key = f"{label}:{pc}{deg or ''}"
+            c = dataclasses.replace(base)
+            c.preconditioner, c.precision, c.rtol = pc, "mixed", rtol
+            c.inner_rtol = args.inner_rtol
+            c.max_iterations = 10000
+            g = RichardsSolver(c)
+            if pc == "chebyshev":
+                g.set_chebyshev(deg, 0.05)
+            for _ in range(2):
+                g.step(g.next_dt() if c.adaptive else dt_fixed)
+            g.synchronize()
+            dts, its = [], 0
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                dts.append(g.next_dt() if c.adaptive else dt_fixed)
+                g.step(dts[-1])
+                its += g.last_solve()["iterations"]
+            g.synchronize()
+            ms = 1e3 * (time.perf_counter() - t0) / steps
+            co = dataclasses.replace(base)
+            co.preconditioner, co.precision, co.rtol = "zline", "fp64", 1e-13
+            o = OracleRichards(co)
+            o.set(abi.F_HEAD, g.get(abi.F_HEAD))
+            o.set(abi.F_WC, g.get(abi.F_WC))
+            o.step(dts[-1])
+            g.step(dts[-1])
+            legs[key] = {"ms_per_step": ms, "iterations_per_solve": its / steps,
+                         "head_err_vs_oracle": nerr(g.get(abi.F_HEAD)[1:-1, 1:-1],
+                                                    o.get(abi.F_HEAD)[1:-1, 1:-1]),
+                         "theta_err_vs_oracle": nerr(g.get(abi.F_WC)[1:-1, 1:-1],
+                                                     o.get(abi.F_WC)[1:-1, 1:-1])}
+            log(f"{key}: {ms:.3f} ms/step, {its / steps:.1f} it/solve, head err "
+                f"{legs[key]['head_err_vs_oracle']:.1e}")
+            g.close()
+    out = {"metric": "mixed-form inner preconditioners: ms per step", "value":
+           legs["richards_hetero_512x512x100:zline"]["ms_per_step"], "unit": "ms/step",
+           "n_gpus": 1, "higher_is_better": False, "dtype": "f32+f64",
+           "data": "synthetic (seeded generator)",
+           "config": {"workload": "richards_stiff_32x32x50 and richards_hetero_512x512x100",
+                      "chebyshev_interval": "[0.1, 2] of D^-1 A"},
+           "legs": legs}
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic():
     p = ROOT / "profiles" / "traffic_pcg_dir_hetero_512x512x100.json"
     if p.exists():
@@ -608,6 +680,9 @@ def main():
         return
     if args.columns:
         columns_bench(args)
+        return
+    if args.pc_sweep:
+        pc_sweep(args)
         return
     # the achievable streaming rate on this device, measured live: the best
     # copy and 6-read/2-write FP64 stream over access width, cache policy and

@@ -23,7 +23,7 @@ FREHG_EUNSUPPORTED = 4
 
 BC_NOFLUX, BC_HEAD, BC_FLUX, BC_GRAVITY, BC_HEAD_HYDROSTATIC = 0, 1, 2, 3, 4
 IC_MOISTURE, IC_HEAD, IC_WATER_TABLE = 0, 1, 2
-PC_JACOBI, PC_ZLINE, PC_ICC0 = 0, 1, 2
+PC_JACOBI, PC_ZLINE, PC_ICC0, PC_CHEBYSHEV = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
 STEP_HOST, STEP_DEVICE, STEP_GRAPH = 0, 1, 2
 
@@ -129,6 +129,7 @@ SYMBOLS = {
     "frehg_gw_get_column_tiles": ([_vp, _ip, _ip, _ip, C.c_int, _ip], C.c_int),
     "frehg_gw_set_column_tiles": ([_vp, _ip, C.c_int], C.c_int),
     "frehg_gw_enable_column_solve": ([_vp, C.c_int], C.c_int),
+    "frehg_gw_set_chebyshev": ([_vp, C.c_int, C.c_double], C.c_int),
     "frehg_gw_get_graph_stats": ([_vp, C.POINTER(GraphStats)], C.c_int),
     "frehg_device_malloc": ([C.c_int64, C.POINTER(_vp)], C.c_int),
     "frehg_device_free": ([_vp], C.c_int),

@@ -175,7 +175,8 @@ class GwCase:
                 setattr(d, f"{which}_value", dptr(vals))
         d.rtol, d.atol, d.max_iterations = self.rtol, self.atol, self.max_iterations
         d.preconditioner = {"jacobi": abi.PC_JACOBI, "zline": abi.PC_ZLINE,
-                            "icc0": abi.PC_ICC0}[self.preconditioner]
+                            "icc0": abi.PC_ICC0,
+                            "chebyshev": abi.PC_CHEBYSHEV}[self.preconditioner]
         d.precision = abi.PREC_MIXED if self.precision == "mixed" else abi.PREC_FP64
         d.initial_guess_previous = int(self.initial_guess_previous)
         d.inner_rtol = self.inner_rtol

@@ -242,16 +242,17 @@ def load_config(src, base_dir=None):
     case.max_iterations = int(sg.get("max_iterations", 1000))
     pc = str(sg.get("preconditioner", "bjacobi-icc"))
     # The reference's preconditioners (LinearSystem.cpp:158-221) run on
-    # PETSc/hypre; this path's PCG offers point Jacobi and the z-line block
-    # Jacobi.  bjacobi-icc (the reference default), amg and gamg map to
-    # z-line: the remap is recorded on the case and announced.
+    # PETSc/hypre; this path's PCG offers point Jacobi, the z-line block
+    # Jacobi and (mixed form, extension key) a Jacobi-Chebyshev polynomial.
+    # bjacobi-icc (the reference default), amg and gamg map to z-line: the
+    # remap is recorded on the case and announced.
     if pc in PC_REMAP:
         case.preconditioner = PC_REMAP[pc]
         case.preconditioner_requested = pc
         if pc != "bjacobi-icc":
             warnings.warn(f"solver.groundwater.preconditioner: {pc} runs as the z-line block "
                           f"Jacobi PCG on this path", stacklevel=2)
-    elif pc in ("zline", "jacobi"):
+    elif pc in ("zline", "jacobi", "chebyshev"):
         case.preconditioner = pc
     else:
         raise ValueError(f"solver.groundwater.preconditioner: unknown value {pc!r}")

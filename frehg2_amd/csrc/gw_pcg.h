@@ -45,6 +45,11 @@ struct PcgOptions {
   int max_iterations = 1000;
   bool zline = true;          // z-line block Jacobi, else point Jacobi
   bool mixed = false;         // FP32 inner PCG inside FP64 refinement
+  // mixed form: the FP32 inner PCG preconditioned by a Jacobi-Chebyshev
+  // polynomial of odd degree cheb_degree on [cheb_lower * 2, 2] (zline off)
+  bool cheb = false;
+  int cheb_degree = 3;
+  double cheb_lower = 0.05;
   double inner_rtol = 1e-4;   // FP32 residual reduction between reliable updates
   int max_refinements = 40;
   bool timing = false;        // bracket SpMV launches with HIP events
@@ -202,6 +207,7 @@ class Pcg {
   void launchRefine(const Stencil<double>& A, const double* b, double* xa, double* xb, double sc,
                     bool hasD, const void* ctl);
   void launchInnerInit(bool zline, double s, const void* ctl);
+  void launchChebSteps(const PcgOptions& opt, const int* skip);
   template <class T>
   int innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T* z, T* q,
                       const T* invd, const T* w, bool zline, double tol2, int maxIt,
@@ -234,6 +240,7 @@ class Pcg {
   float *Ad32_ = nullptr, *Axp32_ = nullptr, *Ayp32_ = nullptr, *Azp32_ = nullptr;
   float *r32_ = nullptr, *z32_ = nullptr, *p032_ = nullptr, *p132_ = nullptr, *q32_ = nullptr;
   float *d32_ = nullptr, *invd32_ = nullptr, *w32_ = nullptr;
+  float* chebD32_ = nullptr;  // the Chebyshev recurrence's d (allocated on first use)
   double* parts_ = nullptr;   // 4 x blocks_ partial sums
   double* red_ = nullptr;     // reduced values (device)
   double* redHost_ = nullptr; // pinned mirror

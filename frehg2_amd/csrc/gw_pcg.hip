@@ -1022,6 +1022,123 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p(Grid g, March mj, Stencil<
   write_parts<1>(parts, pstride, slots, v);
 }
 
+// ---------------------------------------------------------------------------
+// Jacobi-Chebyshev preconditioner (FP32, pair layout): z = p_m(D^-1 A) D^-1 r,
+// p_m the degree-m polynomial of the Chebyshev semi-iteration for D^-1 A on
+// [a, b] (Saad, Iterative Methods for Sparse Linear Systems, 2nd ed.,
+// Algorithm 12.1), theta = (b + a) / 2, delta = (b - a) / 2,
+// sigma = theta / delta, rho_0 = 1 / sigma:
+//   z_1 = d_0 = D^-1 r / theta
+//   rho_k = 1 / (2 sigma - rho_{k-1})
+//   d_k = rho_k rho_{k-1} d_{k-1} + (2 rho_k / delta) D^-1 (r - A z_k)
+//   z_{k+1} = z_k + d_k,          k = 1 .. m - 1.
+// D^-1 A of a diagonally dominant matrix has its spectrum in (0, 2], so
+// b = 2.  The residual polynomial q_m(l) = 1 - l p_m(l) = T_m((b + a - 2 l) /
+// (b - a)) / T_m(sigma) is below 1 on (0, b] (it falls from q_m(0) = 1 and
+// stays within 1 / T_m(sigma) on [a, b]) and, for odd m, negative beyond b;
+// so p_m > 0 on the whole positive axis and the preconditioner is SPD for
+// any spectrum.  The update kernel (point Jacobi)
+// leaves D^-1 r in z; step k reads z_k (FIRST: D^-1 r scaled by 1 / theta,
+// d_{k-1} likewise) and writes d_k and z_{k+1}; the last step leaves the
+// partial sums r.z and r.r for the next direction kernel.  j-march.
+// ---------------------------------------------------------------------------
+template <int R2, bool FIRST, bool LAST>
+__global__ __launch_bounds__(kBlock) void k_cheb_step_p(Grid g, March mj, Stencil<float> A,
+                                                        const float* __restrict__ zin,
+                                                        float* __restrict__ zout,
+                                                        float* __restrict__ d,
+                                                        const float* __restrict__ r,
+                                                        const float* __restrict__ invd,
+                                                        float invTheta, float c1, float c2,
+                                                        double* __restrict__ parts, int pstride,
+                                                        const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nz = g.nz;
+  const int64_t sx = nz, sy = static_cast<int64_t>(g.nx2) * nz;
+  KSum rz, rr;
+  int i, j0, j1;
+  if (march_span(g, mj, i, j0, j1)) {
+    auto formz = [&](int64_t base, float (&z0)[R2], float (&z1)[R2]) {
+      load_pair<R2>(zin, base, lane, nz, z0, z1);
+      if (FIRST) {
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          z0[m] *= invTheta;
+          z1[m] *= invTheta;
+        }
+      }
+    };
+    float zP0[R2], zP1[R2], zC0[R2], zC1[R2], ayP0[R2], ayP1[R2];
+    formz(g.c3(j0 - 1, i, 0), zP0, zP1);
+    formz(g.c3(j0, i, 0), zC0, zC1);
+    load_pair<R2>(A.Ayp, g.c3(j0 - 1, i, 0), lane, nz, ayP0, ayP1);
+    for (int j = j0; j <= j1; ++j) {
+      const int64_t base = g.c3(j, i, 0);
+      float zN0[R2], zN1[R2], zx0[R2], zx1[R2], zm0[R2], zm1[R2];
+      formz(base + sy, zN0, zN1);
+      formz(base + sx, zx0, zx1);
+      formz(base - sx, zm0, zm1);
+      float ad0[R2], ad1[R2], ax0[R2], ax1[R2], am0[R2], am1[R2], ay0[R2], ay1[R2], az0[R2],
+          az1[R2];
+      load_pair<R2>(A.Ad, base, lane, nz, ad0, ad1);
+      load_pair<R2>(A.Axp, base, lane, nz, ax0, ax1);
+      load_pair<R2>(A.Axp, base - sx, lane, nz, am0, am1);
+      load_pair<R2>(A.Ayp, base, lane, nz, ay0, ay1);
+      load_pair<R2>(A.Azp, base, lane, nz, az0, az1);
+      float zDn[R2], zUp[R2], azDn[R2], azUp[R2];
+      pair_shift<R2>(zC0, zC1, lane, zDn, zUp);
+      pair_shift<R2>(az0, az1, lane, azDn, azUp);
+      float r0[R2], r1[R2], iv0[R2], iv1[R2], d0[R2], d1[R2];
+      load_pair<R2>(r, base, lane, nz, r0, r1);
+      load_pair<R2>(invd, base, lane, nz, iv0, iv1);
+      if (FIRST) {
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          d0[m] = zC0[m];
+          d1[m] = zC1[m];
+        }
+      } else {
+        load_pair<R2>(d, base, lane, nz, d0, d1);
+      }
+      float n0[R2], n1[R2];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        const float q0 = ad0[m] * zC0[m] + ax0[m] * zx0[m] + am0[m] * zm0[m] + ay0[m] * zN0[m] +
+                         ayP0[m] * zP0[m] + az0[m] * zC1[m] + azUp[m] * zUp[m];
+        const float q1 = ad1[m] * zC1[m] + ax1[m] * zx1[m] + am1[m] * zm1[m] + ay1[m] * zN1[m] +
+                         ayP1[m] * zP1[m] + az1[m] * zDn[m] + az0[m] * zC0[m];
+        d0[m] = c1 * d0[m] + c2 * (iv0[m] * (r0[m] - q0));
+        d1[m] = c1 * d1[m] + c2 * (iv1[m] * (r1[m] - q1));
+        n0[m] = zC0[m] + d0[m];
+        n1[m] = zC1[m] + d1[m];
+        if (LAST && 128 * m + 2 * lane < nz) {
+          rz.add(static_cast<double>(r0[m]) * static_cast<double>(n0[m]));
+          rz.add(static_cast<double>(r1[m]) * static_cast<double>(n1[m]));
+          rr.add(static_cast<double>(r0[m]) * static_cast<double>(r0[m]));
+          rr.add(static_cast<double>(r1[m]) * static_cast<double>(r1[m]));
+        }
+      }
+      if (!LAST) store_pair<R2>(d, base, lane, nz, d0, d1);
+      store_pair<R2>(zout, base, lane, nz, n0, n1);
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        zP0[m] = zC0[m];
+        zP1[m] = zC1[m];
+        zC0[m] = zN0[m];
+        zC1[m] = zN1[m];
+        ayP0[m] = ay0[m];
+        ayP1[m] = ay1[m];
+      }
+    }
+  }
+  if (LAST) {
+    double v[2] = {rz.value(), rr.value()};
+    const int slots[2] = {0, 1};
+    write_parts<2>(parts, pstride, slots, v);
+  }
+}
+
 // k_pcg_update for FP32 operands in the pair layout (x deferred the same way).
 template <int R2, bool LINE, int XM>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, float* __restrict__ x,
@@ -1499,7 +1616,7 @@ Pcg::~Pcg() {
   for (void* p : {(void*)r_, (void*)z_, (void*)p0_, (void*)p1_, (void*)q_, (void*)invd_, (void*)w_,
                   (void*)Ad32_, (void*)Axp32_, (void*)Ayp32_, (void*)Azp32_, (void*)r32_,
                   (void*)z32_, (void*)p032_, (void*)p132_, (void*)q32_, (void*)d32_,
-                  (void*)invd32_, (void*)w32_, (void*)parts_, (void*)red_, state_, (void*)order_})
+                  (void*)invd32_, (void*)w32_, (void*)parts_, (void*)red_, state_, (void*)order_, (void*)chebD32_})
     if (p) (void)hipFree(p);
   if (redHost_) (void)hipHostFree(redHost_);
   if (stateHost_) (void)hipHostFree(stateHost_);
@@ -1816,6 +1933,36 @@ void Pcg::launchInnerInit(bool zline, double s, const void* ctl) {
   HIP_OK(hipGetLastError());
 }
 
+void Pcg::launchChebSteps(const PcgOptions& opt, const int* skip) {
+  const double b = 2.0, a = opt.cheb_lower * b;
+  const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+  const Stencil<float> A32{Ad32_, Axp32_, Ayp32_, Azp32_};
+  const March& mj = march32_;
+  double rhoPrev = 1.0 / sigma;
+  float* zin = z32_;
+  float* zout = q32_;
+  for (int k = 1; k < opt.cheb_degree; ++k) {
+    const double rho = 1.0 / (2.0 * sigma - rhoPrev);
+    const float c1 = static_cast<float>(rho * rhoPrev), c2 = static_cast<float>(2.0 * rho / delta);
+    const float invTheta = static_cast<float>(1.0 / theta);
+    const bool first = k == 1, last = k + 1 == opt.cheb_degree;
+    dispatch_R2(g_.nz, [&](auto RC) {
+      constexpr int R2 = decltype(RC)::value;
+      auto go = [&](auto kern) {
+        kern<<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A32, zin, zout, chebD32_, r32_, invd32_,
+                                                invTheta, c1, c2, parts_, pstride_, skip);
+      };
+      if (first && last) go(k_cheb_step_p<R2, true, true>);
+      else if (first) go(k_cheb_step_p<R2, true, false>);
+      else if (last) go(k_cheb_step_p<R2, false, true>);
+      else go(k_cheb_step_p<R2, false, false>);
+    });
+    HIP_OK(hipGetLastError());
+    std::swap(zin, zout);
+    rhoPrev = rho;
+  }
+}
+
 template <class T>
 int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T* z, T* q,
                          const T* invd, const T* w, bool zline, double tol2, int maxIt,
@@ -1846,11 +1993,13 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
       // through each group (the first launch of a solve is the restart form)
       sampled[c] = opt.timing && (n - n0) % every == every / 2;
       if (sampled[c]) HIP_OK(hipEventRecord(evs[2 * c], stream_));
-      const int np = n == n0 ? nprev0 : blocks_;
+      const bool cheb = std::is_same<T, float>::value && opt.cheb;
+      const int np = n == n0 ? nprev0 : (cheb ? mj.blocks : blocks_);
       launchDir<T>(A, z, pOld, pNew, q, np, n, n == n0 && restart, mj);
       if (sampled[c]) HIP_OK(hipEventRecord(evs[2 * c + 1], stream_));
       const bool two = ((n - n0) & 1) != 0;  // x takes this and the previous term
       launchUpdate<T>(x, pOld, pNew, r, q, z, invd, w, zline, two, n, mj);
+      if (cheb) launchChebSteps(opt, &dst->done);
       std::swap(pOld, pNew);
     }
     HIP_OK(hipMemcpyAsync(hst, dst, sizeof(PcgState), hipMemcpyDeviceToHost, stream_));
@@ -1970,6 +2119,13 @@ std::pair<double, double> Pcg::refineHost(const Stencil<double>& A, const double
 
 void Pcg::mixedSetup(const Stencil<double>& A, const PcgOptions& opt) {
   const Stencil<float> A32 = stencil32();
+  if (opt.cheb) {
+    if (!pairs_) throw std::runtime_error("the Chebyshev preconditioner needs an even nz");
+    if (opt.zline || opt.cheb_degree < 3 || opt.cheb_degree > 15 || !(opt.cheb_degree & 1) ||
+        !(opt.cheb_lower > 0.0 && opt.cheb_lower < 1.0))
+      throw std::runtime_error("Chebyshev: odd degree in [3, 15], lower ratio in (0, 1)");
+    if (!chebD32_) chebD32_ = dalloc<float>(g_.cellsH());
+  }
   if (!opt.stencil32_ready) {
     k_cast_stencil<<<blocks_ * 4, kBlock, 0, stream_>>>(g_, A, A32);
     HIP_OK(hipGetLastError());
@@ -2015,13 +2171,14 @@ int Pcg::mixedLoop(const Stencil<double>& A, const double* b, double* x, const P
         c.total >= opt.max_iterations)
       break;
     launchInnerInit(opt.zline, c.s, nullptr);
+    if (opt.cheb) launchChebSteps(opt, nullptr);
     const double rrS = c.rr * c.s * c.s;
     const double tolIn2 =
         std::max(opt.inner_rtol * opt.inner_rtol * rrS, 0.25 * c.tol2 * c.s * c.s);
     int its = 0;
     innerIterations<float>(A32, d32_, c.pOld, c.pNew, r32_, z32_, q32_, invd32_, w32_, opt.zline,
                            tolIn2, std::min(opt.max_iterations - c.total, 400), march32_,
-                           blocks_, c.n0, c.restart, opt, st, &its);
+                           opt.cheb ? march32_.blocks : blocks_, c.n0, c.restart, opt, st, &its);
     c.total += its;
     c.n0 += its;
     c.restart = false;
@@ -2053,6 +2210,7 @@ int Pcg::fp64Finish(const Stencil<double>& A, const double* b, double* x, const 
                           hipMemcpyDeviceToDevice, stream_));
   PcgOptions o64 = opt;
   o64.mixed = false;
+  o64.cheb = false;  // FP64 finish: point Jacobi in place of the FP32 polynomial
   o64.max_iterations = std::max(1, opt.max_iterations - total);
   PcgStats s64;
   const int rc = solveFp64(A, b, x, o64, &s64);
@@ -2072,7 +2230,10 @@ int Pcg::fp64Finish(const Stencil<double>& A, const double* b, double* x, const 
 // host mirror.  Capturable in a graph.
 // ---------------------------------------------------------------------------
 void Pcg::prepareDevice(const PcgOptions& opt) {
-  if (opt.mixed && !(direct_ && opt.zline)) (void)stencil32();  // allocated before any capture
+  if (opt.mixed && !(direct_ && opt.zline)) {  // allocated before any capture
+    (void)stencil32();
+    if (opt.cheb && !chebD32_) chebD32_ = dalloc<float>(g_.cellsH());
+  }
 }
 
 void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
@@ -2125,12 +2286,15 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
       k_dc_cycle<<<1, 1, 0, stream_>>>(dst, opt.max_iterations, opt.inner_rtol);
       HIP_OK(hipGetLastError());
       launchInnerInit(opt.zline, 0.0, dst);
+      if (opt.cheb) launchChebSteps(opt, &dst->skip);
       float* pOld = p032_;
       float* pNew = p132_;
+      const int np = opt.cheb ? march32_.blocks : blocks_;
       for (int n = 0; n < B; ++n) {
-        launchDir<float>(A32, z32_, pOld, pNew, q32_, blocks_, n, cycle == 0 && n == 0, march32_);
+        launchDir<float>(A32, z32_, pOld, pNew, q32_, np, n, cycle == 0 && n == 0, march32_);
         launchUpdate<float>(d32_, pOld, pNew, r32_, q32_, z32_, invd32_, w32_, opt.zline,
                             (n & 1) != 0, n, march32_);
+        if (opt.cheb) launchChebSteps(opt, &dst->done);
         std::swap(pOld, pNew);
       }
       k_dc_flush<float, true><<<blocks_ * 4, kBlock, 0, stream_>>>(g_, d32_, p132_, p032_, dst,

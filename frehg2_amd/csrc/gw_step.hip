@@ -2600,6 +2600,8 @@ void GwSolver::create(const frehg_gw_desc& ds) {
     fail(FREHG_EUNSUPPORTED, "saturated_contact: legacy is order-dependent; head only");
   if (ds.preconditioner == FREHG_PC_ICC0)
     fail(FREHG_EUNSUPPORTED, "icc0 is a CPU-only preconditioner; use zline or jacobi");
+  if (ds.preconditioner < FREHG_PC_JACOBI || ds.preconditioner > FREHG_PC_CHEBYSHEV)
+    fail(FREHG_EINVAL, "unknown preconditioner");
   if (!ds.ksx || !ds.ksy || !ds.ksz || !ds.vg_alpha || !ds.vg_n || !ds.theta_s || !ds.theta_r ||
       !ds.aev)
     fail(FREHG_EINVAL, "all eight per-cell soil arrays are required");
@@ -2848,6 +2850,9 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   popt.max_iterations = ds.max_iterations > 0 ? ds.max_iterations : 1000;
   popt.zline = ds.preconditioner == FREHG_PC_ZLINE;
   popt.mixed = ds.precision == FREHG_PREC_MIXED;
+  popt.cheb = ds.preconditioner == FREHG_PC_CHEBYSHEV;
+  if (popt.cheb && (!popt.mixed || nz % 2 != 0))
+    fail(FREHG_EINVAL, "the Chebyshev preconditioner is the mixed form's (FP32) and needs an even nz");
   popt.inner_rtol = ds.inner_rtol > 0 ? ds.inner_rtol : 1e-4;
 
   // initial conditions (RichardsSolver.cpp:331-427)
@@ -3412,6 +3417,20 @@ int frehg_gw_get_column_tiles(const frehg_gw_solver* s, int* rows_per_lane, int*
     if (!buckets) frehg::fail(FREHG_EINVAL, "frehg_gw_get_column_tiles: buckets is required");
     *buckets = frehg::S(s)->pcg->columnTiles(rows_per_lane, columns, columns_per_wave,
                                              capacity < 0 ? 0 : capacity);
+  });
+}
+int frehg_gw_set_chebyshev(frehg_gw_solver* s, int degree, double lower_ratio) {
+  return guard([&] {
+    GwSolver* g = frehg::S(s);
+    if (!g->popt.cheb)
+      frehg::fail(FREHG_EINVAL, "frehg_gw_set_chebyshev: the solver's preconditioner is not "
+                                "FREHG_PC_CHEBYSHEV");
+    if (degree < 3 || degree > 15 || !(degree & 1) || !(lower_ratio > 0.0 && lower_ratio < 1.0))
+      frehg::fail(FREHG_EINVAL, "frehg_gw_set_chebyshev: odd degree in [3, 15] and a lower "
+                                "ratio in (0, 1) are required");
+    g->popt.cheb_degree = degree;
+    g->popt.cheb_lower = lower_ratio;
+    g->invalidateGraphs();
   });
 }
 int frehg_gw_enable_column_solve(frehg_gw_solver* s, int enable) {

@@ -213,6 +213,9 @@ class RichardsSolver:
         check(self.lib, self.lib.frehg_gw_get_column_tiles(self._h, R, n, cpw, cap, C.byref(nb)))
         return [{"R": R[b], "columns": n[b], "cpw": cpw[b]} for b in range(nb.value)]
 
+    def set_chebyshev(self, degree: int = 3, lower_ratio: float = 0.05) -> None:
+        check(self.lib, self.lib.frehg_gw_set_chebyshev(self._h, int(degree), float(lower_ratio)))
+
     def enable_column_solve(self, enable=True) -> None:
         check(self.lib, self.lib.frehg_gw_enable_column_solve(self._h, int(bool(enable))))
 

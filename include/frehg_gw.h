@@ -59,6 +59,8 @@ extern "C" {
 #define FREHG_PC_JACOBI 0   /* point Jacobi */
 #define FREHG_PC_ZLINE 1    /* z-line block Jacobi: one tridiagonal solve per column */
 #define FREHG_PC_ICC0 2     /* incomplete Cholesky (CPU oracle only; the reference default) */
+#define FREHG_PC_CHEBYSHEV 3 /* mixed form: the FP32 inner PCG preconditioned by a Jacobi-
+                              * Chebyshev polynomial (frehg_gw_set_chebyshev); even nz */
 
 /* Arithmetic of the linear solve. */
 #define FREHG_PREC_FP64 0   /* plain FP64 PCG */
@@ -349,6 +351,13 @@ int frehg_gw_get_graph_stats(const frehg_gw_solver* s, frehg_gw_graph_stats* out
 int frehg_gw_get_column_tiles(const frehg_gw_solver* s, int* rows_per_lane, int* columns,
                               int* columns_per_wave, int capacity, int* buckets);
 int frehg_gw_set_column_tiles(frehg_gw_solver* s, const int* columns_per_wave, int buckets);
+/* FREHG_PC_CHEBYSHEV: the polynomial's odd degree (3 .. 15, default 3; each
+ * inner iteration then runs degree - 1 extra stencil sweeps) and its interval
+ * [lower_ratio * 2, 2] on the spectrum of D^-1 A (default lower_ratio 0.05).
+ * The reference offers point Jacobi, block Jacobi / ICC and (hypre) AMG
+ * preconditioners for KSPCG (src/core/LinearSystem.cpp:158-221); a
+ * polynomial preconditioner is an extension. */
+int frehg_gw_set_chebyshev(frehg_gw_solver* s, int degree, double lower_ratio);
 /* 0: solve laterally decoupled systems with the z-line PCG instead (1, the
  * default, restores the column solve; EUNSUPPORTED where there is none). */
 int frehg_gw_enable_column_solve(frehg_gw_solver* s, int enable);

@@ -562,6 +562,8 @@ void Oracle::create(const frehg_gw_desc& d) {
   atol = d.atol;
   maxIt = d.max_iterations > 0 ? d.max_iterations : 1000;
   pc = d.preconditioner;
+  if (pc != FREHG_PC_JACOBI && pc != FREHG_PC_ZLINE && pc != FREHG_PC_ICC0)
+    fail(FREHG_EUNSUPPORTED, "the oracle's preconditioners are jacobi, zline and icc0");
   guessPrevious = d.initial_guess_previous != 0;
 
   buildGeometry(d);
