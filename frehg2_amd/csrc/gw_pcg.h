@@ -219,20 +219,26 @@ class Pcg {
 
   struct Shapes {
     int blocks = 0, pstride = 0;
-    March march64, march32;
+    March march64, march32, march32w;
   };
   static Shapes shapesFor(const Grid& g, int cus, int wpcCol, int wpc64, int wpc32);
-  static March makeMarch(const Grid& g, int cus, int wavesPerCu);
+  static March makeMarch(const Grid& g, int cus, int wavesPerCu, int columnsPerWave = 1);
+  // block count of the direction kernel (whose partial sums the update reduces)
+  int dirBlocks(bool fp32, const March& mj) const {
+    return fp32 && pairs_ && cw2_ ? march32w_.blocks : mj.blocks;
+  }
 
   Grid g_;
   hipStream_t stream_;
   int blocks_ = 0;            // grid size of the column kernels (multiple of 8)
   March march64_, march32_;   // stencil-kernel shapes, FP64 and FP32 operands
+  March march32w_;            // FP32 direction kernel over column pairs (cw2_)
   int pstride_ = 0;           // slot stride of parts_ (largest grid)
   int partsCap_ = 0;          // slot stride parts_ was allocated for
   int cus_ = 256;
   int wpc64_ = 16, wpc32_ = 32, wpcCol_ = 16;  // waves per CU (setShapes)
   bool pairs_ = false;        // FP32 kernels in the row-pair layout (nz even)
+  bool cw2_ = false;          // ... and the direction kernel over column pairs (nx even)
   // FP64 vectors (halo layout)
   double *r_ = nullptr, *z_ = nullptr, *p0_ = nullptr, *p1_ = nullptr, *q_ = nullptr;
   double *invd_ = nullptr, *w_ = nullptr;

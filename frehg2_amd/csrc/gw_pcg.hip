@@ -114,6 +114,19 @@ __device__ __forceinline__ bool march_span(const Grid& g, const March& mj, int& 
   return slab < mj.nslab && j0 <= g.ny;
 }
 
+// j-march over pairs of columns (i, i + 1), i odd (nx even): wave -> (i, rows
+// j0..j1); mj.nslab counts slabs of nx / 2 waves.
+__device__ __forceinline__ bool march_span2(const Grid& g, const March& mj, int& i, int& j0,
+                                            int& j1) {
+  const int64_t wg = xcd_block() * kWpb + threadIdx.x / kWave;
+  const int half = g.nx / 2;
+  i = 2 * static_cast<int>(wg % half) + 1;
+  const int64_t slab = wg / half;
+  j0 = static_cast<int>(slab) * mj.S + 1;
+  j1 = min(g.ny, j0 + mj.S - 1);
+  return slab < mj.nslab && j0 <= g.ny;
+}
+
 // Row k of a column for a lane past the column's end: such a lane reads the
 // last row (in bounds) and its caller zeroes the value.  All lanes' loads then
 // issue in one block; a load under `if (k < nz)` compiles to a branch that
@@ -1139,6 +1152,148 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step_p(Grid g, March mj, Stenci
   }
 }
 
+// k_pcg_dir_p with two neighbouring columns (i, i + 1) per wave: their p is
+// formed once for both stencils, the shared x leg Axp(i) is read once, and
+// only the outer neighbours i - 1 and i + 2 come from the XCD's L2 (half the
+// cache loads of the one-column form).  nx even.
+template <int R2, bool RESTART>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil<float> A,
+                                                       const float* __restrict__ z,
+                                                       const float* __restrict__ pOld,
+                                                       float* __restrict__ pNew,
+                                                       float* __restrict__ q,
+                                                       double* __restrict__ parts, int pstride,
+                                                       int nprev, PcgState* __restrict__ st,
+                                                       int n) {
+  if (st->done) return;
+  if (n >= st->limit) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->it = n;
+    }
+    return;
+  }
+  double red[2];
+  const int rs[2] = {0, 1};
+  reduce_parts<2>(parts, pstride, nprev, rs, red);
+  const double rzv = red[0], rrv = red[1];
+  if (rrv <= st->tol2) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->it = n;
+      st->rr = rrv;
+    }
+    return;
+  }
+  const double betaD = RESTART ? 0.0 : rzv / st->rz[(n + 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rz[n & 1] = rzv;
+    st->it = n;
+    st->rr = rrv;
+  }
+  const float beta = static_cast<float>(betaD);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nz = g.nz;
+  const int64_t sx = nz, sy = static_cast<int64_t>(g.nx2) * nz;
+  KSum pq;
+  int i, j0, j1;
+  if (march_span2(g, mj, i, j0, j1)) {
+    auto formp = [&](int64_t base, float (&p0)[R2], float (&p1)[R2]) {
+      float z0[R2], z1[R2];
+      load_pair<R2>(z, base, lane, nz, z0, z1);
+      if (RESTART) {
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          p0[m] = z0[m];
+          p1[m] = z1[m];
+        }
+      } else {
+        float o0[R2], o1[R2];
+        load_pair<R2>(pOld, base, lane, nz, o0, o1);
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          p0[m] = z0[m] + beta * o0[m];
+          p1[m] = z1[m] + beta * o1[m];
+        }
+      }
+    };
+    // column a = i, column b = i + 1
+    float aP0[R2], aP1[R2], aC0[R2], aC1[R2], aY0[R2], aY1[R2];
+    float bP0[R2], bP1[R2], bC0[R2], bC1[R2], bY0[R2], bY1[R2];
+    formp(g.c3(j0 - 1, i, 0), aP0, aP1);
+    formp(g.c3(j0, i, 0), aC0, aC1);
+    formp(g.c3(j0 - 1, i + 1, 0), bP0, bP1);
+    formp(g.c3(j0, i + 1, 0), bC0, bC1);
+    load_pair<R2>(A.Ayp, g.c3(j0 - 1, i, 0), lane, nz, aY0, aY1);
+    load_pair<R2>(A.Ayp, g.c3(j0 - 1, i + 1, 0), lane, nz, bY0, bY1);
+    for (int j = j0; j <= j1; ++j) {
+      const int64_t ba = g.c3(j, i, 0), bb = ba + sx;
+      float aN0[R2], aN1[R2], bN0[R2], bN1[R2], w0[R2], w1[R2], e0[R2], e1[R2];
+      formp(ba + sy, aN0, aN1);
+      formp(bb + sy, bN0, bN1);
+      formp(ba - sx, w0, w1);  // column i - 1
+      formp(bb + sx, e0, e1);  // column i + 2
+      float adA0[R2], adA1[R2], axA0[R2], axA1[R2], axW0[R2], axW1[R2], ayA0[R2], ayA1[R2],
+          azA0[R2], azA1[R2];
+      float adB0[R2], adB1[R2], axB0[R2], axB1[R2], ayB0[R2], ayB1[R2], azB0[R2], azB1[R2];
+      load_pair<R2>(A.Ad, ba, lane, nz, adA0, adA1);
+      load_pair<R2>(A.Axp, ba, lane, nz, axA0, axA1);
+      load_pair<R2>(A.Axp, ba - sx, lane, nz, axW0, axW1);
+      load_pair<R2>(A.Ayp, ba, lane, nz, ayA0, ayA1);
+      load_pair<R2>(A.Azp, ba, lane, nz, azA0, azA1);
+      load_pair<R2>(A.Ad, bb, lane, nz, adB0, adB1);
+      load_pair<R2>(A.Axp, bb, lane, nz, axB0, axB1);
+      load_pair<R2>(A.Ayp, bb, lane, nz, ayB0, ayB1);
+      load_pair<R2>(A.Azp, bb, lane, nz, azB0, azB1);
+      float aDn[R2], aUp[R2], azAUp[R2], dummyA[R2], bDn[R2], bUp[R2], azBUp[R2], dummyB[R2];
+      pair_shift<R2>(aC0, aC1, lane, aDn, aUp);
+      pair_shift<R2>(azA0, azA1, lane, dummyA, azAUp);
+      pair_shift<R2>(bC0, bC1, lane, bDn, bUp);
+      pair_shift<R2>(azB0, azB1, lane, dummyB, azBUp);
+      float qa0[R2], qa1[R2], qb0[R2], qb1[R2];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        qa0[m] = adA0[m] * aC0[m] + axA0[m] * bC0[m] + axW0[m] * w0[m] + ayA0[m] * aN0[m] +
+                 aY0[m] * aP0[m] + azA0[m] * aC1[m] + azAUp[m] * aUp[m];
+        qa1[m] = adA1[m] * aC1[m] + axA1[m] * bC1[m] + axW1[m] * w1[m] + ayA1[m] * aN1[m] +
+                 aY1[m] * aP1[m] + azA1[m] * aDn[m] + azA0[m] * aC0[m];
+        qb0[m] = adB0[m] * bC0[m] + axB0[m] * e0[m] + axA0[m] * aC0[m] + ayB0[m] * bN0[m] +
+                 bY0[m] * bP0[m] + azB0[m] * bC1[m] + azBUp[m] * bUp[m];
+        qb1[m] = adB1[m] * bC1[m] + axB1[m] * e1[m] + axA1[m] * aC1[m] + ayB1[m] * bN1[m] +
+                 bY1[m] * bP1[m] + azB1[m] * bDn[m] + azB0[m] * bC0[m];
+        if (128 * m + 2 * lane < nz) {
+          pq.add(static_cast<double>(aC0[m]) * static_cast<double>(qa0[m]));
+          pq.add(static_cast<double>(aC1[m]) * static_cast<double>(qa1[m]));
+          pq.add(static_cast<double>(bC0[m]) * static_cast<double>(qb0[m]));
+          pq.add(static_cast<double>(bC1[m]) * static_cast<double>(qb1[m]));
+        }
+      }
+      store_pair<R2>(pNew, ba, lane, nz, aC0, aC1);
+      store_pair<R2>(q, ba, lane, nz, qa0, qa1);
+      store_pair<R2>(pNew, bb, lane, nz, bC0, bC1);
+      store_pair<R2>(q, bb, lane, nz, qb0, qb1);
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        aP0[m] = aC0[m];
+        aP1[m] = aC1[m];
+        aC0[m] = aN0[m];
+        aC1[m] = aN1[m];
+        aY0[m] = ayA0[m];
+        aY1[m] = ayA1[m];
+        bP0[m] = bC0[m];
+        bP1[m] = bC1[m];
+        bC0[m] = bN0[m];
+        bC1[m] = bN1[m];
+        bY0[m] = ayB0[m];
+        bY1[m] = ayB1[m];
+      }
+    }
+  }
+  double v[1] = {pq.value()};
+  const int slots[1] = {2};
+  write_parts<1>(parts, pstride, slots, v);
+}
+
 // k_pcg_update for FP32 operands in the pair layout (x deferred the same way).
 template <int R2, bool LINE, int XM>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, float* __restrict__ x,
@@ -1543,6 +1698,7 @@ void Pcg::initCommon() {
   if (g_.nz > 512) throw std::runtime_error("nz > 512 is not supported by the column kernels");
   cus_ = device_cus();
   pairs_ = g_.nz % 2 == 0;
+  cw2_ = pairs_ && g_.nx % 2 == 0;
   applyShapes();
 }
 
@@ -1635,7 +1791,8 @@ Pcg::Shapes Pcg::shapesFor(const Grid& g, int cus, int wpcCol, int wpc64, int wp
   sh.blocks = static_cast<int>(std::max<int64_t>(8, ((wavesWanted + kWpb - 1) / kWpb + 7) / 8 * 8));
   sh.march64 = makeMarch(g, cus, wpc64);
   sh.march32 = makeMarch(g, cus, wpc32);
-  sh.pstride = std::max({sh.blocks, sh.march64.blocks, sh.march32.blocks});
+  sh.march32w = makeMarch(g, cus, wpc32, 2);
+  sh.pstride = std::max({sh.blocks, sh.march64.blocks, sh.march32.blocks, sh.march32w.blocks});
   return sh;
 }
 
@@ -1644,6 +1801,7 @@ void Pcg::applyShapes() {
   blocks_ = sh.blocks;
   march64_ = sh.march64;
   march32_ = sh.march32;
+  march32w_ = sh.march32w;
   pstride_ = std::max(sh.pstride, colBlocks_);
   if (parts_ && pstride_ > partsCap_ && external_)
     throw std::runtime_error("launch shapes beyond the external workspace's partial sums");
@@ -1663,13 +1821,14 @@ void Pcg::setShapes(int march64WavesPerCu, int march32WavesPerCu, int columnWave
   applyShapes();
 }
 
-March Pcg::makeMarch(const Grid& g, int cus, int wavesPerCu) {
+March Pcg::makeMarch(const Grid& g, int cus, int wavesPerCu, int columnsPerWave) {
   March mj;
+  const int64_t perSlab = std::max(1, g.nx / columnsPerWave);
   const int64_t target = static_cast<int64_t>(cus) * wavesPerCu;
-  mj.nslab = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(g.ny, target / g.nx)));
+  mj.nslab = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(g.ny, target / perSlab)));
   mj.S = (g.ny + mj.nslab - 1) / mj.nslab;
   mj.nslab = (g.ny + mj.S - 1) / mj.S;  // every slab starts inside the grid
-  const int64_t waves = static_cast<int64_t>(g.nx) * mj.nslab;
+  const int64_t waves = perSlab * mj.nslab;
   mj.blocks = static_cast<int>(std::max<int64_t>(8, ((waves + kWpb - 1) / kWpb + 7) / 8 * 8));
   return mj;
 }
@@ -1797,7 +1956,14 @@ void Pcg::launchDir(const Stencil<T>& A, const T* z, const T* pOld, T* pNew, T* 
     if (pairs_) {
       dispatch_R2(g_.nz, [&](auto RC) {
         constexpr int R2 = decltype(RC)::value;
-        if (restart)
+        const March& mw = march32w_;
+        if (cw2_ && restart)
+          k_pcg_dir_p2<R2, true><<<mw.blocks, kBlock, 0, stream_>>>(g_, mw, A, z, pOld, pNew, q,
+                                                                    parts_, pstride_, np, dst, n);
+        else if (cw2_)
+          k_pcg_dir_p2<R2, false><<<mw.blocks, kBlock, 0, stream_>>>(g_, mw, A, z, pOld, pNew, q,
+                                                                     parts_, pstride_, np, dst, n);
+        else if (restart)
           k_pcg_dir_p<R2, true><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, z, pOld, pNew, q,
                                                                    parts_, pstride_, np, dst, n);
         else
@@ -1824,22 +1990,23 @@ template <class T>
 void Pcg::launchUpdate(T* x, const T* pPrev, const T* p, T* r, const T* q, T* z, const T* invd,
                        const T* w, bool zline, bool two, int n, const March& mj) {
   PcgState* dst = static_cast<PcgState*>(state_);
+  const int nd = dirBlocks(std::is_same<T, float>::value, mj);
   if constexpr (std::is_same<T, float>::value) {
     if (pairs_) {
       dispatch_R2(g_.nz, [&](auto RC) {
         constexpr int R2 = decltype(RC)::value;
         if (zline && two)
           k_pcg_update_p<R2, true, 2><<<blocks_, kBlock, 0, stream_>>>(
-              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
         else if (zline)
           k_pcg_update_p<R2, true, 0><<<blocks_, kBlock, 0, stream_>>>(
-              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
         else if (two)
           k_pcg_update_p<R2, false, 2><<<blocks_, kBlock, 0, stream_>>>(
-              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
         else
           k_pcg_update_p<R2, false, 0><<<blocks_, kBlock, 0, stream_>>>(
-              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
       });
       HIP_OK(hipGetLastError());
       return;
@@ -1849,16 +2016,16 @@ void Pcg::launchUpdate(T* x, const T* pPrev, const T* p, T* r, const T* q, T* z,
     constexpr int R = decltype(RC)::value;
     if (zline && two)
       k_pcg_update<T, R, true, 2><<<blocks_, kBlock, 0, stream_>>>(
-          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
     else if (zline)
       k_pcg_update<T, R, true, 0><<<blocks_, kBlock, 0, stream_>>>(
-          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
     else if (two)
       k_pcg_update<T, R, false, 2><<<blocks_, kBlock, 0, stream_>>>(
-          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
     else
       k_pcg_update<T, R, false, 0><<<blocks_, kBlock, 0, stream_>>>(
-          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, mj.blocks, dst, n);
+          g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
   });
   HIP_OK(hipGetLastError());
 }
