@@ -94,7 +94,7 @@ class GwTiming(C.Structure):
 class GraphStats(C.Structure):
     _fields_ = [("mode", C.c_int), ("budget_rounds", C.c_int), ("budget_iterations", C.c_int),
                 ("captures", C.c_int64), ("replays", C.c_int64), ("device_steps", C.c_int64),
-                ("host_finishes", C.c_int64)]
+                ("host_finishes", C.c_int64), ("budget_later", C.c_int)]
 
 
 # Exported symbols of the product library, with argtypes (the C-ABI surface).

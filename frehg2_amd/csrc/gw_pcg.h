@@ -36,6 +36,9 @@ struct PcgStats {
   double residual_norm = 0.0;
   double rhs_norm = 0.0;
   int fp64_iterations = 0;    // mixed: iterations finished in FP64 after an FP32 stall
+  int rounds = 0;             // recursion rounds (FP64) or refinement cycles (mixed)
+  int first_inner = 0;        // iterations of the first of them
+  int later_inner = 0;        // most iterations of a later one
   double spmv_ms = 0.0;       // accumulated SpMV-kernel time (timing mode)
   int64_t spmv_launches = 0;
 };
@@ -73,12 +76,16 @@ struct ColumnTable {
 // refinement cycles, of `iterations` direction/update launch pairs each.
 struct PcgBudget {
   int rounds = 1;
-  int iterations = 32;
+  int iterations = 32;  // launch pairs of the first round / cycle
+  int later = 0;        // ... of each later one (0: as the first)
 };
 
 // What a device-controlled solve left for the budget's next sizing.
 struct DeviceSummary {
   int max_inner = 0;  // most iterations one round / cycle took
+  int first_inner = 0;  // iterations of the first round / cycle
+  int later_inner = 0;  // most iterations of a later one
+  int rounds = 0;       // rounds / cycles that ran
   int stopped = 0;    // the solve ended inside the budget (converged or not)
   int fallback = 0;   // mixed: a cycle failed to halve the residual
 };

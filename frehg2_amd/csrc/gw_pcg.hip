@@ -66,6 +66,9 @@ struct PcgState {
   int refinements;
   int stalls;       // FP64 rounds that made no iteration
   int maxInner;     // most iterations of one round / cycle
+  int rounds;       // rounds / cycles that ran
+  int firstInner;   // iterations of the first one
+  int laterInner;   // most iterations of a later one
   int gate;         // 1: the solve did not converge inside the budget
   double tolF2;     // final tolerance, squared
   double rrTrue, bbTrue, rrPrev, s;
@@ -1480,6 +1483,9 @@ __global__ __launch_bounds__(kBlock) void k_dc_begin64(const double* __restrict_
   st->refinements = 0;
   st->stalls = 0;
   st->maxInner = 0;
+  st->rounds = 0;
+  st->firstInner = 0;
+  st->laterInner = 0;
   st->fallback = 0;
   st->keepNew = 0;
   st->xInB = 0;
@@ -1529,6 +1535,9 @@ __global__ __launch_bounds__(kBlock) void k_dc_end64(const double* __restrict__ 
   const int its = round_its(st, budget);
   st->total += its;
   st->maxInner = max(st->maxInner, its);
+  if (st->rounds == 0) st->firstInner = its;
+  else st->laterInner = max(st->laterInner, its);
+  st->rounds += 1;
   const double rr = v[0];
   st->rrTrue = rr;
   if (rr <= st->tolF2) {
@@ -1560,6 +1569,9 @@ __global__ __launch_bounds__(kBlock) void k_dc_begin_mixed(const double* __restr
   st->refinements = 0;
   st->stalls = 0;
   st->maxInner = 0;
+  st->rounds = 0;
+  st->firstInner = 0;
+  st->laterInner = 0;
   st->fallback = 0;
   st->keepNew = 0;
   st->xInB = 0;
@@ -1588,6 +1600,9 @@ __global__ void k_dc_mid(PcgState* st, int budget) {
   const int its = round_its(st, budget);
   st->total += its;
   st->maxInner = max(st->maxInner, its);
+  if (st->rounds == 0) st->firstInner = its;
+  else st->laterInner = max(st->laterInner, its);
+  st->rounds += 1;
   if (its > 0) st->rz[1] = st->rz[(its - 1) & 1];
 }
 
@@ -2246,6 +2261,8 @@ int Pcg::fp64Loop(const Stencil<double>& A, const double* b, double* x, const Pc
                             opt.max_iterations - total, march64_, march64_.blocks, 0, true, opt,
                             st, &its);
     total += its;
+    if (st->rounds++ == 0) st->first_inner = its;
+    else st->later_inner = std::max(st->later_inner, its);
     rr = residualHost(A, b, x, opt.zline).first;  // true residual (restart point if not converged)
     if (its == 0 && ++stalls > 2 && !(rr <= tol2)) {
       st->residual_norm = std::sqrt(rr);
@@ -2349,6 +2366,8 @@ int Pcg::mixedLoop(const Stencil<double>& A, const double* b, double* x, const P
     c.total += its;
     c.n0 += its;
     c.restart = false;
+    if (st->rounds++ == 0) st->first_inner = its;
+    else st->later_inner = std::max(st->later_inner, its);
     const double rrPrev = c.rr;
     c.rr = refineHost(A, b, c.xa, c.xb, 1.0 / c.s, true).first;
     st->refinements += 1;
@@ -2408,7 +2427,10 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
   if (bud.rounds < 1 || bud.iterations < 1 || bud.iterations > 4096)
     throw std::runtime_error("device PCG budget out of range");
   PcgState* dst = static_cast<PcgState*>(state_);
-  const int B = bud.iterations;
+  if (bud.later < 0 || bud.later > 4096) throw std::runtime_error("device PCG budget out of range");
+  auto budgetOf = [&](int round) {
+    return round == 0 || bud.later <= 0 ? bud.iterations : bud.later;
+  };
   if (direct_ && opt.zline) {
     launchColumnSolve(A, b, x);
     k_dc_begin64<<<1, kBlock, 0, stream_>>>(parts_, pstride_, colBlocks_, dst, opt.rtol, opt.atol,
@@ -2422,6 +2444,7 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
                                             opt.atol, opt.max_iterations);
     HIP_OK(hipGetLastError());
     for (int round = 0; round < bud.rounds; ++round) {
+      const int B = budgetOf(round);
       k_dc_round64<<<1, 1, 0, stream_>>>(dst, opt.max_iterations);
       HIP_OK(hipGetLastError());
       double* pOld = p0_;
@@ -2450,6 +2473,7 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
                                                 opt.max_refinements, ncell);
     HIP_OK(hipGetLastError());
     for (int cycle = 0; cycle < bud.rounds; ++cycle) {
+      const int B = budgetOf(cycle);
       k_dc_cycle<<<1, 1, 0, stream_>>>(dst, opt.max_iterations, opt.inner_rtol);
       HIP_OK(hipGetLastError());
       launchInnerInit(opt.zline, 0.0, dst);
@@ -2493,6 +2517,9 @@ int Pcg::deviceResult(PcgStats* st, DeviceSummary* sum) const {
   st->rhs_norm = std::sqrt(h->bbTrue);
   if (sum) {
     sum->max_inner = h->maxInner;
+    sum->first_inner = h->firstInner;
+    sum->later_inner = h->laterInner;
+    sum->rounds = h->rounds;
     sum->stopped = h->stop;
     sum->fallback = h->fallback;
   }

@@ -2172,6 +2172,9 @@ struct GwSolver {
   PcgBudget budget;
   bool budgetSet = false;    // given by the caller (else sized from the last solve)
   bool budgetReady = false;  // a host step has sized it, or the caller set it
+  // most iterations of the first and of later rounds, and most rounds, over
+  // the device steps since the budget was last sized (it shrinks to them)
+  int recentFirst = 0, recentLater = 0, recentRounds = 0, recentSteps = 0;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   Dev capturedDev[2]{};      // the device view each graph was captured with
   double* hParity0 = nullptr;
@@ -2266,6 +2269,7 @@ struct GwSolver {
       }
   }
   void setStepMode(int mode, int rounds, int iterations);
+  void shrinkBudget(const DeviceSummary& sum);
   // Post-allocation and final clamp (Reallocate.cpp:122-545,
   // Corrector.cpp:758-792): the classification writes the second buffer pair,
   // which then becomes the state; the saturation flags must be current.
@@ -2988,15 +2992,20 @@ void GwSolver::hostStep() {
                               " iterations (residual " + std::to_string(ps.residual_norm) +
                               ", ||b|| " + std::to_string(ps.rhs_norm) + ")");
   if (stepMode != FREHG_STEP_HOST && !budgetSet && !budgetReady) {
-    // the first step of a device-controlled mode sizes the budget
-    budget.iterations = std::min(4096, std::max(8, ps.iterations + ps.iterations / 2 + 4));
-    budget.rounds = popt.mixed ? std::max(2, ps.refinements + 1) : 1;
+    // the first step of a device-controlled mode sizes the budget: the first
+    // round (or cycle) from its own count, the later ones from theirs
+    const int cap = popt.mixed ? 400 : 4096;
+    auto grow = [](int v) { return v + v / 2 + 4; };
+    budget.iterations = std::min(cap, std::max(8, grow(ps.first_inner)));
     if (popt.mixed) {
-      const int perCycle = (ps.iterations + std::max(1, ps.refinements) - 1) /
-                           std::max(1, ps.refinements);
-      budget.iterations = std::min(400, std::max(8, perCycle + perCycle / 2 + 4));
+      budget.later = std::min(cap, std::max(6, grow(ps.later_inner)));
+      budget.rounds = std::max(2, ps.rounds + 1);
+    } else {
+      budget.later = std::max(8, budget.iterations / 4);
+      budget.rounds = std::max(1, ps.rounds);
     }
     budgetReady = true;
+    recentFirst = recentLater = recentRounds = recentSteps = 0;
   }
   enqueuePost();
   if (timing) HIP_CHECK(hipEventRecord(ev1, stream));
@@ -3068,7 +3077,8 @@ void GwSolver::deviceStep() {
   gstats.device_steps += 1;
   PcgStats ps;
   DeviceSummary sum;
-  if (pcg->deviceResult(&ps, &sum)) {
+  const int gate = pcg->deviceResult(&ps, &sum);
+  if (gate) {
     // The solve did not converge inside the budget and the kernels after it
     // returned at once: finish the solve host-polled from where it stopped,
     // then run the rest of the step.
@@ -3087,14 +3097,24 @@ void GwSolver::deviceStep() {
                                 std::to_string(ps.residual_norm) + ", ||b|| " +
                                 std::to_string(ps.rhs_norm) + ")");
     if (!budgetSet) {
-      // a larger budget for the next steps (graphs are captured again)
-      const int need = popt.mixed ? std::max(sum.max_inner, budget.iterations)
-                                  : ps.iterations;
-      budget.iterations =
-          std::min(popt.mixed ? 400 : 4096, std::max(budget.iterations + budget.iterations / 2,
-                                                      need + need / 4 + 4));
-      if (popt.mixed) budget.rounds = std::max(budget.rounds, ps.refinements + 1);
+      // a larger budget for the next steps (graphs are captured again): the
+      // rounds that ran out of launches get half as many more, else one
+      // more round
+      const int cap = popt.mixed ? 400 : 4096;
+      auto grow = [](int v) { return v + v / 2 + 4; };
+      bool grew = false;
+      if (sum.first_inner >= budget.iterations) {
+        budget.iterations = std::min(cap, grow(budget.iterations));
+        grew = true;
+      }
+      const int later = budget.later > 0 ? budget.later : budget.iterations;
+      if (sum.rounds > 1 && sum.later_inner >= later) {
+        budget.later = std::min(cap, grow(later));
+        grew = true;
+      }
+      if (!grew) budget.rounds = std::min(16, budget.rounds + 1);
       invalidateGraphs();
+      recentFirst = recentLater = recentRounds = recentSteps = 0;
     }
     enqueuePost();
     reduceSlots();
@@ -3105,7 +3125,32 @@ void GwSolver::deviceStep() {
   last.fp64_iterations = ps.fp64_iterations;
   last.residual_norm = ps.residual_norm;
   last.rhs_norm = ps.rhs_norm;
+  if (!budgetSet && !gate) shrinkBudget(sum);
   finishStep(ps);
+}
+
+// Every 8 device steps the budget comes down to what those steps used plus a
+// quarter, if that is well below it: a budget sized on a transient would
+// otherwise leave launches that only return at once (each still costs its
+// dispatch, a few microseconds, in a replayed graph too).
+void GwSolver::shrinkBudget(const DeviceSummary& sum) {
+  recentFirst = std::max(recentFirst, sum.first_inner);
+  recentLater = std::max(recentLater, sum.later_inner);
+  recentRounds = std::max(recentRounds, sum.rounds);
+  if (++recentSteps < 8) return;
+  auto fit = [](int v) { return v + v / 4 + 2; };
+  PcgBudget b = budget;
+  const int first = std::max(4, fit(recentFirst));
+  if (first * 5 / 4 < b.iterations) b.iterations = first;
+  const int later = std::max(4, fit(recentLater));
+  if (b.later > 0 && later * 5 / 4 < b.later) b.later = later;
+  const int rounds = popt.mixed ? std::max(2, recentRounds + 1) : std::max(1, recentRounds);
+  if (rounds < b.rounds) b.rounds = rounds;
+  if (b.iterations != budget.iterations || b.later != budget.later || b.rounds != budget.rounds) {
+    budget = b;
+    invalidateGraphs();
+  }
+  recentFirst = recentLater = recentRounds = recentSteps = 0;
 }
 
 void GwSolver::setStepMode(int mode, int rounds, int iterations) {
@@ -3122,9 +3167,11 @@ void GwSolver::setStepMode(int mode, int rounds, int iterations) {
   if (budgetSet) {
     budget.rounds = rounds;
     budget.iterations = iterations;
+    budget.later = 0;
   }
   gstats.budget_rounds = budget.rounds;
   gstats.budget_iterations = budget.iterations;
+  gstats.budget_later = budget.later > 0 ? budget.later : budget.iterations;
 }
 
 // Audit and the next dt from the step's sums (slotsHost).
@@ -3160,6 +3207,7 @@ void GwSolver::finishStep(const PcgStats& ps) {
   dtgNext = next;
   gstats.budget_rounds = budget.rounds;
   gstats.budget_iterations = budget.iterations;
+  gstats.budget_later = budget.later > 0 ? budget.later : budget.iterations;
   if (timing) {
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));

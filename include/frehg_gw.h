@@ -328,11 +328,14 @@ typedef struct frehg_gw_graph_stats {
   int64_t replays;         /* graph launches */
   int64_t device_steps;    /* steps in DEVICE or GRAPH mode */
   int64_t host_finishes;   /* of those, steps whose solve the host finished */
+  int budget_later;        /* launch pairs of each round / cycle after the first */
 } frehg_gw_graph_stats;
 /* rounds = iterations = 0: the first step runs host-polled and sizes the
- * budget (and it grows when a solve outruns it); else the budget is fixed
- * (rounds <= 64, iterations <= 4096; mixed cycles <= 400).  Timing mode
- * (frehg_gw_set_timing) runs host-polled steps. */
+ * budget, the first round (cycle) from its own iteration count and the later
+ * ones from theirs (and it grows when a solve outruns it); else the budget
+ * is fixed, `iterations` for every round (rounds <= 64, iterations <= 4096;
+ * mixed cycles <= 400).  Timing mode (frehg_gw_set_timing) runs host-polled
+ * steps. */
 int frehg_gw_set_step_mode(frehg_gw_solver* s, int mode, int rounds, int iterations);
 int frehg_gw_get_graph_stats(const frehg_gw_solver* s, frehg_gw_graph_stats* out);
 
