@@ -950,7 +950,8 @@ int frehg_ls_solve_stencil_noalloc(int nx, int ny, int nz, const double* Ad, con
       frehg::ls_fail(FREHG_EINVAL, "invalid arguments");
     if (preconditioner != FREHG_PC_JACOBI && preconditioner != FREHG_PC_ZLINE)
       frehg::ls_fail(FREHG_EUNSUPPORTED, "point Jacobi or z-line");
-    if (nz > 512) frehg::ls_fail(FREHG_EUNSUPPORTED, "nz > 512 is not supported by the column kernels");
+    if (nz > 1024)
+      frehg::ls_fail(FREHG_EUNSUPPORTED, "nz > 1024 is not supported by the column kernels");
     const int64_t nH = static_cast<int64_t>(nx + 2) * (ny + 2) * nz;
     if (nH > frehg::kNoAllocCells)
       frehg::ls_fail(FREHG_EINVAL, "(nx+2)(ny+2)nz = " + std::to_string(nH) +

@@ -64,11 +64,11 @@ struct PcgOptions {
 // in buckets of equal R = ceil(active rows / 64), deepest first, and the
 // launch tile of each bucket (columns per wave).
 struct ColumnTable {
-  int buckets = 0;
-  int R[8] = {};
-  int cpw[8] = {};
-  int colStart[9] = {};   // bucket b's columns: order[colStart[b] .. colStart[b + 1])
-  int waveStart[9] = {};  // ... and its waves
+  int buckets = 0;        // R = 1 .. 8 and 16 (513 .. 1024 rows): at most 9
+  int R[9] = {};
+  int cpw[9] = {};
+  int colStart[10] = {};  // bucket b's columns: order[colStart[b] .. colStart[b + 1])
+  int waveStart[10] = {}; // ... and its waves
 };
 
 // Launches of a device-controlled solve (Pcg::enqueueDevice): `rounds` FP64

@@ -477,7 +477,8 @@ __global__ __launch_bounds__(kBlock) void k_column_solve(Grid g, Stencil<double>
         case 5: column_direct<5>(g, A, b, x, base, k0, lane, rr, bb); break;
         case 6: column_direct<6>(g, A, b, x, base, k0, lane, rr, bb); break;
         case 7: column_direct<7>(g, A, b, x, base, k0, lane, rr, bb); break;
-        default: column_direct<8>(g, A, b, x, base, k0, lane, rr, bb); break;
+        case 8: column_direct<8>(g, A, b, x, base, k0, lane, rr, bb); break;
+        default: column_direct<16>(g, A, b, x, base, k0, lane, rr, bb); break;
       }
     }
   }
@@ -1664,7 +1665,11 @@ void dispatch_R(int nz, F&& f) {
     case 6: f(std::integral_constant<int, 6>{}); break;
     case 7: f(std::integral_constant<int, 7>{}); break;
     case 8: f(std::integral_constant<int, 8>{}); break;
-    default: throw std::runtime_error("nz > 512 is not supported by the column kernels");
+    // 513 .. 1024 rows: one deep form (rows past the column are masked)
+    case 9: case 10: case 11: case 12: case 13: case 14: case 15: case 16:
+      f(std::integral_constant<int, 16>{});
+      break;
+    default: throw std::runtime_error("nz > 1024 is not supported by the column kernels");
   }
 }
 
@@ -1675,7 +1680,8 @@ void dispatch_R2(int nz, F&& f) {
     case 2: f(std::integral_constant<int, 2>{}); break;
     case 3: f(std::integral_constant<int, 3>{}); break;
     case 4: f(std::integral_constant<int, 4>{}); break;
-    default: throw std::runtime_error("nz > 512 is not supported by the column kernels");
+    case 5: case 6: case 7: case 8: f(std::integral_constant<int, 8>{}); break;
+    default: throw std::runtime_error("nz > 1024 is not supported by the column kernels");
   }
 }
 
@@ -1710,7 +1716,7 @@ int device_cus() {
 }
 
 void Pcg::initCommon() {
-  if (g_.nz > 512) throw std::runtime_error("nz > 512 is not supported by the column kernels");
+  if (g_.nz > 1024) throw std::runtime_error("nz > 1024 is not supported by the column kernels");
   cus_ = device_cus();
   pairs_ = g_.nz % 2 == 0;
   cw2_ = pairs_ && g_.nx % 2 == 0;
@@ -1869,17 +1875,20 @@ int Pcg::solve(const Stencil<double>& A, const double* b, double* x, const PcgOp
 void Pcg::setColumnSolve(const int* kt, const std::vector<int>& ktHost) {
   if (external_) throw std::runtime_error("the column solve needs its own workspace");
   kt_ = kt;
-  std::vector<std::vector<int>> byR(9);
+  // R = ceil(active rows / 64), 1 .. 8, and one deep bucket (R = 16) for
+  // 513 .. 1024 rows
+  std::vector<std::vector<int>> byR(17);
   for (int j = 1; j <= g_.ny; ++j)
     for (int i = 1; i <= g_.nx; ++i) {
       const int L = g_.nz - ktHost[static_cast<size_t>(g_.c2(j, i))];
       if (L <= 0) continue;
-      byR[(L + kWave - 1) / kWave].push_back((j - 1) * g_.nx + (i - 1));
+      const int R = (L + kWave - 1) / kWave;
+      byR[R <= 8 ? R : 16].push_back((j - 1) * g_.nx + (i - 1));
     }
   bucketCols_.clear();
   bucketCount_.clear();
   bucketR_.clear();
-  for (int R = 8; R >= 1; --R) {  // deepest first: the longest waves start first
+  for (int R = 16; R >= 1; --R) {  // deepest first: the longest waves start first
     if (byR[R].empty()) continue;
     bucketR_.push_back(R);
     bucketCount_.push_back(static_cast<int>(byR[R].size()));

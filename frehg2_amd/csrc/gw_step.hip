@@ -2121,7 +2121,11 @@ void dispatch_col(int nz, F&& f) {
     case 6: f(std::integral_constant<int, 6>{}); break;
     case 7: f(std::integral_constant<int, 7>{}); break;
     case 8: f(std::integral_constant<int, 8>{}); break;
-    default: fail(FREHG_EUNSUPPORTED, "nz > 512 is not supported by the column kernels");
+    // 513 .. 1024 rows: one deep form (rows past the column are masked)
+    case 9: case 10: case 11: case 12: case 13: case 14: case 15: case 16:
+      f(std::integral_constant<int, 16>{});
+      break;
+    default: fail(FREHG_EUNSUPPORTED, "nz > 1024 is not supported by the column kernels");
   }
 }
 
@@ -2597,7 +2601,7 @@ void GwSolver::buildGeometry(const frehg_gw_desc& ds, std::vector<double>& bath,
 void GwSolver::create(const frehg_gw_desc& ds) {
   if (ds.nx < 1 || ds.ny < 1 || ds.nz < 1) fail(FREHG_EINVAL, "extents must be positive");
   if (!(ds.dx > 0) || !(ds.dy > 0) || !(ds.dz > 0)) fail(FREHG_EINVAL, "spacings must be positive");
-  if (ds.nz > 512) fail(FREHG_EUNSUPPORTED, "nz > 512 is not supported");
+  if (ds.nz > 1024) fail(FREHG_EUNSUPPORTED, "nz > 1024 is not supported");
   if (ds.predictor_corrector != 0 && ds.predictor_corrector != 1)
     fail(FREHG_EINVAL, "predictor_corrector: 1 consistent or 0 legacy");
   if (ds.saturated_contact != 1)
@@ -2606,6 +2610,9 @@ void GwSolver::create(const frehg_gw_desc& ds) {
     fail(FREHG_EUNSUPPORTED, "icc0 is a CPU-only preconditioner; use zline or jacobi");
   if (ds.preconditioner < FREHG_PC_JACOBI || ds.preconditioner > FREHG_PC_CHEBYSHEV)
     fail(FREHG_EINVAL, "unknown preconditioner");
+  if (ds.preconditioner == FREHG_PC_CHEBYSHEV &&
+      (ds.precision != FREHG_PREC_MIXED || ds.nz % 2 != 0))
+    fail(FREHG_EINVAL, "the Chebyshev preconditioner is the mixed form's (FP32) and needs an even nz");
   if (!ds.ksx || !ds.ksy || !ds.ksz || !ds.vg_alpha || !ds.vg_n || !ds.theta_s || !ds.theta_r ||
       !ds.aev)
     fail(FREHG_EINVAL, "all eight per-cell soil arrays are required");
@@ -2855,8 +2862,6 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   popt.zline = ds.preconditioner == FREHG_PC_ZLINE;
   popt.mixed = ds.precision == FREHG_PREC_MIXED;
   popt.cheb = ds.preconditioner == FREHG_PC_CHEBYSHEV;
-  if (popt.cheb && (!popt.mixed || nz % 2 != 0))
-    fail(FREHG_EINVAL, "the Chebyshev preconditioner is the mixed form's (FP32) and needs an even nz");
   popt.inner_rtol = ds.inner_rtol > 0 ? ds.inner_rtol : 1e-4;
 
   // initial conditions (RichardsSolver.cpp:331-427)

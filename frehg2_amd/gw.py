@@ -207,7 +207,7 @@ class RichardsSolver:
     def column_tiles(self) -> list:
         """Depth buckets of the batched column solve: [{R, columns, cpw}]
         (empty when the operator has lateral legs)."""
-        cap = 8
+        cap = 9
         R, n, cpw = (C.c_int * cap)(), (C.c_int * cap)(), (C.c_int * cap)()
         nb = C.c_int()
         check(self.lib, self.lib.frehg_gw_get_column_tiles(self._h, R, n, cpw, cap, C.byref(nb)))

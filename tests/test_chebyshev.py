@@ -41,8 +41,9 @@ def _variant(case, pc, precision="mixed"):
 def test_options_are_validated():
     from frehg2_amd.gw import RichardsSolver
     case, _ = WORKLOADS["stiff_32x32x50"]()
-    with pytest.raises(abi.FrehgError):  # the polynomial is the FP32 form's
+    with pytest.raises(abi.FrehgError) as e:  # the polynomial is the FP32 form's
         RichardsSolver(_variant(case, "chebyshev", precision="fp64"))
+    assert e.value.code == abi.FREHG_EINVAL and "Chebyshev" in str(e.value)
     with pytest.raises(abi.FrehgError):  # the oracle has no polynomial preconditioner
         from oracle.oracle import OracleRichards
         OracleRichards(_variant(case, "chebyshev"))

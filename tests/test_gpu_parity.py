@@ -167,7 +167,8 @@ def test_stepwise_loam():
 
 
 def test_stepwise_deepest_columns():
-    """nz = 512, the deepest column the column kernels take (8 rows a lane)."""
+    """nz = 512, the deepest column of the column kernels' regular forms
+    (8 rows a lane; deeper ones take the R = 16 form)."""
     _stepwise(cases.loam_32x32x50(nx=4, ny=3, nz=512), 3)
 
 
@@ -176,9 +177,11 @@ def test_stepwise_single_layer():
 
 
 def test_deeper_columns_are_rejected():
+    """Columns of up to 1024 cells run (tests/test_deep_columns.py); deeper
+    ones are rejected."""
     from frehg2_amd.gw import RichardsSolver
     with pytest.raises(abi.FrehgError) as e:
-        RichardsSolver(cases.loam_32x32x50(nx=2, ny=2, nz=513))
+        RichardsSolver(cases.loam_32x32x50(nx=2, ny=2, nz=1025))
     assert e.value.code == abi.FREHG_EUNSUPPORTED
 
 
