@@ -77,6 +77,8 @@ def parse():
     ap.add_argument("--member-shape", default="128,128,64")
     ap.add_argument("--columns", action="store_true",
                     help="run the uneven-columns workload instead (4096 columns of 50-400 cells)")
+    ap.add_argument("--step-mode", default="host", choices=["host", "device", "graph"],
+                    help="ensemble: step mode of the stacked legs (frehg_gw_set_step_mode)")
     ap.add_argument("--pc-sweep", action="store_true",
                     help="mixed form's inner preconditioners (zline, jacobi, chebyshev 3/5/7) on "
                          "the stiff 32x32x50 and the 512x512x100 workloads: time, iterations "
@@ -397,6 +399,7 @@ def ensemble_bench(args):
     # every other leg replays
     ens.case.precision = "mixed"
     g = RichardsSolver(ens.case)
+    g.set_step_mode(args.step_mode)
     cm = Coupler(g, bottom, dt=dt0, mode="sync")
     warm_dts, dts = [], []
     t = 0.0
@@ -405,7 +408,8 @@ def ensemble_bench(args):
         t += warm_dts[-1]
         cm.step(t, warm_dts[-1])
     g.synchronize()
-    g.set_timing(SPMV_SAMPLE)
+    if args.step_mode == "host":
+        g.set_timing(SPMV_SAMPLE)  # timing runs host-polled steps
     its = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -442,13 +446,15 @@ def ensemble_bench(args):
 
     ens.case.precision = "fp64"
     g = RichardsSolver(ens.case)
+    g.set_step_mode(args.step_mode)
     c64 = Coupler(g, bottom, dt=dt0, mode="sync")
     t = 0.0
     for dt in warm_dts:
         t += dt
         c64.step(t, dt)
     g.synchronize()
-    g.set_timing(SPMV_SAMPLE)
+    if args.step_mode == "host":
+        g.set_timing(SPMV_SAMPLE)
     its = 0
     t0 = time.perf_counter()
     for dt in dts:
@@ -507,6 +513,7 @@ def ensemble_bench(args):
         "dtype": "f32+f64",
         "data": "synthetic (seeded generator, inputs resident in HBM)",
         "config": {"workload": "richards_coupled_128x128x64 ensemble", "members": M,
+                   "step_mode_stacked": args.step_mode,
                    "member_grid": [nx, ny, nz], "stacked_grid": [ens.case.nx, ens.case.ny, nz],
                    "dt": "adaptive (common to all members)",
                    "coupling": "sync coupled step per substep, surface at rest",
