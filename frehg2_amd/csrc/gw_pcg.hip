@@ -339,12 +339,14 @@ __global__ __launch_bounds__(64) void k_factor(Grid g, const T* __restrict__ Ad,
 // the leg to row k + 1) are the ratios p_k / p_{k-1} of the continuants
 // p_k = a_k p_{k-1} - c_{k-1}^2 p_{k-2}, i.e. v_k = M_k v_{k-1} with
 // M_k = [[a_k, -c_{k-1}^2], [1, 0]] and v_{-1} = (1, 0).  The prefix products
-// M_k ... M_0 are an inclusive scan over the lanes of each row segment
-// (Hillis-Steele, six levels, 2 x 2 products), each product rescaled by a
-// power of two (exact) so it stays in range; the segment's last vector
-// carries into the next segment.  Then the LDL^T solve (line_solve), one
-// FP64 refinement x += M^-1 (b - A x) (the scan's pivots can lose digits to
-// cancellation on near-singular columns) and the residual partials.
+// M_k ... M_0 are an inclusive scan over the lanes of each row segment (four
+// DPP steps inside rows of 16 lanes, then the vectors entering rows 1-3 from
+// the composed matrices of lanes 15, 31 and 47, as the affine scans do),
+// each product rescaled by a power of two (exact) so it stays in range; the
+// segment's last vector carries into the next segment.  Then the LDL^T solve
+// (line_solve), one FP64 refinement x += M^-1 (b - A x) (the scan's pivots
+// can lose digits to cancellation on near-singular columns) and the
+// residual partials.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void rescale2(double& a, double& b, double& c, double& d) {
   const double mx = fmax(fmax(fabs(a), fabs(b)), fmax(fabs(c), fabs(d)));
@@ -357,6 +359,34 @@ __device__ __forceinline__ void rescale2(double& a, double& b, double& c, double
   d = ldexp(d, -e);
 }
 
+// One step of the 2 x 2 product scan inside rows of 16 lanes: lane l's
+// matrix times the one S lanes before it (DPP row shift; lanes without such
+// a neighbour in their row take the identity).
+template <int S>
+__device__ __forceinline__ void mat_step(double& m00, double& m01, double& m10, double& m11) {
+  constexpr int ctrl = kDppRowShr + S;
+  const double p00 = dpp_move<ctrl>(1.0, m00), p01 = dpp_move<ctrl>(0.0, m01);
+  const double p10 = dpp_move<ctrl>(0.0, m10), p11 = dpp_move<ctrl>(1.0, m11);
+  double n00 = m00 * p00 + m01 * p10, n01 = m00 * p01 + m01 * p11;
+  double n10 = m10 * p00 + m11 * p10, n11 = m10 * p01 + m11 * p11;
+  rescale2(n00, n01, n10, n11);
+  m00 = n00;
+  m01 = n01;
+  m10 = n10;
+  m11 = n11;
+}
+
+// x <- (lane l's matrix) x, the matrix read with readlane (wave-uniform).
+__device__ __forceinline__ void mat_apply_lane(double m00, double m01, double m10, double m11,
+                                               int l, double& x0, double& x1) {
+  const double a = lane_value(m00, l), b = lane_value(m01, l);
+  const double c = lane_value(m10, l), d = lane_value(m11, l);
+  double y0 = a * x0 + b * x1, y1 = c * x0 + d * x1, z0 = 0.0, z1 = 0.0;
+  rescale2(y0, y1, z0, z1);
+  x0 = y0;
+  x1 = y1;
+}
+
 template <int R>
 __device__ __forceinline__ void continuant_factor(int lane, int L, const double (&a)[R],
                                                   const double (&c)[R], double (&iv)[R],
@@ -364,6 +394,7 @@ __device__ __forceinline__ void continuant_factor(int lane, int L, const double 
   double cUp[R], dummy[R];
   shift_rows<double, R>(c, lane, dummy, cUp);  // cUp[m]: the leg of row k - 1
   double v0 = 1.0, v1 = 0.0;                   // v_{k0 - 1}
+  const int row = lane >> 4;
 #pragma unroll
   for (int m = 0; m < R; ++m) {
     // rows past the column take the identity map (their pivots repeat the
@@ -371,28 +402,30 @@ __device__ __forceinline__ void continuant_factor(int lane, int L, const double 
     const bool in = lane + kWave * m < L;
     double m00 = in ? a[m] : 1.0, m01 = in ? -cUp[m] * cUp[m] : 0.0;
     double m10 = in ? 1.0 : 0.0, m11 = in ? 0.0 : 1.0;
+    // products from the start of each 16-lane row, then the vectors entering
+    // rows 1-3 from the composed matrices of lanes 15, 31 and 47
+    mat_step<1>(m00, m01, m10, m11);
+    mat_step<2>(m00, m01, m10, m11);
+    mat_step<4>(m00, m01, m10, m11);
+    mat_step<8>(m00, m01, m10, m11);
+    double e0[4], e1[4];
+    e0[0] = v0;
+    e1[0] = v1;
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const double p00 = __shfl_up(m00, off, kWave), p01 = __shfl_up(m01, off, kWave);
-      const double p10 = __shfl_up(m10, off, kWave), p11 = __shfl_up(m11, off, kWave);
-      if (lane >= off) {
-        double n00 = m00 * p00 + m01 * p10, n01 = m00 * p01 + m01 * p11;
-        double n10 = m10 * p00 + m11 * p10, n11 = m10 * p01 + m11 * p11;
-        rescale2(n00, n01, n10, n11);
-        m00 = n00;
-        m01 = n01;
-        m10 = n10;
-        m11 = n11;
-      }
+    for (int r = 1; r < 4; ++r) {
+      e0[r] = e0[r - 1];
+      e1[r] = e1[r - 1];
+      mat_apply_lane(m00, m01, m10, m11, 16 * r - 1, e0[r], e1[r]);
     }
-    const double u0 = m00 * v0 + m01 * v1, u1 = m10 * v0 + m11 * v1;
+    const double x0 = row == 0 ? e0[0] : row == 1 ? e0[1] : row == 2 ? e0[2] : e0[3];
+    const double x1 = row == 0 ? e1[0] : row == 1 ? e1[1] : row == 2 ? e1[2] : e1[3];
+    const double u0 = m00 * x0 + m01 * x1, u1 = m10 * x0 + m11 * x1;
     const double delta = u0 / u1;
     iv[m] = 1.0 / delta;
     wv[m] = c[m] * iv[m];
-    v0 = __shfl(u0, kWave - 1, kWave);
-    v1 = __shfl(u1, kWave - 1, kWave);
-    double z0 = 0.0, z1 = 0.0;
-    rescale2(v0, v1, z0, z1);
+    v0 = e0[3];
+    v1 = e1[3];
+    mat_apply_lane(m00, m01, m10, m11, kWave - 1, v0, v1);
   }
 }
 
@@ -446,9 +479,31 @@ __device__ __forceinline__ void column_direct(const Grid& g, const Stencil<doubl
   }
 }
 
-// One launch over every column with active rows, deepest bucket first:
-// wave w of bucket b solves columns order[colStart + (w - waveStart) * cpw
-// + q], q < cpw.  Partials rr (slot 1), bb (slot 3).
+// column_direct<R> for a run-time R <= RMAX (R in 1 .. 8, or 16 for 513 ..
+// 1024 rows): the branches stop at the deepest form the launch holds, so a
+// table without deep columns does not take the R = 16 form's registers.
+template <int RMAX>
+__device__ __forceinline__ void column_direct_upto(int R, const Grid& g,
+                                                   const Stencil<double>& A,
+                                                   const double* __restrict__ b,
+                                                   double* __restrict__ x, int64_t base, int k0,
+                                                   int lane, KSum& rr, KSum& bb) {
+  if constexpr (RMAX == 16) {
+    if (R > 8) column_direct<16>(g, A, b, x, base, k0, lane, rr, bb);
+    else column_direct_upto<8>(R, g, A, b, x, base, k0, lane, rr, bb);
+  } else if constexpr (RMAX > 1) {
+    if (R == RMAX) column_direct<RMAX>(g, A, b, x, base, k0, lane, rr, bb);
+    else column_direct_upto<RMAX - 1>(R, g, A, b, x, base, k0, lane, rr, bb);
+  } else {
+    column_direct<1>(g, A, b, x, base, k0, lane, rr, bb);
+  }
+}
+
+// One launch over every column with active rows, deepest bucket first (the
+// longest waves start first and the shallow ones fill in behind them): wave
+// w of bucket b solves columns order[colStart + (w - waveStart) * cpw + q],
+// q < cpw.  RMAX is the deepest bucket's R.  Partials rr (slot 1), bb (slot 3).
+template <int RMAX>
 __global__ __launch_bounds__(kBlock) void k_column_solve(Grid g, Stencil<double> A,
                                                          const double* __restrict__ b,
                                                          double* __restrict__ x,
@@ -467,19 +522,8 @@ __global__ __launch_bounds__(kBlock) void k_column_solve(Grid g, Stencil<double>
     for (int q = first; q < last; ++q) {
       const int col = order[q];
       const int j = col / g.nx + 1, i = col % g.nx + 1;
-      const int64_t base = g.c3(j, i, 0);
-      const int k0 = kt[g.c2(j, i)];
-      switch (tab.R[bk]) {
-        case 1: column_direct<1>(g, A, b, x, base, k0, lane, rr, bb); break;
-        case 2: column_direct<2>(g, A, b, x, base, k0, lane, rr, bb); break;
-        case 3: column_direct<3>(g, A, b, x, base, k0, lane, rr, bb); break;
-        case 4: column_direct<4>(g, A, b, x, base, k0, lane, rr, bb); break;
-        case 5: column_direct<5>(g, A, b, x, base, k0, lane, rr, bb); break;
-        case 6: column_direct<6>(g, A, b, x, base, k0, lane, rr, bb); break;
-        case 7: column_direct<7>(g, A, b, x, base, k0, lane, rr, bb); break;
-        case 8: column_direct<8>(g, A, b, x, base, k0, lane, rr, bb); break;
-        default: column_direct<16>(g, A, b, x, base, k0, lane, rr, bb); break;
-      }
+      column_direct_upto<RMAX>(tab.R[bk], g, A, b, x, g.c3(j, i, 0), kt[g.c2(j, i)], lane, rr,
+                               bb);
     }
   }
   double v[2] = {rr.value(), bb.value()};
@@ -1943,8 +1987,23 @@ int Pcg::columnTiles(int* R, int* counts, int* cpw, int cap) const {
 }
 
 void Pcg::launchColumnSolve(const Stencil<double>& A, const double* b, double* x) {
-  k_column_solve<<<colBlocks_, kBlock, 0, stream_>>>(g_, A, b, x, kt_, order_, tab_, parts_,
-                                                    pstride_);
+  // buckets are deepest first; no bucket (no active cell) runs one block of
+  // idle waves that writes zero partials
+  auto go = [&](auto RC) {
+    k_column_solve<decltype(RC)::value><<<colBlocks_, kBlock, 0, stream_>>>(
+        g_, A, b, x, kt_, order_, tab_, parts_, pstride_);
+  };
+  switch (tab_.buckets ? tab_.R[0] : 1) {
+    case 1: go(std::integral_constant<int, 1>{}); break;
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 3: go(std::integral_constant<int, 3>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 5: go(std::integral_constant<int, 5>{}); break;
+    case 6: go(std::integral_constant<int, 6>{}); break;
+    case 7: go(std::integral_constant<int, 7>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    default: go(std::integral_constant<int, 16>{}); break;
+  }
   HIP_OK(hipGetLastError());
 }
 
