@@ -26,8 +26,8 @@ import numpy as np
 from . import abi
 
 CANDIDATES = {
-    "march64": (8, 16, 32),
-    "march32": (32, 64, 128),
+    "march64": (4, 8, 16),
+    "march32": (4, 8, 32),
     "column": (8, 16, 32),
 }
 COUPLING_NAMES = ("eta", "dept", "seep_accum", "avail", "gain")

@@ -160,11 +160,13 @@ class Pcg {
   // Algorithmic bytes of one launch of the stencil kernel (the dominant one).
   double spmvBytesPerLaunch(bool mixed) const;
 
-  // Launch shapes, as waves per CU: the FP64 and FP32 stencil (j-march)
-  // kernels and the column kernels (0 keeps a value).  Defaults 16 / 32 / 16
-  // (the fastest of a sweep at 512 x 512 x 100).  Results depend on the
-  // shapes at rounding level (the order of the partial sums) and are
-  // deterministic for given shapes.
+  // Launch shapes, as waves per CU: the FP64 and FP32 direction kernels and
+  // the column kernels (0 keeps a value).  Defaults 4 / 4 / 16 where the
+  // direction kernels run over column pairs (nx and nz even), else 16 / 32
+  // / 16 (the fastest of sweeps at 512 x 512 x 100).  The FP64 residual and
+  // refinement sweeps keep 16 waves per CU.  Results depend on the shapes at
+  // rounding level (the order of the partial sums) and are deterministic for
+  // given shapes.
   void setShapes(int march64WavesPerCu, int march32WavesPerCu, int columnWavesPerCu);
   void shapes(int* march64WavesPerCu, int* march32WavesPerCu, int* columnWavesPerCu) const {
     *march64WavesPerCu = wpc64_;
@@ -226,25 +228,36 @@ class Pcg {
 
   struct Shapes {
     int blocks = 0, pstride = 0;
-    March march64, march32, march32w;
+    March march64, march32, march32w, march64w;
   };
   static Shapes shapesFor(const Grid& g, int cus, int wpcCol, int wpc64, int wpc32);
+  // default direction-kernel shapes of grid g
+  static void defaultShapes(const Grid& g, int* wpc64, int* wpc32);
   static March makeMarch(const Grid& g, int cus, int wavesPerCu, int columnsPerWave = 1);
   // block count of the direction kernel (whose partial sums the update reduces)
   int dirBlocks(bool fp32, const March& mj) const {
-    return fp32 && pairs_ && cw2_ ? march32w_.blocks : mj.blocks;
+    if (fp32) return pairs_ && cw2_ ? march32w_.blocks : mj.blocks;
+    return pairs64_ && cw2(false) ? march64w_.blocks : mj.blocks;
   }
+  // the direction kernel runs over column pairs (FP64: columns of at most
+  // kCw2Rows64 rows; deeper pairs spill registers)
+  bool cw2(bool fp32) const { return cw2_ && (fp32 || g_.nz <= kCw2Rows64); }
+  static constexpr int kCw2Rows64 = 256;
+  // the FP64 operands of a solve allow the pair layout (16-byte aligned)
+  void choosePairs64(const Stencil<double>& A, const double* b, const double* x);
 
   Grid g_;
   hipStream_t stream_;
   int blocks_ = 0;            // grid size of the column kernels (multiple of 8)
   March march64_, march32_;   // stencil-kernel shapes, FP64 and FP32 operands
   March march32w_;            // FP32 direction kernel over column pairs (cw2_)
+  March march64w_;            // FP64 direction kernel over column pairs (cw2_, pairs64_)
   int pstride_ = 0;           // slot stride of parts_ (largest grid)
   int partsCap_ = 0;          // slot stride parts_ was allocated for
   int cus_ = 256;
   int wpc64_ = 16, wpc32_ = 32, wpcCol_ = 16;  // waves per CU (setShapes)
   bool pairs_ = false;        // FP32 kernels in the row-pair layout (nz even)
+  bool pairs64_ = false;      // FP64 iteration kernels likewise (nz even, operands aligned)
   bool cw2_ = false;          // ... and the direction kernel over column pairs (nx even)
   // FP64 vectors (halo layout)
   double *r_ = nullptr, *z_ = nullptr, *p0_ = nullptr, *p1_ = nullptr, *q_ = nullptr;

@@ -879,58 +879,71 @@ __global__ __launch_bounds__(kBlock) void k_inner_init(Grid g, const double* __r
 }
 
 // ---------------------------------------------------------------------------
-// FP32 operands in the row-pair layout: lane l of row segment m holds rows
+// Operands in the row-pair layout: lane l of row segment m holds rows
 // k0 = 128 m + 2 l and k1 = k0 + 1 (R2 = ceil(nz / 128) segments), so every
-// load and store moves 8 bytes a lane, and the z-line recurrences first
-// compose each lane's two steps and then scan over half as many lanes-rows.
-// Used when nz is even (the pairs are then 8-byte aligned).
+// load and store moves two elements a lane (8 bytes in FP32, 16 in FP64),
+// and the z-line recurrences first compose each lane's two steps and then
+// scan over half as many lanes-rows.  Used when nz is even (the pairs are
+// then aligned to their size).
 // ---------------------------------------------------------------------------
-template <int R2>
-__device__ __forceinline__ void load_pair(const float* __restrict__ a, int64_t base, int lane,
-                                          int nz, float (&v0)[R2], float (&v1)[R2]) {
+template <class T> struct PairOf;
+template <> struct PairOf<float> { typedef float v __attribute__((ext_vector_type(2))); };
+template <> struct PairOf<double> { typedef double v __attribute__((ext_vector_type(2))); };
+
+template <int R2, class T>
+__device__ __forceinline__ void load_pair(const T* __restrict__ a, int64_t base, int lane, int nz,
+                                          T (&v0)[R2], T (&v1)[R2]) {
+  using V = typename PairOf<T>::v;
 #pragma unroll
   for (int m = 0; m < R2; ++m) {
     const int k0 = 128 * m + 2 * lane;
     // nz is even in this layout: the last pair (nz - 2, nz - 1) stands in
-    const float2 t = *reinterpret_cast<const float2*>(a + base + (k0 < nz ? k0 : nz - 2));
-    v0[m] = k0 < nz ? t.x : 0.0f;
-    v1[m] = k0 < nz ? t.y : 0.0f;
+    const V t = *reinterpret_cast<const V*>(a + base + (k0 < nz ? k0 : nz - 2));
+    v0[m] = k0 < nz ? t.x : T(0);
+    v1[m] = k0 < nz ? t.y : T(0);
   }
 }
 
-template <int R2>
-__device__ __forceinline__ void store_pair(float* __restrict__ a, int64_t base, int lane, int nz,
-                                           const float (&v0)[R2], const float (&v1)[R2]) {
+// NT: non-temporal stores (streamed past the caches' retention)
+template <int R2, bool NT = false, class T>
+__device__ __forceinline__ void store_pair(T* __restrict__ a, int64_t base, int lane, int nz,
+                                           const T (&v0)[R2], const T (&v1)[R2]) {
+  using V = typename PairOf<T>::v;
 #pragma unroll
   for (int m = 0; m < R2; ++m) {
     const int k0 = 128 * m + 2 * lane;
-    if (k0 < nz) *reinterpret_cast<float2*>(a + base + k0) = make_float2(v0[m], v1[m]);
+    V t;
+    t.x = v0[m];
+    t.y = v1[m];
+    if (k0 < nz) {
+      if constexpr (NT) __builtin_nontemporal_store(t, reinterpret_cast<V*>(a + base + k0));
+      else *reinterpret_cast<V*>(a + base + k0) = t;
+    }
   }
 }
 
 // Row k1 + 1 (dn) and row k0 - 1 (up) of a column in the pair layout.
-template <int R2>
-__device__ __forceinline__ void pair_shift(const float (&v0)[R2], const float (&v1)[R2], int lane,
-                                           float (&dn)[R2], float (&up)[R2]) {
+template <int R2, class T>
+__device__ __forceinline__ void pair_shift(const T (&v0)[R2], const T (&v1)[R2], int lane,
+                                           T (&dn)[R2], T (&up)[R2]) {
 #pragma unroll
   for (int m = 0; m < R2; ++m) {
-    const float nx0 = __shfl_down(v0[m], 1, kWave);
-    const float seg = (m + 1 < R2) ? __shfl(v0[m + 1 < R2 ? m + 1 : m], 0, kWave) : 0.0f;
+    const T nx0 = __shfl_down(v0[m], 1, kWave);
+    const T seg = (m + 1 < R2) ? __shfl(v0[m + 1 < R2 ? m + 1 : m], 0, kWave) : T(0);
     dn[m] = lane < kWave - 1 ? nx0 : seg;
-    const float pv1 = __shfl_up(v1[m], 1, kWave);
-    const float segp = (m > 0) ? __shfl(v1[m > 0 ? m - 1 : 0], kWave - 1, kWave) : 0.0f;
+    const T pv1 = __shfl_up(v1[m], 1, kWave);
+    const T segp = (m > 0) ? __shfl(v1[m > 0 ? m - 1 : 0], kWave - 1, kWave) : T(0);
     up[m] = lane > 0 ? pv1 : segp;
   }
 }
 
 // z = M^-1 r in the pair layout (the recurrences of apply_precond).
-template <int R2, bool LINE>
+template <int R2, bool LINE, class T>
 __device__ __forceinline__ void precond_pair(int lane, int nz, int64_t base,
-                                             const float* __restrict__ invd,
-                                             const float* __restrict__ w, const float (&r0)[R2],
-                                             const float (&r1)[R2], float (&z0)[R2],
-                                             float (&z1)[R2]) {
-  float iv0[R2], iv1[R2];
+                                             const T* __restrict__ invd, const T* __restrict__ w,
+                                             const T (&r0)[R2], const T (&r1)[R2], T (&z0)[R2],
+                                             T (&z1)[R2]) {
+  T iv0[R2], iv1[R2];
   load_pair<R2>(invd, base, lane, nz, iv0, iv1);
   if constexpr (!LINE) {
 #pragma unroll
@@ -940,44 +953,44 @@ __device__ __forceinline__ void precond_pair(int lane, int nz, int64_t base,
     }
     return;
   } else {
-    float w0[R2], w1[R2], wDn[R2], wUp[R2];
+    T w0[R2], w1[R2], wDn[R2], wUp[R2];
     load_pair<R2>(w, base, lane, nz, w0, w1);
     pair_shift<R2>(w0, w1, lane, wDn, wUp);
-    float y0[R2], y1[R2];
-    float carry = 0.0f;
+    T y0[R2], y1[R2];
+    T carry = T(0);
 #pragma unroll
     for (int m = 0; m < R2; ++m) {
       // y_k = r_k - w_{k-1} y_{k-1}: rows k0 then k1, composed per lane
-      const float a0 = r0[m], b0 = -wUp[m];
-      const float a1 = r1[m], b1 = -w0[m];
-      const float cin = carry;
-      const float Y1 = scan_affine_up(a1 + b1 * a0, b1 * b0, carry, lane);
-      const float P = dpp_move<kDppWaveShr1>(cin, Y1);  // row k0 - 1
+      const T a0 = r0[m], b0 = -wUp[m];
+      const T a1 = r1[m], b1 = -w0[m];
+      const T cin = carry;
+      const T Y1 = scan_affine_up(a1 + b1 * a0, b1 * b0, carry, lane);
+      const T P = dpp_move<kDppWaveShr1>(cin, Y1);  // row k0 - 1
       y0[m] = a0 + b0 * P;
       y1[m] = Y1;
     }
-    carry = 0.0f;
+    carry = T(0);
 #pragma unroll
     for (int m = R2 - 1; m >= 0; --m) {
       // z_k = y_k invd_k - w_k z_{k+1}: rows k1 then k0, composed per lane
-      const float a0 = y0[m] * iv0[m], b0 = -w0[m];
-      const float a1 = y1[m] * iv1[m], b1 = -w1[m];
-      const float cin = carry;
-      const float Z0 = scan_affine_down(a0 + b0 * a1, b0 * b1, carry, lane);
-      const float Nx = dpp_move<kDppWaveShl1>(cin, Z0);  // row k1 + 1
+      const T a0 = y0[m] * iv0[m], b0 = -w0[m];
+      const T a1 = y1[m] * iv1[m], b1 = -w1[m];
+      const T cin = carry;
+      const T Z0 = scan_affine_down(a0 + b0 * a1, b0 * b1, carry, lane);
+      const T Nx = dpp_move<kDppWaveShl1>(cin, Z0);  // row k1 + 1
       z0[m] = Z0;
       z1[m] = a1 + b1 * Nx;
     }
   }
 }
 
-// k_pcg_dir for FP32 operands in the pair layout.
-template <int R2, bool RESTART>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir_p(Grid g, March mj, Stencil<float> A,
-                                                      const float* __restrict__ z,
-                                                      const float* __restrict__ pOld,
-                                                      float* __restrict__ pNew,
-                                                      float* __restrict__ q,
+// k_pcg_dir in the pair layout (one column a wave).
+template <class T, int R2, bool RESTART>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir_p(Grid g, March mj, Stencil<T> A,
+                                                      const T* __restrict__ z,
+                                                      const T* __restrict__ pOld,
+                                                      T* __restrict__ pNew,
+                                                      T* __restrict__ q,
                                                       double* __restrict__ parts, int pstride,
                                                       int nprev, PcgState* __restrict__ st,
                                                       int n) {
@@ -1007,15 +1020,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p(Grid g, March mj, Stencil<
     st->it = n;
     st->rr = rrv;
   }
-  const float beta = static_cast<float>(betaD);
+  const T beta = static_cast<T>(betaD);
   const int lane = threadIdx.x & (kWave - 1);
   const int nz = g.nz;
   const int64_t sx = nz, sy = static_cast<int64_t>(g.nx2) * nz;
   KSum pq;
   int i, j0, j1;
   if (march_span(g, mj, i, j0, j1)) {
-    auto formp = [&](int64_t base, float (&p0)[R2], float (&p1)[R2]) {
-      float z0[R2], z1[R2];
+    auto formp = [&](int64_t base, T (&p0)[R2], T (&p1)[R2]) {
+      T z0[R2], z1[R2];
       load_pair<R2>(z, base, lane, nz, z0, z1);
       if (RESTART) {
 #pragma unroll
@@ -1024,7 +1037,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p(Grid g, March mj, Stencil<
           p1[m] = z1[m];
         }
       } else {
-        float o0[R2], o1[R2];
+        T o0[R2], o1[R2];
         load_pair<R2>(pOld, base, lane, nz, o0, o1);
 #pragma unroll
         for (int m = 0; m < R2; ++m) {
@@ -1033,27 +1046,27 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p(Grid g, March mj, Stencil<
         }
       }
     };
-    float pP0[R2], pP1[R2], pC0[R2], pC1[R2], ayP0[R2], ayP1[R2];
+    T pP0[R2], pP1[R2], pC0[R2], pC1[R2], ayP0[R2], ayP1[R2];
     formp(g.c3(j0 - 1, i, 0), pP0, pP1);
     formp(g.c3(j0, i, 0), pC0, pC1);
     load_pair<R2>(A.Ayp, g.c3(j0 - 1, i, 0), lane, nz, ayP0, ayP1);
     for (int j = j0; j <= j1; ++j) {
       const int64_t base = g.c3(j, i, 0);
-      float pN0[R2], pN1[R2], px0[R2], px1[R2], pm0[R2], pm1[R2];
+      T pN0[R2], pN1[R2], px0[R2], px1[R2], pm0[R2], pm1[R2];
       formp(base + sy, pN0, pN1);
       formp(base + sx, px0, px1);
       formp(base - sx, pm0, pm1);
-      float ad0[R2], ad1[R2], ax0[R2], ax1[R2], am0[R2], am1[R2], ay0[R2], ay1[R2], az0[R2],
+      T ad0[R2], ad1[R2], ax0[R2], ax1[R2], am0[R2], am1[R2], ay0[R2], ay1[R2], az0[R2],
           az1[R2];
       load_pair<R2>(A.Ad, base, lane, nz, ad0, ad1);
       load_pair<R2>(A.Axp, base, lane, nz, ax0, ax1);
       load_pair<R2>(A.Axp, base - sx, lane, nz, am0, am1);
       load_pair<R2>(A.Ayp, base, lane, nz, ay0, ay1);
       load_pair<R2>(A.Azp, base, lane, nz, az0, az1);
-      float pDn[R2], pUp[R2], azDn[R2], azUp[R2];
+      T pDn[R2], pUp[R2], azDn[R2], azUp[R2];
       pair_shift<R2>(pC0, pC1, lane, pDn, pUp);
       pair_shift<R2>(az0, az1, lane, azDn, azUp);
-      float q0[R2], q1[R2];
+      T q0[R2], q1[R2];
 #pragma unroll
       for (int m = 0; m < R2; ++m) {
         q0[m] = ad0[m] * pC0[m] + ax0[m] * px0[m] + am0[m] * pm0[m] + ay0[m] * pN0[m] +
@@ -1203,13 +1216,14 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step_p(Grid g, March mj, Stenci
 // k_pcg_dir_p with two neighbouring columns (i, i + 1) per wave: their p is
 // formed once for both stencils, the shared x leg Axp(i) is read once, and
 // only the outer neighbours i - 1 and i + 2 come from the XCD's L2 (half the
-// cache loads of the one-column form).  nx even.
-template <int R2, bool RESTART>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil<float> A,
-                                                       const float* __restrict__ z,
-                                                       const float* __restrict__ pOld,
-                                                       float* __restrict__ pNew,
-                                                       float* __restrict__ q,
+// cache loads of the one-column form).  nx even.  NTS: p and q are stored
+// non-temporally.
+template <class T, int R2, bool RESTART, bool NTS>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil<T> A,
+                                                       const T* __restrict__ z,
+                                                       const T* __restrict__ pOld,
+                                                       T* __restrict__ pNew,
+                                                       T* __restrict__ q,
                                                        double* __restrict__ parts, int pstride,
                                                        int nprev, PcgState* __restrict__ st,
                                                        int n) {
@@ -1239,15 +1253,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil
     st->it = n;
     st->rr = rrv;
   }
-  const float beta = static_cast<float>(betaD);
+  const T beta = static_cast<T>(betaD);
   const int lane = threadIdx.x & (kWave - 1);
   const int nz = g.nz;
   const int64_t sx = nz, sy = static_cast<int64_t>(g.nx2) * nz;
   KSum pq;
   int i, j0, j1;
   if (march_span2(g, mj, i, j0, j1)) {
-    auto formp = [&](int64_t base, float (&p0)[R2], float (&p1)[R2]) {
-      float z0[R2], z1[R2];
+    auto formp = [&](int64_t base, T (&p0)[R2], T (&p1)[R2]) {
+      T z0[R2], z1[R2];
       load_pair<R2>(z, base, lane, nz, z0, z1);
       if (RESTART) {
 #pragma unroll
@@ -1256,7 +1270,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil
           p1[m] = z1[m];
         }
       } else {
-        float o0[R2], o1[R2];
+        T o0[R2], o1[R2];
         load_pair<R2>(pOld, base, lane, nz, o0, o1);
 #pragma unroll
         for (int m = 0; m < R2; ++m) {
@@ -1266,8 +1280,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil
       }
     };
     // column a = i, column b = i + 1
-    float aP0[R2], aP1[R2], aC0[R2], aC1[R2], aY0[R2], aY1[R2];
-    float bP0[R2], bP1[R2], bC0[R2], bC1[R2], bY0[R2], bY1[R2];
+    T aP0[R2], aP1[R2], aC0[R2], aC1[R2], aY0[R2], aY1[R2];
+    T bP0[R2], bP1[R2], bC0[R2], bC1[R2], bY0[R2], bY1[R2];
     formp(g.c3(j0 - 1, i, 0), aP0, aP1);
     formp(g.c3(j0, i, 0), aC0, aC1);
     formp(g.c3(j0 - 1, i + 1, 0), bP0, bP1);
@@ -1276,14 +1290,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil
     load_pair<R2>(A.Ayp, g.c3(j0 - 1, i + 1, 0), lane, nz, bY0, bY1);
     for (int j = j0; j <= j1; ++j) {
       const int64_t ba = g.c3(j, i, 0), bb = ba + sx;
-      float aN0[R2], aN1[R2], bN0[R2], bN1[R2], w0[R2], w1[R2], e0[R2], e1[R2];
+      T aN0[R2], aN1[R2], bN0[R2], bN1[R2], w0[R2], w1[R2], e0[R2], e1[R2];
       formp(ba + sy, aN0, aN1);
       formp(bb + sy, bN0, bN1);
       formp(ba - sx, w0, w1);  // column i - 1
       formp(bb + sx, e0, e1);  // column i + 2
-      float adA0[R2], adA1[R2], axA0[R2], axA1[R2], axW0[R2], axW1[R2], ayA0[R2], ayA1[R2],
+      T adA0[R2], adA1[R2], axA0[R2], axA1[R2], axW0[R2], axW1[R2], ayA0[R2], ayA1[R2],
           azA0[R2], azA1[R2];
-      float adB0[R2], adB1[R2], axB0[R2], axB1[R2], ayB0[R2], ayB1[R2], azB0[R2], azB1[R2];
+      T adB0[R2], adB1[R2], axB0[R2], axB1[R2], ayB0[R2], ayB1[R2], azB0[R2], azB1[R2];
       load_pair<R2>(A.Ad, ba, lane, nz, adA0, adA1);
       load_pair<R2>(A.Axp, ba, lane, nz, axA0, axA1);
       load_pair<R2>(A.Axp, ba - sx, lane, nz, axW0, axW1);
@@ -1293,12 +1307,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil
       load_pair<R2>(A.Axp, bb, lane, nz, axB0, axB1);
       load_pair<R2>(A.Ayp, bb, lane, nz, ayB0, ayB1);
       load_pair<R2>(A.Azp, bb, lane, nz, azB0, azB1);
-      float aDn[R2], aUp[R2], azAUp[R2], dummyA[R2], bDn[R2], bUp[R2], azBUp[R2], dummyB[R2];
+      T aDn[R2], aUp[R2], azAUp[R2], dummyA[R2], bDn[R2], bUp[R2], azBUp[R2], dummyB[R2];
       pair_shift<R2>(aC0, aC1, lane, aDn, aUp);
       pair_shift<R2>(azA0, azA1, lane, dummyA, azAUp);
       pair_shift<R2>(bC0, bC1, lane, bDn, bUp);
       pair_shift<R2>(azB0, azB1, lane, dummyB, azBUp);
-      float qa0[R2], qa1[R2], qb0[R2], qb1[R2];
+      T qa0[R2], qa1[R2], qb0[R2], qb1[R2];
 #pragma unroll
       for (int m = 0; m < R2; ++m) {
         qa0[m] = adA0[m] * aC0[m] + axA0[m] * bC0[m] + axW0[m] * w0[m] + ayA0[m] * aN0[m] +
@@ -1316,10 +1330,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil
           pq.add(static_cast<double>(bC1[m]) * static_cast<double>(qb1[m]));
         }
       }
-      store_pair<R2>(pNew, ba, lane, nz, aC0, aC1);
-      store_pair<R2>(q, ba, lane, nz, qa0, qa1);
-      store_pair<R2>(pNew, bb, lane, nz, bC0, bC1);
-      store_pair<R2>(q, bb, lane, nz, qb0, qb1);
+      store_pair<R2, NTS>(pNew, ba, lane, nz, aC0, aC1);
+      store_pair<R2, NTS>(q, ba, lane, nz, qa0, qa1);
+      store_pair<R2, NTS>(pNew, bb, lane, nz, bC0, bC1);
+      store_pair<R2, NTS>(q, bb, lane, nz, qb0, qb1);
 #pragma unroll
       for (int m = 0; m < R2; ++m) {
         aP0[m] = aC0[m];
@@ -1342,16 +1356,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil
   write_parts<1>(parts, pstride, slots, v);
 }
 
-// k_pcg_update for FP32 operands in the pair layout (x deferred the same way).
-template <int R2, bool LINE, int XM>
-__global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, float* __restrict__ x,
-                                                         const float* __restrict__ pPrev,
-                                                         const float* __restrict__ p,
-                                                         float* __restrict__ r,
-                                                         const float* __restrict__ q,
-                                                         float* __restrict__ z,
-                                                         const float* __restrict__ invd,
-                                                         const float* __restrict__ w,
+// k_pcg_update in the pair layout (x deferred the same way).
+template <class T, int R2, bool LINE, int XM>
+__global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, T* __restrict__ x,
+                                                         const T* __restrict__ pPrev,
+                                                         const T* __restrict__ p,
+                                                         T* __restrict__ r,
+                                                         const T* __restrict__ q,
+                                                         T* __restrict__ z,
+                                                         const T* __restrict__ invd,
+                                                         const T* __restrict__ w,
                                                          double* __restrict__ parts, int pstride,
                                                          int nprev, PcgState* __restrict__ st,
                                                          int n) {
@@ -1360,12 +1374,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, float* __restri
   const int rs[1] = {2};
   reduce_parts<1>(parts, pstride, nprev, rs, red);
   const double alphaD = st->rz[n & 1] / red[0];
-  const float alphaPrev = XM == 2 ? static_cast<float>(st->alphaH[(n - 1) & 1]) : 0.0f;
+  const T alphaPrev = XM == 2 ? static_cast<T>(st->alphaH[(n - 1) & 1]) : T(0);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->alpha = alphaD;
     st->alphaH[n & 1] = alphaD;
   }
-  const float alpha = static_cast<float>(alphaD);
+  const T alpha = static_cast<T>(alphaD);
   const int lane = threadIdx.x & (kWave - 1);
   const int nz = g.nz;
   KSum rz, rr;
@@ -1373,9 +1387,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, float* __restri
   column_loop(g.columns(), col, end, stride);
   for (; col < end; col += stride) {
     const int64_t base = column_base(g, col);
-    float r0[R2], r1[R2], q0[R2], q1[R2];
+    T r0[R2], r1[R2], q0[R2], q1[R2];
     if (XM == 2) {
-      float x0[R2], x1[R2], a0[R2], a1[R2], p0[R2], p1[R2];
+      T x0[R2], x1[R2], a0[R2], a1[R2], p0[R2], p1[R2];
       load_pair<R2>(x, base, lane, nz, x0, x1);
       load_pair<R2>(pPrev, base, lane, nz, a0, a1);
       load_pair<R2>(p, base, lane, nz, p0, p1);
@@ -1393,7 +1407,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, float* __restri
       r0[m] = r0[m] - alpha * q0[m];
       r1[m] = r1[m] - alpha * q1[m];
     }
-    float z0[R2], z1[R2];
+    T z0[R2], z1[R2];
     precond_pair<R2, LINE>(lane, nz, base, invd, w, r0, r1, z0, z1);
     store_pair<R2>(r, base, lane, nz, r0, r1);
     store_pair<R2>(z, base, lane, nz, z0, z1);
@@ -1764,6 +1778,7 @@ void Pcg::initCommon() {
   cus_ = device_cus();
   pairs_ = g_.nz % 2 == 0;
   cw2_ = pairs_ && g_.nx % 2 == 0;
+  defaultShapes(g_, &wpc64_, &wpc32_);
   applyShapes();
 }
 
@@ -1772,7 +1787,9 @@ constexpr int64_t padded(int64_t n) { return (n + 31) / 32 * 32; }
 
 int64_t Pcg::workspaceDoubles(const Grid& g) {
   const int cus = device_cus();
-  const Pcg::Shapes sh = shapesFor(g, cus, 16, 16, 32);
+  int wpc64 = 16, wpc32 = 32;
+  defaultShapes(g, &wpc64, &wpc32);
+  const Pcg::Shapes sh = shapesFor(g, cus, 16, wpc64, wpc32);
   return 7 * padded(g.cellsH()) + padded(4 * static_cast<int64_t>(sh.pstride)) + padded(4) +
          padded((sizeof(PcgState) + sizeof(double) - 1) / sizeof(double));
 }
@@ -1849,15 +1866,31 @@ double Pcg::spmvBytesPerLaunch(bool mixed) const {
   return per * static_cast<double>(g_.columns()) * g_.nz;
 }
 
+// The direction kernels over column pairs (nx and nz even) stream best at one
+// wave per SIMD, where each wave has the memory pipeline to itself (sweep at
+// 512 x 512 x 100: FP64 0.317 ms at 4 waves per CU against 0.337-0.342 at
+// 8-16; FP32 0.174 against 0.181-0.186); the one-column forms at 16 (FP64)
+// and 32 (FP32).
+void Pcg::defaultShapes(const Grid& g, int* wpc64, int* wpc32) {
+  const bool cw2 = g.nz % 2 == 0 && g.nx % 2 == 0;
+  *wpc64 = cw2 && g.nz <= kCw2Rows64 ? 4 : 16;
+  *wpc32 = cw2 ? 4 : 32;
+}
+
 Pcg::Shapes Pcg::shapesFor(const Grid& g, int cus, int wpcCol, int wpc64, int wpc32) {
   Shapes sh;
   const int64_t ncol = g.columns();
   const int64_t wavesWanted = std::max<int64_t>(8, std::min<int64_t>(ncol, int64_t(cus) * wpcCol));
   sh.blocks = static_cast<int>(std::max<int64_t>(8, ((wavesWanted + kWpb - 1) / kWpb + 7) / 8 * 8));
-  sh.march64 = makeMarch(g, cus, wpc64);
-  sh.march32 = makeMarch(g, cus, wpc32);
+  const bool cw2 = g.nz % 2 == 0 && g.nx % 2 == 0;
+  // the residual / refinement sweeps (and, without column pairs, the
+  // direction kernels) run one column a wave
+  sh.march64 = makeMarch(g, cus, cw2 && g.nz <= kCw2Rows64 ? 16 : wpc64);
+  sh.march32 = makeMarch(g, cus, cw2 ? 32 : wpc32);
   sh.march32w = makeMarch(g, cus, wpc32, 2);
-  sh.pstride = std::max({sh.blocks, sh.march64.blocks, sh.march32.blocks, sh.march32w.blocks});
+  sh.march64w = makeMarch(g, cus, wpc64, 2);
+  sh.pstride = std::max({sh.blocks, sh.march64.blocks, sh.march32.blocks, sh.march32w.blocks,
+                         sh.march64w.blocks});
   return sh;
 }
 
@@ -1867,6 +1900,7 @@ void Pcg::applyShapes() {
   march64_ = sh.march64;
   march32_ = sh.march32;
   march32w_ = sh.march32w;
+  march64w_ = sh.march64w;
   pstride_ = std::max(sh.pstride, colBlocks_);
   if (parts_ && pstride_ > partsCap_ && external_)
     throw std::runtime_error("launch shapes beyond the external workspace's partial sums");
@@ -1909,6 +1943,7 @@ void Pcg::reduceHost(int nparts, int nvals, double* out) {
 int Pcg::solve(const Stencil<double>& A, const double* b, double* x, const PcgOptions& opt,
                PcgStats* st) {
   *st = PcgStats{};
+  choosePairs64(A, b, x);
   if (direct_ && opt.zline) return solveDirect(A, b, x, opt, st);
   return opt.mixed ? solveMixed(A, b, x, opt, st) : solveFp64(A, b, x, opt, st);
 }
@@ -2031,31 +2066,47 @@ int Pcg::solveDirect(const Stencil<double>& A, const double* b, double* x, const
 // ---------------------------------------------------------------------------
 // Launch helpers shared by the host-polled and the device-controlled solves.
 // ---------------------------------------------------------------------------
+// The FP64 iteration kernels take the pair layout when nz is even, at most
+// 512 (deeper columns spill the pair kernels' registers to scratch) and
+// every FP64 operand of the solve is 16-byte aligned (the pairs are loaded
+// and stored as 16-byte vectors); the PCG's own vectors always are.
+void Pcg::choosePairs64(const Stencil<double>& A, const double* b, const double* x) {
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  pairs64_ = pairs_ && g_.nz <= 512 && al(A.Ad) && al(A.Axp) && al(A.Ayp) && al(A.Azp) &&
+             al(b) && al(x);
+}
+
 template <class T>
 void Pcg::launchDir(const Stencil<T>& A, const T* z, const T* pOld, T* pNew, T* q, int np, int n,
                     bool restart, const March& mj) {
   PcgState* dst = static_cast<PcgState*>(state_);
-  if constexpr (std::is_same<T, float>::value) {
-    if (pairs_) {
-      dispatch_R2(g_.nz, [&](auto RC) {
-        constexpr int R2 = decltype(RC)::value;
-        const March& mw = march32w_;
-        if (cw2_ && restart)
-          k_pcg_dir_p2<R2, true><<<mw.blocks, kBlock, 0, stream_>>>(g_, mw, A, z, pOld, pNew, q,
-                                                                    parts_, pstride_, np, dst, n);
-        else if (cw2_)
-          k_pcg_dir_p2<R2, false><<<mw.blocks, kBlock, 0, stream_>>>(g_, mw, A, z, pOld, pNew, q,
-                                                                     parts_, pstride_, np, dst, n);
+  constexpr bool fp32 = std::is_same<T, float>::value;
+  if (fp32 ? pairs_ : pairs64_) {
+    // FP64 p and q are stored non-temporally (0.319 -> 0.316 ms a launch at
+    // 512 x 512 x 100); FP32 gains nothing from it
+    dispatch_R2(g_.nz, [&](auto RC) {
+      constexpr int R2 = decltype(RC)::value;
+      constexpr bool NTS = !fp32;
+      if constexpr (fp32 || R2 <= 4) {
+        const March& mw = fp32 ? march32w_ : march64w_;
+        if (cw2(fp32) && restart)
+          k_pcg_dir_p2<T, R2, true, NTS><<<mw.blocks, kBlock, 0, stream_>>>(
+              g_, mw, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n);
+        else if (cw2(fp32))
+          k_pcg_dir_p2<T, R2, false, NTS><<<mw.blocks, kBlock, 0, stream_>>>(
+              g_, mw, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n);
         else if (restart)
-          k_pcg_dir_p<R2, true><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, z, pOld, pNew, q,
-                                                                   parts_, pstride_, np, dst, n);
+          k_pcg_dir_p<T, R2, true><<<mj.blocks, kBlock, 0, stream_>>>(
+              g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n);
         else
-          k_pcg_dir_p<R2, false><<<mj.blocks, kBlock, 0, stream_>>>(g_, mj, A, z, pOld, pNew, q,
-                                                                    parts_, pstride_, np, dst, n);
-      });
-      HIP_OK(hipGetLastError());
-      return;
-    }
+          k_pcg_dir_p<T, R2, false><<<mj.blocks, kBlock, 0, stream_>>>(
+              g_, mj, A, z, pOld, pNew, q, parts_, pstride_, np, dst, n);
+      } else {
+        throw std::runtime_error("FP64 pair kernels take nz <= 512");
+      }
+    });
+    HIP_OK(hipGetLastError());
+    return;
   }
   dispatch_R(g_.nz, [&](auto RC) {
     constexpr int R = decltype(RC)::value;
@@ -2073,27 +2124,27 @@ template <class T>
 void Pcg::launchUpdate(T* x, const T* pPrev, const T* p, T* r, const T* q, T* z, const T* invd,
                        const T* w, bool zline, bool two, int n, const March& mj) {
   PcgState* dst = static_cast<PcgState*>(state_);
-  const int nd = dirBlocks(std::is_same<T, float>::value, mj);
-  if constexpr (std::is_same<T, float>::value) {
-    if (pairs_) {
-      dispatch_R2(g_.nz, [&](auto RC) {
-        constexpr int R2 = decltype(RC)::value;
-        if (zline && two)
-          k_pcg_update_p<R2, true, 2><<<blocks_, kBlock, 0, stream_>>>(
-              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
-        else if (zline)
-          k_pcg_update_p<R2, true, 0><<<blocks_, kBlock, 0, stream_>>>(
-              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
-        else if (two)
-          k_pcg_update_p<R2, false, 2><<<blocks_, kBlock, 0, stream_>>>(
-              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
-        else
-          k_pcg_update_p<R2, false, 0><<<blocks_, kBlock, 0, stream_>>>(
-              g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
-      });
-      HIP_OK(hipGetLastError());
-      return;
-    }
+  constexpr bool fp32 = std::is_same<T, float>::value;
+  const int nd = dirBlocks(fp32, mj);
+  if (fp32 ? pairs_ : pairs64_) {
+    dispatch_R2(g_.nz, [&](auto RC) {
+      constexpr int R2 = decltype(RC)::value;
+      if constexpr (!fp32 && R2 > 4) throw std::runtime_error("FP64 pair kernels take nz <= 512");
+      else if (zline && two)
+        k_pcg_update_p<T, R2, true, 2><<<blocks_, kBlock, 0, stream_>>>(
+            g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
+      else if (zline)
+        k_pcg_update_p<T, R2, true, 0><<<blocks_, kBlock, 0, stream_>>>(
+            g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
+      else if (two)
+        k_pcg_update_p<T, R2, false, 2><<<blocks_, kBlock, 0, stream_>>>(
+            g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
+      else
+        k_pcg_update_p<T, R2, false, 0><<<blocks_, kBlock, 0, stream_>>>(
+            g_, x, pPrev, p, r, q, z, invd, w, parts_, pstride_, nd, dst, n);
+    });
+    HIP_OK(hipGetLastError());
+    return;
   }
   dispatch_R(g_.nz, [&](auto RC) {
     constexpr int R = decltype(RC)::value;
@@ -2494,6 +2545,7 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
                         const PcgOptions& opt, const PcgBudget& bud) {
   if (bud.rounds < 1 || bud.iterations < 1 || bud.iterations > 4096)
     throw std::runtime_error("device PCG budget out of range");
+  choosePairs64(A, b, x);
   PcgState* dst = static_cast<PcgState*>(state_);
   if (bud.later < 0 || bud.later > 4096) throw std::runtime_error("device PCG budget out of range");
   auto budgetOf = [&](int round) {
@@ -2598,6 +2650,7 @@ int Pcg::resumeDevice(const Stencil<double>& A, const double* b, double* x, cons
                       PcgStats* st) {
   const PcgState h = *static_cast<const PcgState*>(stateHost_);
   *st = PcgStats{};
+  choosePairs64(A, b, x);
   st->rhs_norm = std::sqrt(h.bbTrue);
   st->refinements = h.refinements;
   if (direct_ && opt.zline) {

@@ -33,9 +33,13 @@ def test_shape_cache_round_trip(tmp_path):
 def test_shapes_set_get_and_reject_bad_values():
     from frehg2_amd.gw import RichardsSolver
     g = RichardsSolver(cases.loam_32x32x50(steps=1))
-    assert g.pcg_shapes() == {"march64": 16, "march32": 32, "column": 16}
+    # nx and nz even: the direction kernels run over column pairs, 4 waves a CU
+    assert g.pcg_shapes() == {"march64": 4, "march32": 4, "column": 16}
+    odd = RichardsSolver(cases.loam_32x32x50(steps=1, nx=31))
+    assert odd.pcg_shapes() == {"march64": 16, "march32": 32, "column": 16}
+    odd.close()
     g.set_pcg_shapes(8, 0, 32)
-    assert g.pcg_shapes() == {"march64": 8, "march32": 32, "column": 32}
+    assert g.pcg_shapes() == {"march64": 8, "march32": 4, "column": 32}
     with pytest.raises(abi.FrehgError):
         g.set_pcg_shapes(-1, 0, 0)
     with pytest.raises(abi.FrehgError):
