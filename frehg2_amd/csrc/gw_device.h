@@ -265,6 +265,30 @@ __device__ __forceinline__ T scan_affine_down(T a, T b, T& z0, int lane) {
   return a + b * e;
 }
 
+// The two scans above within each half of the wave: lanes 0-31 and 32-63
+// hold two independent sequences (two columns), each entered with 0.  After
+// the in-row steps only rows 1 and 3 take a value from the row before them.
+template <class T>
+__device__ __forceinline__ T scan_affine_up_halves(T a, T b, int lane) {
+  affine_step<1, true>(a, b);
+  affine_step<2, true>(a, b);
+  affine_step<4, true>(a, b);
+  affine_step<8, true>(a, b);
+  const T e1 = lane_value(a, 15), e3 = lane_value(a, 47);
+  const T e = (lane & 16) ? (lane < 32 ? e1 : e3) : T(0);
+  return a + b * e;
+}
+template <class T>
+__device__ __forceinline__ T scan_affine_down_halves(T a, T b, int lane) {
+  affine_step<1, false>(a, b);
+  affine_step<2, false>(a, b);
+  affine_step<4, false>(a, b);
+  affine_step<8, false>(a, b);
+  const T e0 = lane_value(a, 16), e2 = lane_value(a, 48);
+  const T e = (lane & 16) ? T(0) : (lane < 32 ? e0 : e2);
+  return a + b * e;
+}
+
 // The wave reductions below and the stencil kernels' row shifts stay on LDS
 // shuffles: with DPP moves in either place the FP64 PCG direction kernel
 // (512 x 512 x 100) ran 5 % slower (0.398 -> 0.418 ms a launch), while the

@@ -259,6 +259,7 @@ class Pcg {
   bool pairs_ = false;        // FP32 kernels in the row-pair layout (nz even)
   bool pairs64_ = false;      // FP64 iteration kernels likewise (nz even, operands aligned)
   bool cw2_ = false;          // ... and the direction kernel over column pairs (nx even)
+  bool quads_ = false;        // FP32 direction kernel in the quad layout (nz % 4 == 0, <= 128)
   // FP64 vectors (halo layout)
   double *r_ = nullptr, *z_ = nullptr, *p0_ = nullptr, *p1_ = nullptr, *q_ = nullptr;
   double *invd_ = nullptr, *w_ = nullptr;
