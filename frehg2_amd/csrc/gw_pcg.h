@@ -228,7 +228,7 @@ class Pcg {
 
   struct Shapes {
     int blocks = 0, pstride = 0;
-    March march64, march32, march32w, march64w;
+    March march64, march32, march32w, march64w, sweep64w;
   };
   static Shapes shapesFor(const Grid& g, int cus, int wpcCol, int wpc64, int wpc32);
   // default direction-kernel shapes of grid g
@@ -242,6 +242,10 @@ class Pcg {
   // the direction kernel runs over column pairs (FP64: columns of at most
   // kCw2Rows64 rows; deeper pairs spill registers)
   bool cw2(bool fp32) const { return cw2_ && (fp32 || g_.nz <= kCw2Rows64); }
+  // the FP64 residual / refinement sweeps run over column pairs, and the
+  // block count of their partial sums
+  bool sweepPairs() const { return pairs64_ && cw2(false); }
+  int sweepBlocks() const { return sweepPairs() ? sweep64w_.blocks : march64_.blocks; }
   static constexpr int kCw2Rows64 = 256;
   // the FP64 operands of a solve allow the pair layout (16-byte aligned)
   void choosePairs64(const Stencil<double>& A, const double* b, const double* x);
@@ -252,6 +256,7 @@ class Pcg {
   March march64_, march32_;   // stencil-kernel shapes, FP64 and FP32 operands
   March march32w_;            // FP32 direction kernel over column pairs (cw2_)
   March march64w_;            // FP64 direction kernel over column pairs (cw2_, pairs64_)
+  March sweep64w_;            // FP64 residual / refinement sweeps over column pairs
   int pstride_ = 0;           // slot stride of parts_ (largest grid)
   int partsCap_ = 0;          // slot stride parts_ was allocated for
   int cus_ = 256;

@@ -887,6 +887,15 @@ __global__ __launch_bounds__(kBlock) void k_inner_init(Grid g, const double* __r
 // then aligned to their size).
 // ---------------------------------------------------------------------------
 typedef float float4v __attribute__((ext_vector_type(4)));
+template <bool NT>
+__device__ __forceinline__ void store4(float* p, float4v v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<float4v*>(p));
+  else *reinterpret_cast<float4v*>(p) = v;
+}
+#ifndef FREHG_UPDATE_NTS
+#define FREHG_UPDATE_NTS 1
+#endif
+constexpr bool kUpdateNts = FREHG_UPDATE_NTS;
 template <class T> struct PairOf;
 template <> struct PairOf<float> { typedef float v __attribute__((ext_vector_type(2))); };
 template <> struct PairOf<double> { typedef double v __attribute__((ext_vector_type(2))); };
@@ -1477,7 +1486,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_q(Grid g, March mj, Stencil<
 // columns (scan_affine_up_halves / _down_halves): each lane first composes
 // its four steps of a recurrence, the half-wave scans the composed maps, and
 // the lane then walks its first three rows from the value entering it.
-template <bool LINE, int XM>
+template <bool LINE, int XM, bool NTS = kUpdateNts>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_q(Grid g, float* __restrict__ x,
                                                          const float* __restrict__ pPrev,
                                                          const float* __restrict__ p,
@@ -1517,7 +1526,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_q(Grid g, float* __restri
     const int64_t c = g.c3(j, i, 0) + kc;
     if (XM == 2) {
       const float4v xn = (ld(x + c) + alphaPrev * ld(pPrev + c)) + alpha * ld(p + c);
-      if (in) *reinterpret_cast<float4v*>(x + c) = xn;
+      if (in) store4<NTS>(x + c, xn);
     }
     float4v rv = ld(r + c) - alpha * ld(q + c);
     if (!in) rv = float4v(0.0f);
@@ -1559,8 +1568,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_q(Grid g, float* __restri
       zv.x = Z0;
     }
     if (in) {
-      *reinterpret_cast<float4v*>(r + c) = rv;
-      *reinterpret_cast<float4v*>(z + c) = zv;
+      store4<NTS>(r + c, rv);
+      store4<NTS>(z + c, zv);
       rz.add(static_cast<double>(rv.x) * static_cast<double>(zv.x));
       rz.add(static_cast<double>(rv.y) * static_cast<double>(zv.y));
       rz.add(static_cast<double>(rv.z) * static_cast<double>(zv.z));
@@ -1577,7 +1586,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_q(Grid g, float* __restri
 }
 
 // k_pcg_update in the pair layout (x deferred the same way).
-template <class T, int R2, bool LINE, int XM>
+template <class T, int R2, bool LINE, int XM, bool NTS = kUpdateNts>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, T* __restrict__ x,
                                                          const T* __restrict__ pPrev,
                                                          const T* __restrict__ p,
@@ -1618,7 +1627,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, T* __restrict__
         x0[m] = (x0[m] + alphaPrev * a0[m]) + alpha * p0[m];
         x1[m] = (x1[m] + alphaPrev * a1[m]) + alpha * p1[m];
       }
-      store_pair<R2>(x, base, lane, nz, x0, x1);
+      store_pair<R2, NTS>(x, base, lane, nz, x0, x1);
     }
     load_pair<R2>(r, base, lane, nz, r0, r1);
     load_pair<R2>(q, base, lane, nz, q0, q1);
@@ -1629,8 +1638,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, T* __restrict__
     }
     T z0[R2], z1[R2];
     precond_pair<R2, LINE>(lane, nz, base, invd, w, r0, r1, z0, z1);
-    store_pair<R2>(r, base, lane, nz, r0, r1);
-    store_pair<R2>(z, base, lane, nz, z0, z1);
+    store_pair<R2, NTS>(r, base, lane, nz, r0, r1);
+    store_pair<R2, NTS>(z, base, lane, nz, z0, z1);
 #pragma unroll
     for (int m = 0; m < R2; ++m) {
       if (128 * m + 2 * lane < nz) {
@@ -1695,6 +1704,188 @@ __global__ __launch_bounds__(kBlock) void k_inner_init_p(Grid g, const double* _
   double v[2] = {rz.value(), rr.value()};
   const int slots[2] = {0, 1};
   write_parts<2>(parts, pstride, slots, v);
+}
+
+// ---------------------------------------------------------------------------
+// The FP64 sweeps over column pairs in the row-pair layout (nx and nz even,
+// nz <= 256), the shape of k_pcg_dir_p2: k_residual_p2 is k_residual64 and
+// k_refine_p2 is k_refine64.  XV: x at a column of the pair layout (the
+// refinement adds sc d of the FP32 correction first).
+// ---------------------------------------------------------------------------
+template <int R2>
+__device__ __forceinline__ void stencil_pair_q(
+    int lane, const double (&aP0)[R2], const double (&aP1)[R2], const double (&aC0)[R2],
+    const double (&aC1)[R2], const double (&aN0)[R2], const double (&aN1)[R2],
+    const double (&bP0)[R2], const double (&bP1)[R2], const double (&bC0)[R2],
+    const double (&bC1)[R2], const double (&bN0)[R2], const double (&bN1)[R2],
+    const double (&w0)[R2], const double (&w1)[R2], const double (&e0)[R2],
+    const double (&e1)[R2], const double (&aY0)[R2], const double (&aY1)[R2],
+    const double (&bY0)[R2], const double (&bY1)[R2], const Stencil<double>& A, int64_t ba,
+    int64_t sx, int nz, double (&qa0)[R2], double (&qa1)[R2], double (&qb0)[R2],
+    double (&qb1)[R2], double (&ayA0)[R2], double (&ayA1)[R2], double (&ayB0)[R2],
+    double (&ayB1)[R2]) {
+  const int64_t bb = ba + sx;
+  double adA0[R2], adA1[R2], axA0[R2], axA1[R2], axW0[R2], axW1[R2], azA0[R2], azA1[R2];
+  double adB0[R2], adB1[R2], axB0[R2], axB1[R2], azB0[R2], azB1[R2];
+  load_pair<R2>(A.Ad, ba, lane, nz, adA0, adA1);
+  load_pair<R2>(A.Axp, ba, lane, nz, axA0, axA1);
+  load_pair<R2>(A.Axp, ba - sx, lane, nz, axW0, axW1);
+  load_pair<R2>(A.Ayp, ba, lane, nz, ayA0, ayA1);
+  load_pair<R2>(A.Azp, ba, lane, nz, azA0, azA1);
+  load_pair<R2>(A.Ad, bb, lane, nz, adB0, adB1);
+  load_pair<R2>(A.Axp, bb, lane, nz, axB0, axB1);
+  load_pair<R2>(A.Ayp, bb, lane, nz, ayB0, ayB1);
+  load_pair<R2>(A.Azp, bb, lane, nz, azB0, azB1);
+  double aDn[R2], aUp[R2], azAUp[R2], dA[R2], bDn[R2], bUp[R2], azBUp[R2], dB[R2];
+  pair_shift<R2>(aC0, aC1, lane, aDn, aUp);
+  pair_shift<R2>(azA0, azA1, lane, dA, azAUp);
+  pair_shift<R2>(bC0, bC1, lane, bDn, bUp);
+  pair_shift<R2>(azB0, azB1, lane, dB, azBUp);
+#pragma unroll
+  for (int m = 0; m < R2; ++m) {
+    qa0[m] = adA0[m] * aC0[m] + axA0[m] * bC0[m] + axW0[m] * w0[m] + ayA0[m] * aN0[m] +
+             aY0[m] * aP0[m] + azA0[m] * aC1[m] + azAUp[m] * aUp[m];
+    qa1[m] = adA1[m] * aC1[m] + axA1[m] * bC1[m] + axW1[m] * w1[m] + ayA1[m] * aN1[m] +
+             aY1[m] * aP1[m] + azA1[m] * aDn[m] + azA0[m] * aC0[m];
+    qb0[m] = adB0[m] * bC0[m] + axB0[m] * e0[m] + axA0[m] * aC0[m] + ayB0[m] * bN0[m] +
+             bY0[m] * bP0[m] + azB0[m] * bC1[m] + azBUp[m] * bUp[m];
+    qb1[m] = adB1[m] * bC1[m] + axB1[m] * e1[m] + axA1[m] * aC1[m] + ayB1[m] * bN1[m] +
+             bY1[m] * bP1[m] + azB1[m] * bDn[m] + azB0[m] * bC0[m];
+  }
+}
+
+// KIND 0: r = b - A x, z = M^-1 r (LINE: z-line), partials rz (0), rr (1),
+// bb (3).  KIND 1: xo = xi + sc d (HASD) or xi, r = b - A xo, partials rr
+// (1), bb (3).
+template <int R2, int KIND, bool LINE, bool HASD>
+__global__ __launch_bounds__(kBlock) void k_sweep_p2(Grid g, March mj, Stencil<double> A,
+                                                     const double* __restrict__ b,
+                                                     double* xa, double* xb,
+                                                     const float* __restrict__ d, double sc,
+                                                     double* __restrict__ r,
+                                                     double* __restrict__ z,
+                                                     const double* __restrict__ invd,
+                                                     const double* __restrict__ w,
+                                                     double* __restrict__ parts, int pstride,
+                                                     const PcgState* __restrict__ ctl) {
+  const double* __restrict__ xi = xa;
+  double* __restrict__ xo = xb;
+  if (ctl) {
+    if (ctl->skip) return;
+    if (KIND == 1) {
+      if (ctl->xInB) {
+        xi = xb;
+        xo = xa;
+      }
+      sc = 1.0 / ctl->s;
+    }
+  }
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nz = g.nz;
+  const int64_t sx = nz, sy = static_cast<int64_t>(g.nx2) * nz;
+  KSum rz, rr, bbs;
+  int i, j0, j1;
+  if (march_span2(g, mj, i, j0, j1)) {
+    auto formx = [&](int64_t base, double (&v0)[R2], double (&v1)[R2]) {
+      load_pair<R2>(xi, base, lane, nz, v0, v1);
+      if (KIND == 1 && HASD) {
+        float d0[R2], d1[R2];
+        load_pair<R2>(d, base, lane, nz, d0, d1);
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          v0[m] = v0[m] + sc * static_cast<double>(d0[m]);
+          v1[m] = v1[m] + sc * static_cast<double>(d1[m]);
+        }
+      }
+    };
+    double aP0[R2], aP1[R2], aC0[R2], aC1[R2], aY0[R2], aY1[R2];
+    double bP0[R2], bP1[R2], bC0[R2], bC1[R2], bY0[R2], bY1[R2];
+    formx(g.c3(j0 - 1, i, 0), aP0, aP1);
+    formx(g.c3(j0, i, 0), aC0, aC1);
+    formx(g.c3(j0 - 1, i + 1, 0), bP0, bP1);
+    formx(g.c3(j0, i + 1, 0), bC0, bC1);
+    load_pair<R2>(A.Ayp, g.c3(j0 - 1, i, 0), lane, nz, aY0, aY1);
+    load_pair<R2>(A.Ayp, g.c3(j0 - 1, i + 1, 0), lane, nz, bY0, bY1);
+    for (int j = j0; j <= j1; ++j) {
+      const int64_t ba = g.c3(j, i, 0), bb = ba + sx;
+      double aN0[R2], aN1[R2], bN0[R2], bN1[R2], w0[R2], w1[R2], e0[R2], e1[R2];
+      formx(ba + sy, aN0, aN1);
+      formx(bb + sy, bN0, bN1);
+      formx(ba - sx, w0, w1);  // column i - 1
+      formx(bb + sx, e0, e1);  // column i + 2
+      double qa0[R2], qa1[R2], qb0[R2], qb1[R2], ayA0[R2], ayA1[R2], ayB0[R2], ayB1[R2];
+      stencil_pair_q<R2>(lane, aP0, aP1, aC0, aC1, aN0, aN1, bP0, bP1, bC0, bC1, bN0, bN1, w0,
+                         w1, e0, e1, aY0, aY1, bY0, bY1, A, ba, sx, nz, qa0, qa1, qb0, qb1, ayA0,
+                         ayA1, ayB0, ayB1);
+      double bA0[R2], bA1[R2], bB0[R2], bB1[R2];
+      load_pair<R2>(b, ba, lane, nz, bA0, bA1);
+      load_pair<R2>(b, bb, lane, nz, bB0, bB1);
+      double rA0[R2], rA1[R2], rB0[R2], rB1[R2];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        const bool in = 128 * m + 2 * lane < nz;
+        rA0[m] = in ? bA0[m] - qa0[m] : 0.0;
+        rA1[m] = in ? bA1[m] - qa1[m] : 0.0;
+        rB0[m] = in ? bB0[m] - qb0[m] : 0.0;
+        rB1[m] = in ? bB1[m] - qb1[m] : 0.0;
+        if (in) {
+          rr.add(rA0[m] * rA0[m]);
+          rr.add(rA1[m] * rA1[m]);
+          rr.add(rB0[m] * rB0[m]);
+          rr.add(rB1[m] * rB1[m]);
+          bbs.add(bA0[m] * bA0[m]);
+          bbs.add(bA1[m] * bA1[m]);
+          bbs.add(bB0[m] * bB0[m]);
+          bbs.add(bB1[m] * bB1[m]);
+        }
+      }
+      store_pair<R2>(r, ba, lane, nz, rA0, rA1);
+      store_pair<R2>(r, bb, lane, nz, rB0, rB1);
+      if constexpr (KIND == 0) {
+        double zA0[R2], zA1[R2], zB0[R2], zB1[R2];
+        precond_pair<R2, LINE>(lane, nz, ba, invd, w, rA0, rA1, zA0, zA1);
+        precond_pair<R2, LINE>(lane, nz, bb, invd, w, rB0, rB1, zB0, zB1);
+        store_pair<R2>(z, ba, lane, nz, zA0, zA1);
+        store_pair<R2>(z, bb, lane, nz, zB0, zB1);
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          if (128 * m + 2 * lane < nz) {
+            rz.add(rA0[m] * zA0[m]);
+            rz.add(rA1[m] * zA1[m]);
+            rz.add(rB0[m] * zB0[m]);
+            rz.add(rB1[m] * zB1[m]);
+          }
+        }
+      } else if constexpr (HASD) {
+        store_pair<R2>(xo, ba, lane, nz, aC0, aC1);
+        store_pair<R2>(xo, bb, lane, nz, bC0, bC1);
+      }
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        aP0[m] = aC0[m];
+        aP1[m] = aC1[m];
+        aC0[m] = aN0[m];
+        aC1[m] = aN1[m];
+        aY0[m] = ayA0[m];
+        aY1[m] = ayA1[m];
+        bP0[m] = bC0[m];
+        bP1[m] = bC1[m];
+        bC0[m] = bN0[m];
+        bC1[m] = bN1[m];
+        bY0[m] = ayB0[m];
+        bY1[m] = ayB1[m];
+      }
+    }
+  }
+  if constexpr (KIND == 0) {
+    double v[3] = {rz.value(), rr.value(), bbs.value()};
+    const int slots[3] = {0, 1, 3};
+    write_parts<3>(parts, pstride, slots, v);
+  } else {
+    double v[2] = {rr.value(), bbs.value()};
+    const int slots[2] = {1, 3};
+    write_parts<2>(parts, pstride, slots, v);
+  }
 }
 
 // FP64 stencil -> FP32 stencil over interior cells.
@@ -2110,8 +2301,12 @@ Pcg::Shapes Pcg::shapesFor(const Grid& g, int cus, int wpcCol, int wpc64, int wp
   sh.march32 = makeMarch(g, cus, cw2 ? 32 : wpc32);
   sh.march32w = makeMarch(g, cus, wpc32, 2);
   sh.march64w = makeMarch(g, cus, wpc64, 2);
+  // the FP64 sweeps at 8 waves a CU (512 x 512 x 100, in the step's
+  // sequence: residual 0.43-0.46 ms and refinements 0.36 / 0.38 ms, where 4
+  // waves take 0.45-0.46 / 0.53 / 0.36 and 12 waves 0.45-0.47 / 0.33 / 0.39)
+  sh.sweep64w = makeMarch(g, cus, 8, 2);
   sh.pstride = std::max({sh.blocks, sh.march64.blocks, sh.march32.blocks, sh.march32w.blocks,
-                         sh.march64w.blocks});
+                         sh.march64w.blocks, sh.sweep64w.blocks});
   return sh;
 }
 
@@ -2122,6 +2317,7 @@ void Pcg::applyShapes() {
   march32_ = sh.march32;
   march32w_ = sh.march32w;
   march64w_ = sh.march64w;
+  sweep64w_ = sh.sweep64w;
   pstride_ = std::max(sh.pstride, colBlocks_);
   if (parts_ && pstride_ > partsCap_ && external_)
     throw std::runtime_error("launch shapes beyond the external workspace's partial sums");
@@ -2430,6 +2626,23 @@ void Pcg::launchFactor(const Stencil<double>& A, bool zline, bool fp32) {
 void Pcg::launchResidual(const Stencil<double>& A, const double* b, const double* x, bool zline,
                          const void* ctl) {
   const PcgState* c = static_cast<const PcgState*>(ctl);
+  if (sweepPairs()) {
+    const March& mw = sweep64w_;
+    double* xm = const_cast<double*>(x);  // read only in this form
+    auto go = [&](auto RC) {
+      constexpr int R2 = decltype(RC)::value;
+      if (zline)
+        k_sweep_p2<R2, 0, true, false><<<mw.blocks, kBlock, 0, stream_>>>(
+            g_, mw, A, b, xm, xm, nullptr, 0.0, r_, z_, invd_, w_, parts_, pstride_, c);
+      else
+        k_sweep_p2<R2, 0, false, false><<<mw.blocks, kBlock, 0, stream_>>>(
+            g_, mw, A, b, xm, xm, nullptr, 0.0, r_, z_, invd_, w_, parts_, pstride_, c);
+    };
+    if (g_.nz <= 128) go(std::integral_constant<int, 1>{});
+    else go(std::integral_constant<int, 2>{});
+    HIP_OK(hipGetLastError());
+    return;
+  }
   dispatch_R(g_.nz, [&](auto RC) {
     constexpr int R = decltype(RC)::value;
     const March& mj = march64_;
@@ -2446,6 +2659,22 @@ void Pcg::launchResidual(const Stencil<double>& A, const double* b, const double
 void Pcg::launchRefine(const Stencil<double>& A, const double* b, double* xa, double* xb,
                        double sc, bool hasD, const void* ctl) {
   const PcgState* c = static_cast<const PcgState*>(ctl);
+  if (sweepPairs()) {
+    const March& mw = sweep64w_;
+    auto go = [&](auto RC) {
+      constexpr int R2 = decltype(RC)::value;
+      if (hasD)
+        k_sweep_p2<R2, 1, false, true><<<mw.blocks, kBlock, 0, stream_>>>(
+            g_, mw, A, b, xa, xb, d32_, sc, r_, nullptr, nullptr, nullptr, parts_, pstride_, c);
+      else
+        k_sweep_p2<R2, 1, false, false><<<mw.blocks, kBlock, 0, stream_>>>(
+            g_, mw, A, b, xa, xb, d32_, sc, r_, nullptr, nullptr, nullptr, parts_, pstride_, c);
+    };
+    if (g_.nz <= 128) go(std::integral_constant<int, 1>{});
+    else go(std::integral_constant<int, 2>{});
+    HIP_OK(hipGetLastError());
+    return;
+  }
   dispatch_R(g_.nz, [&](auto RC) {
     constexpr int R = decltype(RC)::value;
     if (hasD)
@@ -2596,7 +2825,7 @@ std::pair<double, double> Pcg::residualHost(const Stencil<double>& A, const doub
                                             const double* x, bool zline) {
   launchResidual(A, b, x, zline, nullptr);
   double red[4];
-  reduceHost(march64_.blocks, 4, red);
+  reduceHost(sweepBlocks(), 4, red);
   return std::make_pair(red[1], red[3]);
 }
 
@@ -2627,7 +2856,7 @@ int Pcg::fp64Loop(const Stencil<double>& A, const double* b, double* x, const Pc
     }
     int its = 0;
     innerIterations<double>(A, x, pOld, pNew, r_, z_, q_, invd_, w_, opt.zline, tol2,
-                            opt.max_iterations - total, march64_, march64_.blocks, 0, true, opt,
+                            opt.max_iterations - total, march64_, sweepBlocks(), 0, true, opt,
                             st, &its);
     total += its;
     if (st->rounds++ == 0) st->first_inner = its;
@@ -2666,7 +2895,7 @@ std::pair<double, double> Pcg::refineHost(const Stencil<double>& A, const double
                                           double* xo, double sc, bool hasD) {
   launchRefine(A, b, xi, xo, sc, hasD, nullptr);
   double red[4];
-  reduceHost(march64_.blocks, 4, red);
+  reduceHost(sweepBlocks(), 4, red);
   return std::make_pair(red[1], red[3]);
 }
 
@@ -2810,7 +3039,7 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
   } else if (!opt.mixed) {
     launchFactor(A, opt.zline, false);
     launchResidual(A, b, x, opt.zline, nullptr);
-    k_dc_begin64<<<1, kBlock, 0, stream_>>>(parts_, pstride_, march64_.blocks, dst, opt.rtol,
+    k_dc_begin64<<<1, kBlock, 0, stream_>>>(parts_, pstride_, sweepBlocks(), dst, opt.rtol,
                                             opt.atol, opt.max_iterations);
     HIP_OK(hipGetLastError());
     for (int round = 0; round < bud.rounds; ++round) {
@@ -2820,7 +3049,7 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
       double* pOld = p0_;
       double* pNew = p1_;
       for (int n = 0; n < B; ++n) {
-        launchDir<double>(A, z_, pOld, pNew, q_, n == 0 ? march64_.blocks : blocks_, n, n == 0,
+        launchDir<double>(A, z_, pOld, pNew, q_, n == 0 ? sweepBlocks() : blocks_, n, n == 0,
                           march64_);
         launchUpdate<double>(x, pOld, pNew, r_, q_, z_, invd_, w_, opt.zline, (n & 1) != 0, n,
                              march64_);
@@ -2829,7 +3058,7 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
       k_dc_flush<double, false><<<blocks_ * 4, kBlock, 0, stream_>>>(g_, x, p1_, p0_, dst, B);
       HIP_OK(hipGetLastError());
       launchResidual(A, b, x, opt.zline, dst);
-      k_dc_end64<<<1, kBlock, 0, stream_>>>(parts_, pstride_, march64_.blocks, dst, B,
+      k_dc_end64<<<1, kBlock, 0, stream_>>>(parts_, pstride_, sweepBlocks(), dst, B,
                                             opt.max_iterations);
       HIP_OK(hipGetLastError());
     }
@@ -2838,7 +3067,7 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
     const Stencil<float> A32{Ad32_, Axp32_, Ayp32_, Azp32_};
     launchRefine(A, b, x, q_, 0.0, false, nullptr);
     const double ncell = static_cast<double>(g_.columns()) * g_.nz;
-    k_dc_begin_mixed<<<1, kBlock, 0, stream_>>>(parts_, pstride_, march64_.blocks, dst, opt.rtol,
+    k_dc_begin_mixed<<<1, kBlock, 0, stream_>>>(parts_, pstride_, sweepBlocks(), dst, opt.rtol,
                                                 opt.atol, opt.max_iterations,
                                                 opt.max_refinements, ncell);
     HIP_OK(hipGetLastError());
@@ -2863,7 +3092,7 @@ void Pcg::enqueueDevice(const Stencil<double>& A, const double* b, double* x,
       k_dc_mid<<<1, 1, 0, stream_>>>(dst, B);
       HIP_OK(hipGetLastError());
       launchRefine(A, b, x, q_, 0.0, true, dst);
-      k_dc_end_mixed<<<1, kBlock, 0, stream_>>>(parts_, pstride_, march64_.blocks, dst,
+      k_dc_end_mixed<<<1, kBlock, 0, stream_>>>(parts_, pstride_, sweepBlocks(), dst,
                                                 opt.max_iterations, opt.max_refinements);
       HIP_OK(hipGetLastError());
     }
