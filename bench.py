@@ -11,12 +11,14 @@ metric      cell-iterations/s = active cells x PCG iterations / PCG wall time
 accuracy    "@1e-8 rel err": each form runs at the tolerance that keeps its
             head within 1e-8 (max-abs, relative to max |h|) of a converged
             solution.  After the timed steps both forms take one step from the
-            same state and are compared with that step converged to rtol 1e-13
-            (FP64 residual, iterative refinement); the errors are reported.
-roofline    the dominant kernel is the PCG direction kernel k_pcg_dir (FP64):
-            algorithmic bytes per launch / its mean launch time, measured with
-            HIP events on the solver's own stream over the timed steps, one
-            launch in 8 (launches that exit at convergence are not counted)
+            same state and are compared with the FP64 oracle's step from that
+            state converged to rtol 1e-13; the errors are reported.
+roofline    the dominant kernel is the FP64 PCG direction kernel (k_pcg_dir_p2,
+            column pairs in the row-pair layout): algorithmic bytes per launch /
+            its mean launch time, measured with HIP events on the solver's own
+            stream over the timed steps, one launch in 8 (launches that exit at
+            convergence are not counted); traffic = its HBM bytes per launch from
+            two rocprofv3 counter runs (bench.py --write-traffic)
 stream      frehg_hbm_probe, run first: the GB/s of an FP64 copy and of a
             6-read/2-write FP64 stream over eight fields of the workload's size
             (roofline.measured_stream_gbs, roofline.frac_of_measured_stream)
@@ -85,6 +87,11 @@ def parse():
                          "and error against the FP64 oracle")
     ap.add_argument("--ens-rtol", type=float, default=0.0,
                     help="ensemble: PCG rtol of every leg (0: the workload's 1e-10)")
+    ap.add_argument("--write-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the direction kernel's HBM traffic JSON (roofline.traffic) "
+                         "from the counter_collection.csv files of two rocprofv3 runs "
+                         "(--pmc FETCH_SIZE, then --pmc WRITE_SIZE) of this bench, and print "
+                         "every PCG kernel's traffic")
     return ap.parse_args()
 
 
@@ -665,11 +672,70 @@ def pc_sweep(args):
     print(json.dumps(out), flush=True)
 
 
+TRAFFIC_JSON = ROOT / "profiles" / "traffic_pcg_dir_hetero_512x512x100.json"
+# the FP64 direction kernel the solver runs at 512 x 512 x 100 (column pairs, row pairs)
+DIR_KERNEL = "k_pcg_dir_p2<double, 1, false, true>"
+
+
+def write_traffic(fetch_csv, write_csv, nx=512, ny=512, nz=100):
+    """HBM bytes per dispatch of the PCG kernels from two rocprofv3 counter
+    runs.  FETCH_SIZE and WRITE_SIZE (KiB a dispatch) are scaled by what they
+    read for the step's device-to-device copy of one FP64 field of known size
+    ((nx+2)(ny+2)nz doubles, __amd_rocclr_copyBuffer); a kernel's figure is the
+    median over its dispatches that moved more than 1 % of its largest (the
+    launches queued past convergence return at once).  Writes the direction
+    kernel's bytes per launch to the JSON bench.py reads as roofline.traffic."""
+    import csv
+    field = (nx + 2) * (ny + 2) * nz * 8.0
+
+    def per_kernel(path, counter):
+        vals = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                name = r["Kernel_Name"].replace("void ", "").replace(
+                    "frehg::(anonymous namespace)::", "").split("(")[0]
+                vals.setdefault(name, []).append(float(r["Counter_Value"]) * 1024.0)
+        return vals
+
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+
+    def copy_factor(vals):
+        big = [v for v in vals.get("__amd_rocclr_copyBuffer", []) if v > 0.25 * field]
+        return field / float(np.median(big)) if big else None
+
+    ff, fw = copy_factor(fetch), copy_factor(write)
+    if ff is None or fw is None:
+        raise SystemExit("write-traffic: no field-sized copy in the counter runs to calibrate")
+    active = nx * ny * nz
+    rows = {}
+    for name in sorted(set(fetch) & set(write)):
+        if not name.startswith("k_pcg") and not name.startswith("k_sweep"):
+            continue
+        fv, wv = np.array(fetch[name]), np.array(write[name])
+        f_med = float(np.median(fv[fv > 0.01 * fv.max()])) * ff
+        w_med = float(np.median(wv[wv > 0.01 * wv.max()])) * fw
+        rows[name] = (f_med, w_med)
+        print(f"{name:48s} read {f_med / 1e6:8.1f} MB  write {w_med / 1e6:8.1f} MB  "
+              f"{(f_med + w_med) / active:6.1f} B/cell", flush=True)
+    if DIR_KERNEL not in rows:
+        raise SystemExit(f"write-traffic: {DIR_KERNEL} not in the counter runs")
+    f_med, w_med = rows[DIR_KERNEL]
+    out = {"kernel": DIR_KERNEL, "workload": "richards_hetero_512x512x100",
+           "bytes_per_launch": f_med + w_med, "fetch_scaled": f_med, "write_scaled": w_med,
+           "fetch_factor": ff, "write_factor": fw, "algorithmic_bytes": 64.0 * active}
+    TRAFFIC_JSON.write_text(json.dumps(out, indent=1) + "\n")
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic():
-    p = ROOT / "profiles" / "traffic_pcg_dir_hetero_512x512x100.json"
+    p = TRAFFIC_JSON
     if p.exists():
         try:
-            return json.loads(p.read_text()).get("bytes_per_launch")
+            t = json.loads(p.read_text())
+            # only a figure measured on the kernel this build runs
+            return t.get("bytes_per_launch") if t.get("kernel") == DIR_KERNEL else None
         except (OSError, ValueError):
             return None
     return None
@@ -690,6 +756,9 @@ def main():
         return
     if args.pc_sweep:
         pc_sweep(args)
+        return
+    if args.write_traffic:
+        write_traffic(*args.write_traffic, nx=args.nx, ny=args.ny, nz=args.nz)
         return
     # the achievable streaming rate on this device, measured live: the best
     # copy and 6-read/2-write FP64 stream over access width, cache policy and
@@ -733,7 +802,8 @@ def main():
     bw = r64["spmv_gbs"]
     out["roofline"] = {"bound": "hbm", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": bw / HBM_PEAK_GBS, "traffic": load_traffic(),
-                       "kernel": "k_pcg_dir<double> (FP64 p = z + beta p, q = A p, p.q)",
+                       "kernel": DIR_KERNEL + " (FP64 p = z + beta p, q = A p, p.q over "
+                                              "column pairs in the row-pair layout)",
                        "avg_launch_ms": r64["spmv_avg_ms"],
                        "algorithmic_bytes_per_launch": r64["spmv_bytes"],
                        "measured_stream_gbs": {"copy": stream["copy_gbs"],

@@ -247,6 +247,13 @@ class Pcg {
   bool sweepPairs() const { return pairs64_ && cw2(false); }
   int sweepBlocks() const { return sweepPairs() ? sweep64w_.blocks : march64_.blocks; }
   static constexpr int kCw2Rows64 = 256;
+  // grids whose FP64 iteration kernels can take the row-pair layout (nz even,
+  // 64 < nz <= 512), and whose FP64 direction kernel then runs over column
+  // pairs (nx even, nz <= kCw2Rows64)
+  static bool pairForm64(const Grid& g) { return g.nz % 2 == 0 && g.nz > 64 && g.nz <= 512; }
+  static bool colPairs64(const Grid& g) {
+    return pairForm64(g) && g.nx % 2 == 0 && g.nz <= kCw2Rows64;
+  }
   // the FP64 operands of a solve allow the pair layout (16-byte aligned)
   void choosePairs64(const Stencil<double>& A, const double* b, const double* x);
 

@@ -892,10 +892,11 @@ __device__ __forceinline__ void store4(float* p, float4v v) {
   if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<float4v*>(p));
   else *reinterpret_cast<float4v*>(p) = v;
 }
-#ifndef FREHG_UPDATE_NTS
-#define FREHG_UPDATE_NTS 1
-#endif
-constexpr bool kUpdateNts = FREHG_UPDATE_NTS;
+// The update kernels store r, z (and x) non-temporally: the direction kernel
+// after them then runs 2-3 % faster (FP64 0.336 -> 0.330 ms, FP32 0.181 ->
+// 0.176 ms at 512 x 512 x 100) and they lose nothing themselves.
+template <class T>
+constexpr bool kUpdateNts = true;
 template <class T> struct PairOf;
 template <> struct PairOf<float> { typedef float v __attribute__((ext_vector_type(2))); };
 template <> struct PairOf<double> { typedef double v __attribute__((ext_vector_type(2))); };
@@ -1486,7 +1487,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_q(Grid g, March mj, Stencil<
 // columns (scan_affine_up_halves / _down_halves): each lane first composes
 // its four steps of a recurrence, the half-wave scans the composed maps, and
 // the lane then walks its first three rows from the value entering it.
-template <bool LINE, int XM, bool NTS = kUpdateNts>
+template <bool LINE, int XM, bool NTS = kUpdateNts<float>>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_q(Grid g, float* __restrict__ x,
                                                          const float* __restrict__ pPrev,
                                                          const float* __restrict__ p,
@@ -1586,7 +1587,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_q(Grid g, float* __restri
 }
 
 // k_pcg_update in the pair layout (x deferred the same way).
-template <class T, int R2, bool LINE, int XM, bool NTS = kUpdateNts>
+template <class T, int R2, bool LINE, int XM, bool NTS = kUpdateNts<T>>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, T* __restrict__ x,
                                                          const T* __restrict__ pPrev,
                                                          const T* __restrict__ p,
@@ -2189,7 +2190,7 @@ void Pcg::initCommon() {
   cus_ = device_cus();
   pairs_ = g_.nz % 2 == 0;
   cw2_ = pairs_ && g_.nx % 2 == 0;
-  quads_ = cw2_ && g_.nz % 4 == 0 && g_.nz <= 128;
+  quads_ = cw2_ && g_.nz % 4 == 0 && g_.nz > 64 && g_.nz <= 128;
   defaultShapes(g_, &wpc64_, &wpc32_);
   applyShapes();
 }
@@ -2284,9 +2285,8 @@ double Pcg::spmvBytesPerLaunch(bool mixed) const {
 // 8-16; FP32 0.174 against 0.181-0.186); the one-column forms at 16 (FP64)
 // and 32 (FP32).
 void Pcg::defaultShapes(const Grid& g, int* wpc64, int* wpc32) {
-  const bool cw2 = g.nz % 2 == 0 && g.nx % 2 == 0;
-  *wpc64 = cw2 && g.nz <= kCw2Rows64 ? 4 : 16;
-  *wpc32 = cw2 ? 4 : 32;
+  *wpc64 = colPairs64(g) ? 4 : 16;
+  *wpc32 = g.nx % 2 == 0 && g.nz % 2 == 0 && g.nz > 64 ? 4 : 32;
 }
 
 Pcg::Shapes Pcg::shapesFor(const Grid& g, int cus, int wpcCol, int wpc64, int wpc32) {
@@ -2295,9 +2295,9 @@ Pcg::Shapes Pcg::shapesFor(const Grid& g, int cus, int wpcCol, int wpc64, int wp
   const int64_t wavesWanted = std::max<int64_t>(8, std::min<int64_t>(ncol, int64_t(cus) * wpcCol));
   sh.blocks = static_cast<int>(std::max<int64_t>(8, ((wavesWanted + kWpb - 1) / kWpb + 7) / 8 * 8));
   const bool cw2 = g.nz % 2 == 0 && g.nx % 2 == 0;
-  // the residual / refinement sweeps (and, without column pairs, the
-  // direction kernels) run one column a wave
-  sh.march64 = makeMarch(g, cus, cw2 && g.nz <= kCw2Rows64 ? 16 : wpc64);
+  // the one-column FP64 kernels (and, without column pairs, the direction
+  // kernels) run one column a wave
+  sh.march64 = makeMarch(g, cus, colPairs64(g) ? 16 : wpc64);
   sh.march32 = makeMarch(g, cus, cw2 ? 32 : wpc32);
   sh.march32w = makeMarch(g, cus, wpc32, 2);
   sh.march64w = makeMarch(g, cus, wpc64, 2);
@@ -2483,14 +2483,16 @@ int Pcg::solveDirect(const Stencil<double>& A, const double* b, double* x, const
 // ---------------------------------------------------------------------------
 // Launch helpers shared by the host-polled and the device-controlled solves.
 // ---------------------------------------------------------------------------
-// The FP64 iteration kernels take the pair layout when nz is even, at most
-// 512 (deeper columns spill the pair kernels' registers to scratch) and
-// every FP64 operand of the solve is 16-byte aligned (the pairs are loaded
-// and stored as 16-byte vectors); the PCG's own vectors always are.
+// The FP64 iteration kernels take the pair layout when nz is even, above 64
+// (a column of 64 rows or fewer fills half a wave of row pairs, and one row a
+// lane fills all of it), at most 512 (deeper columns spill the pair kernels'
+// registers to scratch) and every FP64 operand of the solve is 16-byte
+// aligned (the pairs are loaded and stored as 16-byte vectors); the PCG's own
+// vectors always are.
 void Pcg::choosePairs64(const Stencil<double>& A, const double* b, const double* x) {
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-  pairs64_ = pairs_ && g_.nz <= 512 && al(A.Ad) && al(A.Axp) && al(A.Ayp) && al(A.Azp) &&
-             al(b) && al(x);
+  pairs64_ = pairForm64(g_) && al(A.Ad) && al(A.Axp) && al(A.Ayp) && al(A.Azp) && al(b) &&
+             al(x);
 }
 
 template <class T>

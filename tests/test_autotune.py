@@ -32,14 +32,18 @@ def test_shape_cache_round_trip(tmp_path):
 @pytest.mark.gpu
 def test_shapes_set_get_and_reject_bad_values():
     from frehg2_amd.gw import RichardsSolver
+    # nx even, 64 < nz <= 256: the direction kernels run over column pairs, 4 waves a CU
+    deep = RichardsSolver(cases.hetero_512x512x100(nx=16, ny=10, nz=100))
+    assert deep.pcg_shapes() == {"march64": 4, "march32": 4, "column": 16}
+    deep.close()
+    # nz = 50 (one row a lane fills the waves better) or nx odd: one column a wave
     g = RichardsSolver(cases.loam_32x32x50(steps=1))
-    # nx and nz even: the direction kernels run over column pairs, 4 waves a CU
-    assert g.pcg_shapes() == {"march64": 4, "march32": 4, "column": 16}
-    odd = RichardsSolver(cases.loam_32x32x50(steps=1, nx=31))
+    assert g.pcg_shapes() == {"march64": 16, "march32": 32, "column": 16}
+    odd = RichardsSolver(cases.hetero_512x512x100(nx=15, ny=10, nz=100))
     assert odd.pcg_shapes() == {"march64": 16, "march32": 32, "column": 16}
     odd.close()
     g.set_pcg_shapes(8, 0, 32)
-    assert g.pcg_shapes() == {"march64": 8, "march32": 4, "column": 32}
+    assert g.pcg_shapes() == {"march64": 8, "march32": 32, "column": 32}
     with pytest.raises(abi.FrehgError):
         g.set_pcg_shapes(-1, 0, 0)
     with pytest.raises(abi.FrehgError):
