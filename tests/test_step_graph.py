@@ -16,7 +16,8 @@ Pins (every one bitwise: the same launches must give the same bits):
   finishes the step: still bitwise the DEVICE step, which closes it too;
 * a budget the solves fit in: DEVICE is HOST, FP64 and mixed (the mixed
   form's FP32 scale, 1 / sqrt(rr / n), and tolerances are computed on the
-  device, whose double sqrt and divide round as the host's);
+  device, whose double sqrt and divide round as the host's), also on a
+  100-row terrain grid whose PCG runs the pair and quad kernels;
 * a configuration change after capture (a sink) is captured again.
 """
 import numpy as np
@@ -38,6 +39,15 @@ def _stacked(precision="fp64", rtol=1e-12):
         c.rtol, c.atol, c.max_iterations = rtol, 1e-300, 5000
         ms.append(c)
     return Ensemble(ms).case
+
+
+def _terrain(precision="fp64"):
+    """nz = 100, nx even: the PCG's FP64 column-pair and row-pair kernels and
+    sweeps, and the FP32 quad kernels (the stacked members' 16-row columns
+    take the one-row and row-pair forms)."""
+    c = cases.hetero_512x512x100(nx=16, ny=10, nz=100, precision=precision)
+    c.rtol, c.atol = (1e-12 if precision == "fp64" else 3e-12), 1e-300
+    return c
 
 
 def _columns(precision="fp64"):
@@ -80,6 +90,8 @@ def _same(a, b, k, coupled=False):
     ("stacked_fp64", lambda: _stacked("fp64"), True),
     ("stacked_mixed", lambda: _stacked("mixed", 1e-10), True),
     ("columns_fp64", lambda: _columns("fp64"), False),
+    ("terrain_fp64", lambda: _terrain("fp64"), False),
+    ("terrain_mixed", lambda: _terrain("mixed"), False),
 ])
 def test_graph_replay_is_the_device_step(name, make, coupled):
     case = make()
@@ -113,6 +125,15 @@ def test_device_step_is_the_host_step(precision):
     case = _stacked(precision, 1e-12 if precision == "fp64" else 1e-10)
     sims = _run(case, ("host", "device"), 6,
                 every=lambda k, s: _same(s[0], s[1], k, True))
+    assert sims[1].graph_stats()["host_finishes"] == 0
+    assert sims[1].last_solve()["converged"] == 1
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_step_is_the_host_step_in_the_pair_layouts(precision):
+    case = _terrain(precision)
+    print(f"[graph] terrain {precision} host vs device", flush=True)
+    sims = _run(case, ("host", "device"), 5, every=lambda k, s: _same(s[0], s[1], k))
     assert sims[1].graph_stats()["host_finishes"] == 0
     assert sims[1].last_solve()["converged"] == 1
 
