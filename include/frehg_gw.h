@@ -366,10 +366,12 @@ int frehg_gw_set_chebyshev(frehg_gw_solver* s, int degree, double lower_ratio);
 int frehg_gw_enable_column_solve(frehg_gw_solver* s, int enable);
 
 /* Launch shapes of the PCG (no counterpart in the reference, where PETSc
- * makes these choices): waves per CU of the FP64 and FP32 stencil kernels
- * and of the column kernels, each in [1, 256]; 0 keeps a value.  Defaults
- * 16 / 32 / 16.  Results depend on the shapes at rounding level (the order
- * of the partial sums) and are deterministic for given shapes.  The tuner
+ * makes these choices): waves per CU of the FP64 and FP32 direction kernels
+ * and of the column (update) kernels, each in [1, 256]; 0 keeps a value.
+ * Defaults 4 / 4 / 16 where the direction kernels run over column pairs
+ * (nx even, nz even and above 64; FP64 also nz <= 256), else 16 / 32 / 16.
+ * Results depend on the shapes at rounding level (the order of the partial
+ * sums) and are deterministic for given shapes.  The tuner
  * frehg2_amd.autotune picks them by timing solves. */
 int frehg_gw_set_pcg_shapes(frehg_gw_solver* s, int march64_waves_per_cu,
                             int march32_waves_per_cu, int column_waves_per_cu);
