@@ -172,3 +172,4 @@ def test_seam_nonsymmetric_values_and_copied_values():
     with pytest.raises(abi.FrehgError):
         ls.set_precision("mixed")
         _solve(ls, rhs)
+

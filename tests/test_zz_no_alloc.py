@@ -92,3 +92,32 @@ def test_noalloc_refuses_grids_beyond_its_scratch():
     assert rc == abi.FREHG_EINVAL
     assert b"scratch" in lib.frehg_last_error()
     assert (200 + 2) * (200 + 2) * 100 > abi.NOALLOC_MAX_CELLS
+
+
+def test_noalloc_misaligned_operands_take_the_one_row_kernels():
+    """Every operand one double into its buffer (8-byte, not 16-byte
+    aligned): the FP64 pair kernels load and store 16-byte vectors, so the
+    solve runs the one-row-a-lane kernels instead.  Same iterations and the
+    aligned solve's solution to rounding; bitwise equality would mean the
+    pair kernels ran on the misaligned operands."""
+    nx, ny, nz = 40, 30, 100
+    vals, rhs, cols = _system(nx, ny, nz)
+    arrays = _stencil(nx, ny, nz, vals, rhs, cols)
+    x_al, st_al = _noalloc(nx, ny, nz, arrays, abi.PC_ZLINE)
+    lib = abi.load_library()
+    dev = [DeviceArray(np.concatenate([[0.0], a.reshape(-1)])) for a in arrays]
+    sol = DeviceArray.zeros(arrays[0].size + 1)
+    st = abi.SolveStats()
+    print(f"[noalloc] grid {nx}x{ny}x{nz} misaligned", flush=True)
+    p = [d.ptr.value + 8 for d in dev]
+    rc = lib.frehg_ls_solve_stencil_noalloc(nx, ny, nz, p[0], p[1], p[2], p[3], p[4],
+                                            sol.ptr.value + 8, 1e-12, 1e-300, 5000,
+                                            abi.PC_ZLINE, C.byref(st))
+    abi.check(lib, rc)
+    x_mis = sol.numpy()[1:].reshape(arrays[0].shape)[1:-1, 1:-1].reshape(-1)
+    err = float(np.max(np.abs(x_mis - x_al)) / np.max(np.abs(x_al)))
+    print(f"[noalloc] misaligned {st.iterations} it (aligned {st_al.iterations}), "
+          f"difference {err:.2e}", flush=True)
+    assert st.converged == 1 and abs(st.iterations - st_al.iterations) <= 1
+    assert err <= 1e-10
+    assert not np.array_equal(x_mis, x_al)
