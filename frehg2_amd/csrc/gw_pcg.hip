@@ -895,7 +895,6 @@ __device__ __forceinline__ void store4(float* p, float4v v) {
 // The update kernels store r, z (and x) non-temporally: the direction kernel
 // after them then runs 2-3 % faster (FP64 0.336 -> 0.330 ms, FP32 0.181 ->
 // 0.176 ms at 512 x 512 x 100) and they lose nothing themselves.
-template <class T>
 constexpr bool kUpdateNts = true;
 template <class T> struct PairOf;
 template <> struct PairOf<float> { typedef float v __attribute__((ext_vector_type(2))); };
@@ -1487,7 +1486,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_q(Grid g, March mj, Stencil<
 // columns (scan_affine_up_halves / _down_halves): each lane first composes
 // its four steps of a recurrence, the half-wave scans the composed maps, and
 // the lane then walks its first three rows from the value entering it.
-template <bool LINE, int XM, bool NTS = kUpdateNts<float>>
+template <bool LINE, int XM, bool NTS = kUpdateNts>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_q(Grid g, float* __restrict__ x,
                                                          const float* __restrict__ pPrev,
                                                          const float* __restrict__ p,
@@ -1587,7 +1586,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_q(Grid g, float* __restri
 }
 
 // k_pcg_update in the pair layout (x deferred the same way).
-template <class T, int R2, bool LINE, int XM, bool NTS = kUpdateNts<T>>
+template <class T, int R2, bool LINE, int XM, bool NTS = kUpdateNts>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_p(Grid g, T* __restrict__ x,
                                                          const T* __restrict__ pPrev,
                                                          const T* __restrict__ p,
