@@ -313,15 +313,54 @@ __global__ void k_classify_coupled(Dev d) {
   }
 }
 
+// ---- cells gathered into full waves ----------------------------------------
+// A closure's logarithms and exponentials are needed in some cells only (the
+// unsaturated ones), and the cell kernels' waves run over consecutive rows,
+// so nearly every wave holds a few such cells and the whole wave pays for
+// them.  A kernel can instead do the cheap cells at once and append the
+// others to its wave's ring of up to kRing cell indices in LDS (head and cnt
+// wave-uniform): whenever kWave entries wait, the wave evaluates them
+// together.  Each cell's arithmetic is unchanged.
+constexpr int kRing = 2 * kWave;
+struct WaveRing {
+  int64_t* q;
+  int head, cnt;
+};
+__device__ __forceinline__ void ring_push(WaveRing& r, bool want, int64_t val) {
+  const unsigned long long m = __ballot(want);
+  const unsigned lo = static_cast<unsigned>(m), hi = static_cast<unsigned>(m >> 32);
+  const int below = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0));
+  if (want) r.q[(r.head + r.cnt + below) & (kRing - 1)] = val;
+  r.cnt += __popcll(m);
+}
+// The entry of this lane in the next full wave's worth (or in the remainder
+// when last), and whether the lane has one; advances the ring.
+__device__ __forceinline__ bool ring_take(WaveRing& r, int lane, int64_t& val) {
+  __builtin_amdgcn_wave_barrier();
+  const bool has = lane < r.cnt;
+  if (has) val = r.q[(r.head + lane) & (kRing - 1)];
+  __builtin_amdgcn_wave_barrier();
+  const int took = r.cnt < kWave ? r.cnt : kWave;
+  r.head = (r.head + took) & (kRing - 1);
+  r.cnt -= took;
+  return has;
+}
+
 // ---- per-cell conductivities, halo cells included -------------------------
 // With CAP (the predictor, where h is h^n) it also stores C(h^n) for the
-// assembly.
+// assembly.  A cell with h > 0 takes K = Ks and C = 0 at once (what
+// vg_k_from and vg_mualem_capacity return there); the others go through the
+// wave's ring and are evaluated 64 at a time (303 / 256 us against 379 /
+// 328 us for the closure in every lane, with and without C, at 512 x 512 x
+// 100, where 60 % of the cells are saturated).
 template <bool ISO, bool CAP>
-__global__ void k_cell_k(Dev d) {
+__global__ __launch_bounds__(kThreads) void k_cell_k(Dev d) {
   if (gated(d)) return;
+  __shared__ int64_t ring[kThreads / kWave][kRing];
+  const int lane = threadIdx.x & (kWave - 1);
+  WaveRing wr{ring[threadIdx.x / kWave], 0, 0};
   const int64_t n = d.g.cellsH();
-  for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
-       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+  auto eval = [&](int64_t c) {
     MualemFactors f;
     if (CAP) {
       double cap;
@@ -335,6 +374,33 @@ __global__ void k_cell_k(Dev d) {
       d.Ky[c] = vg_k_from(f, d.ksy[c]);
       d.Kz[c] = vg_k_from(f, d.ksz[c]);
     }
+  };
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x - lane; c0 < n;
+       c0 += stride) {
+    const int64_t c = c0 + lane;
+    bool slow = false;
+    if (c < n) {
+      if (d.h[c] > 0.0) {
+        d.Kx[c] = d.ksx[c];
+        if (!ISO) {
+          d.Ky[c] = d.ksy[c];
+          d.Kz[c] = d.ksz[c];
+        }
+        if (CAP) d.ch[c] = 0.0;
+      } else {
+        slow = true;
+      }
+    }
+    ring_push(wr, slow, c);
+    if (wr.cnt >= kWave) {
+      int64_t cq;
+      if (ring_take(wr, lane, cq)) eval(cq);
+    }
+  }
+  if (wr.cnt > 0) {
+    int64_t cq;
+    if (ring_take(wr, lane, cq)) eval(cq);
   }
 }
 
