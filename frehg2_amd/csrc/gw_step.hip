@@ -348,7 +348,7 @@ __device__ __forceinline__ bool ring_take(WaveRing& r, int lane, int64_t& val) {
 
 // ---- per-cell conductivities, halo cells included -------------------------
 // With CAP (the predictor, where h is h^n) it also stores C(h^n) for the
-// assembly.  A cell with h > 0 takes K = Ks and C = 0 at once (what
+// assembly and copies the step's start state into (hn, wcn).  A cell with h > 0 takes K = Ks and C = 0 at once (what
 // vg_k_from and vg_mualem_capacity return there); the others go through the
 // wave's ring and are evaluated 64 at a time (303 / 256 us against 379 /
 // 328 us for the closure in every lane, with and without C, at 512 x 512 x
@@ -381,7 +381,13 @@ __global__ __launch_bounds__(kThreads) void k_cell_k(Dev d) {
     const int64_t c = c0 + lane;
     bool slow = false;
     if (c < n) {
-      if (d.h[c] > 0.0) {
+      const double h = d.h[c];
+      if (CAP) {
+        // the step's start state (h^n, theta^n), halo included
+        d.hn[c] = h;
+        d.wcn[c] = d.wc[c];
+      }
+      if (h > 0.0) {
         d.Kx[c] = d.ksx[c];
         if (!ISO) {
           d.Ky[c] = d.ksy[c];
@@ -1552,6 +1558,13 @@ __global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
     const int64_t c = g.c3(j, i, k);
     const double before = d.wc[c];
     const double hc = d.h[c];
+    // the operands of an active cell, loaded with kt rather than after it (in
+    // bounds: halo), so their latency is not added to kt's
+    const VgSoil s = soil_at(d, c);
+    const double volume = g.az * d.dz3d[c];
+    const bool satU = d.sat[c + 1], satE = d.sat[c + nz], satW = d.sat[c - nz];
+    const bool satN = d.sat[c + sy], satS = d.sat[c - sy], satD = d.sat[c - 1];
+    const double kxE = d.kx[c], kxW = d.kx[c - nz], kyN = d.ky[c], kyS = d.ky[c - sy];
     // ghost heads next to edge columns are carried into the new buffer (the
     // ghost theta is rebuilt from the interior afterwards)
     if (i == 1) d.hOut[c - nz] = d.h[c - nz];
@@ -1560,12 +1573,6 @@ __global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
     if (j == g.ny) d.hOut[c + sy] = d.h[c + sy];
     double wOut = before, hOutV = hc, su = 0.0, sd = 0.0;
     if (k >= kt) {
-      const VgSoil s = soil_at(d, c);
-      const double volume = g.az * d.dz3d[c];
-      // neighbour saturation operands, loaded together (in bounds: halo)
-      const bool satU = d.sat[c + 1], satE = d.sat[c + nz], satW = d.sat[c - nz];
-      const bool satN = d.sat[c + sy], satS = d.sat[c - sy], satD = d.sat[c - 1];
-      const double kxE = d.kx[c], kxW = d.kx[c - nz], kyN = d.ky[c], kyS = d.ky[c - sy];
       if (before >= s.ts) {
         if (!d.reallocOff) {
           const double excess = (before - s.ts) * volume;
@@ -2978,14 +2985,11 @@ PcgOptions GwSolver::stepOptions() const {
 }
 
 void GwSolver::enqueuePre() {
-  const int64_t nH = g.cellsH();
   k_reset_parts<<<64, kThreads, 0, stream>>>(d.parts, rb);
   HIP_CHECK(hipGetLastError());
   // step-start refresh (RichardsSolver.cpp:743-748)
   ghosts(true, true);
-  copy(d.hn, d.h, nH);
-  copy(d.wcn, d.wc, nH);
-  // predictor
+  // predictor (its cell-K pass also copies h, theta into hn, wcn)
   if (d.coupled) {
     k_classify_coupled<<<std::max(1, std::min(4096, (g.columns() + kThreads - 1) / kThreads)),
                          kThreads, 0, stream>>>(d);
