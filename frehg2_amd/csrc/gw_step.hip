@@ -1665,13 +1665,17 @@ __device__ __forceinline__ void clamp_column(const Dev& d, int lane, int kt, int
   }
 }
 
-// Room, sends and their ballot masks of a walking column.
+// Room, sends and their ballot masks of a walking column.  upIn / dnIn,
+// when given, are the column's send volumes already loaded (rows past nz
+// hold 0).
 template <int R>
 __device__ __forceinline__ void walk_inputs(const Dev& d, int lane, int kt, int64_t b0,
                                             const double (&wcv)[R], const double (&dzv)[R],
                                             double (&room)[R], double (&sUp)[R], double (&sDn)[R],
                                             unsigned long long (&sendMask)[R],
-                                            unsigned long long (&roomMask)[R]) {
+                                            unsigned long long (&roomMask)[R],
+                                            const double* upIn = nullptr,
+                                            const double* dnIn = nullptr) {
   const Grid& g = d.g;
 #pragma unroll
   for (int m = 0; m < R; ++m) {
@@ -1681,8 +1685,8 @@ __device__ __forceinline__ void walk_inputs(const Dev& d, int lane, int kt, int6
     sDn[m] = 0.0;
     if (k < g.nz && k >= kt) {
       room[m] = (wcs_at(d, b0 + k) - wcv[m]) * (g.az * dzv[m]);
-      sUp[m] = d.sendUp[b0 + k];
-      sDn[m] = d.sendDown[b0 + k];
+      sUp[m] = upIn ? upIn[m] : d.sendUp[b0 + k];
+      sDn[m] = dnIn ? dnIn[m] : d.sendDown[b0 + k];
     }
     sendMask[m] = __ballot(sUp[m] > 0.0 || sDn[m] > 0.0);
     roomMask[m] = __ballot(room[m] > 0.0);
@@ -1706,32 +1710,45 @@ __global__ __launch_bounds__(kThreads) void k_realloc_walk(Dev d) {
   constexpr int wpb = kThreads / kWave;
   const int cBeg = static_cast<int>(static_cast<int64_t>(xcd) * ncol / kXcds);
   const int cEnd = static_cast<int>(static_cast<int64_t>(xcd + 1) * ncol / kXcds);
-  for (int col = cBeg + local * wpb + wv; col < cEnd; col += bpx * wpb) {
+  // a column's flag is loaded one column ahead, so the walk's operands are
+  // loaded together with the column's state instead of after the flag
+  const int step = bpx * wpb;
+  int col = cBeg + local * wpb + wv;
+  int flag = col < cEnd ? d.colFlag[col] : 0;
+  for (; col < cEnd; col += step) {
+    const int flagNext = col + step < cEnd ? d.colFlag[col + step] : 0;
     const int j = col / g.nx + 1, i = col % g.nx + 1;
     const int kt = d.kt[g.c2(j, i)];
     const int64_t b0 = g.c3(j, i, 0);
-    const bool walk = d.colFlag[col] != 0 && !d.reallocOff;
-    double wcv[R], dzv[R];
+    const bool walk = flag != 0 && !d.reallocOff;
+    flag = flagNext;
+    const int64_t f0 = g.cF(j, i, 0);
+    double wcv[R], dzv[R], upv[R], dnv[R], qz[RF];
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const int k = lane + kWave * m;
       wcv[m] = k < nz ? d.wcOut[b0 + k] : 0.0;
       dzv[m] = k < nz ? d.dz3d[b0 + k] : 1.0;
+      if constexpr (R <= kParallelWalkRows) {
+        upv[m] = walk && k < nz ? d.sendUp[b0 + k] : 0.0;
+        dnv[m] = walk && k < nz ? d.sendDown[b0 + k] : 0.0;
+      }
+    }
+    if constexpr (R <= kParallelWalkRows) {
+#pragma unroll
+      for (int m = 0; m < RF; ++m) {
+        const int k = lane + kWave * m;
+        qz[m] = walk && k <= nz ? d.qzF[f0 + k] : 0.0;
+      }
     }
     if (walk) {
       if constexpr (R > kParallelWalkRows) {
         continue;  // k_realloc_walk_serial
       } else {
         if (d.tw) continue;  // path recording: k_realloc_walk_serial
-        const int64_t f0 = g.cF(j, i, 0);
-        double room[R], sUp[R], sDn[R], qz[RF];
+        double room[R], sUp[R], sDn[R];
         unsigned long long sendMask[R], roomMask[R];
-        walk_inputs<R>(d, lane, kt, b0, wcv, dzv, room, sUp, sDn, sendMask, roomMask);
-#pragma unroll
-        for (int m = 0; m < RF; ++m) {
-          const int k = lane + kWave * m;
-          qz[m] = k <= nz ? d.qzF[f0 + k] : 0.0;
-        }
+        walk_inputs<R>(d, lane, kt, b0, wcv, dzv, room, sUp, sDn, sendMask, roomMask, upv, dnv);
         if (!walk_parallel<R>(g, lane, kt, dtg, wcv, dzv, qz, room, sUp, sDn, sendMask, roomMask,
                               shWalk[wv], lost))
           continue;  // k_realloc_walk_serial
