@@ -2242,6 +2242,11 @@ struct GwSolver {
   double dtgNext = 0, dtgCurrent = 0;
   int rb = 0, nb2d = 0;
   int rbCol = 0;  // blocks of the column kernels (<= rb)
+  // Grids of the cell-K and face passes: the blocks that can be resident at
+  // once (<= rb), so the grid-stride walk sweeps the cells in one round
+  // (cell K 256 -> 231 us, faces 1012 -> 999 us at 512 x 512 x 100; the
+  // assembly, the classification and the theta update gain nothing).
+  int gFaces[2] = {0, 0}, gCellK[4] = {0, 0, 0, 0};
   bool iso = false;  // ksx = ksy = ksz: one Ks and one per-cell K array
   int64_t nActive = 0;
   std::vector<int> codesHost[6];
@@ -2305,15 +2310,15 @@ struct GwSolver {
     HIP_CHECK(hipGetLastError());
   }
   void faces(int corr) {
-    if (iso && !corr) k_cell_k<true, true><<<rb, kThreads, 0, stream>>>(d);
-    else if (iso) k_cell_k<true, false><<<rb, kThreads, 0, stream>>>(d);
-    else if (!corr) k_cell_k<false, true><<<rb, kThreads, 0, stream>>>(d);
-    else k_cell_k<false, false><<<rb, kThreads, 0, stream>>>(d);
+    if (iso && !corr) k_cell_k<true, true><<<gCellK[3], kThreads, 0, stream>>>(d);
+    else if (iso) k_cell_k<true, false><<<gCellK[2], kThreads, 0, stream>>>(d);
+    else if (!corr) k_cell_k<false, true><<<gCellK[1], kThreads, 0, stream>>>(d);
+    else k_cell_k<false, false><<<gCellK[0], kThreads, 0, stream>>>(d);
     HIP_CHECK(hipGetLastError());
     // the predictor's face K are formed inside the assembly
     if (!corr) return;
-    if (d.rRho) k_faces<true><<<rb, kThreads, 0, stream>>>(d, corr);
-    else k_faces<false><<<rb, kThreads, 0, stream>>>(d, corr);
+    if (d.rRho) k_faces<true><<<gFaces[1], kThreads, 0, stream>>>(d, corr);
+    else k_faces<false><<<gFaces[0], kThreads, 0, stream>>>(d, corr);
     HIP_CHECK(hipGetLastError());
   }
   void copy(double* dst, const double* src, int64_t n) {
@@ -2733,6 +2738,22 @@ void GwSolver::create(const frehg_gw_desc& ds) {
   rbCol = static_cast<int>(std::min<int64_t>(
       rb, std::max<int64_t>(8, ((g.columns() + kThreads / kWave - 1) / (kThreads / kWave) + 7) /
                                    8 * 8)));
+  {
+    auto resident = [&](const void* f) {
+      int per = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, kThreads, 0) != hipSuccess ||
+          per <= 0)
+        return rb;
+      const int64_t b = (int64_t(cus) * per + 7) / 8 * 8;
+      return static_cast<int>(std::min<int64_t>(rb, std::max<int64_t>(8, b)));
+    };
+    gFaces[0] = resident(reinterpret_cast<const void*>(&k_faces<false>));
+    gFaces[1] = resident(reinterpret_cast<const void*>(&k_faces<true>));
+    gCellK[0] = resident(reinterpret_cast<const void*>(&k_cell_k<false, false>));
+    gCellK[1] = resident(reinterpret_cast<const void*>(&k_cell_k<false, true>));
+    gCellK[2] = resident(reinterpret_cast<const void*>(&k_cell_k<true, false>));
+    gCellK[3] = resident(reinterpret_cast<const void*>(&k_cell_k<true, true>));
+  }
 
   std::vector<double> bath, dz, bot;
   std::vector<int> ktop;
