@@ -996,10 +996,9 @@ __global__ void k_scatter(Dev d) {
     const int k = static_cast<int>(r.t % g.nz);
     const int col = static_cast<int>(r.t / g.nz);
     const int j = col / g.nx + 1, i = col % g.nx + 1;
-    if (act(d, j, i, k)) {
-      const int64_t c = g.c3(j, i, k);
-      d.h[c] = d.x[c];
-    }
+    const int64_t c = g.c3(j, i, k);
+    const double xv = d.x[c];  // loaded with kt, not after it
+    if (act(d, j, i, k)) d.h[c] = xv;
   }
 }
 
@@ -1151,24 +1150,29 @@ __global__ void k_update_wc(Dev d) {
     const int col = static_cast<int>(r.t / g.nz);
     int j, i;
     tiled_column(g, col, j, i);
-    if (!act(d, j, i, k)) continue;
     const int64_t c = g.c3(j, i, k);
-    const double volume = g.az * d.dz3d[c];
-    const double dh = d.h[c] - d.hn[c];
     const int64_t f = g.cF(j, i, k);
+    // the operands, loaded with kt rather than after the activity test (in
+    // bounds: halo and face planes), so their latency is not added to kt's
+    const double dz = d.dz3d[c], hc = d.h[c], hnc = d.hn[c], wcnc = d.wcn[c];
+    const double qxc = d.qx[c], qxW = d.qx[c - g.nz], qyc = d.qy[c], qyS = d.qy[c - sy];
+    const double qzB = d.qzF[f + 1], qzT = d.qzF[f];
+    if (!act(d, j, i, k)) continue;
+    const double volume = g.az * dz;
+    const double dh = hc - hnc;
     const double rr = DENS ? d.rRho[c] : 1.0;
     const double coeff = volume * (rr + rr * d.ss * dh / wcs_at(d, c));
     double dqx, dqy, dqz;
     if (DENS) {
-      dqx = dtg * (d.qx[c] * d.rXp[c] - d.qx[c - g.nz] * d.rXp[c - g.nz]);
-      dqy = dtg * (d.qy[c] * d.rYp[c] - d.qy[c - sy] * d.rYp[c - sy]);
-      dqz = dtg * (d.qzF[f + 1] * d.rZp[f + 1] - d.qzF[f] * d.rZp[f]);
+      dqx = dtg * (qxc * d.rXp[c] - qxW * d.rXp[c - g.nz]);
+      dqy = dtg * (qyc * d.rYp[c] - qyS * d.rYp[c - sy]);
+      dqz = dtg * (qzB * d.rZp[f + 1] - qzT * d.rZp[f]);
     } else {
-      dqx = dtg * (d.qx[c] - d.qx[c - g.nz]);
-      dqy = dtg * (d.qy[c] - d.qy[c - sy]);
-      dqz = dtg * (d.qzF[f + 1] - d.qzF[f]);
+      dqx = dtg * (qxc - qxW);
+      dqy = dtg * (qyc - qyS);
+      dqz = dtg * (qzB - qzT);
     }
-    double upd = (d.wcn[c] * rr * volume + dqx + dqy + dqz) / coeff;
+    double upd = (wcnc * rr * volume + dqx + dqy + dqz) / coeff;
     ss.add(upd * volume * rr * d.ss * dh / wcs_at(d, c));
     if (SINK) {
       const double sk = d.sink[c];
@@ -1180,7 +1184,7 @@ __global__ void k_update_wc(Dev d) {
       }
     }
     d.wc[c] = upd;
-    d.sat[c] = (upd >= wcs_at(d, c) || d.h[c] > aev_at(d, c)) ? 1 : 0;
+    d.sat[c] = (upd >= wcs_at(d, c) || hc > aev_at(d, c)) ? 1 : 0;
   }
   if (SINK) {
     double v[2] = {ss.value(), tr.value()};
