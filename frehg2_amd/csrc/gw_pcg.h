@@ -221,7 +221,7 @@ class Pcg {
   int innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T* z, T* q,
                       const T* invd, const T* w, bool zline, double tol2, int maxIt,
                       const March& mj, int nprev0, int n0, bool restart,
-                      const PcgOptions& opt, PcgStats* st, int* itDone);
+                      const PcgOptions& opt, PcgStats* st, int* itDone, double rr0 = -1.0);
   void reduceHost(int nparts, int nvals, double* out);
   void applyShapes();
   void initCommon();

@@ -77,7 +77,8 @@ struct PcgState {
 constexpr int kBlock = 256;
 constexpr int kWpb = kBlock / kWave;
 constexpr int kChunk = 32;   // rows per LDS tile in the factorization (16: +20 % time)
-constexpr int kPoll = 8;     // iterations between host polls
+constexpr int kPoll = 8;      // iterations between host polls (until a rate is known)
+constexpr int kPollMax = 32;  // the most iterations launched between two polls
 
 #define HIP_OK(x)                                                                     \
   do {                                                                                \
@@ -2748,7 +2749,7 @@ template <class T>
 int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T* z, T* q,
                          const T* invd, const T* w, bool zline, double tol2, int maxIt,
                          const March& mj, int nprev0, int n0, bool restart,
-                         const PcgOptions& opt, PcgStats* st, int* itDone) {
+                         const PcgOptions& opt, PcgStats* st, int* itDone, double rr0) {
   // Iterations are numbered on from n0 so that a resumed run finds the
   // previous rz in the state slot the numbering expects.
   PcgState* dst = static_cast<PcgState*>(state_);
@@ -2759,14 +2760,22 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
   HIP_OK(hipGetLastError());
   T* const bufOld = pOld;  // dir launch n writes p into (n - n0 even ? bufNew : bufOld)
   T* const bufNew = pNew;
-  hipEvent_t evs[2 * kPoll];
-  bool sampled[kPoll] = {};
+  hipEvent_t evs[2 * kPollMax];
+  bool sampled[kPollMax] = {};
   const int every = std::max(1, opt.timing_every);
   if (opt.timing)
     for (auto& e : evs) HIP_OK(hipEventCreate(&e));
   int done = 0;
+  // Launches between polls: kPoll, then the iterations the recursion's rate
+  // since the last poll predicts it still needs, plus the launch that sees it
+  // converge.  Fewer polls (each leaves the device idle while the host
+  // reacts) and fewer launches queued past convergence; the iterates are
+  // the same.
+  int chunkLen = kPoll;
+  double rrPrev = rr0;  // r.r entering iteration itPrev (< 0: not known)
+  int itPrev = n0;
   while (true) {
-    const int chunk = std::min(kPoll, maxIt - n);
+    const int chunk = std::min(chunkLen, maxIt - n);
     if (chunk <= 0) break;
     for (int c = 0; c < chunk; ++c, ++n) {
       // an event pair around a launch delays the next dispatch by a few
@@ -2800,6 +2809,17 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
       done = 1;
       break;
     }
+    const double rrNow = hst->rr;
+    const int itNow = hst->it;
+    chunkLen = kPoll;
+    if (rrPrev > 0.0 && rrNow > 0.0 && rrNow < rrPrev && itNow > itPrev && tol2 > 0.0) {
+      const double perIt = std::log(rrNow / rrPrev) / (itNow - itPrev);
+      const double need = std::log(tol2 / rrNow) / perIt;
+      if (std::isfinite(need))
+        chunkLen = static_cast<int>(std::min<double>(kPollMax, std::max(2.0, std::ceil(need) + 1.0)));
+    }
+    rrPrev = rrNow;
+    itPrev = itNow;
   }
   if (opt.timing)
     for (auto& e : evs) (void)hipEventDestroy(e);
@@ -2858,7 +2878,7 @@ int Pcg::fp64Loop(const Stencil<double>& A, const double* b, double* x, const Pc
     int its = 0;
     innerIterations<double>(A, x, pOld, pNew, r_, z_, q_, invd_, w_, opt.zline, tol2,
                             opt.max_iterations - total, march64_, sweepBlocks(), 0, true, opt,
-                            st, &its);
+                            st, &its, rr);
     total += its;
     if (st->rounds++ == 0) st->first_inner = its;
     else st->later_inner = std::max(st->later_inner, its);
@@ -2961,7 +2981,8 @@ int Pcg::mixedLoop(const Stencil<double>& A, const double* b, double* x, const P
     int its = 0;
     innerIterations<float>(A32, d32_, c.pOld, c.pNew, r32_, z32_, q32_, invd32_, w32_, opt.zline,
                            tolIn2, std::min(opt.max_iterations - c.total, 400), march32_,
-                           opt.cheb ? march32_.blocks : blocks_, c.n0, c.restart, opt, st, &its);
+                           opt.cheb ? march32_.blocks : blocks_, c.n0, c.restart, opt, st, &its,
+                           rrS);
     c.total += its;
     c.n0 += its;
     c.restart = false;
@@ -3179,10 +3200,10 @@ int Pcg::resumeDevice(const Stencil<double>& A, const double* b, double* x, cons
 template int Pcg::innerIterations<double>(const Stencil<double>&, double*, double*&, double*&,
                                           double*, double*, double*, const double*, const double*,
                                           bool, double, int, const March&, int, int, bool,
-                                          const PcgOptions&, PcgStats*, int*);
+                                          const PcgOptions&, PcgStats*, int*, double);
 template int Pcg::innerIterations<float>(const Stencil<float>&, float*, float*&, float*&, float*,
                                          float*, float*, const float*, const float*, bool, double,
                                          int, const March&, int, int, bool, const PcgOptions&,
-                                         PcgStats*, int*);
+                                         PcgStats*, int*, double);
 
 }  // namespace frehg
