@@ -1201,23 +1201,39 @@ __global__ void k_update_wc(Dev d) {
 struct Split {
   double up, down;
 };
-__device__ Split gradient_split(const Dev& d, int j, int i, int k) {
+// The operands of a cell's gradient split (the halo and the face planes keep
+// every address in bounds); h is the state entering reallocation.  The
+// caller loads them with its own operands, so they are in flight together
+// instead of one behind each test.
+struct SplitOps {
+  double h0, hE, hW, hN, hS, hU, hD, kxE, kxW, kyN, kyS, kzUp, kzDn, dz0, dzU, dzD;
+  int ktE, ktW, ktN, ktS, kt0;
+};
+__device__ __forceinline__ SplitOps split_ops(const Dev& d, int j, int i, int k) {
   const Grid& g = d.g;
-  const double* hR = d.h;  // the state entering reallocation
+  const double* hR = d.h;
   const int64_t c = g.c3(j, i, k);
   const int64_t sx = g.nz, sy = static_cast<int64_t>(g.nx2) * g.nz;
   const int64_t f = g.cF(j, i, k);
-  // Every operand is loaded up front (the halo and the face planes keep all
-  // of these addresses in bounds), so the loads are in flight together
-  // instead of one behind each test.
-  const double h0 = hR[c], hE = hR[c + sx], hW = hR[c - sx], hN = hR[c + sy], hS = hR[c - sy];
-  const double hU = hR[c + 1], hD = hR[c - 1];
-  const double kxE = d.kx[c], kxW = d.kx[c - sx], kyN = d.ky[c], kyS = d.ky[c - sy];
-  const double kzUp = d.kzF[f + 1], kzDn = d.kzF[f];
-  const double dz0 = d.dz3d[c], dzU = d.dz3d[c + 1], dzD = d.dz3d[c - 1];
-  const int ktE = d.kt[g.c2(j, i + 1)], ktW = d.kt[g.c2(j, i - 1)];
-  const int ktN = d.kt[g.c2(j + 1, i)], ktS = d.kt[g.c2(j - 1, i)];
-  const int kt0 = d.kt[g.c2(j, i)];
+  SplitOps o;
+  o.h0 = hR[c], o.hE = hR[c + sx], o.hW = hR[c - sx], o.hN = hR[c + sy], o.hS = hR[c - sy];
+  o.hU = hR[c + 1], o.hD = hR[c - 1];
+  o.kxE = d.kx[c], o.kxW = d.kx[c - sx], o.kyN = d.ky[c], o.kyS = d.ky[c - sy];
+  o.kzUp = d.kzF[f + 1], o.kzDn = d.kzF[f];
+  o.dz0 = d.dz3d[c], o.dzU = d.dz3d[c + 1], o.dzD = d.dz3d[c - 1];
+  o.ktE = d.kt[g.c2(j, i + 1)], o.ktW = d.kt[g.c2(j, i - 1)];
+  o.ktN = d.kt[g.c2(j + 1, i)], o.ktS = d.kt[g.c2(j - 1, i)];
+  o.kt0 = d.kt[g.c2(j, i)];
+  return o;
+}
+__device__ Split gradient_split(const Dev& d, int j, int i, int k, const SplitOps& o) {
+  const Grid& g = d.g;
+  const int64_t f = g.cF(j, i, k);
+  const double h0 = o.h0, hE = o.hE, hW = o.hW, hN = o.hN, hS = o.hS, hU = o.hU, hD = o.hD;
+  const double kxE = o.kxE, kxW = o.kxW, kyN = o.kyN, kyS = o.kyS;
+  const double kzUp = o.kzUp, kzDn = o.kzDn;
+  const double dz0 = o.dz0, dzU = o.dzU, dzD = o.dzD;
+  const int ktE = o.ktE, ktW = o.ktW, ktN = o.ktN, ktS = o.ktS, kt0 = o.kt0;
   double d6[6];
   d6[0] = (k >= ktE && kxE > 0.0) ? (hE - h0) / g.dx : 0.0;
   d6[1] = (k >= ktW && kxW > 0.0) ? (h0 - hW) / g.dx : 0.0;
@@ -1569,6 +1585,8 @@ __global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
     const bool satU = d.sat[c + 1], satE = d.sat[c + nz], satW = d.sat[c - nz];
     const bool satN = d.sat[c + sy], satS = d.sat[c - sy], satD = d.sat[c - 1];
     const double kxE = d.kx[c], kxW = d.kx[c - nz], kyN = d.ky[c], kyS = d.ky[c - sy];
+    // and the gradient split's (a third of the cells split an excess here)
+    const SplitOps so = split_ops(d, j, i, k);
     // ghost heads next to edge columns are carried into the new buffer (the
     // ghost theta is rebuilt from the interior afterwards)
     if (i == 1) d.hOut[c - nz] = d.h[c - nz];
@@ -1580,7 +1598,7 @@ __global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
       if (before >= s.ts) {
         if (!d.reallocOff) {
           const double excess = (before - s.ts) * volume;
-          const Split sp = gradient_split(d, j, i, k);
+          const Split sp = gradient_split(d, j, i, k, so);
           su = excess * sp.up;
           sd = excess * sp.down;
           adj.add(-(excess - su - sd));
@@ -1609,7 +1627,7 @@ __global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
           wOut = after;
           if (d.redistribute && after < before) {
             const double surplus = (before - after) * volume;
-            const Split sp = gradient_split(d, j, i, k);
+            const Split sp = gradient_split(d, j, i, k, so);
             su = surplus * sp.up;
             sd = surplus * sp.down;
             adj.add(-(surplus - su - sd));
