@@ -515,13 +515,19 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
     const int64_t cb = g.c3(j, i, 0) + k, ca = cb - 1;
     const double Kza = d.Kz[ca], Kzb = d.Kz[cb], ha = d.h[ca], hb = d.h[cb];
     const double dza = d.dz3d[ca], dzb = d.dz3d[cb];
+    // the column's top and bottom inputs (2D), loaded with the rest rather
+    // than behind kt and the face-K store in the top and bottom faces' rules
+    const int topc = d.topCode[c2], botc = d.botCode[c2];
+    const double topv = d.topValue[c2], botv = d.botValue[c2];
+    const bool etc = d.etCol && d.etCol[c2];
+    const double etq = d.etCol ? d.etQ[c2] : 0.0;
     const double base = d.harmonic ? face_k_harmonic(Kza, d.ksz[ca], dza, Kzb, d.ksz[cb], dzb)
                                    : 0.5 * (Kza + Kzb);
     double v;
     if (k == 0) {
       v = 0.0;
     } else if (k == g.nz) {
-      v = d.botCode[c2] == FREHG_BC_NOFLUX ? 0.0 : Kza;
+      v = botc == FREHG_BC_NOFLUX ? 0.0 : Kza;
     } else if (actK && !actM) {
       v = Kzb;
     } else if (!actM) {
@@ -536,7 +542,7 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
         const double wet = d.pcLegacy ? 0.5 * (d.ksz[c] + own) : d.ksz[c];
         v = d.dept[c2] > 0.0 ? wet : own;
       } else {
-        const int code = d.topCode[c2];
+        const int code = topc;
         if (code == FREHG_BC_HEAD) v = d.ksz[c];
         else if (code == FREHG_BC_NOFLUX) v = 0.0;
         else v = own;
@@ -582,30 +588,30 @@ __device__ __forceinline__ void face_z_at(const Dev& d, int corr, double dtg, in
           const double kface = 0.5 * (kG + Kzb);
           q = g.az * kface * vz * ((hb - depth) / delta - rz);
           if (q <= 0.0) q = 0.0;
-          if (d.etCol && d.etCol[c2]) {
-            q += d.etQ[c2] * g.az;
-          } else if (d.topCode[c2] == FREHG_BC_FLUX) {
-            const double tv = d.topValue[c2];
+          if (etc) {
+            q += etq * g.az;
+          } else if (topc == FREHG_BC_FLUX) {
+            const double tv = topv;
             if ((tv > 0.0 && d.wc[c] > wcr_at(d, c)) || tv < 0.0) q += tv * g.az;
           }
         }
       } else {
-        const int code = d.topCode[c2];
-        if (d.etCol && d.etCol[c2]) {
-          q = d.etQ[c2] * g.az;
+        const int code = topc;
+        if (etc) {
+          q = etq * g.az;
         } else if (code == FREHG_BC_FLUX) {
-          q = d.topValue[c2] * g.az;
+          q = topv * g.az;
         } else if (code == FREHG_BC_HEAD) {
-          q = g.az * d.ksz[c] * vz * ((hb - d.topValue[c2]) / (0.5 * dzb) - rz);
+          q = g.az * d.ksz[c] * vz * ((hb - topv) / (0.5 * dzb) - rz);
         }
       }
     } else if (k == g.nz && actM) {
       const double kface = Kza;
-      const int code = d.botCode[c2];
-      if (code == FREHG_BC_FLUX) q = d.botValue[c2] * g.az;
+      const int code = botc;
+      if (code == FREHG_BC_FLUX) q = botv * g.az;
       else if (code == FREHG_BC_GRAVITY) q = -kface * vz * rz * g.az;
       else if (code == FREHG_BC_HEAD)
-        q = g.az * kface * vz * ((d.botValue[c2] - ha) / (0.5 * dza) - rz);
+        q = g.az * kface * vz * ((botv - ha) / (0.5 * dza) - rz);
     } else if (actM && actK) {
       double kface = base;
       if (d.pcLegacy && d.ksz[ca] != d.ksz[cb])  // legacy: the lower cell's Ks on both flanks
