@@ -348,9 +348,10 @@ __device__ __forceinline__ bool ring_take(WaveRing& r, int lane, int64_t& val) {
 
 // ---- per-cell conductivities, halo cells included -------------------------
 // With CAP (the predictor, where h is h^n) it also stores C(h^n) for the
-// assembly and copies the step's start state into (hn, wcn).  A cell with h > 0 takes K = Ks and C = 0 at once (what
-// vg_k_from and vg_mualem_capacity return there); the others go through the
-// wave's ring and are evaluated 64 at a time (303 / 256 us against 379 /
+// assembly and copies the step's start state into (hn, wcn).  A cell with
+// h > 0 takes K = Ks and C = 0 at once (what vg_k_from and
+// vg_mualem_capacity return there); the others go through the wave's ring
+// and are evaluated 64 at a time (303 / 256 us against 379 /
 // 328 us for the closure in every lane, with and without C, at 512 x 512 x
 // 100, where 60 % of the cells are saturated).
 template <bool ISO, bool CAP>
@@ -1570,87 +1571,146 @@ __global__ void k_sat_halo(Dev d) {
 // volumes, and flags the columns that have senders.
 __global__ __launch_bounds__(kThreads) void k_realloc_classify(Dev d) {
   if (gated(d)) return;
+  // The cells whose head is rebuilt from theta (h(theta): two logarithms and
+  // two exponentials) go through the wave's ring, their closure operands
+  // beside them, and are evaluated 64 at a time.
+  __shared__ int64_t ringIdx[kThreads / kWave][kRing];
+  __shared__ double ringVal[kThreads / kWave][5][kRing];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  WaveRing wr{ringIdx[wv], 0, 0};
   const Grid& g = d.g;
   const int nz = g.nz;
   const int64_t n = static_cast<int64_t>(g.columns()) * nz;
   const int64_t sy = static_cast<int64_t>(g.nx2) * nz;
   KSum adj, disc;
-  for (Range r = xcd_range(n); r.t < r.end; r.t += r.stride) {
-    const int k = static_cast<int>(r.t % nz);
-    const int col = static_cast<int>(r.t / nz);
-    const int j = col / g.nx + 1, i = col % g.nx + 1;
-    const int c2 = g.c2(j, i);
-    const int kt = d.kt[c2];
-    const int64_t c = g.c3(j, i, k);
-    const double before = d.wc[c];
-    const double hc = d.h[c];
-    // the operands of an active cell, loaded with kt rather than after it (in
-    // bounds: halo), so their latency is not added to kt's
-    const VgSoil s = soil_at(d, c);
-    const double volume = g.az * d.dz3d[c];
-    const bool satU = d.sat[c + 1], satE = d.sat[c + nz], satW = d.sat[c - nz];
-    const bool satN = d.sat[c + sy], satS = d.sat[c - sy], satD = d.sat[c - 1];
-    const double kxE = d.kx[c], kxW = d.kx[c - nz], kyN = d.ky[c], kyS = d.ky[c - sy];
-    // and the gradient split's (a third of the cells split an excess here)
-    const SplitOps so = split_ops(d, j, i, k);
-    // ghost heads next to edge columns are carried into the new buffer (the
-    // ghost theta is rebuilt from the interior afterwards)
-    if (i == 1) d.hOut[c - nz] = d.h[c - nz];
-    if (i == g.nx) d.hOut[c + nz] = d.h[c + nz];
-    if (j == 1) d.hOut[c - sy] = d.h[c - sy];
-    if (j == g.ny) d.hOut[c + sy] = d.h[c + sy];
-    double wOut = before, hOutV = hc, su = 0.0, sd = 0.0;
-    if (k >= kt) {
-      if (before >= s.ts) {
-        if (!d.reallocOff) {
-          const double excess = (before - s.ts) * volume;
-          const Split sp = gradient_split(d, j, i, k, so);
-          su = excess * sp.up;
-          sd = excess * sp.down;
-          adj.add(-(excess - su - sd));
-          disc.add(excess - su - sd);
-          wOut = s.ts;
-        }
-      } else {
-        // the reference's test chain (first true wins) is an OR of its tests
-        bool adjSat = (k + 1 < nz && satU) || (kxE != 0.0 && satE) || (kxW != 0.0 && satW) ||
-                      (kyN != 0.0 && satN) || (kyS != 0.0 && satS);
-        if (!adjSat) {
-          if (k == 0 || k - 1 < kt) {
-            if (d.coupled) {
-              if (d.dept[c2] > 0.0) adjSat = true;
-            } else if (d.topCode[c2] == FREHG_BC_HEAD && d.topValue[c2] >= 0.0) {
+  auto headOf = [&]() {
+    __builtin_amdgcn_wave_barrier();
+    const int e = (wr.head + lane) & (kRing - 1);
+    const bool has = lane < wr.cnt;
+    int64_t c = 0;
+    double a = 0.0, nn = 0.0, ts = 0.0, tr = 0.0, w = 0.0;
+    if (has) {
+      c = ringIdx[wv][e];
+      a = ringVal[wv][0][e];
+      nn = ringVal[wv][1][e];
+      ts = ringVal[wv][2][e];
+      tr = ringVal[wv][3][e];
+      w = ringVal[wv][4][e];
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int took = wr.cnt < kWave ? wr.cnt : kWave;
+    wr.head = (wr.head + took) & (kRing - 1);
+    wr.cnt -= took;
+    if (has) d.hOut[c] = vg_head(VgSoil{a, nn, ts, tr, 0.0}, w);
+  };
+  // the walk of xcd_range, wave-uniform: lane l takes t0 + l
+  const Range rg = xcd_range(n);
+  for (int64_t t0 = rg.t - lane; t0 < rg.end; t0 += rg.stride) {
+    const int64_t t = t0 + lane;
+    bool defer = false;
+    int64_t cDef = 0;
+    VgSoil sDef{0.0, 0.0, 0.0, 0.0, 0.0};
+    double wDef = 0.0;
+    if (t < rg.end) {
+      const int k = static_cast<int>(t % nz);
+      const int col = static_cast<int>(t / nz);
+      const int j = col / g.nx + 1, i = col % g.nx + 1;
+      const int c2 = g.c2(j, i);
+      const int kt = d.kt[c2];
+      const int64_t c = g.c3(j, i, k);
+      const double before = d.wc[c];
+      const double hc = d.h[c];
+      // the operands of an active cell, loaded with kt rather than after it (in
+      // bounds: halo), so their latency is not added to kt's
+      const VgSoil s = soil_at(d, c);
+      const double volume = g.az * d.dz3d[c];
+      const bool satU = d.sat[c + 1], satE = d.sat[c + nz], satW = d.sat[c - nz];
+      const bool satN = d.sat[c + sy], satS = d.sat[c - sy], satD = d.sat[c - 1];
+      const double kxE = d.kx[c], kxW = d.kx[c - nz], kyN = d.ky[c], kyS = d.ky[c - sy];
+      // and the gradient split's (a third of the cells split an excess here)
+      const SplitOps so = split_ops(d, j, i, k);
+      // ghost heads next to edge columns are carried into the new buffer (the
+      // ghost theta is rebuilt from the interior afterwards)
+      if (i == 1) d.hOut[c - nz] = d.h[c - nz];
+      if (i == g.nx) d.hOut[c + nz] = d.h[c + nz];
+      if (j == 1) d.hOut[c - sy] = d.h[c - sy];
+      if (j == g.ny) d.hOut[c + sy] = d.h[c + sy];
+      double wOut = before, hOutV = hc, su = 0.0, sd = 0.0;
+      if (k >= kt) {
+        if (before >= s.ts) {
+          if (!d.reallocOff) {
+            const double excess = (before - s.ts) * volume;
+            const Split sp = gradient_split(d, j, i, k, so);
+            su = excess * sp.up;
+            sd = excess * sp.down;
+            adj.add(-(excess - su - sd));
+            disc.add(excess - su - sd);
+            wOut = s.ts;
+          }
+        } else {
+          // the reference's test chain (first true wins) is an OR of its tests
+          bool adjSat = (k + 1 < nz && satU) || (kxE != 0.0 && satE) || (kxW != 0.0 && satW) ||
+                        (kyN != 0.0 && satN) || (kyS != 0.0 && satS);
+          if (!adjSat) {
+            if (k == 0 || k - 1 < kt) {
+              if (d.coupled) {
+                if (d.dept[c2] > 0.0) adjSat = true;
+              } else if (d.topCode[c2] == FREHG_BC_HEAD && d.topValue[c2] >= 0.0) {
+                adjSat = true;
+              }
+            } else if (satD) {
               adjSat = true;
             }
-          } else if (satD) {
-            adjSat = true;
           }
-        }
-        if (!adjSat) {
-          if (before < 0.9999 * s.ts) hOutV = vg_head(s, before);
-        } else if (!d.reallocOff) {
-          const double after = vg_theta(s, hc);
-          wOut = after;
-          if (d.redistribute && after < before) {
-            const double surplus = (before - after) * volume;
-            const Split sp = gradient_split(d, j, i, k, so);
-            su = surplus * sp.up;
-            sd = surplus * sp.down;
-            adj.add(-(surplus - su - sd));
-            disc.add(surplus - su - sd);
-          } else {
-            adj.add((after - before) * volume);
-            if (after < before) disc.add((before - after) * volume);
+          if (!adjSat) {
+            if (before < 0.9999 * s.ts) {  // hOut = vg_head(s, before), through the ring
+              defer = true;
+              cDef = c;
+              sDef = s;
+              wDef = before;
+            }
+          } else if (!d.reallocOff) {
+            const double after = vg_theta(s, hc);
+            wOut = after;
+            if (d.redistribute && after < before) {
+              const double surplus = (before - after) * volume;
+              const Split sp = gradient_split(d, j, i, k, so);
+              su = surplus * sp.up;
+              sd = surplus * sp.down;
+              adj.add(-(surplus - su - sd));
+              disc.add(surplus - su - sd);
+            } else {
+              adj.add((after - before) * volume);
+              if (after < before) disc.add((before - after) * volume);
+            }
           }
         }
       }
+      d.wcOut[c] = wOut;
+      if (!defer) d.hOut[c] = hOutV;
+      d.sendUp[c] = su;
+      d.sendDown[c] = sd;
+      if (su > 0.0 || sd > 0.0) d.colFlag[col] = 1;
     }
-    d.wcOut[c] = wOut;
-    d.hOut[c] = hOutV;
-    d.sendUp[c] = su;
-    d.sendDown[c] = sd;
-    if (su > 0.0 || sd > 0.0) d.colFlag[col] = 1;
+    {
+      const unsigned long long m = __ballot(defer);
+      const unsigned lo = static_cast<unsigned>(m), hi = static_cast<unsigned>(m >> 32);
+      const int below = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0));
+      if (defer) {
+        const int e = (wr.head + wr.cnt + below) & (kRing - 1);
+        ringIdx[wv][e] = cDef;
+        ringVal[wv][0][e] = sDef.a;
+        ringVal[wv][1][e] = sDef.n;
+        ringVal[wv][2][e] = sDef.ts;
+        ringVal[wv][3][e] = sDef.tr;
+        ringVal[wv][4][e] = wDef;
+      }
+      wr.cnt += __popcll(m);
+    }
+    if (wr.cnt >= kWave) headOf();
   }
+  if (wr.cnt > 0) headOf();
   double v[2] = {adj.value(), disc.value()};
   const int sl[2] = {S_ADJ, S_DISC};
   put_parts<2>(d, sl, v);
