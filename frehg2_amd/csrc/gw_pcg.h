@@ -286,6 +286,9 @@ class Pcg {
   void* state_ = nullptr;     // PcgState (device)
   void* stateHost_ = nullptr; // pinned mirror (external mode: the caller's host buffer)
   bool external_ = false;     // memory owned by the caller (PcgWorkspace)
+  // timing mode's event pairs around sampled direction launches, created on
+  // first use and kept (creating them per solve leaves the device idle)
+  std::vector<hipEvent_t> evs_;
   // direct column solve
   bool direct_ = false;
   const int* kt_ = nullptr;

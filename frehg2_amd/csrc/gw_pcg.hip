@@ -2263,6 +2263,7 @@ Pcg::Pcg(const Grid& g, hipStream_t stream) : g_(g), stream_(stream) {
 }
 
 Pcg::~Pcg() {
+  for (auto& e : evs_) (void)hipEventDestroy(e);
   if (external_) return;
   for (void* p : {(void*)r_, (void*)z_, (void*)p0_, (void*)p1_, (void*)q_, (void*)invd_, (void*)w_,
                   (void*)Ad32_, (void*)Axp32_, (void*)Ayp32_, (void*)Azp32_, (void*)r32_,
@@ -2760,11 +2761,13 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
   HIP_OK(hipGetLastError());
   T* const bufOld = pOld;  // dir launch n writes p into (n - n0 even ? bufNew : bufOld)
   T* const bufNew = pNew;
-  hipEvent_t evs[2 * kPollMax];
   bool sampled[kPollMax] = {};
   const int every = std::max(1, opt.timing_every);
-  if (opt.timing)
-    for (auto& e : evs) HIP_OK(hipEventCreate(&e));
+  if (opt.timing && evs_.empty()) {
+    evs_.resize(2 * kPollMax);
+    for (auto& e : evs_) HIP_OK(hipEventCreate(&e));
+  }
+  hipEvent_t* evs = evs_.data();
   int done = 0;
   // Launches between polls: kPoll, then the iterations the recursion's rate
   // since the last poll predicts it still needs, plus the launch that sees it
@@ -2821,8 +2824,6 @@ int Pcg::innerIterations(const Stencil<T>& A, T* x, T*& pOld, T*& pNew, T* r, T*
     rrPrev = rrNow;
     itPrev = itNow;
   }
-  if (opt.timing)
-    for (auto& e : evs) (void)hipEventDestroy(e);
   *itDone = (done ? hst->it : std::min(n, maxIt)) - n0;
   // Launches queued after convergence are no-ops, yet the loop swapped the
   // pointers for them too: point pOld at the last p actually formed.
