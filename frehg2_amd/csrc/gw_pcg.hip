@@ -948,14 +948,13 @@ __device__ __forceinline__ void pair_shift(const T (&v0)[R2], const T (&v1)[R2],
   }
 }
 
-// z = M^-1 r in the pair layout (the recurrences of apply_precond).
+// z = M^-1 r in the pair layout (the recurrences of apply_precond), from the
+// factors iv = invd, w of the rows (w unused without LINE).
 template <int R2, bool LINE, class T>
-__device__ __forceinline__ void precond_pair(int lane, int nz, int64_t base,
-                                             const T* __restrict__ invd, const T* __restrict__ w,
-                                             const T (&r0)[R2], const T (&r1)[R2], T (&z0)[R2],
-                                             T (&z1)[R2]) {
-  T iv0[R2], iv1[R2];
-  load_pair<R2>(invd, base, lane, nz, iv0, iv1);
+__device__ __forceinline__ void precond_pair_f(int lane, const T (&iv0)[R2], const T (&iv1)[R2],
+                                               const T (&w0)[R2], const T (&w1)[R2],
+                                               const T (&r0)[R2], const T (&r1)[R2],
+                                               T (&z0)[R2], T (&z1)[R2]) {
   if constexpr (!LINE) {
 #pragma unroll
     for (int m = 0; m < R2; ++m) {
@@ -964,8 +963,7 @@ __device__ __forceinline__ void precond_pair(int lane, int nz, int64_t base,
     }
     return;
   } else {
-    T w0[R2], w1[R2], wDn[R2], wUp[R2];
-    load_pair<R2>(w, base, lane, nz, w0, w1);
+    T wDn[R2], wUp[R2];
     pair_shift<R2>(w0, w1, lane, wDn, wUp);
     T y0[R2], y1[R2];
     T carry = T(0);
@@ -993,6 +991,18 @@ __device__ __forceinline__ void precond_pair(int lane, int nz, int64_t base,
       z1[m] = a1 + b1 * Nx;
     }
   }
+}
+
+// precond_pair_f with the factors loaded from invd, w.
+template <int R2, bool LINE, class T>
+__device__ __forceinline__ void precond_pair(int lane, int nz, int64_t base,
+                                             const T* __restrict__ invd, const T* __restrict__ w,
+                                             const T (&r0)[R2], const T (&r1)[R2], T (&z0)[R2],
+                                             T (&z1)[R2]) {
+  T iv0[R2], iv1[R2], w0[R2], w1[R2];
+  load_pair<R2>(invd, base, lane, nz, iv0, iv1);
+  if constexpr (LINE) load_pair<R2>(w, base, lane, nz, w0, w1);
+  precond_pair_f<R2, LINE>(lane, iv0, iv1, w0, w1, r0, r1, z0, z1);
 }
 
 // k_pcg_dir in the pair layout (one column a wave).
@@ -1717,6 +1727,32 @@ __global__ __launch_bounds__(kBlock) void k_inner_init_p(Grid g, const double* _
 // k_refine_p2 is k_refine64.  XV: x at a column of the pair layout (the
 // refinement adds sc d of the FP32 correction first).
 // ---------------------------------------------------------------------------
+// The stencil's coefficients at column i (A) and i + 1 (B), and the x leg at
+// i - 1 (W), in the pair layout.
+template <int R2>
+struct CoefPair {
+  double adA0[R2], adA1[R2], axA0[R2], axA1[R2], axW0[R2], axW1[R2], ayA0[R2], ayA1[R2],
+      azA0[R2], azA1[R2];
+  double adB0[R2], adB1[R2], axB0[R2], axB1[R2], ayB0[R2], ayB1[R2], azB0[R2], azB1[R2];
+};
+template <int R2>
+__device__ __forceinline__ void load_coef_pair(const Stencil<double>& A, int64_t ba, int64_t sx,
+                                               int lane, int nz, CoefPair<R2>& c) {
+  const int64_t bb = ba + sx;
+  load_pair<R2>(A.Ad, ba, lane, nz, c.adA0, c.adA1);
+  load_pair<R2>(A.Axp, ba, lane, nz, c.axA0, c.axA1);
+  load_pair<R2>(A.Axp, ba - sx, lane, nz, c.axW0, c.axW1);
+  load_pair<R2>(A.Ayp, ba, lane, nz, c.ayA0, c.ayA1);
+  load_pair<R2>(A.Azp, ba, lane, nz, c.azA0, c.azA1);
+  load_pair<R2>(A.Ad, bb, lane, nz, c.adB0, c.adB1);
+  load_pair<R2>(A.Axp, bb, lane, nz, c.axB0, c.axB1);
+  load_pair<R2>(A.Ayp, bb, lane, nz, c.ayB0, c.ayB1);
+  load_pair<R2>(A.Azp, bb, lane, nz, c.azB0, c.azB1);
+}
+
+// q = A x for columns i (qa) and i + 1 (qb) of the pair layout, from x at
+// j - 1 (P), j (C), j + 1 (N) and the outer columns (w: i - 1, e: i + 2), the
+// y leg toward j - 1 (Y) and the step's coefficients (c).
 template <int R2>
 __device__ __forceinline__ void stencil_pair_q(
     int lane, const double (&aP0)[R2], const double (&aP1)[R2], const double (&aC0)[R2],
@@ -1725,37 +1761,23 @@ __device__ __forceinline__ void stencil_pair_q(
     const double (&bC1)[R2], const double (&bN0)[R2], const double (&bN1)[R2],
     const double (&w0)[R2], const double (&w1)[R2], const double (&e0)[R2],
     const double (&e1)[R2], const double (&aY0)[R2], const double (&aY1)[R2],
-    const double (&bY0)[R2], const double (&bY1)[R2], const Stencil<double>& A, int64_t ba,
-    int64_t sx, int nz, double (&qa0)[R2], double (&qa1)[R2], double (&qb0)[R2],
-    double (&qb1)[R2], double (&ayA0)[R2], double (&ayA1)[R2], double (&ayB0)[R2],
-    double (&ayB1)[R2]) {
-  const int64_t bb = ba + sx;
-  double adA0[R2], adA1[R2], axA0[R2], axA1[R2], axW0[R2], axW1[R2], azA0[R2], azA1[R2];
-  double adB0[R2], adB1[R2], axB0[R2], axB1[R2], azB0[R2], azB1[R2];
-  load_pair<R2>(A.Ad, ba, lane, nz, adA0, adA1);
-  load_pair<R2>(A.Axp, ba, lane, nz, axA0, axA1);
-  load_pair<R2>(A.Axp, ba - sx, lane, nz, axW0, axW1);
-  load_pair<R2>(A.Ayp, ba, lane, nz, ayA0, ayA1);
-  load_pair<R2>(A.Azp, ba, lane, nz, azA0, azA1);
-  load_pair<R2>(A.Ad, bb, lane, nz, adB0, adB1);
-  load_pair<R2>(A.Axp, bb, lane, nz, axB0, axB1);
-  load_pair<R2>(A.Ayp, bb, lane, nz, ayB0, ayB1);
-  load_pair<R2>(A.Azp, bb, lane, nz, azB0, azB1);
+    const double (&bY0)[R2], const double (&bY1)[R2], const CoefPair<R2>& c,
+    double (&qa0)[R2], double (&qa1)[R2], double (&qb0)[R2], double (&qb1)[R2]) {
   double aDn[R2], aUp[R2], azAUp[R2], dA[R2], bDn[R2], bUp[R2], azBUp[R2], dB[R2];
   pair_shift<R2>(aC0, aC1, lane, aDn, aUp);
-  pair_shift<R2>(azA0, azA1, lane, dA, azAUp);
+  pair_shift<R2>(c.azA0, c.azA1, lane, dA, azAUp);
   pair_shift<R2>(bC0, bC1, lane, bDn, bUp);
-  pair_shift<R2>(azB0, azB1, lane, dB, azBUp);
+  pair_shift<R2>(c.azB0, c.azB1, lane, dB, azBUp);
 #pragma unroll
   for (int m = 0; m < R2; ++m) {
-    qa0[m] = adA0[m] * aC0[m] + axA0[m] * bC0[m] + axW0[m] * w0[m] + ayA0[m] * aN0[m] +
-             aY0[m] * aP0[m] + azA0[m] * aC1[m] + azAUp[m] * aUp[m];
-    qa1[m] = adA1[m] * aC1[m] + axA1[m] * bC1[m] + axW1[m] * w1[m] + ayA1[m] * aN1[m] +
-             aY1[m] * aP1[m] + azA1[m] * aDn[m] + azA0[m] * aC0[m];
-    qb0[m] = adB0[m] * bC0[m] + axB0[m] * e0[m] + axA0[m] * aC0[m] + ayB0[m] * bN0[m] +
-             bY0[m] * bP0[m] + azB0[m] * bC1[m] + azBUp[m] * bUp[m];
-    qb1[m] = adB1[m] * bC1[m] + axB1[m] * e1[m] + axA1[m] * aC1[m] + ayB1[m] * bN1[m] +
-             bY1[m] * bP1[m] + azB1[m] * bDn[m] + azB0[m] * bC0[m];
+    qa0[m] = c.adA0[m] * aC0[m] + c.axA0[m] * bC0[m] + c.axW0[m] * w0[m] + c.ayA0[m] * aN0[m] +
+             aY0[m] * aP0[m] + c.azA0[m] * aC1[m] + azAUp[m] * aUp[m];
+    qa1[m] = c.adA1[m] * aC1[m] + c.axA1[m] * bC1[m] + c.axW1[m] * w1[m] + c.ayA1[m] * aN1[m] +
+             aY1[m] * aP1[m] + c.azA1[m] * aDn[m] + c.azA0[m] * aC0[m];
+    qb0[m] = c.adB0[m] * bC0[m] + c.axB0[m] * e0[m] + c.axA0[m] * aC0[m] + c.ayB0[m] * bN0[m] +
+             bY0[m] * bP0[m] + c.azB0[m] * bC1[m] + azBUp[m] * bUp[m];
+    qb1[m] = c.adB1[m] * bC1[m] + c.axB1[m] * e1[m] + c.axA1[m] * aC1[m] + c.ayB1[m] * bN1[m] +
+             bY1[m] * bP1[m] + c.azB1[m] * bDn[m] + c.azB0[m] * bC0[m];
   }
 }
 
@@ -1791,17 +1813,30 @@ __global__ __launch_bounds__(kBlock) void k_sweep_p2(Grid g, March mj, Stencil<d
   KSum rz, rr, bbs;
   int i, j0, j1;
   if (march_span2(g, mj, i, j0, j1)) {
-    auto formx = [&](int64_t base, double (&v0)[R2], double (&v1)[R2]) {
-      load_pair<R2>(xi, base, lane, nz, v0, v1);
-      if (KIND == 1 && HASD) {
-        float d0[R2], d1[R2];
-        load_pair<R2>(d, base, lane, nz, d0, d1);
+    // x (and the correction d) of a column: loaded, then formed (xi + sc d)
+    struct Xd {
+      double v0[R2], v1[R2];
+      float d0[R2], d1[R2];
+    };
+    auto loadx = [&](int64_t base, Xd& o) {
+      load_pair<R2>(xi, base, lane, nz, o.v0, o.v1);
+      if (KIND == 1 && HASD) load_pair<R2>(d, base, lane, nz, o.d0, o.d1);
+    };
+    auto formxd = [&](const Xd& o, double (&v0)[R2], double (&v1)[R2]) {
 #pragma unroll
-        for (int m = 0; m < R2; ++m) {
-          v0[m] = v0[m] + sc * static_cast<double>(d0[m]);
-          v1[m] = v1[m] + sc * static_cast<double>(d1[m]);
+      for (int m = 0; m < R2; ++m) {
+        v0[m] = o.v0[m];
+        v1[m] = o.v1[m];
+        if (KIND == 1 && HASD) {
+          v0[m] = v0[m] + sc * static_cast<double>(o.d0[m]);
+          v1[m] = v1[m] + sc * static_cast<double>(o.d1[m]);
         }
       }
+    };
+    auto formx = [&](int64_t base, double (&v0)[R2], double (&v1)[R2]) {
+      Xd o;
+      loadx(base, o);
+      formxd(o, v0, v1);
     };
     double aP0[R2], aP1[R2], aC0[R2], aC1[R2], aY0[R2], aY1[R2];
     double bP0[R2], bP1[R2], bC0[R2], bC1[R2], bY0[R2], bY1[R2];
@@ -1813,18 +1848,36 @@ __global__ __launch_bounds__(kBlock) void k_sweep_p2(Grid g, March mj, Stencil<d
     load_pair<R2>(A.Ayp, g.c3(j0 - 1, i + 1, 0), lane, nz, bY0, bY1);
     for (int j = j0; j <= j1; ++j) {
       const int64_t ba = g.c3(j, i, 0), bb = ba + sx;
-      double aN0[R2], aN1[R2], bN0[R2], bN1[R2], w0[R2], w1[R2], e0[R2], e1[R2];
-      formx(ba + sy, aN0, aN1);
-      formx(bb + sy, bN0, bN1);
-      formx(ba - sx, w0, w1);  // column i - 1
-      formx(bb + sx, e0, e1);  // column i + 2
-      double qa0[R2], qa1[R2], qb0[R2], qb1[R2], ayA0[R2], ayA1[R2], ayB0[R2], ayB1[R2];
-      stencil_pair_q<R2>(lane, aP0, aP1, aC0, aC1, aN0, aN1, bP0, bP1, bC0, bC1, bN0, bN1, w0,
-                         w1, e0, e1, aY0, aY1, bY0, bY1, A, ba, sx, nz, qa0, qa1, qb0, qb1, ayA0,
-                         ayA1, ayB0, ayB1);
+      // every load of the step ahead of the arithmetic (left to itself the
+      // scheduler spreads them between waits)
+      Xd xaN, xbN, xW, xE;
+      loadx(ba + sy, xaN);
+      loadx(bb + sy, xbN);
+      loadx(ba - sx, xW);  // column i - 1
+      loadx(bb + sx, xE);  // column i + 2
       double bA0[R2], bA1[R2], bB0[R2], bB1[R2];
       load_pair<R2>(b, ba, lane, nz, bA0, bA1);
       load_pair<R2>(b, bb, lane, nz, bB0, bB1);
+      double ivA0[R2], ivA1[R2], wA0[R2], wA1[R2], ivB0[R2], ivB1[R2], wB0[R2], wB1[R2];
+      if constexpr (KIND == 0) {
+        load_pair<R2>(invd, ba, lane, nz, ivA0, ivA1);
+        load_pair<R2>(invd, bb, lane, nz, ivB0, ivB1);
+        if constexpr (LINE) {
+          load_pair<R2>(w, ba, lane, nz, wA0, wA1);
+          load_pair<R2>(w, bb, lane, nz, wB0, wB1);
+        }
+      }
+      CoefPair<R2> cp;
+      load_coef_pair<R2>(A, ba, sx, lane, nz, cp);
+      __builtin_amdgcn_sched_barrier(0);
+      double aN0[R2], aN1[R2], bN0[R2], bN1[R2], w0[R2], w1[R2], e0[R2], e1[R2];
+      formxd(xaN, aN0, aN1);
+      formxd(xbN, bN0, bN1);
+      formxd(xW, w0, w1);
+      formxd(xE, e0, e1);
+      double qa0[R2], qa1[R2], qb0[R2], qb1[R2];
+      stencil_pair_q<R2>(lane, aP0, aP1, aC0, aC1, aN0, aN1, bP0, bP1, bC0, bC1, bN0, bN1, w0,
+                         w1, e0, e1, aY0, aY1, bY0, bY1, cp, qa0, qa1, qb0, qb1);
       double rA0[R2], rA1[R2], rB0[R2], rB1[R2];
 #pragma unroll
       for (int m = 0; m < R2; ++m) {
@@ -1848,8 +1901,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_p2(Grid g, March mj, Stencil<d
       store_pair<R2>(r, bb, lane, nz, rB0, rB1);
       if constexpr (KIND == 0) {
         double zA0[R2], zA1[R2], zB0[R2], zB1[R2];
-        precond_pair<R2, LINE>(lane, nz, ba, invd, w, rA0, rA1, zA0, zA1);
-        precond_pair<R2, LINE>(lane, nz, bb, invd, w, rB0, rB1, zB0, zB1);
+        precond_pair_f<R2, LINE>(lane, ivA0, ivA1, wA0, wA1, rA0, rA1, zA0, zA1);
+        precond_pair_f<R2, LINE>(lane, ivB0, ivB1, wB0, wB1, rB0, rB1, zB0, zB1);
         store_pair<R2>(z, ba, lane, nz, zA0, zA1);
         store_pair<R2>(z, bb, lane, nz, zB0, zB1);
 #pragma unroll
@@ -1871,14 +1924,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep_p2(Grid g, March mj, Stencil<d
         aP1[m] = aC1[m];
         aC0[m] = aN0[m];
         aC1[m] = aN1[m];
-        aY0[m] = ayA0[m];
-        aY1[m] = ayA1[m];
+        aY0[m] = cp.ayA0[m];
+        aY1[m] = cp.ayA1[m];
         bP0[m] = bC0[m];
         bP1[m] = bC1[m];
         bC0[m] = bN0[m];
         bC1[m] = bN1[m];
-        bY0[m] = ayB0[m];
-        bY1[m] = ayB1[m];
+        bY0[m] = cp.ayB0[m];
+        bY1[m] = cp.ayB1[m];
       }
     }
   }
