@@ -1281,24 +1281,30 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil
   KSum pq;
   int i, j0, j1;
   if (march_span2(g, mj, i, j0, j1)) {
-    auto formp = [&](int64_t base, T (&p0)[R2], T (&p1)[R2]) {
-      T z0[R2], z1[R2];
-      load_pair<R2>(z, base, lane, nz, z0, z1);
-      if (RESTART) {
+    // p = z + beta p_old of a column: loaded (loadp), then formed (mkp)
+    struct Zo {
+      T z0[R2], z1[R2], o0[R2], o1[R2];
+    };
+    auto loadp = [&](int64_t base, Zo& v) {
+      load_pair<R2>(z, base, lane, nz, v.z0, v.z1);
+      if (!RESTART) load_pair<R2>(pOld, base, lane, nz, v.o0, v.o1);
+    };
+    auto mkp = [&](const Zo& v, T (&p0)[R2], T (&p1)[R2]) {
 #pragma unroll
-        for (int m = 0; m < R2; ++m) {
-          p0[m] = z0[m];
-          p1[m] = z1[m];
-        }
-      } else {
-        T o0[R2], o1[R2];
-        load_pair<R2>(pOld, base, lane, nz, o0, o1);
-#pragma unroll
-        for (int m = 0; m < R2; ++m) {
-          p0[m] = z0[m] + beta * o0[m];
-          p1[m] = z1[m] + beta * o1[m];
+      for (int m = 0; m < R2; ++m) {
+        if (RESTART) {
+          p0[m] = v.z0[m];
+          p1[m] = v.z1[m];
+        } else {
+          p0[m] = v.z0[m] + beta * v.o0[m];
+          p1[m] = v.z1[m] + beta * v.o1[m];
         }
       }
+    };
+    auto formp = [&](int64_t base, T (&p0)[R2], T (&p1)[R2]) {
+      Zo v;
+      loadp(base, v);
+      mkp(v, p0, p1);
     };
     // column a = i, column b = i + 1
     T aP0[R2], aP1[R2], aC0[R2], aC1[R2], aY0[R2], aY1[R2];
@@ -1311,11 +1317,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil
     load_pair<R2>(A.Ayp, g.c3(j0 - 1, i + 1, 0), lane, nz, bY0, bY1);
     for (int j = j0; j <= j1; ++j) {
       const int64_t ba = g.c3(j, i, 0), bb = ba + sx;
-      T aN0[R2], aN1[R2], bN0[R2], bN1[R2], w0[R2], w1[R2], e0[R2], e1[R2];
-      formp(ba + sy, aN0, aN1);
-      formp(bb + sy, bN0, bN1);
-      formp(ba - sx, w0, w1);  // column i - 1
-      formp(bb + sx, e0, e1);  // column i + 2
+      Zo vaN, vbN, vW, vE;
+      loadp(ba + sy, vaN);
+      loadp(bb + sy, vbN);
+      loadp(ba - sx, vW);  // column i - 1
+      loadp(bb + sx, vE);  // column i + 2
       T adA0[R2], adA1[R2], axA0[R2], axA1[R2], axW0[R2], axW1[R2], ayA0[R2], ayA1[R2],
           azA0[R2], azA1[R2];
       T adB0[R2], adB1[R2], axB0[R2], axB1[R2], ayB0[R2], ayB1[R2], azB0[R2], azB1[R2];
@@ -1328,10 +1334,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_p2(Grid g, March mj, Stencil
       load_pair<R2>(A.Axp, bb, lane, nz, axB0, axB1);
       load_pair<R2>(A.Ayp, bb, lane, nz, ayB0, ayB1);
       load_pair<R2>(A.Azp, bb, lane, nz, azB0, azB1);
-      // The restart form (p = z) is scheduled with its loads spread between
-      // waits (0.43 against 0.33 ms a launch); the barrier keeps every load
-      // of the step ahead of the arithmetic, as in the form that reads p_old.
-      if constexpr (RESTART) __builtin_amdgcn_sched_barrier(0);
+      // Every load of the step ahead of the arithmetic: left to itself the
+      // scheduler spreads them between waits in some forms (the restart
+      // form at R2 = 1: 0.43 against 0.29 ms a launch).
+      __builtin_amdgcn_sched_barrier(0);
+      T aN0[R2], aN1[R2], bN0[R2], bN1[R2], w0[R2], w1[R2], e0[R2], e1[R2];
+      mkp(vaN, aN0, aN1);
+      mkp(vbN, bN0, bN1);
+      mkp(vW, w0, w1);
+      mkp(vE, e0, e1);
       T aDn[R2], aUp[R2], azAUp[R2], dummyA[R2], bDn[R2], bUp[R2], azBUp[R2], dummyB[R2];
       pair_shift<R2>(aC0, aC1, lane, aDn, aUp);
       pair_shift<R2>(azA0, azA1, lane, dummyA, azAUp);
