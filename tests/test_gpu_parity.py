@@ -296,14 +296,16 @@ def test_trajectory_threshold_sensitive(make):
 # pairs of row pairs (nx, nz even, nz <= 256), FP64 row pairs one column a
 # wave (nx odd, or 256 < nz <= 512), one row a lane (nz odd or > 512); FP32
 # quads (nx even, nz % 4 == 0, nz <= 128), column pairs of row pairs, row
-# pairs, one row a lane.  Each case is held to the oracle step by step.
+# pairs, one row a lane.  Each case is held to the oracle step by step
+# (nz = 200: the column-pair kernels with two row pairs a lane, FP64 and FP32).
 @pytest.mark.parametrize("nx,ny,nz", [(12, 10, 100), (11, 10, 100), (12, 9, 300), (6, 5, 520),
-                                      (12, 10, 97)])
+                                      (12, 10, 97), (12, 9, 200)])
 def test_stepwise_fp64_layouts(nx, ny, nz):
     _stepwise(cases.hetero_512x512x100(nx=nx, ny=ny, nz=nz), 3)
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(16, 10, 100), (16, 10, 64), (15, 10, 100), (16, 10, 98)])
+@pytest.mark.parametrize("nx,ny,nz", [(16, 10, 100), (16, 10, 64), (15, 10, 100), (16, 10, 98),
+                                      (12, 9, 200)])
 def test_mixed_precision_layouts(nx, ny, nz):
     c = cases.hetero_512x512x100(nx=nx, ny=ny, nz=nz, precision="mixed")
     assert _mixed_stepwise(c, 4) == 0
